@@ -1,0 +1,140 @@
+/*
+ * ugpack.h -- C ABI of the MI355X RSA worldline packer (libugpack.so).
+ *
+ * The reference exposes this path as a process boundary, not an FFI: the engine
+ * binaries cuda/braid_cuda (2+1) and cuda/braid_cuda3d (3+1) take the argv that
+ * braidlab/engine.py:50-101 (build_command) assembles, print the
+ * "done: N=<n> in <attempts> attempts" line that braidlab/engine.py:104-112
+ * (parse_done) reads, and write the curve / parameter-dump CSVs
+ * (cuda/braid_cuda.cu:1045-1104).  The functions below are what a binding of that
+ * path binds instead: one handle per packing run, all device work on the caller's
+ * HIP stream, plain pointers and sizes, no exit() on error.
+ *
+ * Conventions
+ *   - Return 0 on success, a negative UG_E* code on failure; ug_last_error() holds
+ *     the message for the calling thread.
+ *   - A handle is used by one host thread at a time; handles are independent.
+ *   - Host output buffers are caller-owned; a cap of 0 with a non-NULL count
+ *     pointer is a size query.  Device outputs (dev_*) are caller-owned device
+ *     memory on the handle's device.
+ */
+#ifndef UGPACK_H
+#define UGPACK_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UG_ABI_VERSION 1
+
+enum {
+    UG_OK = 0,
+    UG_EINVAL = -1,    /* invalid argument or configuration (engine "error:" exits) */
+    UG_ENOMEM = -2,    /* device or host allocation failed */
+    UG_EHIP = -3,      /* HIP runtime error */
+    UG_EOVERFLOW = -4, /* a fixed capacity was exceeded (never silently truncated) */
+    UG_ESTATE = -5     /* call not valid in the handle's current state */
+};
+
+/* Precision of the prefilter.  Both give the identical accepted set. */
+enum {
+    UG_PREC_FP64 = 0,      /* plain all-FP64 GPU form */
+    UG_PREC_SPLIT_BF16 = 1 /* split-BF16 MFMA evaluation + FP64 recheck near the radius */
+};
+
+/* Stop rules. */
+enum {
+    UG_STOP_BUDGET = 0,       /* candidates [0, max_attempts) */
+    UG_STOP_FIXED_WINDOW = 1, /* at each multiple c of stop_batch with c >= stop_window:
+                                 stop when accepts in [c-stop_window, c) < stop_rate*stop_window */
+    UG_STOP_REF_WINDOW = 2    /* the reference's --until-accept-rate window
+                                 (cuda/braid_cuda.cu:585-590,930-943) over stop_batch batches */
+};
+
+/* One packing run: the flags of cuda/braid_cuda.cu:534-584 / cuda/braid_cuda3d.cu:862-913. */
+typedef struct ug_pack_config {
+    int32_t dim;         /* 2 = braid_cuda, 3 = braid_cuda3d */
+    int32_t T;           /* -t: timesteps; the max frequency is T (--maxfreq must equal T) */
+    int32_t terms;       /* --terms incl. sin1; 0 = T (full spectrum) */
+    int32_t precision;   /* UG_PREC_* */
+    uint64_t seed;       /* --seed: Philox key */
+    int32_t pin_sin1;    /* --pin-sin1 */
+    int32_t stop_kind;   /* UG_STOP_* */
+    double max_attempts; /* --attempts */
+    double stop_rate;    /* --until-accept-rate */
+    int64_t stop_batch;  /* fixed-window rule: boundary spacing (default 2^20) */
+    int64_t stop_window; /* fixed-window rule: window length (default 2^22) */
+    int64_t max_n;       /* --max-n: stop after this many uniques (0 = off) */
+    double margin;       /* split-BF16 certainty margin; 0 = derived error bound */
+    int64_t max_paths;   /* initial path capacity (0 = automatic; grows on demand) */
+} ug_pack_config;
+
+typedef struct ug_pack_status {
+    int64_t attempts;  /* candidates consumed (the done: line's attempts) */
+    int64_t n;         /* accepted unique worldlines (the done: line's N) */
+    int64_t nsub;      /* accepted subpaths */
+    int32_t done;      /* 1 once the stop rule (or budget / max_n) has fired */
+    int32_t reserved;
+    int64_t survivors; /* candidates that reached the in-batch accept */
+    int64_t rechecked; /* candidates settled by the FP64 recheck (split-BF16 form) */
+    double margin;     /* certainty margin in use (split-BF16 form) */
+    double seconds;    /* wall time spent inside ug_pack_run */
+} ug_pack_status;
+
+typedef struct ug_pack ug_pack;
+
+/* ABI version of the loaded library (UG_ABI_VERSION). */
+int32_t ug_abi_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* ug_last_error(void);
+
+/* Validates cfg like the engine's argv checks (cuda/braid_cuda.cu:597-636), builds the
+ * basis tables and the device grid, and binds the handle to hip_stream (NULL = the
+ * default stream) on the current device.  Replaces launching braid_cuda[3d]. */
+int32_t ug_pack_create(const ug_pack_config* cfg, void* hip_stream, ug_pack** out);
+
+/* Runs the packing loop (cuda/braid_cuda.cu:885-944) until the stop rule fires or
+ * max_candidates more candidates have been consumed (0 = no limit).  st may be NULL. */
+int32_t ug_pack_run(ug_pack* h, int64_t max_candidates, ug_pack_status* st);
+
+/* Subpath phase (cuda/braid_cuda.cu:949-1043, --subpaths): tests candidates continuing
+ * the index stream for count more candidates with the adoption rule of host_sub_gid
+ * (:726-750); dim 2 only, like the reference (braidlab/config.py:56-57). */
+int32_t ug_pack_run_sub(ug_pack* h, int32_t sub_terms, int64_t count, ug_pack_status* st);
+
+/* Current status without running. */
+int32_t ug_pack_get_status(const ug_pack* h, ug_pack_status* st);
+
+/* Accepted candidate indices (admission order) and group ids; replaces the dump's row
+ * order and gid column (cuda/braid_cuda.cu:1062-1104).  *n receives the total. */
+int32_t ug_pack_accepted(const ug_pack* h, int64_t* idx, int32_t* gid, int64_t cap, int64_t* n);
+
+/* Dump parameters of accepted paths [first, first+count): per path and axis,
+ * a1 then nw triples (amplitude, frequency, phase) -- the values the reference
+ * prints per row (cuda/braid_cuda.cu:1086-1099).  out holds count*dim*(1+3*nw) doubles. */
+int32_t ug_pack_params(const ug_pack* h, int64_t first, int64_t count, double* out);
+
+/* Canonical wrapped comoving positions of all accepted paths, [n][T][dim] FP64, into
+ * caller device memory of cap paths; the reference keeps them in px/py
+ * (cuda/braid_cuda.cu:685-694). */
+int32_t ug_pack_positions(const ug_pack* h, double* dev_out, int64_t cap);
+
+/* Log-spaced growth curve (attempts, n, nsub, filled) as cuda/braid_cuda.cu:925-929. */
+int32_t ug_pack_curve(const ug_pack* h, int64_t* rows4, int64_t cap, int64_t* n);
+
+/* Positions of arbitrary candidates of the handle's model: dev_idx[count] -> dev_out
+ * [count][T][dim] (FP64).  precision UG_PREC_FP64 gives the canonical values
+ * (collides_dev's evaluation, cuda/braid_cuda.cu:418-441); UG_PREC_SPLIT_BF16 gives the
+ * prefilter's split-BF16 MFMA values (for error reporting). */
+int32_t ug_eval_candidates(const ug_pack* h, const int64_t* dev_idx, int64_t count,
+                           int32_t precision, double* dev_out);
+
+/* Releases the handle and its device memory. */
+void ug_pack_destroy(ug_pack* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

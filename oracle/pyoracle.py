@@ -1,0 +1,200 @@
+"""ctypes view of the CPU oracle (liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, the smoke check and bench.py's cpu_baseline leg as the
+checker; never by the product path.  See ug_oracle.h for what each function
+restates from the reference.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+        L.or_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.or_philox4x32_10.restype = None
+        L.or_sincospi_rat.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double)]
+        L.or_sincospi_rat.restype = None
+        L.or_log.argtypes = [C.c_double]
+        L.or_log.restype = C.c_double
+        L.or_model_init.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int]
+        L.or_model_init.restype = C.c_int
+        L.or_model_free.argtypes = [C.c_void_p]
+        L.or_model_free.restype = None
+        L.or_candidate.argtypes = [C.c_void_p, C.c_uint64] + [C.c_void_p] * 6
+        L.or_candidate.restype = None
+        L.or_eval_t.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 5
+        L.or_eval_t.restype = None
+        L.or_cell.argtypes = [C.c_void_p, C.c_double]
+        L.or_cell.restype = C.c_int
+        L.or_pack.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_int64, C.c_double,
+                              C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                              C.POINTER(C.c_int64)]
+        L.or_pack.restype = C.c_int64
+        L.or_pack_sub.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
+                                  C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                  C.POINTER(C.c_int64)]
+        L.or_pack_sub.restype = C.c_int64
+        L.or_check_rsa.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
+        L.or_check_rsa.restype = C.c_int64
+        _lib = L
+    return _lib
+
+
+class _Model(C.Structure):
+    _fields_ = [
+        ("d", C.c_int), ("T", C.c_int), ("nw", C.c_int), ("modmax", C.c_int), ("pin", C.c_int),
+        ("seed", C.c_uint64), ("r", C.c_double), ("halfT", C.c_double),
+        ("S", C.POINTER(C.c_double)), ("C", C.POINTER(C.c_double)),
+        ("sz", C.POINTER(C.c_double)), ("order", C.POINTER(C.c_int)),
+    ]
+
+
+def philox(ctr, key) -> np.ndarray:
+    out = np.zeros(4, np.uint32)
+    lib().or_philox4x32_10(np.ascontiguousarray(ctr, np.uint32),
+                           np.ascontiguousarray(key, np.uint32), out)
+    return out
+
+
+def sincospi_rat(n: int, m: int) -> tuple[float, float]:
+    s, c = C.c_double(), C.c_double()
+    lib().or_sincospi_rat(n, m, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def log(v: float) -> float:
+    return lib().or_log(v)
+
+
+def _p(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+@dataclass
+class Candidate:
+    a1: np.ndarray      # (d,)
+    b: np.ndarray       # (d, nw) int
+    c: np.ndarray       # (d, nw) a_j cos f_j
+    s: np.ndarray       # (d, nw) a_j sin f_j
+    amp: np.ndarray     # (d, nw) signed amplitudes a_j
+    phase: np.ndarray   # (d, nw) phases f_j
+
+
+class Model:
+    """Oracle model for (d, T, terms, seed, pin)."""
+
+    def __init__(self, d: int, T: int, terms: int | None = None, seed: int = 1, pin: bool = False):
+        self.d, self.T = d, T
+        self.terms = T if terms is None else terms
+        self._m = _Model()
+        rc = lib().or_model_init(C.byref(self._m), d, T, self.terms, seed, int(pin))
+        if rc != 0:
+            raise ValueError(f"or_model_init failed ({rc}) for d={d} T={T} terms={self.terms}")
+        self.nw = self._m.nw
+        self.r = self._m.r
+
+    def __del__(self):
+        try:
+            lib().or_model_free(C.byref(self._m))
+        except Exception:
+            pass
+
+    @property
+    def ref(self):
+        return C.byref(self._m)
+
+    @property
+    def S(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._m.S, shape=(self.T, self.T + 1)).copy()
+
+    @property
+    def C(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._m.C, shape=(self.T, self.T + 1)).copy()
+
+    @property
+    def sz(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._m.sz, shape=(self.T,)).copy()
+
+    @property
+    def order(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._m.order, shape=(self.T,)).copy()
+
+    def candidate(self, idx: int) -> Candidate:
+        d, nw = self.d, self.nw
+        a1 = np.zeros(d)
+        b = np.zeros((d, nw), np.int32)
+        c = np.zeros((d, nw))
+        s = np.zeros((d, nw))
+        amp = np.zeros((d, nw))
+        ph = np.zeros((d, nw))
+        lib().or_candidate(self.ref, idx, _p(a1), _p(b), _p(c), _p(s), _p(amp), _p(ph))
+        return Candidate(a1, b, c, s, amp, ph)
+
+    def positions(self, idx: int) -> np.ndarray:
+        """Canonical wrapped positions (T, d) of candidate idx."""
+        cd = self.candidate(idx)
+        X = np.zeros((self.T, self.d))
+        for t in range(self.T):
+            lib().or_eval_t(self.ref, t, _p(cd.a1), _p(cd.b), _p(cd.c), _p(cd.s), _p(X[t]))
+        return X
+
+    def cell(self, x: float) -> int:
+        return lib().or_cell(self.ref, x)
+
+    def pack(self, max_candidates: int, stop_kind: int = 0, stop_batch: int = 1 << 20,
+             window: int = 1 << 22, stop_rate: float = 1e-4, max_n: int = 0,
+             cap: int = 1 << 22, want_pos: bool = False):
+        """Serial RSA; returns (idx int64[N], gid int32[N], attempts[, pos (N,T,d)])."""
+        idx = np.zeros(cap, np.int64)
+        gid = np.zeros(cap, np.int32)
+        pos = np.zeros((cap, self.T, self.d)) if want_pos else None
+        att = C.c_int64(0)
+        n = lib().or_pack(self.ref, max_candidates, stop_kind, stop_batch, window, stop_rate,
+                          max_n, _p(idx), _p(gid), _p(pos) if want_pos else None, cap,
+                          C.byref(att))
+        if n < 0:
+            raise RuntimeError(f"or_pack failed ({n})")
+        n = min(n, cap)
+        out = (idx[:n].copy(), gid[:n].copy(), att.value)
+        if want_pos:
+            out = out + (pos[:n].copy(),)
+        return out
+
+    def pack_sub(self, pos0: np.ndarray, gid0: np.ndarray, first: int, count: int,
+                 cap: int = 1 << 20):
+        pos0 = np.ascontiguousarray(pos0, np.float64)
+        gid0 = np.ascontiguousarray(gid0, np.int32)
+        idx = np.zeros(cap, np.int64)
+        gid = np.zeros(cap, np.int32)
+        filled = C.c_int64(0)
+        n = lib().or_pack_sub(self.ref, _p(pos0), _p(gid0), len(gid0), first, count, _p(idx),
+                              _p(gid), cap, C.byref(filled))
+        if n < 0:
+            raise RuntimeError(f"or_pack_sub failed ({n})")
+        n = min(n, cap)
+        return idx[:n].copy(), gid[:n].copy(), filled.value
+
+    def check_rsa(self, M: int, acc: np.ndarray) -> int:
+        acc = np.ascontiguousarray(acc, np.int64)
+        return lib().or_check_rsa(self.ref, M, _p(acc), len(acc))

@@ -1,0 +1,703 @@
+/*
+ * ug_oracle.c -- CPU oracle for the RSA worldline packer.  TEST INFRASTRUCTURE ONLY:
+ * loaded by tests/, the smoke check and bench.py's cpu_baseline leg, never by the
+ * product path.
+ *
+ * What it restates (reference files under universe-analysis/universe-generator):
+ *   - proposal measure: cuda/braid_cuda.cu:295-373 (multi-term branch), with the
+ *     draw stages keyed by Philox counters instead of a serial xoshiro stream;
+ *     full-pool frequency set cuda/braid_cuda.cu:203-219, unique-frequency
+ *     rejection draws :220-243, Dirichlet(1) by exponential spacings :250-264,
+ *     a_j = w_j/b_j with random signs :348-359, phases on even b only :366-371,
+ *     sin1 amplitude U(-1,1) (cuda/braid_knot.cu:164);
+ *   - time grid z_i = (i+1)*pi/(T+1), endpoint-first scan: cuda/braid_cuda.cu:647-664;
+ *   - evaluation X = [a1 sin z + sum a_j(sin(b_j z + f_j) - sin f_j)]/sin z in the
+ *     basis-table form of cuda/braid_knot.cu:219-230,341-354;
+ *   - torus_wrap / torus_delta: cuda/braid_cuda.cu:158-168;
+ *   - Chebyshev exclusion of half-width 2/T, inclusive: cuda/braid_cuda.cu:449-451,604;
+ *   - per-timestep hash grid, 3^d neighbour scan: cuda/braid_cuda.cu:697-722,865-879;
+ *   - index-ordered serial admission: cuda/braid_knot.cu:433-462;
+ *   - subpath adoption rule: cuda/braid_cuda.cu:726-750.
+ *
+ * The arithmetic is the canonical FP64 definition shared with the GPU kernels
+ * (explicit fma, fixed summation tree, own sin/cos/log); this file is compiled
+ * with -ffp-contract=off so no other contraction happens.
+ */
+#include "ug_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------- Philox4x32-10 */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t x0 = ctr[0], x1 = ctr[1], x2 = ctr[2], x3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; round++) {
+        if (round > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)x0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)x2;
+        uint32_t y0 = (uint32_t)(p1 >> 32) ^ x1 ^ k0;
+        uint32_t y1 = (uint32_t)p1;
+        uint32_t y2 = (uint32_t)(p0 >> 32) ^ x3 ^ k1;
+        uint32_t y3 = (uint32_t)p0;
+        x0 = y0;
+        x1 = y1;
+        x2 = y2;
+        x3 = y3;
+    }
+    out[0] = x0;
+    out[1] = x1;
+    out[2] = x2;
+    out[3] = x3;
+}
+
+/* ------------------------------------------------------------ sin(pi x), cos(pi x)
+ * Taylor coefficients of sin(pi r) and cos(pi r) in r (|r| <= 1/4):
+ * (-1)^i pi^(2i+1)/(2i+1)! and (-1)^i pi^(2i)/(2i)!, rounded to double. */
+static const double kSinPi[10] = {
+    0x1.921fb54442d18p+1,  -0x1.4abbce625be53p+2, 0x1.466bc6775aae2p+1,  -0x1.32d2cce62bd86p-1,
+    0x1.50783487ee782p-4,  -0x1.e3074fde8871fp-8, 0x1.e8f434d018d63p-12, -0x1.6fadb9f155744p-16,
+    0x1.aaec32af93359p-21, -0x1.8a404211f9547p-26};
+static const double kCosPi[10] = {
+    0x1.0000000000000p+0,  -0x1.3bd3cc9be45dep+2, 0x1.03c1f081b5ac4p+2,  -0x1.55d3c7e3cbffap+0,
+    0x1.e1f506891babbp-3,  -0x1.a6d1f2a204a8cp-6, 0x1.f9d38a3763cc3p-10, -0x1.b6e24f44b128fp-14,
+    0x1.20c62c2f2d7f5p-18, -0x1.2a0c591af8314p-23};
+
+void or_sincospi_rat(uint64_t n, uint64_t m, double* s, double* c) {
+    /* Exact reduction of pi*n/m to pi*(r + q/2), |r| <= 1/4, q in 0..4. */
+    uint64_t two_m = 2 * m;
+    uint64_t nn = n % two_m;
+    uint64_t q = (4 * nn + m) / two_m;
+    int64_t num = (int64_t)(2 * nn) - (int64_t)(q * m);
+    double r = (double)num / (double)two_m;
+    double x = r * r;
+    double ps = kSinPi[9];
+    for (int i = 8; i >= 0; i--)
+        ps = fma(ps, x, kSinPi[i]);
+    double pc = kCosPi[9];
+    for (int i = 8; i >= 0; i--)
+        pc = fma(pc, x, kCosPi[i]);
+    double sn = r * ps, cs = pc;
+    switch (q & 3) {
+        case 0: *s = sn; *c = cs; break;
+        case 1: *s = cs; *c = -sn; break;
+        case 2: *s = -sn; *c = -cs; break;
+        default: *s = -cs; *c = sn; break;
+    }
+}
+
+/* ----------------------------------------------------------------------- log(v)
+ * v = 2^e * m, m in [sqrt(1/2), sqrt(2)];  log m = 2 atanh(s), s = (m-1)/(m+1),
+ * 2 atanh(s) = 2s + s * z * sum_{i=1..10} (2/(2i+1)) z^(i-1),  z = s^2 <= 0.0295. */
+static const double kLogAt[10] = {
+    0x1.5555555555555p-1, 0x1.999999999999ap-2, 0x1.2492492492492p-2, 0x1.c71c71c71c71cp-3,
+    0x1.745d1745d1746p-3, 0x1.3b13b13b13b14p-3, 0x1.1111111111111p-3, 0x1.e1e1e1e1e1e1ep-4,
+    0x1.af286bca1af28p-4, 0x1.8618618618618p-4};
+static const double kLn2Hi = 0x1.62e42fee00000p-1;
+static const double kLn2Lo = 0x1.a39ef35793c76p-33;
+
+double or_log(double v) {
+    uint64_t bits;
+    memcpy(&bits, &v, 8);
+    int e = (int)((bits >> 52) & 0x7ff) - 1023;
+    uint64_t mb = (bits & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+    if (mb > 0x3ff6a09e667f3bcdULL) { /* m > sqrt(2): use m/2 */
+        mb -= 0x0010000000000000ULL;
+        e += 1;
+    }
+    double m;
+    memcpy(&m, &mb, 8);
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double p = kLogAt[9];
+    for (int i = 8; i >= 0; i--)
+        p = fma(p, z, kLogAt[i]);
+    double lm = fma(s, z * p, 2.0 * s);
+    double de = (double)e;
+    return fma(de, kLn2Hi, fma(de, kLn2Lo, lm));
+}
+
+/* ------------------------------------------------------------------------ model */
+static double wrap1(double x) { /* torus_wrap, cuda/braid_cuda.cu:158-160 */
+    return x - 2.0 * floor((x + 1.0) * 0.5);
+}
+static double tdelta(double d) { /* torus_delta, cuda/braid_cuda.cu:161-168 */
+    if (d > 1.0)
+        return d - 2.0;
+    if (d < -1.0)
+        return d + 2.0;
+    return d;
+}
+
+int or_model_init(or_model* m, int d, int T, int terms, uint64_t seed, int pin) {
+    memset(m, 0, sizeof(*m));
+    if ((d != 2 && d != 3) || T < 4 || T > 4096)
+        return -1;
+    int modmax = T - 1 > 2 ? T - 1 : 2;
+    if (terms < 2)
+        terms = 2;
+    if (terms - 1 > modmax)
+        return -1;
+    m->d = d;
+    m->T = T;
+    m->nw = terms - 1;
+    m->modmax = modmax;
+    m->pin = pin;
+    m->seed = seed;
+    m->r = 2.0 / (double)T;
+    m->halfT = 0.5 * (double)T;
+    int W = T + 1;
+    m->S = (double*)calloc((size_t)T * W, sizeof(double));
+    m->C = (double*)calloc((size_t)T * W, sizeof(double));
+    m->sz = (double*)calloc((size_t)T, sizeof(double));
+    m->order = (int*)calloc((size_t)T, sizeof(int));
+    if (!m->S || !m->C || !m->sz || !m->order)
+        return -2;
+    /* z_t = (t+1) pi/(T+1) (cuda/braid_cuda.cu:655-660). */
+    uint64_t D = (uint64_t)T + 1;
+    for (int t = 0; t < T; t++) {
+        double s1, c1;
+        or_sincospi_rat((uint64_t)(t + 1), D, &s1, &c1);
+        m->sz[t] = s1;
+        for (int k = 0; k <= T; k++) {
+            double sk, ck, hk, hc;
+            or_sincospi_rat((uint64_t)k * (uint64_t)(t + 1), D, &sk, &ck);
+            or_sincospi_rat((uint64_t)k * (uint64_t)(t + 1), 2 * D, &hk, &hc);
+            m->S[(size_t)t * W + k] = sk / s1;
+            m->C[(size_t)t * W + k] = (-2.0 * (hk * hk)) / s1;
+        }
+    }
+    /* Endpoint-first order (cuda/braid_cuda.cu:663-664): 0, T-1, 1, T-2, ... */
+    int o = 0;
+    for (int i = 0; o < T; i++) {
+        m->order[o++] = i;
+        if (o < T)
+            m->order[o++] = T - 1 - i;
+    }
+    return 0;
+}
+
+void or_model_free(or_model* m) {
+    free(m->S);
+    free(m->C);
+    free(m->sz);
+    free(m->order);
+    memset(m, 0, sizeof(*m));
+}
+
+static void philox_at(const or_model* m, uint64_t idx, uint32_t c2, uint32_t c3, uint32_t w[4]) {
+    uint32_t ctr[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), c2, c3};
+    uint32_t key[2] = {(uint32_t)m->seed, (uint32_t)(m->seed >> 32)};
+    or_philox4x32_10(ctr, key, w);
+}
+
+void or_candidate(const or_model* m, uint64_t idx, double* a1, int* b, double* c, double* s,
+                  double* amp, double* phase) {
+    const int d = m->d, nw = m->nw;
+    const int npairs = (nw + 1) / 2;
+    uint32_t w0[4];
+    philox_at(m, idx, 0u, 0u, w0);
+    for (int a = 0; a < d; a++) {
+        /* sin1 amplitude: (2w+1) 2^-32 - 1, uniform on (-1, 1). */
+        a1[a] = m->pin ? 0.0 : (double)(2 * (uint64_t)w0[a] + 1) * 0x1p-32 - 1.0;
+        int* ba = b + (size_t)a * nw;
+        /* Frequencies: the full pool {2..modmax+1} in ascending slot order, or nw
+         * unique rejection draws (cuda/braid_cuda.cu:220-243) from stream 2. */
+        if (nw == m->modmax) {
+            for (int j = 0; j < nw; j++)
+                ba[j] = j + 2;
+        } else {
+            uint32_t g = 0, wb[4] = {0, 0, 0, 0};
+            for (int j = 0; j < nw; j++) {
+                int bb = 0, dup = 1;
+                for (int guard = 0; dup && guard < 64; guard++) {
+                    if ((g & 3) == 0)
+                        philox_at(m, idx, g >> 2, (2u << 8) | (uint32_t)a, wb);
+                    bb = (int)(wb[g & 3] % (uint32_t)m->modmax) + 2;
+                    g++;
+                    dup = 0;
+                    for (int k = 0; k < j; k++)
+                        if (ba[k] == bb)
+                            dup = 1;
+                }
+                if (dup) {
+                    for (int cc = 2; cc <= m->modmax + 1; cc++) {
+                        int used = 0;
+                        for (int k = 0; k < j; k++)
+                            if (ba[k] == cc)
+                                used = 1;
+                        if (!used) {
+                            bb = cc;
+                            break;
+                        }
+                    }
+                }
+                ba[j] = bb;
+            }
+        }
+        /* Per slot pair p: Dirichlet words (bit 0 = sign), phase words. */
+        double e[4096];
+        uint32_t neg[4096], ph[4096];
+        double Q[64];
+        for (int i = 0; i < 64; i++)
+            Q[i] = 0.0;
+        for (int p = 0; p < npairs; p++) {
+            uint32_t w[4];
+            philox_at(m, idx, (uint32_t)p, (1u << 8) | (uint32_t)a, w);
+            for (int q = 0; q < 2; q++) {
+                int j = 2 * p + q;
+                if (j >= nw)
+                    break;
+                uint32_t mm = w[q] >> 1;
+                double v = (double)(0x80000000u - mm) * 0x1p-31;
+                e[j] = -or_log(v);
+                neg[j] = w[q] & 1u;
+                ph[j] = w[2 + q];
+                Q[p & 63] = Q[p & 63] + e[j];
+            }
+        }
+        for (int h = 32; h >= 1; h >>= 1)
+            for (int i = 0; i < h; i++)
+                Q[i] = Q[i] + Q[i + h];
+        double invW = 1.0 / Q[0];
+        for (int j = 0; j < nw; j++) {
+            double invb = 1.0 / (double)ba[j];
+            double aj = (nw == 1) ? invb : (e[j] * invW) * invb;
+            if (neg[j])
+                aj = -aj;
+            double cj, sj, fj = 0.0;
+            if ((ba[j] & 1) == 0) {
+                double sf, cf;
+                or_sincospi_rat((uint64_t)ph[j], 0x100000000ULL, &sf, &cf);
+                cj = aj * cf;
+                sj = aj * sf;
+                fj = (double)ph[j] * 0x1p-32 * 3.141592653589793;
+            } else {
+                cj = aj;
+                sj = 0.0;
+            }
+            c[(size_t)a * nw + j] = cj;
+            s[(size_t)a * nw + j] = sj;
+            if (amp)
+                amp[(size_t)a * nw + j] = aj;
+            if (phase)
+                phase[(size_t)a * nw + j] = fj;
+        }
+    }
+}
+
+void or_eval_t(const or_model* m, int t, const double* a1, const int* b, const double* c,
+               const double* s, double* X) {
+    const int nw = m->nw, npairs = (nw + 1) / 2, W = m->T + 1;
+    const double* St = m->S + (size_t)t * W;
+    const double* Ct = m->C + (size_t)t * W;
+    for (int a = 0; a < m->d; a++) {
+        double P[64];
+        for (int i = 0; i < 64; i++)
+            P[i] = 0.0;
+        for (int p = 0; p < npairs; p++) {
+            for (int q = 0; q < 2; q++) {
+                int j = 2 * p + q;
+                if (j >= nw)
+                    break;
+                size_t o = (size_t)a * nw + j;
+                int k = b[o];
+                P[p & 63] = fma(c[o], St[k], P[p & 63]);
+                if ((k & 1) == 0)
+                    P[p & 63] = fma(s[o], Ct[k], P[p & 63]);
+            }
+        }
+        for (int h = 32; h >= 1; h >>= 1)
+            for (int i = 0; i < h; i++)
+                P[i] = P[i] + P[i + h];
+        X[a] = wrap1(a1[a] + P[0]);
+    }
+}
+
+int or_cell(const or_model* m, double x) {
+    int c = (int)floor((x + 1.0) * m->halfT);
+    if (c < 0)
+        c += m->T;
+    if (c >= m->T)
+        c -= m->T;
+    return c;
+}
+
+/* ----------------------------------------------------------- per-timestep hash grid
+ * Open addressing keyed by (t, cell); value = head path of a chain through nxt[n*T+t]
+ * (the role of the reference's unordered_map<long, vector<HPt>> per timestep). */
+typedef struct {
+    uint64_t* keys; /* key+1, 0 = empty */
+    int32_t* vals;
+    uint64_t cap, used;
+} hgrid;
+
+static uint64_t hmix(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ULL;
+    k ^= k >> 33;
+    return k;
+}
+static int hg_init(hgrid* h, uint64_t cap) {
+    h->cap = cap;
+    h->used = 0;
+    h->keys = (uint64_t*)calloc(cap, 8);
+    h->vals = (int32_t*)calloc(cap, 4);
+    return (h->keys && h->vals) ? 0 : -2;
+}
+static void hg_free(hgrid* h) {
+    free(h->keys);
+    free(h->vals);
+}
+static int32_t* hg_slot(hgrid* h, uint64_t key, int create);
+static int hg_grow(hgrid* h) {
+    hgrid n;
+    if (hg_init(&n, h->cap * 2))
+        return -2;
+    for (uint64_t i = 0; i < h->cap; i++)
+        if (h->keys[i])
+            *hg_slot(&n, h->keys[i] - 1, 1) = h->vals[i];
+    hg_free(h);
+    *h = n;
+    return 0;
+}
+static int32_t* hg_slot(hgrid* h, uint64_t key, int create) {
+    uint64_t i = hmix(key) & (h->cap - 1);
+    for (;;) {
+        if (h->keys[i] == key + 1)
+            return &h->vals[i];
+        if (h->keys[i] == 0) {
+            if (!create)
+                return NULL;
+            h->keys[i] = key + 1;
+            h->vals[i] = -1;
+            h->used++;
+            return &h->vals[i];
+        }
+        i = (i + 1) & (h->cap - 1);
+    }
+}
+
+typedef struct {
+    const or_model* m;
+    hgrid h;
+    int64_t n, cap;
+    double* pos;   /* [cap][T][d] */
+    int32_t* nxt;  /* [cap][T] */
+    int32_t* gid;  /* [cap] */
+    uint64_t ncell;
+} packing;
+
+static int pk_init(packing* P, const or_model* m) {
+    memset(P, 0, sizeof(*P));
+    P->m = m;
+    P->ncell = 1;
+    for (int a = 0; a < m->d; a++)
+        P->ncell *= (uint64_t)m->T;
+    P->cap = 1024;
+    P->pos = (double*)malloc((size_t)P->cap * m->T * m->d * 8);
+    P->nxt = (int32_t*)malloc((size_t)P->cap * m->T * 4);
+    P->gid = (int32_t*)malloc((size_t)P->cap * 4);
+    if (!P->pos || !P->nxt || !P->gid)
+        return -2;
+    return hg_init(&P->h, 1u << 16);
+}
+static void pk_free(packing* P) {
+    free(P->pos);
+    free(P->nxt);
+    free(P->gid);
+    hg_free(&P->h);
+}
+static uint64_t cell_key(const packing* P, int t, const int* cx) {
+    uint64_t id = 0;
+    for (int a = 0; a < P->m->d; a++)
+        id = id * (uint64_t)P->m->T + (uint64_t)cx[a];
+    return (uint64_t)t * P->ncell + id;
+}
+/* Inserts trajectory X[T][d] under group g; returns newly occupied cells or <0. */
+static int64_t pk_insert(packing* P, const double* X, int32_t g) {
+    const or_model* m = P->m;
+    const int T = m->T, d = m->d;
+    if (P->n == P->cap) {
+        int64_t nc = P->cap * 2;
+        double* np = (double*)realloc(P->pos, (size_t)nc * T * d * 8);
+        if (!np)
+            return -2;
+        P->pos = np;
+        int32_t* nn = (int32_t*)realloc(P->nxt, (size_t)nc * T * 4);
+        if (!nn)
+            return -2;
+        P->nxt = nn;
+        int32_t* ng = (int32_t*)realloc(P->gid, (size_t)nc * 4);
+        if (!ng)
+            return -2;
+        P->gid = ng;
+        P->cap = nc;
+    }
+    int64_t n = P->n, fresh = 0;
+    memcpy(P->pos + (size_t)n * T * d, X, (size_t)T * d * 8);
+    for (int t = 0; t < T; t++) {
+        if (2 * (P->h.used + 1) > P->h.cap && hg_grow(&P->h))
+            return -2;
+        int cx[3];
+        for (int a = 0; a < d; a++)
+            cx[a] = or_cell(m, X[t * d + a]);
+        int32_t* head = hg_slot(&P->h, cell_key(P, t, cx), 1);
+        if (*head < 0)
+            fresh++;
+        P->nxt[(size_t)n * T + t] = *head;
+        *head = (int32_t)n;
+    }
+    P->gid[n] = g;
+    P->n++;
+    return fresh;
+}
+/* Scans the 3^d neighbour cells of x at timestep t.  mode 0: returns 1 on the first
+ * contact.  mode 1 (subpath): tracks the single contacted group in *g, returns 1 on a
+ * second distinct group. */
+static int pk_scan(const packing* P, int t, const double* x, int mode, int32_t* g) {
+    const or_model* m = P->m;
+    const int T = m->T, d = m->d;
+    int c0[3], cx[3];
+    for (int a = 0; a < d; a++)
+        c0[a] = or_cell(m, x[a]);
+    int nb = d == 2 ? 9 : 27;
+    for (int q = 0; q < nb; q++) {
+        int qq = q;
+        for (int a = d - 1; a >= 0; a--) {
+            int off = qq % 3 - 1;
+            qq /= 3;
+            cx[a] = (c0[a] + off + T) % T;
+        }
+        int32_t* head = hg_slot((hgrid*)&P->h, cell_key(P, t, cx), 0);
+        if (!head)
+            continue;
+        for (int32_t n = *head; n >= 0; n = P->nxt[(size_t)n * T + t]) {
+            const double* p = P->pos + ((size_t)n * T + t) * d;
+            int hit = 1;
+            for (int a = 0; a < d; a++)
+                if (fabs(tdelta(x[a] - p[a])) > m->r)
+                    hit = 0;
+            if (!hit)
+                continue;
+            if (mode == 0)
+                return 1;
+            if (*g == -1)
+                *g = P->gid[n];
+            else if (P->gid[n] != *g)
+                return 1;
+        }
+    }
+    return 0;
+}
+
+typedef struct {
+    double a1[3];
+    int* b;
+    double *c, *s, *X;
+} cand_buf;
+
+static int cb_init(cand_buf* cb, const or_model* m) {
+    size_t n = (size_t)m->d * m->nw;
+    cb->b = (int*)malloc(n * sizeof(int));
+    cb->c = (double*)malloc(n * 8);
+    cb->s = (double*)malloc(n * 8);
+    cb->X = (double*)malloc((size_t)m->T * m->d * 8);
+    return (cb->b && cb->c && cb->s && cb->X) ? 0 : -2;
+}
+static void cb_free(cand_buf* cb) {
+    free(cb->b);
+    free(cb->c);
+    free(cb->s);
+    free(cb->X);
+}
+
+int64_t or_pack(const or_model* m, int64_t max_candidates, int stop_kind, int64_t stop_batch,
+                int64_t window, double stop_rate, int64_t max_n, int64_t* idx, int32_t* gid,
+                double* pos, int64_t cap, int64_t* attempts) {
+    packing P;
+    cand_buf cb;
+    if (pk_init(&P, m) || cb_init(&cb, m)) {
+        pk_free(&P);
+        cb_free(&cb);
+        return -2;
+    }
+    const int T = m->T, d = m->d;
+    int64_t nacc_cap = 1024, nacc = 0, lo = 0;
+    int64_t* acc = (int64_t*)malloc((size_t)nacc_cap * 8);
+    int64_t c = 0, ret = 0;
+    for (;; c++) {
+        if (stop_kind == 1 && stop_batch > 0 && c > 0 && c % stop_batch == 0 && c >= window) {
+            while (lo < nacc && acc[lo] < c - window)
+                lo++;
+            if ((double)(nacc - lo) < stop_rate * (double)window)
+                break;
+        }
+        if (c >= max_candidates)
+            break;
+        or_candidate(m, (uint64_t)c, cb.a1, cb.b, cb.c, cb.s, NULL, NULL);
+        int hit = 0;
+        for (int oi = 0; oi < T && !hit; oi++) {
+            int t = m->order[oi];
+            or_eval_t(m, t, cb.a1, cb.b, cb.c, cb.s, cb.X + (size_t)t * d);
+            hit = pk_scan(&P, t, cb.X + (size_t)t * d, 0, NULL);
+        }
+        if (hit)
+            continue;
+        if (pk_insert(&P, cb.X, (int32_t)P.n) < 0) {
+            ret = -2;
+            break;
+        }
+        if (nacc == nacc_cap) {
+            nacc_cap *= 2;
+            acc = (int64_t*)realloc(acc, (size_t)nacc_cap * 8);
+        }
+        acc[nacc++] = c;
+        if (max_n > 0 && nacc >= max_n) {
+            c++;
+            break;
+        }
+    }
+    if (ret == 0) {
+        for (int64_t i = 0; i < nacc && i < cap; i++) {
+            if (idx)
+                idx[i] = acc[i];
+            if (gid)
+                gid[i] = P.gid[i];
+            if (pos)
+                memcpy(pos + (size_t)i * T * d, P.pos + (size_t)i * T * d, (size_t)T * d * 8);
+        }
+        ret = nacc;
+        if (attempts)
+            *attempts = c;
+    }
+    free(acc);
+    pk_free(&P);
+    cb_free(&cb);
+    return ret;
+}
+
+int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, int64_t n0,
+                    uint64_t first, int64_t count, int64_t* idx, int32_t* gid, int64_t cap,
+                    int64_t* filled_new) {
+    packing P;
+    cand_buf cb;
+    if (pk_init(&P, m) || cb_init(&cb, m)) {
+        pk_free(&P);
+        cb_free(&cb);
+        return -2;
+    }
+    const int T = m->T, d = m->d;
+    int64_t ret = 0, nsub = 0, fresh = 0;
+    for (int64_t i = 0; i < n0; i++)
+        if (pk_insert(&P, pos0 + (size_t)i * T * d, gid0[i]) < 0)
+            ret = -2;
+    for (int64_t k = 0; k < count && ret == 0; k++) {
+        uint64_t c = first + (uint64_t)k;
+        or_candidate(m, c, cb.a1, cb.b, cb.c, cb.s, NULL, NULL);
+        int32_t g = -1;
+        int bad = 0;
+        for (int oi = 0; oi < T && !bad; oi++) {
+            int t = m->order[oi];
+            or_eval_t(m, t, cb.a1, cb.b, cb.c, cb.s, cb.X + (size_t)t * d);
+            bad = pk_scan(&P, t, cb.X + (size_t)t * d, 1, &g);
+        }
+        if (bad || g < 0)
+            continue;
+        int64_t f = pk_insert(&P, cb.X, g);
+        if (f < 0) {
+            ret = -2;
+            break;
+        }
+        fresh += f;
+        if (nsub < cap) {
+            if (idx)
+                idx[nsub] = (int64_t)c;
+            if (gid)
+                gid[nsub] = g;
+        }
+        nsub++;
+    }
+    if (filled_new)
+        *filled_new = fresh;
+    pk_free(&P);
+    cb_free(&cb);
+    return ret ? ret : nsub;
+}
+
+/* Canonical pair predicate at one timestep: adjacent cells on every axis and
+ * |torus_delta| <= r on every axis. */
+static int pair_hit(const or_model* m, const double* x, const double* y) {
+    for (int a = 0; a < m->d; a++) {
+        int dc = or_cell(m, x[a]) - or_cell(m, y[a]);
+        if (dc < 0)
+            dc = -dc;
+        if (dc > 1 && dc != m->T - 1)
+            return 0;
+        if (fabs(tdelta(x[a] - y[a])) > m->r)
+            return 0;
+    }
+    return 1;
+}
+
+int64_t or_check_rsa(const or_model* m, int64_t M, const int64_t* acc, int64_t na) {
+    const int T = m->T, d = m->d;
+    cand_buf cb;
+    if (cb_init(&cb, m))
+        return -2;
+    double* all = (double*)malloc((size_t)M * T * d * 8);
+    char* member = (char*)calloc((size_t)M, 1);
+    if (!all || !member) {
+        free(all);
+        free(member);
+        cb_free(&cb);
+        return -2;
+    }
+    for (int64_t c = 0; c < M; c++) {
+        or_candidate(m, (uint64_t)c, cb.a1, cb.b, cb.c, cb.s, NULL, NULL);
+        for (int t = 0; t < T; t++)
+            or_eval_t(m, t, cb.a1, cb.b, cb.c, cb.s, all + ((size_t)c * T + t) * d);
+    }
+    int64_t bad = 0;
+    for (int64_t i = 0; i < na && !bad; i++) {
+        if (acc[i] < 0 || acc[i] >= M || (i > 0 && acc[i] <= acc[i - 1]))
+            bad = 1 + (acc[i] >= 0 ? acc[i] : 0);
+        else
+            member[acc[i]] = 1;
+    }
+    /* (i) members pairwise collision-free */
+    for (int64_t i = 0; i < na && !bad; i++)
+        for (int64_t j = 0; j < i && !bad; j++)
+            for (int t = 0; t < T; t++)
+                if (pair_hit(m, all + ((size_t)acc[i] * T + t) * d,
+                             all + ((size_t)acc[j] * T + t) * d)) {
+                    bad = 1 + acc[i];
+                    break;
+                }
+    /* (ii) every non-member collides with an earlier member */
+    for (int64_t c = 0; c < M && !bad; c++) {
+        if (member[c])
+            continue;
+        int found = 0;
+        for (int64_t j = 0; j < na && acc[j] < c && !found; j++)
+            for (int t = 0; t < T; t++)
+                if (pair_hit(m, all + ((size_t)c * T + t) * d,
+                             all + ((size_t)acc[j] * T + t) * d)) {
+                    found = 1;
+                    break;
+                }
+        if (!found)
+            bad = 1 + c;
+    }
+    free(all);
+    free(member);
+    cb_free(&cb);
+    return bad;
+}

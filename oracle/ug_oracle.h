@@ -1,0 +1,88 @@
+/*
+ * ug_oracle.h -- CPU oracle for the RSA worldline packer (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker, never the product: only tests/, the package's
+ * smoke check and bench.py's cpu_baseline leg load it.  It restates, in plain
+ * C99, the serial random-sequential-adsorption admission of the reference
+ * (cuda/braid_cuda.cu:703-722 host_collides, :831-879 evalPath/insertPath,
+ * cuda/braid_knot.cu:433-462 index-ordered admission) on the Philox candidate
+ * stream whose layout is fixed in DESIGN.md ("Candidate stream and canonical
+ * arithmetic").  Every function below is written independently of the HIP
+ * implementation; the two must agree bit for bit.
+ */
+#ifndef UG_ORACLE_H
+#define UG_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Philox4x32-10 (Salmon et al., SC'11), one block. */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* sin(pi*n/m), cos(pi*n/m) for integers n >= 0, 0 < m <= 2^32. */
+void or_sincospi_rat(uint64_t n, uint64_t m, double* s, double* c);
+
+/* Natural log of v in [2^-1022, +inf) (own atanh-series method). */
+double or_log(double v);
+
+/* Model geometry: basis tables and scan order for T timesteps, max frequency F = T. */
+typedef struct or_model {
+    int d;          /* spatial axes: 2 or 3 */
+    int T;          /* timesteps (= max frequency) */
+    int nw;         /* wiggle terms per axis (terms - 1) */
+    int modmax;     /* frequency pool size: frequencies 2..modmax+1 */
+    int pin;        /* --pin-sin1 */
+    uint64_t seed;  /* Philox key */
+    double r;       /* inclusive Chebyshev radius 2/T */
+    double halfT;   /* T/2: cell index multiplier */
+    double* S;      /* [T][T+1]: sin(k z_t)/sin z_t */
+    double* C;      /* [T][T+1]: (cos(k z_t) - 1)/sin z_t */
+    double* sz;     /* [T]: sin z_t */
+    int* order;     /* [T]: endpoint-first scan order */
+} or_model;
+
+int or_model_init(or_model* m, int d, int T, int terms, uint64_t seed, int pin);
+void or_model_free(or_model* m);
+
+/* Candidate idx: a1[d], b[d*nw], c[d*nw], s[d*nw] (per-slot coefficients),
+ * and optionally the dump form amp[d*nw], phase[d*nw] (may be NULL). */
+void or_candidate(const or_model* m, uint64_t idx, double* a1, int* b, double* c, double* s,
+                  double* amp, double* phase);
+
+/* Canonical wrapped comoving position of a candidate at timestep t (X[d]). */
+void or_eval_t(const or_model* m, int t, const double* a1, const int* b, const double* c,
+               const double* s, double* X);
+
+/* Canonical torus cell of a wrapped coordinate. */
+int or_cell(const or_model* m, double x);
+
+/* Serial RSA in candidate-index order.
+ *   stop_kind 0: run candidates [0, max_candidates)
+ *   stop_kind 1: as 0, but also stop at the first boundary c = k*stop_batch (c >= window)
+ *                where accepts among [c-window, c) < stop_rate*window
+ *   max_n > 0 stops right after the max_n-th accept.
+ * Outputs (caller-allocated, capacity cap): idx[], gid[]; positions pos[n][t][d]
+ * (may be NULL, capacity cap*T*d).  Returns N (>= 0) or a negative error;
+ * *attempts = candidates consumed. */
+int64_t or_pack(const or_model* m, int64_t max_candidates, int stop_kind, int64_t stop_batch,
+                int64_t window, double stop_rate, int64_t max_n, int64_t* idx, int32_t* gid,
+                double* pos, int64_t cap, int64_t* attempts);
+
+/* Serial RSA continuing an existing packing: subpath phase.  Candidates
+ * [first, first+count) join the single group they touch (host_sub_gid,
+ * cuda/braid_cuda.cu:726-750); uniques are given by pos0[n0][T][d], gid0[n0]. */
+int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, int64_t n0,
+                    uint64_t first, int64_t count, int64_t* idx, int32_t* gid, int64_t cap,
+                    int64_t* filled_new);
+
+/* Brute-force check that acc[0..na) is exactly the serial RSA output over candidate
+ * indices [0, M): (i) no two members collide, (ii) every non-member collides with an
+ * earlier member.  Returns 0 when both hold, else 1 + the first offending index. */
+int64_t or_check_rsa(const or_model* m, int64_t M, const int64_t* acc, int64_t na);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

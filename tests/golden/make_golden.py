@@ -1,0 +1,57 @@
+"""Writes the golden accepted-index fixtures of tests/golden/ from the CPU oracle.
+
+Each fixture is the serial-RSA output (accepted candidate indices and group labels)
+for one model and seed on the Philox candidate stream, as computed by oracle/ (which
+restates cuda/braid_cuda.cu's host admission).  The small ones are re-derived by the
+oracle in tests as well; the long runs (config 2 to jam, config 3 prefix) exist so the
+GPU tests need not rerun a minute-long CPU pack.
+
+    python tests/golden/make_golden.py [name ...]
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "oracle"))
+
+import pyoracle  # noqa: E402
+
+# name -> (d, T, terms, seed, pin, candidates, stop_kind)
+CASES = {
+    **{f"cfg1_d2_T16_s{s}": (2, 16, 16, s, False, 20000, 0) for s in (1, 2, 3, 4)},
+    "d2_T32_terms2_s1": (2, 32, 2, 1, False, 50000, 0),
+    "d2_T24_terms5_s3": (2, 24, 5, 3, False, 30000, 0),
+    "d2_T16_pin_s2": (2, 16, 16, 2, True, 20000, 0),
+    "d3_T12_s1": (3, 12, 12, 1, False, 20000, 0),
+    "d3_T16_terms4_s5": (3, 16, 4, 5, False, 20000, 0),
+    **{f"cfg2_d2_T64_s{s}_jam": (2, 64, 64, s, False, 1 << 40, 1) for s in (1, 2, 3, 4)},
+    "cfg3_d3_T128_s1_prefix1M": (3, 128, 128, 1, False, 1 << 20, 0),
+}
+
+
+def make(name: str) -> dict:
+    d, T, terms, seed, pin, cand, stop = CASES[name]
+    m = pyoracle.Model(d, T, terms=terms, seed=seed, pin=pin)
+    t0 = time.time()
+    idx, gid, att = m.pack(cand, stop_kind=stop)
+    dt = time.time() - t0
+    rec = {
+        "name": name, "dim": d, "T": T, "terms": terms, "seed": seed, "pin_sin1": pin,
+        "stop": "fixed_window(batch=2^20, window=2^22, rate=1e-4)" if stop else "budget",
+        "candidates": int(att), "n": int(len(idx)), "idx": [int(x) for x in idx],
+        "gid": [int(x) for x in gid], "oracle_seconds": round(dt, 2),
+    }
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(rec, f)
+    print(f"{name}: N={len(idx)} candidates={att} ({dt:.1f} s)")
+    return rec
+
+
+if __name__ == "__main__":
+    for n in sys.argv[1:] or list(CASES):
+        make(n)

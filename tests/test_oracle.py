@@ -1,0 +1,353 @@
+"""Pins the CPU oracle (oracle/) against things it did not compute itself.
+
+- Philox4x32-10: Random123's published known-answer vectors and rocRAND's Philox.
+- log / sin / cos: the C library (math.log, math.sin, math.cos) to a few ulp.
+- basis tables and evaluation: the reference's direct expression
+  X = [a1 sin z + sum a_j (sin(b_j z + f_j) - sin f_j)] / sin z (cuda/braid_cuda.cu:418-441,
+  braidlab/corrdim.py:116-137) evaluated with math.sin, and its closed forms at z = pi/2
+  (tests/test_corrdim.py:154-171 of the reference).
+- proposal measure: exact invariants and distribution tests (cuda/braid_cuda.cu:295-373).
+- torus wrap / min-image: the reference's own examples (tests/test_corrdim.py:138-145,
+  tests/test_group_partners.py:65-82).
+- serial RSA: the brute-force characterization (no two accepted collide; every rejected
+  collides with an earlier accepted), which planted mistakes must fail.
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import pyoracle as po
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------------ Philox4x32-10
+def test_philox_random123_kat():
+    # Random123 kat_vectors, philox4x32 10 rounds.
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in kat:
+        assert list(po.philox(ctr, key)) == want
+
+
+_ROCRAND_PROBE = r"""
+#include <rocrand/rocrand_philox4x32_10.h>
+#include <cstdio>
+struct E : rocrand_device::philox4x32_10_engine {
+    E() : rocrand_device::philox4x32_10_engine(0, 0, 0) {}
+    uint4 go(uint4 c, uint2 k) { return ten_rounds(c, k); }
+};
+int main() {
+    E e;
+    unsigned s = 12345u;
+    for (int i = 0; i < 200; i++) {
+        unsigned v[6];
+        for (int j = 0; j < 6; j++) { s = s * 1664525u + 1013904223u; v[j] = s; }
+        uint4 r = e.go(uint4{v[0], v[1], v[2], v[3]}, uint2{v[4], v[5]});
+        printf("%u %u %u %u %u %u %u %u %u %u\n", v[0], v[1], v[2], v[3], v[4], v[5], r.x, r.y, r.z, r.w);
+    }
+}
+"""
+
+
+def test_philox_matches_rocrand(tmp_path):
+    hipcc = "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "probe.cpp"
+    src.write_text(_ROCRAND_PROBE)
+    exe = tmp_path / "probe"
+    r = subprocess.run([hipcc, "--offload-arch=gfx950", "-O1", "-o", str(exe), "-x", "hip", str(src)],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("rocRAND host probe did not compile: " + r.stderr[-300:])
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
+    rows = [list(map(int, ln.split())) for ln in out.strip().splitlines()]
+    assert len(rows) == 200
+    for row in rows:
+        assert list(po.philox(row[0:4], row[4:6])) == row[6:10]
+
+
+# ------------------------------------------------------------------ log, sinpi, cospi
+def _ulp_diff(a: float, b: float) -> float:
+    return abs(a - b) / math.ulp(max(abs(a), abs(b), 1e-300))
+
+
+def test_log_against_libm():
+    rng = np.random.default_rng(1)
+    vals = list(rng.uniform(2.0**-31, 1.0, 3000)) + list(2.0 ** rng.uniform(-31, 0, 3000))
+    vals += [1.0, 0.5, 2.0**-31, 1 - 2.0**-31, 0.7071067811865476, 0.7071067811865475]
+    for v in vals:
+        got, want = po.log(v), math.log(v)
+        if want == 0.0:
+            assert got == 0.0
+        else:
+            assert _ulp_diff(got, want) <= 2.0, (v, got, want)
+
+
+def test_sincospi_against_libm_and_exact_points():
+    import mpmath
+
+    mpmath.mp.dps = 40
+    rng = np.random.default_rng(2)
+    for _ in range(1500):
+        m = int(rng.integers(1, 2**32))
+        n = int(rng.integers(0, 4 * m))
+        s, c = po.sincospi_rat(n, m)
+        x = mpmath.pi * mpmath.mpf(n) / m
+        ws, wc = float(mpmath.sin(x)), float(mpmath.cos(x))
+        assert abs(s - ws) <= 2 * math.ulp(max(abs(ws), 2**-60)) + 1e-300, (n, m)
+        assert abs(c - wc) <= 2 * math.ulp(max(abs(wc), 2**-60)) + 1e-300, (n, m)
+    # exact angles
+    assert po.sincospi_rat(0, 7) == (0.0, 1.0)
+    assert po.sincospi_rat(1, 2) == (1.0, -0.0) or po.sincospi_rat(1, 2)[0] == 1.0
+    s, c = po.sincospi_rat(1, 6)
+    assert abs(s - 0.5) <= 1.2e-16 and abs(c - math.sqrt(3) / 2) <= 2.3e-16
+    s, c = po.sincospi_rat(1, 4)
+    assert abs(s - math.sqrt(0.5)) <= 1.2e-16 and abs(c - math.sqrt(0.5)) <= 1.2e-16
+
+
+# ------------------------------------------------------------------ basis and evaluation
+@pytest.mark.parametrize("T", [16, 64, 101])
+def test_basis_tables_match_direct_trig(T):
+    m = po.Model(2, T)
+    z = (np.arange(T) + 1) * np.pi / (T + 1)
+    assert np.max(np.abs(m.sz - np.sin(z))) <= 4.5e-16
+    import mpmath
+
+    mpmath.mp.dps = 30
+    S, Cc = m.S, m.C
+    for t in range(T):
+        zt = mpmath.pi * (t + 1) / (T + 1)
+        for k in range(T + 1):
+            s_ref = float(mpmath.sin(k * zt) / mpmath.sin(zt))
+            c_ref = float((mpmath.cos(k * zt) - 1) / mpmath.sin(zt))
+            assert abs(S[t, k] - s_ref) <= 4e-15 * max(1.0, abs(s_ref)), (t, k)
+            assert abs(Cc[t, k] - c_ref) <= 4e-15 * max(1.0, abs(c_ref)), (t, k)
+    o = m.order
+    assert sorted(o) == list(range(T))
+    assert list(o[:4]) == [0, T - 1, 1, T - 2]
+    keys = np.minimum(o, T - 1 - o)
+    assert np.all(np.diff(keys) >= 0)
+
+
+def _direct_positions(cd, z, wrap=True):
+    """The reference's per-term expression with libm sin (cuda/braid_cuda.cu:422-441)."""
+    out = []
+    for a in range(cd.a1.shape[0]):
+        x = cd.a1[a] * np.sin(z)
+        for j in range(cd.amp.shape[1]):
+            x = x + cd.amp[a, j] * (np.sin(cd.b[a, j] * z + cd.phase[a, j]) - np.sin(cd.phase[a, j]))
+        X = x / np.sin(z)
+        out.append(X - 2.0 * np.floor((X + 1.0) * 0.5) if wrap else X)
+    return np.stack(out, axis=-1)
+
+
+@pytest.mark.parametrize("d,T,terms", [(2, 16, 16), (2, 64, 64), (3, 32, 32), (2, 40, 6), (3, 20, 2)])
+def test_eval_matches_reference_expression(d, T, terms):
+    m = po.Model(d, T, terms=terms, seed=7)
+    z = (np.arange(T) + 1) * np.pi / (T + 1)
+    worst = 0.0
+    for idx in range(0, 4000, 97):
+        cd = m.candidate(idx)
+        X = m.positions(idx)
+        ref = _direct_positions(cd, z)
+        dX = np.abs(X - ref)
+        dX = np.minimum(dX, 2.0 - dX)  # same point across the seam
+        worst = max(worst, float(dX.max()))
+    assert worst < 1e-11, worst
+
+
+def test_eval_closed_form_at_half_pi():
+    # T odd: t = (T-1)/2 sits at z = pi/2 where sin z = 1 (reference tests/test_corrdim.py:154-171)
+    T = 31
+    m = po.Model(2, T, seed=3)
+    t = (T - 1) // 2
+    for idx in (0, 5, 99):
+        cd = m.candidate(idx)
+        X = m.positions(idx)[t]
+        for a in range(2):
+            x = cd.a1[a] + sum(cd.amp[a, j] * (math.sin(cd.b[a, j] * math.pi / 2 + cd.phase[a, j])
+                                               - math.sin(cd.phase[a, j])) for j in range(m.nw))
+            want = x - 2.0 * math.floor((x + 1.0) * 0.5)
+            assert abs(X[a] - want) < 1e-12
+
+
+# ------------------------------------------------------------------ proposal measure
+def test_candidate_invariants_full_spectrum():
+    for d, T in ((2, 16), (2, 64), (3, 128)):
+        m = po.Model(d, T, seed=5)
+        for idx in range(0, 300, 7):
+            cd = m.candidate(idx)
+            assert np.all(np.abs(cd.a1) < 1.0)
+            for a in range(d):
+                assert sorted(cd.b[a]) == list(range(2, T + 1))
+                assert abs(np.sum(np.abs(cd.amp[a]) * cd.b[a]) - 1.0) < 1e-13
+                odd = cd.b[a] % 2 == 1
+                assert np.all(cd.phase[a][odd] == 0.0) and np.all(cd.s[a][odd] == 0.0)
+                assert np.all((cd.phase[a][~odd] >= 0) & (cd.phase[a][~odd] < np.pi))
+                assert np.allclose(cd.c[a], cd.amp[a] * np.cos(cd.phase[a]), rtol=0, atol=1e-16)
+                assert np.allclose(cd.s[a], cd.amp[a] * np.sin(cd.phase[a]), rtol=0, atol=1e-16)
+
+
+def test_candidate_invariants_partial_and_pin():
+    m = po.Model(2, 50, terms=7, seed=9)
+    for idx in range(200):
+        cd = m.candidate(idx)
+        for a in range(2):
+            b = cd.b[a]
+            assert len(set(b)) == 6 and b.min() >= 2 and b.max() <= 50
+            assert abs(np.sum(np.abs(cd.amp[a]) * b) - 1.0) < 1e-14
+    legacy = po.Model(3, 30, terms=2, seed=1)
+    for idx in range(100):
+        cd = legacy.candidate(idx)
+        assert np.allclose(np.abs(cd.amp[:, 0]) * cd.b[:, 0], 1.0)
+    pinned = po.Model(2, 16, seed=4, pin=True)
+    assert np.all(pinned.candidate(3).a1 == 0.0)
+
+
+def test_candidate_distributions():
+    d, T, n = 2, 32, 3000
+    m = po.Model(d, T, seed=21)
+    a1, share, ph, neg = [], [], [], []
+    for idx in range(n):
+        cd = m.candidate(idx)
+        a1.extend(cd.a1)
+        w = np.abs(cd.amp[0]) * cd.b[0]
+        share.append(w[5])
+        even = cd.b[0] % 2 == 0
+        ph.extend(cd.phase[0][even][:3])
+        neg.extend(cd.amp[0] < 0)
+    assert stats.kstest(a1, "uniform", args=(-1, 2)).pvalue > 1e-3
+    nw = T - 1
+    assert stats.kstest(share, "beta", args=(1, nw - 1)).pvalue > 1e-3
+    assert stats.kstest(np.array(ph) / np.pi, "uniform").pvalue > 1e-3
+    assert abs(np.mean(neg) - 0.5) < 0.01
+
+
+def test_candidates_are_counter_addressed():
+    m = po.Model(2, 24, seed=3)
+    a = m.candidate(12345)
+    b = m.candidate(12345)
+    c = po.Model(2, 24, seed=4).candidate(12345)
+    assert np.array_equal(a.c, b.c) and not np.array_equal(a.c, c.c)
+
+
+# ------------------------------------------------------------------ wrap and predicate
+def test_wrap_reference_examples():
+    m = po.Model(2, 16)
+    x = np.array([-2.0, -1.2, -1.0, -0.3, 0.0, 0.99, 1.0, 1.2, 1.9])
+    w = x - 2.0 * np.floor((x + 1.0) * 0.5)
+    assert np.all(w >= -1.0) and np.all(w < 1.0)
+    assert np.allclose(w[3:6], x[3:6])
+    # cell index wraps modulo T at both ends (cell width 2/T)
+    assert m.cell(-1.0) == 0 and m.cell(0.999999) == 15 and m.cell(-1.0 - 1e-17) in (0, 15)
+    assert m.cell(np.nextafter(-1.0, -2.0)) == 15
+
+
+def test_rsa_predicate_reference_cases():
+    # tests/test_group_partners.py:65-82: constant comoving positions a1 across the seam
+    # collide; paths parked 1.0 apart never do.  Build them as pinned-wiggle-free packings
+    # through the characterization check on a hand-made acceptance list is not possible,
+    # so check the predicate via two accepted candidates of a real packing instead.
+    m = po.Model(2, 16, seed=1)
+    idx, gid, att = m.pack(5000)
+    X = {i: m.positions(int(i)) for i in idx[:10]}
+    r = m.r
+    for i in idx[:10]:
+        for j in idx[:10]:
+            if i >= j:
+                continue
+            d = np.abs(X[i] - X[j])
+            d = np.minimum(d, 2.0 - d)
+            assert not np.any(np.all(d <= r, axis=1)), "two accepted paths collide"
+
+
+# ------------------------------------------------------------------ serial RSA
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_config1_golden_and_characterization(seed):
+    g = json.load(open(os.path.join(GOLD, f"cfg1_d2_T16_s{seed}.json")))
+    m = po.Model(2, 16, seed=seed)
+    idx, gid, att = m.pack(20000)
+    assert att == 20000
+    assert list(idx) == g["idx"] and list(gid) == g["gid"]
+    assert list(gid) == list(range(len(idx)))
+    assert m.check_rsa(20000, idx) == 0
+
+
+@pytest.mark.parametrize("name", ["d2_T32_terms2_s1", "d2_T24_terms5_s3", "d2_T16_pin_s2",
+                                  "d3_T12_s1", "d3_T16_terms4_s5"])
+def test_golden_variants(name):
+    g = json.load(open(os.path.join(GOLD, name + ".json")))
+    m = po.Model(g["dim"], g["T"], terms=g["terms"], seed=g["seed"], pin=g["pin_sin1"])
+    idx, gid, att = m.pack(g["candidates"])
+    assert list(idx) == g["idx"]
+    assert m.check_rsa(min(g["candidates"], 8000), idx[idx < 8000]) == 0
+
+
+def test_characterization_rejects_planted_mistakes():
+    m = po.Model(2, 16, seed=1)
+    idx, _, _ = m.pack(6000)
+    assert m.check_rsa(6000, idx) == 0
+    assert m.check_rsa(6000, idx[:-1]) != 0            # a missing accept
+    assert m.check_rsa(6000, np.delete(idx, 3)) != 0   # a missing early accept
+    rejected = np.setdiff1d(np.arange(6000), idx)[5]
+    bad = np.sort(np.append(idx, rejected))
+    assert m.check_rsa(6000, bad) != 0                 # an extra (colliding) accept
+
+
+def test_fixed_window_stop_rule_by_hand():
+    # tiny batches: the rule fires at the first boundary whose window holds too few accepts
+    m = po.Model(2, 16, seed=1)
+    idx, _, att = m.pack(10**7, stop_kind=1, stop_batch=1024, window=4096, stop_rate=2e-3)
+    all_idx, _, _ = m.pack(att + 8192)
+    assert list(all_idx[all_idx < att]) == list(idx)
+    for c in range(4096, att + 1, 1024):
+        cnt = np.sum((all_idx >= c - 4096) & (all_idx < c))
+        if c < att:
+            assert cnt >= 2e-3 * 4096
+        else:
+            assert cnt < 2e-3 * 4096
+
+
+def test_max_n_stops_at_the_nth_accept():
+    m = po.Model(2, 16, seed=2)
+    idx, _, att = m.pack(20000, max_n=10)
+    assert len(idx) == 10 and att == idx[-1] + 1
+
+
+def test_subpath_rule_adopts_a_single_group():
+    m = po.Model(2, 16, seed=1)
+    idx, gid, att, pos = m.pack(20000, want_pos=True)
+    sidx, sgid, filled = m.pack_sub(pos, gid, first=att, count=20000)
+    assert len(sidx) > 0 and np.all(sgid >= 0) and np.all(sgid < len(idx))
+    # each subpath touches only its adopted group among the paths before it
+    X0 = {int(i): m.positions(int(i)) for i in idx}
+    r = m.r
+    for k in range(min(len(sidx), 15)):
+        Xs = m.positions(int(sidx[k]))
+        groups = set()
+        for i, g in zip(idx, gid):
+            d = np.abs(Xs - X0[int(i)])
+            d = np.minimum(d, 2.0 - d)
+            if np.any(np.all(d <= r, axis=1)):
+                groups.add(int(g))
+        for kk in range(k):
+            d = np.abs(Xs - m.positions(int(sidx[kk])))
+            d = np.minimum(d, 2.0 - d)
+            if np.any(np.all(d <= r, axis=1)):
+                groups.add(int(sgid[kk]))
+        assert groups == {int(sgid[k])}

@@ -1,0 +1,522 @@
+// ug_api.hip -- C ABI entry points and the packing loop of libugpack.so.
+//
+// The loop replaces the reference engine's main() (cuda/braid_cuda.cu:885-944,
+// cuda/braid_cuda3d.cu:1446-1566): candidates are consumed in index order in
+// sub-batches; each sub-batch is prefiltered against the device cell table, its
+// survivors are resolved in index order and inserted.  Because the accepted list is
+// serial RSA in index order, the sub-batch size only affects speed, never results.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ug_arith.h"
+#include "ug_internal.h"
+
+using namespace ug;
+
+namespace {
+
+// Host copy of the canonical generator (same arithmetic as the device's gen_wave64):
+// dump parameters of accepted paths, per axis a1 then (amplitude, frequency, phase).
+void gen_host(const Pack* h, uint64_t idx, double* out_row) {
+    const int D = h->d, nw = h->nw, npairs = h->npairs;
+    const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
+    const uint32_t k0 = h->dm.key0, k1 = h->dm.key1;
+    const u32x4 w0 = philox(lo, hi, 0u, 0u, k0, k1);
+    const uint32_t wa[3] = {w0.x, w0.y, w0.z};
+    std::vector<int> b(nw);
+    std::vector<double> e(nw);
+    std::vector<uint32_t> neg(nw), ph(nw);
+    for (int a = 0; a < D; a++) {
+        double* row = out_row + (size_t)a * (1 + 3 * nw);
+        row[0] = h->dm.pin ? 0.0 : a1_of_word(wa[a]);
+        if (h->full) {
+            for (int j = 0; j < nw; j++)
+                b[j] = j + 2;
+        } else {
+            std::vector<char> used(h->modmax + 2, 0);
+            uint32_t g = 0;
+            u32x4 wb{0, 0, 0, 0};
+            for (int j = 0; j < nw; j++) {
+                int bb = 0;
+                bool dup = true;
+                for (int guard = 0; dup && guard < 64; guard++) {
+                    if ((g & 3u) == 0)
+                        wb = philox(lo, hi, g >> 2, (2u << 8) | (uint32_t)a, k0, k1);
+                    const uint32_t wv = (g & 3u) == 0 ? wb.x : (g & 3u) == 1 ? wb.y : (g & 3u) == 2 ? wb.z : wb.w;
+                    bb = (int)(wv % (uint32_t)h->modmax) + 2;
+                    g++;
+                    dup = used[bb] != 0;
+                }
+                if (dup)
+                    for (int cc = 2; cc <= h->modmax + 1; cc++)
+                        if (!used[cc]) {
+                            bb = cc;
+                            break;
+                        }
+                used[bb] = 1;
+                b[j] = bb;
+            }
+        }
+        double Q[64];
+        for (int i = 0; i < 64; i++)
+            Q[i] = 0.0;
+        for (int p = 0; p < npairs; p++) {
+            const u32x4 r = philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, k0, k1);
+            const uint32_t dw[2] = {r.x, r.y}, pw[2] = {r.z, r.w};
+            for (int k = 0; k < 2; k++) {
+                const int j = 2 * p + k;
+                if (j >= nw)
+                    break;
+                e[j] = expo_of_word(dw[k]);
+                neg[j] = dw[k] & 1u;
+                ph[j] = pw[k];
+                Q[p & 63] = Q[p & 63] + e[j];
+            }
+        }
+        for (int s = 32; s >= 1; s >>= 1)
+            for (int i = 0; i < s; i++)
+                Q[i] = Q[i] + Q[i + s];
+        const double invW = 1.0 / Q[0];
+        for (int j = 0; j < nw; j++) {
+            const double invb = 1.0 / (double)b[j];
+            double aj = (nw == 1) ? invb : (e[j] * invW) * invb;
+            if (neg[j])
+                aj = -aj;
+            row[1 + 3 * j] = aj;
+            row[2 + 3 * j] = (double)b[j];
+            row[3 + 3 * j] = (b[j] & 1) ? 0.0 : (double)ph[j] * 0x1p-32 * 3.141592653589793;
+        }
+    }
+}
+
+template <typename P>
+void free_dev(P*& p) {
+    if (p)
+        (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void release(Pack* h) {
+    fast_release(h);
+    free_dev(h->d_S);
+    free_dev(h->d_C);
+    free_dev(h->d_order);
+    free_dev(h->d_tab);
+    free_dev(h->d_marg);
+    free_dev(h->d_pos64);
+    free_dev(h->d_pos32);
+    free_dev(h->d_acc_idx);
+    free_dev(h->d_acc_gid);
+    free_dev(h->d_filled);
+    free_dev(h->d_flags);
+    free_dev(h->d_list);
+    free_dev(h->d_list2);
+    free_dev(h->d_bcount);
+    free_dev(h->d_count);
+    free_dev(h->d_spos);
+    free_dev(h->d_mask);
+    free_dev(h->d_acc_list);
+    free_dev(h->d_nacc);
+    if (h->h_pinned)
+        (void)hipHostFree(h->h_pinned);
+    h->h_pinned = nullptr;
+}
+
+__global__ void k_fill_tab(int* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = -1;
+}
+
+int32_t record_curve(Pack* h) {
+    while (h->attempts >= h->next_ms) {
+        unsigned long long filled = 0;
+        if (hipMemcpy(&filled, h->d_filled, 8, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(UG_EHIP, "curve: filled-count copy failed");
+        h->curve.push_back({h->attempts, h->n, h->nsub, (int64_t)filled});
+        h->next_ms = (int64_t)((double)h->next_ms * 1.15) + 1;
+    }
+    return UG_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream, ug_pack** out) {
+    if (!cfg_in || !out)
+        return fail(UG_EINVAL, "ug_pack_create: NULL argument");
+    *out = nullptr;
+    ug_pack_config cfg = *cfg_in;
+    if (cfg.dim != 2 && cfg.dim != 3)
+        return fail(UG_EINVAL, "error: dim must be 2 or 3, got %d", cfg.dim);
+    if (cfg.T < 4 || cfg.T > UG_MAX_T)
+        return fail(UG_EINVAL, "error: T=%d outside the supported range [4, %d]", cfg.T, UG_MAX_T);
+    const int modmax = cfg.T - 1 > 2 ? cfg.T - 1 : 2;
+    if (cfg.terms == 0)
+        cfg.terms = cfg.T;
+    if (cfg.terms < 2)
+        cfg.terms = 2;
+    if ((uint32_t)(cfg.terms - 1) > (uint32_t)modmax)
+        return fail(UG_EINVAL, "error: --terms %d exceeds unique-frequency pool %d (max terms %d)",
+                    cfg.terms, modmax, modmax + 1);
+    if (cfg.precision != UG_PREC_FP64 && cfg.precision != UG_PREC_SPLIT_BF16)
+        return fail(UG_EINVAL, "error: unknown precision %d", cfg.precision);
+    if (cfg.stop_kind < UG_STOP_BUDGET || cfg.stop_kind > UG_STOP_REF_WINDOW)
+        return fail(UG_EINVAL, "error: unknown stop rule %d", cfg.stop_kind);
+    if (cfg.stop_kind != UG_STOP_BUDGET && !(cfg.stop_rate > 0))
+        return fail(UG_EINVAL, "error: the stop rule needs stop_rate > 0");
+    if (cfg.stop_batch <= 0)
+        cfg.stop_batch = 1 << 20;
+    if (cfg.stop_window <= 0)
+        cfg.stop_window = 1 << 22;
+    if (cfg.stop_kind == UG_STOP_FIXED_WINDOW && (cfg.stop_batch & (cfg.stop_batch - 1)))
+        return fail(UG_EINVAL, "error: stop_batch must be a power of two");
+    if (!(cfg.max_attempts > 0))
+        cfg.max_attempts = 9.2e18;
+
+    Pack* h = new Pack();
+    h->cfg = cfg;
+    h->stream = (hipStream_t)hip_stream;
+    (void)hipGetDevice(&h->device);
+    h->d = cfg.dim;
+    h->T = cfg.T;
+    h->nw = cfg.terms - 1;
+    h->npairs = (h->nw + 1) / 2;
+    h->modmax = modmax;
+    h->full = h->nw == modmax;
+    h->max_n = cfg.max_n;
+    const int T = h->T, W = T + 1;
+
+    // canonical basis tables on the host (ug_arith.h), uploaded once
+    std::vector<double> S((size_t)T * W), C((size_t)T * W);
+    const uint64_t Dn = (uint64_t)T + 1;
+    for (int t = 0; t < T; t++) {
+        const double s1 = sincospi_rat((uint64_t)(t + 1), Dn).s;
+        for (int k = 0; k <= T; k++) {
+            const uint64_t nk = (uint64_t)k * (uint64_t)(t + 1);
+            const double sk = sincospi_rat(nk, Dn).s;
+            const double hk = sincospi_rat(nk, 2 * Dn).s;
+            S[(size_t)t * W + k] = sk / s1;
+            C[(size_t)t * W + k] = (-2.0 * (hk * hk)) / s1;
+        }
+    }
+    std::vector<int> order(T);
+    for (int i = 0, o = 0; o < T; i++) {
+        order[o++] = i;
+        if (o < T)
+            order[o++] = T - 1 - i;
+    }
+    int64_t tab_len = (int64_t)T * T * (T + 2);
+    if (h->d == 3)
+        tab_len *= T;
+    h->tab_len = tab_len;
+    int64_t cap0 = cfg.max_paths > 0 ? cfg.max_paths : 4096;
+
+    int32_t rc = UG_OK;
+#define UG_ALLOC(ptr, bytes)                                                             \
+    do {                                                                                 \
+        if (rc == UG_OK && hipMalloc((void**)&(ptr), (bytes)) != hipSuccess)            \
+            rc = fail(UG_ENOMEM, "device allocation of %zu bytes failed", (size_t)(bytes)); \
+    } while (0)
+    UG_ALLOC(h->d_S, S.size() * 8);
+    UG_ALLOC(h->d_C, C.size() * 8);
+    UG_ALLOC(h->d_order, (size_t)T * 4);
+    UG_ALLOC(h->d_tab, (size_t)tab_len * 4);
+    UG_ALLOC(h->d_filled, 8);
+    UG_ALLOC(h->d_flags, (size_t)UG_SUB_MAX);
+    UG_ALLOC(h->d_list, (size_t)UG_SUB_MAX * 8);
+    UG_ALLOC(h->d_list2, (size_t)UG_SUB_MAX * 8);
+    h->cmp_blocks_cap = UG_SUB_MAX / 4096 + 1;
+    UG_ALLOC(h->d_bcount, (size_t)h->cmp_blocks_cap * 4);
+    UG_ALLOC(h->d_count, 4);
+    UG_ALLOC(h->d_spos, (size_t)UG_S_MAX * T * h->d * 8);
+    UG_ALLOC(h->d_mask, (size_t)UG_S_MAX * UG_MASK_WORDS * 8);
+    UG_ALLOC(h->d_acc_list, (size_t)UG_S_MAX * 4);
+    UG_ALLOC(h->d_nacc, 4);
+#undef UG_ALLOC
+    if (rc == UG_OK && hipHostMalloc(&h->h_pinned, 64) != hipSuccess)
+        rc = fail(UG_ENOMEM, "pinned host allocation failed");
+    if (rc == UG_OK) {
+        rc = hip_check(hipMemcpyAsync(h->d_S, S.data(), S.size() * 8, hipMemcpyHostToDevice, h->stream), "upload S");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(h->d_C, C.data(), C.size() * 8, hipMemcpyHostToDevice, h->stream), "upload C");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(h->d_order, order.data(), (size_t)T * 4, hipMemcpyHostToDevice, h->stream),
+                           "upload order");
+        if (!rc) {
+            hipLaunchKernelGGL(k_fill_tab, dim3(grid_for(tab_len, 256 * 8)), dim3(256), 0, h->stream, h->d_tab, tab_len);
+            rc = hip_check(hipMemsetAsync(h->d_filled, 0, 8, h->stream), "clear filled");
+        }
+    }
+    DevModel& m = h->dm;
+    m.T = T;
+    m.nw = h->nw;
+    m.npairs = h->npairs;
+    m.modmax = modmax;
+    m.pin = cfg.pin_sin1 ? 1 : 0;
+    m.full = h->full ? 1 : 0;
+    m.key0 = (uint32_t)cfg.seed;
+    m.key1 = (uint32_t)(cfg.seed >> 32);
+    m.r = 2.0 / (double)T;
+    m.halfT = 0.5 * (double)T;
+    m.ldb = W;
+    m.rowlen = T + 2;
+    m.S = h->d_S;
+    m.C = h->d_C;
+    m.order = h->d_order;
+    m.tab = h->d_tab;
+    m.pos64 = nullptr;
+    m.pos32 = nullptr;
+    m.marg = nullptr;
+    if (rc == UG_OK)
+        rc = ensure_paths(h, cap0);
+    if (rc == UG_OK && cfg.precision == UG_PREC_SPLIT_BF16)
+        rc = fast_setup(h);
+    if (rc == UG_OK)
+        rc = hip_check(hipStreamSynchronize(h->stream), "create sync");
+    if (rc != UG_OK) {
+        release(h);
+        delete h;
+        return rc;
+    }
+    *out = (ug_pack*)h;
+    return UG_OK;
+}
+
+extern "C" void ug_pack_destroy(ug_pack* hp) {
+    if (!hp)
+        return;
+    Pack* h = (Pack*)hp;
+    (void)hipStreamSynchronize(h->stream);
+    release(h);
+    delete h;
+}
+
+static void fill_status(const Pack* h, ug_pack_status* st) {
+    if (!st)
+        return;
+    st->attempts = h->attempts;
+    st->n = h->n;
+    st->nsub = h->nsub;
+    st->done = h->done ? 1 : 0;
+    st->reserved = 0;
+    st->survivors = h->survivors;
+    st->rechecked = h->rechecked;
+    st->margin = h->margin;
+    st->seconds = h->seconds;
+}
+
+extern "C" int32_t ug_pack_get_status(const ug_pack* hp, ug_pack_status* st) {
+    if (!hp || !st)
+        return fail(UG_EINVAL, "ug_pack_get_status: NULL argument");
+    fill_status((const Pack*)hp, st);
+    return UG_OK;
+}
+
+// Fixed-window rule: at a boundary c (multiple of stop_batch, c >= window) stop when
+// the accepts among [c - window, c) are fewer than stop_rate * window.
+static bool fixed_window_stop(const Pack* h) {
+    const ug_pack_config& c = h->cfg;
+    const int64_t at = h->attempts;
+    if (at == 0 || at % c.stop_batch != 0 || at < c.stop_window)
+        return false;
+    const int64_t lo = at - c.stop_window;
+    const auto it = std::lower_bound(h->acc_host.begin(), h->acc_host.end(), lo);
+    const int64_t cnt = (int64_t)(h->acc_host.end() - it);
+    return (double)cnt < c.stop_rate * (double)c.stop_window;
+}
+
+extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_status* st) {
+    if (!hp)
+        return fail(UG_EINVAL, "ug_pack_run: NULL handle");
+    Pack* h = (Pack*)hp;
+    const auto t0 = std::chrono::steady_clock::now();
+    const ug_pack_config& cfg = h->cfg;
+    const int64_t budget = (int64_t)std::min(cfg.max_attempts, 9.2e18);
+    const int64_t call_end = max_candidates > 0 ? h->attempts + max_candidates : INT64_MAX;
+    const int64_t sub_cap = std::min<int64_t>(UG_SUB_MAX, cfg.stop_kind == UG_STOP_FIXED_WINDOW ? cfg.stop_batch : UG_SUB_MAX);
+    const int64_t win_target = cfg.stop_kind == UG_STOP_REF_WINDOW
+                                   ? (int64_t)std::min(5e9, std::max(2e7, 30.0 / cfg.stop_rate))
+                                   : 0;
+    int32_t rc = UG_OK;
+    while (!h->done && rc == UG_OK) {
+        if (cfg.stop_kind == UG_STOP_FIXED_WINDOW && fixed_window_stop(h)) {
+            h->done = true;
+            break;
+        }
+        if (h->attempts >= budget || (h->max_n > 0 && h->n >= h->max_n)) {
+            h->done = true;
+            break;
+        }
+        if (h->attempts >= call_end)
+            break;
+        int64_t sub = std::min(h->sub, sub_cap);
+        sub = std::min(sub, budget - h->attempts);
+        sub = std::min(sub, call_end - h->attempts);
+        if (cfg.stop_kind == UG_STOP_FIXED_WINDOW) {
+            const int64_t nb = (h->attempts / cfg.stop_batch + 1) * cfg.stop_batch;
+            sub = std::min(sub, nb - h->attempts);
+        }
+        const uint64_t base = (uint64_t)h->attempts;
+        if (cfg.precision == UG_PREC_SPLIT_BF16)
+            rc = launch_fast(h, base, sub, h->d_flags);
+        else
+            rc = launch_cand64(h, base, sub, nullptr, h->d_flags, nullptr, false);
+        if (rc)
+            break;
+        int64_t S = 0;
+        rc = compact(h, h->d_flags, sub, base, nullptr, h->d_list, &S);
+        if (rc)
+            break;
+        if (cfg.precision == UG_PREC_SPLIT_BF16 && S > 0) {
+            // FP64 recheck of every candidate the split-BF16 prefilter could not reject
+            h->rechecked += S;
+            rc = launch_cand64(h, 0, S, h->d_list, h->d_flags, nullptr, false);
+            if (!rc)
+                rc = compact(h, h->d_flags, S, 0, h->d_list, h->d_list2, &S);
+            if (!rc && S > 0)
+                rc = hip_check(hipMemcpyAsync(h->d_list, h->d_list2, (size_t)S * 8, hipMemcpyDeviceToDevice, h->stream),
+                               "list copy");
+            if (rc)
+                break;
+        }
+        h->survivors += S;
+        int64_t acc = 0;
+        const int64_t n_before = h->n;
+        rc = resolve_and_insert(h, h->d_list, S, &acc);
+        if (rc)
+            break;
+        if (h->max_n > 0 && h->n >= h->max_n) {
+            h->attempts = h->acc_host[(size_t)h->max_n - 1] + 1;
+            h->done = true;
+        } else {
+            h->attempts += sub;
+        }
+        // sub-batch adaptation toward ~256 survivors (cuda/braid_cuda.cu:920-924)
+        if (S > 512 && h->sub > 4096)
+            h->sub /= 2;
+        else if (S < 64 && h->sub < sub_cap)
+            h->sub *= 2;
+        if (win_target) {
+            const int64_t adm = h->n - n_before;
+            h->win.push_back({sub, adm});
+            h->win_att += sub;
+            h->win_adm += adm;
+            while (h->win.size() > 1 && h->win_att - h->win.front().first >= win_target) {
+                h->win_att -= h->win.front().first;
+                h->win_adm -= h->win.front().second;
+                h->win.pop_front();
+            }
+            if (h->win_att >= win_target && h->attempts > 1000000 &&
+                (double)h->win_adm / (double)h->win_att < cfg.stop_rate)
+                h->done = true;
+        }
+        rc = record_curve(h);
+    }
+    if (rc == UG_OK)
+        rc = hip_check(hipStreamSynchronize(h->stream), "run sync");
+    h->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_status(h, st);
+    return rc;
+}
+
+extern "C" int32_t ug_pack_accepted(const ug_pack* hp, int64_t* idx, int32_t* gid, int64_t cap, int64_t* n) {
+    if (!hp)
+        return fail(UG_EINVAL, "ug_pack_accepted: NULL handle");
+    const Pack* h = (const Pack*)hp;
+    const int64_t tot = h->n + h->nsub;
+    if (n)
+        *n = tot;
+    const int64_t k = std::min(cap, tot);
+    if (k <= 0)
+        return UG_OK;
+    int32_t rc = UG_OK;
+    if (idx)
+        rc = hip_check(hipMemcpyAsync(idx, h->d_acc_idx, (size_t)k * 8, hipMemcpyDeviceToHost, h->stream), "idx copy");
+    if (!rc && gid)
+        rc = hip_check(hipMemcpyAsync(gid, h->d_acc_gid, (size_t)k * 4, hipMemcpyDeviceToHost, h->stream), "gid copy");
+    if (!rc)
+        rc = hip_check(hipStreamSynchronize(h->stream), "accepted sync");
+    return rc;
+}
+
+extern "C" int32_t ug_pack_params(const ug_pack* hp, int64_t first, int64_t count, double* out) {
+    if (!hp || (!out && count > 0))
+        return fail(UG_EINVAL, "ug_pack_params: NULL argument");
+    const Pack* h = (const Pack*)hp;
+    const int64_t tot = h->n + h->nsub;
+    if (first < 0 || count < 0 || first + count > tot)
+        return fail(UG_EINVAL, "ug_pack_params: range [%lld, %lld) outside %lld accepted paths",
+                    (long long)first, (long long)(first + count), (long long)tot);
+    if (count == 0)
+        return UG_OK;
+    std::vector<int64_t> idx(count);
+    int32_t rc = hip_check(hipMemcpyAsync(idx.data(), h->d_acc_idx + first, (size_t)count * 8, hipMemcpyDeviceToHost,
+                                          h->stream),
+                           "params idx copy");
+    if (!rc)
+        rc = hip_check(hipStreamSynchronize(h->stream), "params sync");
+    if (rc)
+        return rc;
+    const size_t row = (size_t)h->d * (1 + 3 * h->nw);
+    for (int64_t i = 0; i < count; i++)
+        gen_host(h, (uint64_t)idx[i], out + (size_t)i * row);
+    return UG_OK;
+}
+
+extern "C" int32_t ug_pack_positions(const ug_pack* hp, double* dev_out, int64_t cap) {
+    if (!hp || !dev_out)
+        return fail(UG_EINVAL, "ug_pack_positions: NULL argument");
+    const Pack* h = (const Pack*)hp;
+    const int64_t tot = h->n + h->nsub;
+    if (cap < tot)
+        return fail(UG_EOVERFLOW, "ug_pack_positions: capacity %lld < %lld paths", (long long)cap, (long long)tot);
+    if (tot == 0)
+        return UG_OK;
+    int32_t rc = hip_check(hipMemcpyAsync(dev_out, h->d_pos64, (size_t)tot * h->T * h->d * 8, hipMemcpyDeviceToDevice,
+                                          h->stream),
+                           "positions copy");
+    if (!rc)
+        rc = hip_check(hipStreamSynchronize(h->stream), "positions sync");
+    return rc;
+}
+
+extern "C" int32_t ug_pack_curve(const ug_pack* hp, int64_t* rows4, int64_t cap, int64_t* n) {
+    if (!hp)
+        return fail(UG_EINVAL, "ug_pack_curve: NULL handle");
+    const Pack* h = (const Pack*)hp;
+    if (n)
+        *n = (int64_t)h->curve.size();
+    const int64_t k = std::min<int64_t>(cap, (int64_t)h->curve.size());
+    for (int64_t i = 0; i < k && rows4; i++)
+        for (int j = 0; j < 4; j++)
+            rows4[i * 4 + j] = h->curve[(size_t)i][(size_t)j];
+    return UG_OK;
+}
+
+extern "C" int32_t ug_eval_candidates(const ug_pack* hp, const int64_t* dev_idx, int64_t count, int32_t precision,
+                                      double* dev_out) {
+    if (!hp || (count > 0 && (!dev_idx || !dev_out)))
+        return fail(UG_EINVAL, "ug_eval_candidates: NULL argument");
+    Pack* h = (Pack*)hp;
+    int32_t rc;
+    if (precision == UG_PREC_FP64)
+        rc = launch_cand64(h, 0, count, dev_idx, nullptr, dev_out, true);
+    else if (precision == UG_PREC_SPLIT_BF16)
+        rc = launch_fast_eval(h, dev_idx, count, dev_out);
+    else
+        return fail(UG_EINVAL, "ug_eval_candidates: unknown precision %d", precision);
+    if (!rc)
+        rc = hip_check(hipStreamSynchronize(h->stream), "eval sync");
+    return rc;
+}
+
+extern "C" int32_t ug_pack_run_sub(ug_pack* hp, int32_t sub_terms, int64_t count, ug_pack_status* st) {
+    (void)sub_terms;
+    (void)count;
+    if (!hp)
+        return fail(UG_EINVAL, "ug_pack_run_sub: NULL handle");
+    fill_status((Pack*)hp, st);
+    return fail(UG_ESTATE, "ug_pack_run_sub: subpath phase not available in this build");
+}

@@ -1,0 +1,101 @@
+// ug_internal.h -- shared types of the packer's translation units (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <array>
+#include <deque>
+#include <utility>
+#include <vector>
+
+#include "../../include/ugpack.h"
+
+#define UG_MAX_T 512      // basis pairs per lane per axis <= 4 (T <= 512)
+#define MAXPL 4
+#define UG_S_MAX 1024     // survivors resolved per in-batch chunk
+#define UG_MASK_WORDS (UG_S_MAX / 64)
+#define UG_SUB_MAX (1 << 20)  // largest candidate sub-batch
+
+namespace ug {
+
+// Everything a kernel needs about the model and the packing, by value.
+struct DevModel {
+    int T, nw, npairs, modmax, pin, full;
+    uint32_t key0, key1;
+    double r, halfT;
+    int ldb;          // basis row length (T + 1)
+    int rowlen;       // cell-table row length along the last axis (T + 2: one halo cell each side)
+    const double* S;  // [T][T+1]  sin(k z_t)/sin z_t
+    const double* C;  // [T][T+1]  (cos(k z_t) - 1)/sin z_t
+    const int* order; // [T] endpoint-first scan order
+    int* tab;         // cell table: head path of each (t, cell) chain, -1 = empty
+    const double* pos64;  // [n][T][D] canonical positions of accepted paths
+    const float4* pos32;  // [n][T] (x, y, w, next path in the same cell as int bits)
+    // split-BF16 form
+    const float* marg;    // [T] certainty margin per timestep
+};
+
+struct Pack {
+    ug_pack_config cfg;
+    hipStream_t stream;
+    int device;
+    int d, T, nw, npairs, modmax;
+    bool full;
+    DevModel dm;
+    // tables
+    double *d_S = nullptr, *d_C = nullptr;
+    int* d_order = nullptr;
+    int* d_tab = nullptr;
+    int64_t tab_len = 0;
+    float* d_marg = nullptr;
+    std::vector<float> marg_host;
+    double margin = 0.0;
+    // accepted paths
+    double* d_pos64 = nullptr;
+    float4* d_pos32 = nullptr;
+    int64_t* d_acc_idx = nullptr;
+    int32_t* d_acc_gid = nullptr;
+    unsigned long long* d_filled = nullptr;
+    int64_t cap = 0, n = 0, nsub = 0;
+    // batch scratch
+    uint8_t* d_flags = nullptr;
+    int64_t* d_list = nullptr;
+    int64_t* d_list2 = nullptr;
+    int* d_bcount = nullptr;
+    int cmp_blocks_cap = 0;
+    int* d_count = nullptr;
+    double* d_spos = nullptr;
+    uint64_t* d_mask = nullptr;
+    int* d_acc_list = nullptr;
+    int* d_nacc = nullptr;
+    void* h_pinned = nullptr;
+    // split-BF16 scratch
+    void* d_basis16 = nullptr;
+    // host state
+    std::vector<int64_t> acc_host;
+    std::vector<std::array<int64_t, 4>> curve;
+    int64_t next_ms = 1000;
+    int64_t attempts = 0, max_n = 0, sub = 4096;
+    bool done = false;
+    int64_t survivors = 0, rechecked = 0;
+    double seconds = 0.0;
+    std::deque<std::pair<int64_t, int64_t>> win;
+    int64_t win_att = 0, win_adm = 0;
+};
+
+int32_t fail(int32_t code, const char* fmt, ...);
+int32_t hip_check(hipError_t e, const char* what);
+int grid_for(int64_t items, int per_block);
+int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
+                      double* out, bool full);
+int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, const int64_t* list,
+                int64_t* out, int64_t* S);
+int32_t ensure_paths(Pack* h, int64_t need);
+int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc);
+// split-BF16 prefilter (ug_fast.hip)
+int32_t fast_setup(Pack* h);
+int32_t launch_fast(Pack* h, uint64_t base, int64_t count, uint8_t* flags);
+int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* out);
+void fast_release(Pack* h);
+
+}  // namespace ug

@@ -1,0 +1,677 @@
+// ug_pack.hip -- MI355X RSA worldline packer: kernels, host driver and C ABI.
+//
+// Path (reference universe-analysis/universe-generator):
+//   braidlab/engine.py:50-112 -> cuda/braid_cuda.cu:519-1106 (2+1) / cuda/braid_cuda3d.cu:849-1672 (3+1)
+// The reference prefilters a batch on the GPU with atomicAdd survivor slots
+// (cuda/braid_cuda.cu:475-510) and admits survivors serially on the host
+// (:903-919).  Here every stage stays on the device and is deterministic:
+//   1. candidate test   one wave per candidate: Philox draws -> FP64 coefficients
+//                       -> lazy evaluation in the endpoint-first order -> 3^d cell
+//                       lookup per timestep with early exit        (k_cand64)
+//      [split-BF16 form: tile generation in FP32 + MFMA evaluation + certain-hit
+//       classification, FP64 recheck of everything uncertain       (ug_fast.hip)]
+//   2. ordered compaction of survivors by candidate index          (k_compact_*)
+//   3. full FP64 trajectories of the survivors                     (k_cand64<FULL>)
+//   4. pairwise conflicts among survivors + greedy in index order  (k_conflicts, k_greedy)
+//   5. insertion into the per-timestep cell table                  (k_insert)
+// so the accepted list equals serial RSA in candidate-index order (cuda/braid_knot.cu:433-462).
+#include <stdarg.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ug_arith.h"
+#include "ug_internal.h"
+
+using namespace ug;
+
+// =====================================================================================
+// errors
+static thread_local std::string g_err;
+
+int32_t ug::fail(int32_t code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+extern "C" const char* ug_last_error(void) { return g_err.c_str(); }
+extern "C" int32_t ug_abi_version(void) { return UG_ABI_VERSION; }
+
+// =====================================================================================
+// wave helpers
+__device__ __forceinline__ double wave_sum_canonical(double v) {
+    // Halving tree over the 64 lane partials: step h adds lane l^h.  Lanes l and l^h
+    // compute the same commutative sum, so every lane ends with the canonical total.
+#pragma unroll
+    for (int h = 32; h >= 1; h >>= 1)
+        v = v + __shfl_xor(v, h, 64);
+    return v;
+}
+
+// =====================================================================================
+// per-wave candidate coefficients (FP64, canonical)
+template <int D>
+struct WaveCoef {
+    double a1[D];
+    double c[D][MAXPL][2];
+    double s[D][MAXPL][2];
+    int b[D][MAXPL][2];  // frequency of the slot, 0 = no slot
+};
+
+// Frequencies of a partial spectrum (nw < modmax): lane 0 replays the unique-frequency
+// rejection draws of cuda/braid_cuda.cu:220-243 from stage-2 words into LDS.
+template <int D>
+__device__ void draw_freqs_partial(const DevModel& m, uint64_t idx, int* sb, uint32_t* used) {
+    const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
+    for (int a = 0; a < D; a++) {
+        const int nwords = (m.modmax + 2 + 31) / 32;
+        for (int i = 0; i < nwords; i++)
+            used[i] = 0u;
+        uint32_t g = 0;
+        u32x4 wb{0, 0, 0, 0};
+        for (int j = 0; j < m.nw; j++) {
+            int bb = 0;
+            bool dup = true;
+            for (int guard = 0; dup && guard < 64; guard++) {
+                if ((g & 3u) == 0)
+                    wb = philox(lo, hi, g >> 2, (2u << 8) | (uint32_t)a, m.key0, m.key1);
+                const uint32_t wv = (g & 3u) == 0 ? wb.x : (g & 3u) == 1 ? wb.y : (g & 3u) == 2 ? wb.z : wb.w;
+                bb = (int)(wv % (uint32_t)m.modmax) + 2;
+                g++;
+                dup = (used[bb >> 5] >> (bb & 31)) & 1u;
+            }
+            if (dup) {
+                for (int cc = 2; cc <= m.modmax + 1; cc++)
+                    if (!((used[cc >> 5] >> (cc & 31)) & 1u)) {
+                        bb = cc;
+                        break;
+                    }
+            }
+            used[bb >> 5] |= 1u << (bb & 31);
+            sb[a * m.nw + j] = bb;
+        }
+    }
+}
+
+template <int D>
+__host__ __device__ inline int partial_lds_words(const DevModel& m) {
+    return m.full ? 0 : D * m.nw + (m.modmax + 2 + 31) / 32;
+}
+
+template <int D>
+__device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D>& w, int* sb,
+                           uint32_t* used) {
+    const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
+    const u32x4 w0 = philox(lo, hi, 0u, 0u, m.key0, m.key1);
+    const uint32_t wa[3] = {w0.x, w0.y, w0.z};
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        w.a1[a] = m.pin ? 0.0 : a1_of_word(wa[a]);
+    if (!m.full) {
+        if (lane == 0)
+            draw_freqs_partial<D>(m, idx, sb, used);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double e[MAXPL][2];
+        uint32_t neg[MAXPL][2], ph[MAXPL][2];
+        double q = 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXPL; i++) {
+            const int p = lane + 64 * i;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                e[i][k] = 0.0;
+                neg[i][k] = 0;
+                ph[i][k] = 0;
+                w.b[a][i][k] = 0;
+            }
+            if (p < m.npairs) {
+                const u32x4 r = philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, m.key0, m.key1);
+                const uint32_t dw[2] = {r.x, r.y}, pw[2] = {r.z, r.w};
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const int j = 2 * p + k;
+                    if (j < m.nw) {
+                        e[i][k] = expo_of_word(dw[k]);
+                        neg[i][k] = dw[k] & 1u;
+                        ph[i][k] = pw[k];
+                        w.b[a][i][k] = m.full ? j + 2 : sb[a * m.nw + j];
+                        q = q + e[i][k];
+                    }
+                }
+            }
+        }
+        const double W = wave_sum_canonical(q);
+        const double invW = 1.0 / W;
+#pragma unroll
+        for (int i = 0; i < MAXPL; i++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int bj = w.b[a][i][k];
+                double cj = 0.0, sj = 0.0;
+                if (bj) {
+                    const double invb = 1.0 / (double)bj;
+                    double aj = (m.nw == 1) ? invb : (e[i][k] * invW) * invb;
+                    if (neg[i][k])
+                        aj = -aj;
+                    if ((bj & 1) == 0) {
+                        const sc2 f = sincospi_rat((uint64_t)ph[i][k], 0x100000000ull);
+                        cj = aj * f.c;
+                        sj = aj * f.s;
+                    } else {
+                        cj = aj;
+                    }
+                }
+                w.c[a][i][k] = cj;
+                w.s[a][i][k] = sj;
+            }
+    }
+}
+
+// Canonical wrapped position of the wave's candidate at timestep t (all lanes).
+template <int D>
+__device__ __forceinline__ void eval_wave64(const DevModel& m, const WaveCoef<D>& w, int t,
+                                            double* X) {
+    const double* St = m.S + (size_t)t * m.ldb;
+    const double* Ct = m.C + (size_t)t * m.ldb;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        double part = 0.0;
+#pragma unroll
+        for (int i = 0; i < MAXPL; i++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int bj = w.b[a][i][k];
+                if (bj) {
+                    part = fma(w.c[a][i][k], St[bj], part);
+                    if ((bj & 1) == 0)
+                        part = fma(w.s[a][i][k], Ct[bj], part);
+                }
+            }
+        X[a] = torus_wrap(w.a1[a] + wave_sum_canonical(part));
+    }
+}
+
+// Does position X (canonical FP64) collide with a path of the cell table at timestep t?
+// Lanes < 3^(D-1) each scan one 3-cell row of the neighbourhood (cuda/braid_cuda.cu:442-462).
+template <int D>
+__device__ __forceinline__ bool lookup_wave64(const DevModel& m, int t, const double* X,
+                                              int lane) {
+    int c[D];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        c[a] = cell_of(X[a], m.halfT, m.T);
+    bool hit = false;
+    constexpr int ROWS = D == 2 ? 3 : 9;
+    if (lane < ROWS) {
+        const int T = m.T;
+        int64_t row;
+        if (D == 2) {
+            const int gx = (c[0] + lane - 1 + T) % T;
+            row = ((int64_t)t * T + gx) * m.rowlen;
+        } else {
+            const int gx = (c[0] + lane / 3 - 1 + T) % T;
+            const int gy = (c[1] + lane % 3 - 1 + T) % T;
+            row = (((int64_t)t * T + gx) * T + gy) * m.rowlen;
+        }
+        const int* tr = m.tab + row + c[D - 1];  // padded cells c-1..c+1 sit at c..c+2
+        for (int k = 0; k < 3 && !hit; k++) {
+            for (int n = tr[k]; n >= 0 && !hit;) {
+                const size_t o = (size_t)n * T + t;
+                const double* P = m.pos64 + o * D;
+                bool h = true;
+#pragma unroll
+                for (int a = 0; a < D; a++)
+                    h = h && fabs(torus_delta(X[a] - P[a])) <= m.r;
+                hit = h;
+                n = __float_as_int(m.pos32[o].w);
+            }
+        }
+    }
+    return __ballot(hit) != 0;
+}
+
+// One wave per candidate.  FULL = false: flags[i] = 1 iff candidate survives the cell
+// table (early exit on the first contact).  FULL = true: write the canonical trajectory
+// [i][T][D] to out (no lookup).
+template <int D, bool FULL>
+__global__ __launch_bounds__(256) void k_cand64(DevModel m, uint64_t base, int64_t count,
+                                                const int64_t* list, uint8_t* flags,
+                                                double* out) {
+    // Dynamic LDS (partial spectra only): per wave D*nw slot frequencies + a bitmask.
+    extern __shared__ int dyn_lds[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int per_wave = partial_lds_words<D>(m);
+    int* sb = dyn_lds + wv * per_wave;
+    uint32_t* used = (uint32_t*)(sb + D * m.nw);
+    const int64_t nwaves = (int64_t)gridDim.x * 4;
+    for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < count; i += nwaves) {
+        const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
+        WaveCoef<D> w;
+        gen_wave64<D>(m, idx, lane, w, sb, used);
+        if (FULL) {
+            for (int t = 0; t < m.T; t++) {
+                double X[D];
+                eval_wave64<D>(m, w, t, X);
+                if (lane < D)
+                    out[((size_t)i * m.T + t) * D + lane] = X[lane];
+            }
+        } else {
+            bool hit = false;
+            for (int oi = 0; oi < m.T && !hit; oi++) {
+                const int t = m.order[oi];
+                double X[D];
+                eval_wave64<D>(m, w, t, X);
+                hit = lookup_wave64<D>(m, t, X, lane);
+            }
+            if (lane == 0)
+                flags[i] = hit ? 0 : 1;
+        }
+    }
+}
+
+// =====================================================================================
+// ordered compaction: out[k] = candidate index of the k-th flagged item (stable)
+constexpr int CMP_ITEMS = 4096;  // items per block (256 threads x 16)
+
+__global__ __launch_bounds__(256) void k_compact_count(const uint8_t* flags, int64_t count,
+                                                       int* bcount) {
+    __shared__ int red[256];
+    const int64_t b0 = (int64_t)blockIdx.x * CMP_ITEMS;
+    int n = 0;
+    for (int k = threadIdx.x; k < CMP_ITEMS; k += 256) {
+        const int64_t i = b0 + k;
+        if (i < count)
+            n += flags[i];
+    }
+    red[threadIdx.x] = n;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        bcount[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_compact_scatter(const uint8_t* flags, int64_t count,
+                                                         uint64_t base, const int64_t* list,
+                                                         const int* bcount, int nblocks,
+                                                         int64_t* out, int* total) {
+    __shared__ int red[256];
+    __shared__ int scan[256];
+    // offset of this block = sum of earlier block counts
+    int s0 = 0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += 256)
+        s0 += bcount[b];
+    red[threadIdx.x] = s0;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const int boff = red[0];
+    // each thread owns 16 consecutive items
+    const int64_t i0 = (int64_t)blockIdx.x * CMP_ITEMS + (int64_t)threadIdx.x * 16;
+    int n = 0;
+    for (int k = 0; k < 16; k++)
+        if (i0 + k < count)
+            n += flags[i0 + k];
+    scan[threadIdx.x] = n;
+    __syncthreads();
+    for (int s = 1; s < 256; s <<= 1) {  // inclusive Hillis-Steele scan
+        const int v = threadIdx.x >= s ? scan[threadIdx.x - s] : 0;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int o = boff + scan[threadIdx.x] - n;
+    for (int k = 0; k < 16; k++) {
+        const int64_t i = i0 + k;
+        if (i < count && flags[i])
+            out[o++] = list ? list[i] : (int64_t)(base + (uint64_t)i);
+    }
+    if (blockIdx.x == nblocks - 1 && threadIdx.x == 255)
+        *total = boff + scan[255];
+}
+
+// =====================================================================================
+// in-batch accept
+// Canonical pair predicate at one timestep: neighbouring cells on every axis and
+// |torus_delta| <= r on every axis -- exactly what a cell-table lookup would find.
+template <int D>
+__device__ __forceinline__ bool pair_hit(const DevModel& m, const double* x, const double* y) {
+    bool h = true;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int dc = cell_of(x[a], m.halfT, m.T) - cell_of(y[a], m.halfT, m.T);
+        dc = dc < 0 ? -dc : dc;
+        h = h && (dc <= 1 || dc == m.T - 1) && fabs(torus_delta(x[a] - y[a])) <= m.r;
+    }
+    return h;
+}
+
+// Conflict bits among the first S survivors: mask[i][j/64] bit j%64 set iff i > j collide.
+template <int D>
+__global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos, int S,
+                                                   uint64_t* mask) {
+    const int64_t npairs = (int64_t)S * (S - 1) / 2;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < npairs;
+         k += (int64_t)gridDim.x * 256) {
+        int64_t i = (int64_t)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
+        while (i * (i - 1) / 2 > k)
+            i--;
+        while ((i + 1) * i / 2 <= k)
+            i++;
+        const int64_t j = k - i * (i - 1) / 2;
+        const double* xi = pos + (size_t)i * m.T * D;
+        const double* xj = pos + (size_t)j * m.T * D;
+        for (int t = 0; t < m.T; t++)
+            if (pair_hit<D>(m, xi + (size_t)t * D, xj + (size_t)t * D)) {
+                atomicOr((unsigned long long*)&mask[(size_t)i * UG_MASK_WORDS + (j >> 6)],
+                         1ull << (j & 63));
+                break;
+            }
+    }
+}
+
+// Greedy in index order over the conflict bits (one wave): survivor i is accepted iff it
+// conflicts with no earlier accepted survivor -- serial RSA restricted to the chunk.
+__global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, int* acc_list,
+                                               int* nacc) {
+    __shared__ uint64_t accw[UG_MASK_WORDS];
+    const int lane = threadIdx.x;
+    int na = 0;
+    for (int blk = 0; blk * 64 < S; blk++) {
+        const int i = blk * 64 + lane;
+        const bool valid = i < S;
+        bool dead = !valid;
+        for (int wd = 0; wd < blk && valid; wd++)
+            if (mask[(size_t)i * UG_MASK_WORDS + wd] & accw[wd])
+                dead = true;
+        const uint64_t inrow = valid ? mask[(size_t)i * UG_MASK_WORDS + blk] : 0ull;
+        const uint64_t deadm = __ballot(dead);
+        uint64_t accb = 0;
+        const int nin = min(64, S - blk * 64);
+        for (int k = 0; k < nin; k++) {
+            const uint64_t rowk = __shfl(inrow, k, 64);
+            if (!((deadm >> k) & 1ull) && (rowk & accb) == 0)
+                accb |= 1ull << k;
+        }
+        if (lane == 0)
+            accw[blk] = accb;
+        __syncthreads();
+        if (valid && ((accb >> lane) & 1ull)) {
+            const int pos = na + __popcll(accb & ((1ull << lane) - 1ull));
+            acc_list[pos] = i;
+        }
+        na += __popcll(accb);
+    }
+    if (lane == 0)
+        *nacc = na;
+}
+
+// Insert the chunk's accepted survivors (admission order) into the cell table; one
+// thread per timestep, so chain order equals admission order (cuda/braid_cuda.cu:865-879).
+template <int D>
+__global__ __launch_bounds__(64) void k_insert(DevModel m, const double* pos, const int* acc_list,
+                                               const int* nacc, const int64_t* surv_idx,
+                                               int64_t n0, int64_t* acc_idx, int32_t* acc_gid,
+                                               double* pos64, float4* pos32,
+                                               unsigned long long* filled) {
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const int na = *nacc;
+    if (t < m.T) {
+        unsigned long long fresh = 0;
+        for (int k = 0; k < na; k++) {
+            const int s = acc_list[k];
+            const int64_t n = n0 + k;
+            const double* X = pos + ((size_t)s * m.T + t) * D;
+            int c[D];
+            float f[3] = {0.f, 0.f, 0.f};
+            const size_t o = (size_t)n * m.T + t;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                c[a] = cell_of(X[a], m.halfT, m.T);
+                pos64[o * D + a] = X[a];
+                f[a] = (float)X[a];
+            }
+            int64_t row = (int64_t)t * m.T + c[0];
+            if (D == 3)
+                row = row * m.T + c[1];
+            row *= m.rowlen;
+            const int cl = c[D - 1];
+            const int head = m.tab[row + cl + 1];
+            fresh += head < 0 ? 1ull : 0ull;
+            pos32[o] = make_float4(f[0], f[1], f[2], __int_as_float(head));
+            m.tab[row + cl + 1] = (int)n;
+            if (cl == 0)
+                m.tab[row + m.T + 1] = (int)n;
+            if (cl == m.T - 1)
+                m.tab[row] = (int)n;
+        }
+        if (fresh)
+            atomicAdd(filled, fresh);
+    }
+    if (t == 0)
+        for (int k = 0; k < na; k++) {
+            acc_idx[n0 + k] = surv_idx[acc_list[k]];
+            acc_gid[n0 + k] = (int32_t)(n0 + k);
+        }
+}
+
+__global__ void k_fill_i32(int* p, int64_t n, int v) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = v;
+}
+__global__ void k_fill_u64(uint64_t* p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        p[i] = 0ull;
+}
+
+// =====================================================================================
+// host driver
+namespace ug {
+
+int grid_for(int64_t items, int per_block) {
+    int64_t b = (items + per_block - 1) / per_block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 65535));
+}
+
+int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
+                      double* out, bool full) {
+    if (count <= 0)
+        return UG_OK;
+    const int blocks = (int)std::min<int64_t>((count + 3) / 4, 256 * 16);
+    const size_t lds = (size_t)4 * 4 * (h->d == 2 ? partial_lds_words<2>(h->dm) : partial_lds_words<3>(h->dm));
+    if (h->d == 2) {
+        if (full)
+            hipLaunchKernelGGL((k_cand64<2, true>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, list, flags, out);
+        else
+            hipLaunchKernelGGL((k_cand64<2, false>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, list, flags, out);
+    } else {
+        if (full)
+            hipLaunchKernelGGL((k_cand64<3, true>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, list, flags, out);
+        else
+            hipLaunchKernelGGL((k_cand64<3, false>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, list, flags, out);
+    }
+    return hip_check(hipGetLastError(), "k_cand64");
+}
+
+int32_t hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        return fail(UG_EHIP, "%s: %s", what, hipGetErrorString(e));
+    return UG_OK;
+}
+
+// Ordered compaction of flags[count] into h->list_out; returns the count via *S (syncs).
+int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, const int64_t* list,
+                int64_t* out, int64_t* S) {
+    const int nb = (int)((count + CMP_ITEMS - 1) / CMP_ITEMS);
+    if (nb > h->cmp_blocks_cap)
+        return fail(UG_EOVERFLOW, "compaction of %lld items exceeds the batch capacity",
+                    (long long)count);
+    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(256), 0, h->stream, flags, count, h->d_bcount);
+    hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(256), 0, h->stream, flags, count, base, list,
+                       h->d_bcount, nb, out, h->d_count);
+    int32_t rc = hip_check(hipGetLastError(), "compact");
+    if (rc)
+        return rc;
+    rc = hip_check(hipMemcpyAsync(h->h_pinned, h->d_count, sizeof(int), hipMemcpyDeviceToHost, h->stream),
+                   "compact count copy");
+    if (rc)
+        return rc;
+    rc = hip_check(hipStreamSynchronize(h->stream), "compact sync");
+    if (rc)
+        return rc;
+    *S = *(int*)h->h_pinned;
+    return UG_OK;
+}
+
+int32_t ensure_paths(Pack* h, int64_t need) {
+    if (need <= h->cap)
+        return UG_OK;
+    int64_t nc = std::max<int64_t>(need, h->cap * 2);
+    double* p64 = nullptr;
+    float4* p32 = nullptr;
+    int64_t* ai = nullptr;
+    int32_t* ag = nullptr;
+    const size_t T = (size_t)h->T, D = (size_t)h->d;
+    if (hipMalloc(&p64, (size_t)nc * T * D * 8) != hipSuccess ||
+        hipMalloc(&p32, (size_t)nc * T * 16) != hipSuccess ||
+        hipMalloc(&ai, (size_t)nc * 8) != hipSuccess || hipMalloc(&ag, (size_t)nc * 4) != hipSuccess) {
+        (void)hipFree(p64);
+        (void)hipFree(p32);
+        (void)hipFree(ai);
+        (void)hipFree(ag);
+        return fail(UG_ENOMEM, "cannot grow path storage to %lld paths", (long long)nc);
+    }
+    if (h->n > 0) {
+        (void)hipMemcpyAsync(p64, h->d_pos64, (size_t)h->n * T * D * 8, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(p32, h->d_pos32, (size_t)h->n * T * 16, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(ai, h->d_acc_idx, (size_t)h->n * 8, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(ag, h->d_acc_gid, (size_t)h->n * 4, hipMemcpyDeviceToDevice, h->stream);
+    }
+    int32_t rc = hip_check(hipStreamSynchronize(h->stream), "grow copy");
+    (void)hipFree(h->d_pos64);
+    (void)hipFree(h->d_pos32);
+    (void)hipFree(h->d_acc_idx);
+    (void)hipFree(h->d_acc_gid);
+    h->d_pos64 = p64;
+    h->d_pos32 = p32;
+    h->d_acc_idx = ai;
+    h->d_acc_gid = ag;
+    h->cap = nc;
+    h->dm.pos64 = p64;
+    h->dm.pos32 = p32;
+    return rc;
+}
+
+// Resolve the survivors list[0..S) (already clear of the cell table) in index order and
+// insert the accepted ones.  Survivors beyond one chunk are retested against the updated
+// table before their turn.  Returns the number accepted via *acc.
+int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total) {
+    int32_t rc;
+    *acc_total = 0;
+    while (S > 0) {
+        const int s = (int)std::min<int64_t>(S, UG_S_MAX);
+        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true);
+        if (rc)
+            return rc;
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((int64_t)s * UG_MASK_WORDS, 256)), dim3(256), 0,
+                           h->stream, h->d_mask, (int64_t)s * UG_MASK_WORDS);
+        const int64_t npairs = (int64_t)s * (s - 1) / 2;
+        if (npairs > 0) {
+            if (h->d == 2)
+                hipLaunchKernelGGL(k_conflicts<2>, dim3(grid_for(npairs, 256)), dim3(256), 0, h->stream,
+                                   h->dm, h->d_spos, s, h->d_mask);
+            else
+                hipLaunchKernelGGL(k_conflicts<3>, dim3(grid_for(npairs, 256)), dim3(256), 0, h->stream,
+                                   h->dm, h->d_spos, s, h->d_mask);
+        }
+        hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, h->d_acc_list,
+                           h->d_nacc);
+        rc = hip_check(hipMemcpyAsync(h->h_pinned, h->d_nacc, sizeof(int), hipMemcpyDeviceToHost, h->stream),
+                       "nacc copy");
+        if (rc)
+            return rc;
+        rc = hip_check(hipStreamSynchronize(h->stream), "resolve sync");
+        if (rc)
+            return rc;
+        int na = *(int*)h->h_pinned;
+        if (h->max_n > 0 && h->n + na > h->max_n) {
+            na = (int)(h->max_n - h->n);
+            rc = hip_check(hipMemcpyAsync(h->d_nacc, &na, sizeof(int), hipMemcpyHostToDevice, h->stream),
+                           "nacc clamp");
+            if (rc)
+                return rc;
+        }
+        rc = ensure_paths(h, h->n + na);
+        if (rc)
+            return rc;
+        const int tb = (h->T + 63) / 64;
+        if (h->d == 2)
+            hipLaunchKernelGGL(k_insert<2>, dim3(tb), dim3(64), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, list, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64, h->d_pos32,
+                               h->d_filled);
+        else
+            hipLaunchKernelGGL(k_insert<3>, dim3(tb), dim3(64), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, list, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64, h->d_pos32,
+                               h->d_filled);
+        rc = hip_check(hipGetLastError(), "k_insert");
+        if (rc)
+            return rc;
+        // host mirror of the accepted indices (needed by the stop rule)
+        if (na > 0) {
+            h->acc_host.resize(h->n + na);
+            rc = hip_check(hipMemcpyAsync(h->acc_host.data() + h->n, h->d_acc_idx + h->n, (size_t)na * 8,
+                                          hipMemcpyDeviceToHost, h->stream),
+                           "accepted copy");
+            if (rc)
+                return rc;
+            rc = hip_check(hipStreamSynchronize(h->stream), "accepted sync");
+            if (rc)
+                return rc;
+        }
+        h->n += na;
+        *acc_total += na;
+        if (h->max_n > 0 && h->n >= h->max_n)
+            return UG_OK;
+        if (S > s) {
+            // retest the rest against the updated table
+            int64_t rest = S - s;
+            rc = launch_cand64(h, 0, rest, list + s, h->d_flags, nullptr, false);
+            if (rc)
+                return rc;
+            int64_t S2 = 0;
+            rc = compact(h, h->d_flags, rest, 0, list + s, h->d_list2, &S2);
+            if (rc)
+                return rc;
+            rc = hip_check(hipMemcpyAsync(list, h->d_list2, (size_t)S2 * 8, hipMemcpyDeviceToDevice, h->stream),
+                           "list copy");
+            if (rc)
+                return rc;
+            S = S2;
+        } else {
+            S = 0;
+        }
+    }
+    return UG_OK;
+}
+
+}  // namespace ug

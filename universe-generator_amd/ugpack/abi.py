@@ -1,0 +1,206 @@
+"""ctypes binding of libugpack.so (the C ABI declared in include/ugpack.h).
+
+This is the binding a maintainer of the reference would add in place of spawning the
+engine binaries from braidlab/engine.py:50-112.  Loading fails loudly when the HIP
+library has not been built: there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from ._build import LIB_PATH
+
+UG_OK, UG_EINVAL, UG_ENOMEM, UG_EHIP, UG_EOVERFLOW, UG_ESTATE = 0, -1, -2, -3, -4, -5
+PREC_FP64, PREC_SPLIT_BF16 = 0, 1
+STOP_BUDGET, STOP_FIXED_WINDOW, STOP_REF_WINDOW = 0, 1, 2
+ABI_VERSION = 1
+
+
+class UgError(RuntimeError):
+    """A libugpack call returned a negative status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class PackConfig(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("T", C.c_int32), ("terms", C.c_int32), ("precision", C.c_int32),
+        ("seed", C.c_uint64), ("pin_sin1", C.c_int32), ("stop_kind", C.c_int32),
+        ("max_attempts", C.c_double), ("stop_rate", C.c_double), ("stop_batch", C.c_int64),
+        ("stop_window", C.c_int64), ("max_n", C.c_int64), ("margin", C.c_double),
+        ("max_paths", C.c_int64),
+    ]
+
+
+class PackStatus(C.Structure):
+    _fields_ = [
+        ("attempts", C.c_int64), ("n", C.c_int64), ("nsub", C.c_int64), ("done", C.c_int32),
+        ("reserved", C.c_int32), ("survivors", C.c_int64), ("rechecked", C.c_int64),
+        ("margin", C.c_double), ("seconds", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libugpack.so once; raise if it is missing (build with ugpack.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libugpack.so not built ({LIB_PATH}); run ugpack.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+    sigs = {
+        "ug_abi_version": ([], i32),
+        "ug_last_error": ([], C.c_char_p),
+        "ug_pack_create": ([C.POINTER(PackConfig), vp, C.POINTER(vp)], i32),
+        "ug_pack_run": ([vp, i64, C.POINTER(PackStatus)], i32),
+        "ug_pack_run_sub": ([vp, i32, i64, C.POINTER(PackStatus)], i32),
+        "ug_pack_get_status": ([vp, C.POINTER(PackStatus)], i32),
+        "ug_pack_accepted": ([vp, vp, vp, i64, C.POINTER(i64)], i32),
+        "ug_pack_params": ([vp, i64, i64, vp], i32),
+        "ug_pack_positions": ([vp, vp, i64], i32),
+        "ug_pack_curve": ([vp, vp, i64, C.POINTER(i64)], i32),
+        "ug_eval_candidates": ([vp, vp, i64, i32, vp], i32),
+        "ug_pack_destroy": ([vp], None),
+    }
+    for name, (args, res) in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    if L.ug_abi_version() != ABI_VERSION:
+        raise RuntimeError("libugpack ABI version mismatch")
+    _lib = L
+    return L
+
+
+def exported_symbols() -> list[str]:
+    return [
+        "ug_abi_version", "ug_last_error", "ug_pack_create", "ug_pack_run", "ug_pack_run_sub",
+        "ug_pack_get_status", "ug_pack_accepted", "ug_pack_params", "ug_pack_positions",
+        "ug_pack_curve", "ug_eval_candidates", "ug_pack_destroy",
+    ]
+
+
+def check(rc: int) -> None:
+    if rc != UG_OK:
+        raise UgError(rc, lib().ug_last_error().decode())
+
+
+@dataclass
+class Status:
+    attempts: int
+    n: int
+    nsub: int
+    done: bool
+    survivors: int
+    rechecked: int
+    margin: float
+    seconds: float
+
+    @staticmethod
+    def of(st: PackStatus) -> "Status":
+        return Status(st.attempts, st.n, st.nsub, bool(st.done), st.survivors, st.rechecked,
+                      st.margin, st.seconds)
+
+
+class Packer:
+    """One packing run on the current HIP device (a handle of the C ABI)."""
+
+    def __init__(self, dim: int, T: int, terms: int = 0, seed: int = 1,
+                 precision: int = PREC_FP64, pin_sin1: bool = False,
+                 stop_kind: int = STOP_BUDGET, max_attempts: float = 0.0,
+                 stop_rate: float = 0.0, stop_batch: int = 1 << 20, stop_window: int = 1 << 22,
+                 max_n: int = 0, margin: float = 0.0, max_paths: int = 0, stream: int = 0):
+        self._h = None
+        cfg = PackConfig(dim, T, terms, precision, seed, int(pin_sin1), stop_kind,
+                         float(max_attempts), float(stop_rate), stop_batch, stop_window, max_n,
+                         float(margin), max_paths)
+        h = C.c_void_p()
+        check(lib().ug_pack_create(C.byref(cfg), C.c_void_p(stream), C.byref(h)))
+        self._h = h
+        self.dim, self.T = dim, T
+        self.terms = terms if terms else T
+        self.nw = self.terms - 1
+
+    def close(self) -> None:
+        if self._h is not None:
+            lib().ug_pack_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, max_candidates: int = 0) -> Status:
+        st = PackStatus()
+        check(lib().ug_pack_run(self._h, max_candidates, C.byref(st)))
+        return Status.of(st)
+
+    def status(self) -> Status:
+        st = PackStatus()
+        check(lib().ug_pack_get_status(self._h, C.byref(st)))
+        return Status.of(st)
+
+    def accepted(self) -> tuple[np.ndarray, np.ndarray]:
+        n = C.c_int64()
+        check(lib().ug_pack_accepted(self._h, None, None, 0, C.byref(n)))
+        idx = np.zeros(n.value, np.int64)
+        gid = np.zeros(n.value, np.int32)
+        if n.value:
+            check(lib().ug_pack_accepted(self._h, idx.ctypes.data, gid.ctypes.data, n.value,
+                                         C.byref(n)))
+        return idx, gid
+
+    def params(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        """(count, dim, 1 + 3 nw): per axis a1 then (amplitude, frequency, phase) per slot."""
+        if count is None:
+            count = self.status().n + self.status().nsub - first
+        out = np.zeros((count, self.dim, 1 + 3 * self.nw))
+        check(lib().ug_pack_params(self._h, first, count, out.ctypes.data))
+        return out
+
+    def positions(self):
+        """Canonical positions of the accepted paths as a torch float64 tensor (n, T, dim)."""
+        import torch
+
+        st = self.status()
+        n = st.n + st.nsub
+        out = torch.empty((max(n, 1), self.T, self.dim), dtype=torch.float64, device="cuda")
+        check(lib().ug_pack_positions(self._h, out.data_ptr(), out.shape[0]))
+        return out[:n]
+
+    def curve(self) -> np.ndarray:
+        n = C.c_int64()
+        check(lib().ug_pack_curve(self._h, None, 0, C.byref(n)))
+        rows = np.zeros((n.value, 4), np.int64)
+        if n.value:
+            check(lib().ug_pack_curve(self._h, rows.ctypes.data, n.value, C.byref(n)))
+        return rows
+
+    def eval_candidates(self, idx, precision: int = PREC_FP64):
+        """Positions (count, T, dim) float64 of candidate indices (torch cuda int64 tensor)."""
+        import torch
+
+        idx = idx.to(device="cuda", dtype=torch.int64).contiguous()
+        out = torch.empty((idx.numel(), self.T, self.dim), dtype=torch.float64, device="cuda")
+        check(lib().ug_eval_candidates(self._h, idx.data_ptr(), idx.numel(), precision,
+                                       out.data_ptr()))
+        return out
