@@ -1,0 +1,171 @@
+"""Benchmark: candidates tested/s and seconds to jam, 2+1 F=64 (one MI355X).
+
+One step = one complete pack of a 2+1, maxFreq F = T = 64 universe to jamming: Philox
+candidates in batches of 2^20, stopping at the first 2^20 boundary where the accepts
+among the last 2^22 candidates are fewer than 1e-4 * 2^22.  Step k packs seed k+1
+(warmup packs use seeds 1001...).  value = candidates consumed by the timed packs /
+their wall time; ms_per_step = mean time to jam.
+
+    python bench.py [--gpus 1] [--steps 8] [--warmup 2] [--precision split|fp64|both]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "universe-generator_amd"))
+
+HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def pack_once(ugpack, seed: int, precision: int, profile: bool = False):
+    p = ugpack.Packer(2, 64, seed=seed, precision=precision,
+                      stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4,
+                      stop_batch=1 << 20, stop_window=1 << 22)
+    if profile:
+        p.profile(True)
+    st = p.run()
+    return p, st
+
+
+def roofline_of(stats: dict, dim: int, T: int, precision_split: bool) -> dict:
+    """Algorithmic bytes of the candidate-test kernel per launch over its average duration.
+
+    Per visited (candidate, timestep): 3^(d-1) rows of 3 int32 cell heads (12 B each);
+    per grid point compared: its coordinates and chain link (d*8 + 4 B in the FP64 form,
+    16 B FP32 record in the split-BF16 form)."""
+    launches = max(stats["launches"], 1)
+    rows = 3 ** (dim - 1)
+    per_point = 16 if precision_split else 8 * dim + 4
+    bytes_total = stats["visits"] * rows * 12 + stats["points"] * per_point
+    avg_ms = stats["ms"] / launches
+    achieved = (bytes_total / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return {
+        "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+        "kernel": "k_fast_prefilter" if precision_split else "k_cand64<test>",
+        "avg_launch_ms": round(avg_ms, 5), "launches": stats["launches"],
+        "algorithmic_bytes_per_launch": round(bytes_total / launches, 1),
+        "visits_per_candidate": round(stats["visits"] / max(stats["items"], 1), 4),
+    }
+
+
+def cpu_baseline(seconds_target: float = 12.0) -> dict:
+    """The CPU oracle (serial RSA restatement, 1 thread) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    m = pyoracle.Model(2, 64, seed=1)
+    n = 1 << 20
+    t0 = time.perf_counter()
+    m.pack(n)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt, 1), "unit": "candidates/s", "cores": 1, "kind": "port",
+            "sample": "first 2^20 candidates of the seed-1 2+1 F=64 pack (serial RSA, oracle/)",
+            "seconds": round(dt, 2)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp64", choices=["split", "fp64", "both"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the packer runs one universe per "
+              "GPU; see DESIGN.md for how it shards)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import ugpack
+
+    torch.cuda.set_device(0)
+    prec_main = ugpack.PREC_SPLIT_BF16 if args.precision in ("split", "both") else ugpack.PREC_FP64
+    forms = [prec_main] + ([ugpack.PREC_FP64] if args.precision == "both" else [])
+
+    golden = {}
+    for s in range(1, 9):
+        f = os.path.join(ROOT, "tests", "golden", f"cfg2_d2_T64_s{s}_jam.json")
+        if os.path.exists(f):
+            golden[s] = json.load(open(f))["idx"]
+
+    results = {}
+    for prec in forms:
+        for w in range(args.warmup):
+            p, _ = pack_once(ugpack, 1001 + w, prec)
+            p.close()
+        torch.cuda.synchronize()
+        cands, accepted, per_pack, parity = 0, [], [], []
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            p, st = pack_once(ugpack, k + 1, prec)
+            torch.cuda.synchronize()
+            per_pack.append(time.perf_counter() - t0)
+            cands += st.attempts
+            accepted.append(st.n)
+            if (k + 1) in golden:
+                idx, _ = p.accepted()
+                parity.append(list(idx) == golden[k + 1])
+            p.close()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t_start
+        # separate profiled pack (events + exact counters) for the roofline figures
+        p, st = pack_once(ugpack, 1, prec, profile=True)
+        stats = p.kernel_stats(ugpack.abi.KERNEL_PREFILTER)
+        rstats = p.kernel_stats(ugpack.abi.KERNEL_RECHECK)
+        p.close()
+        results[prec] = dict(value=cands / wall, wall=wall, per_pack=per_pack, accepted=accepted,
+                             parity=parity, stats=stats, rstats=rstats, cands=cands)
+
+    main_r = results[prec_main]
+    split = prec_main == ugpack.PREC_SPLIT_BF16
+    line = {
+        "metric": "candidates tested/s & s to jam, 2+1 F=64; split-BF16 vs FP64 GPU, same accepts",
+        "value": round(main_r["value"], 1),
+        "unit": "candidates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * main_r["wall"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3+f32acc, f64 recheck" if split else "f64",
+        "data": "synthetic (Philox candidate stream, seeds 1..steps)",
+        "config": {
+            "workload": "2+1 pack to jamming, F=T=64, batches of 2^20, stop when accepts in the "
+                        "last 2^22 candidates < 1e-4*2^22",
+            "dim": 2, "T": 64, "seeds": list(range(1, args.steps + 1)),
+            "precision": "split-bf16" if split else "fp64",
+        },
+        "s_to_jam_mean": round(main_r["wall"] / args.steps, 5),
+        "candidates_per_pack_mean": main_r["cands"] / args.steps,
+        "accepted_per_pack": main_r["accepted"],
+        "parity_vs_oracle_fixtures": main_r["parity"],
+        "roofline": roofline_of(main_r["stats"], 2, 64, split),
+    }
+    if ugpack.PREC_FP64 in results and split:
+        f = results[ugpack.PREC_FP64]
+        line["fp64_form"] = {"value": round(f["value"], 1), "s_to_jam_mean": round(f["wall"] / args.steps, 5),
+                             "accepted_per_pack": f["accepted"], "parity_vs_oracle_fixtures": f["parity"],
+                             "same_accepts_as_split": f["accepted"] == main_r["accepted"]}
+        line["speedup_split_vs_fp64"] = round(main_r["value"] / f["value"], 3)
+    if split:
+        line["recheck"] = main_r["rstats"]
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -131,6 +131,26 @@ int32_t ug_pack_curve(const ug_pack* h, int64_t* rows4, int64_t cap, int64_t* n)
 int32_t ug_eval_candidates(const ug_pack* h, const int64_t* dev_idx, int64_t count,
                            int32_t precision, double* dev_out);
 
+/* Per-kernel measurement of the candidate-test kernels.  When enabled, every launch is
+ * bracketed by HIP events on the handle's stream and the kernels count the timesteps they
+ * visit and the grid points they compare (exactly), so algorithmic bytes per launch can be
+ * derived.  No counterpart in the reference (which prints only attempts/s, src/main.rs:150-159). */
+typedef struct ug_kernel_stats {
+    int64_t launches; /* kernel launches measured */
+    int64_t items;    /* candidates tested */
+    double ms;        /* summed event time */
+    int64_t visits;   /* (candidate, timestep) pairs visited */
+    int64_t points;   /* grid points compared */
+} ug_kernel_stats;
+
+enum {
+    UG_KERNEL_PREFILTER = 0, /* first test of fresh candidates (FP64 or split-BF16 form) */
+    UG_KERNEL_RECHECK = 1    /* FP64 recheck of the split-BF16 form's uncertain candidates */
+};
+
+int32_t ug_pack_profile(ug_pack* h, int32_t enable);
+int32_t ug_pack_kernel_stats(const ug_pack* h, int32_t which, ug_kernel_stats* out);
+
 /* Releases the handle and its device memory. */
 void ug_pack_destroy(ug_pack* h);
 

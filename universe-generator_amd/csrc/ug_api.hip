@@ -120,6 +120,12 @@ void release(Pack* h) {
     free_dev(h->d_mask);
     free_dev(h->d_acc_list);
     free_dev(h->d_nacc);
+    free_dev(h->d_ctr);
+    if (h->ev0)
+        (void)hipEventDestroy(h->ev0);
+    if (h->ev1)
+        (void)hipEventDestroy(h->ev1);
+    h->ev0 = h->ev1 = nullptr;
     if (h->h_pinned)
         (void)hipHostFree(h->h_pinned);
     h->h_pinned = nullptr;
@@ -359,10 +365,15 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
             sub = std::min(sub, nb - h->attempts);
         }
         const uint64_t base = (uint64_t)h->attempts;
+        rc = timed_begin(h);
+        if (rc)
+            break;
         if (cfg.precision == UG_PREC_SPLIT_BF16)
             rc = launch_fast(h, base, sub, h->d_flags);
         else
-            rc = launch_cand64(h, base, sub, nullptr, h->d_flags, nullptr, false);
+            rc = launch_cand64(h, base, sub, nullptr, h->d_flags, nullptr, false, h->profile ? h->d_ctr : nullptr);
+        if (!rc)
+            rc = timed_end(h, UG_KERNEL_PREFILTER, sub);
         if (rc)
             break;
         int64_t S = 0;
@@ -372,7 +383,11 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         if (cfg.precision == UG_PREC_SPLIT_BF16 && S > 0) {
             // FP64 recheck of every candidate the split-BF16 prefilter could not reject
             h->rechecked += S;
-            rc = launch_cand64(h, 0, S, h->d_list, h->d_flags, nullptr, false);
+            rc = timed_begin(h);
+            if (!rc)
+                rc = launch_cand64(h, 0, S, h->d_list, h->d_flags, nullptr, false, h->profile ? h->d_ctr : nullptr);
+            if (!rc)
+                rc = timed_end(h, UG_KERNEL_RECHECK, S);
             if (!rc)
                 rc = compact(h, h->d_flags, S, 0, h->d_list, h->d_list2, &S);
             if (!rc && S > 0)
@@ -510,6 +525,29 @@ extern "C" int32_t ug_eval_candidates(const ug_pack* hp, const int64_t* dev_idx,
     if (!rc)
         rc = hip_check(hipStreamSynchronize(h->stream), "eval sync");
     return rc;
+}
+
+extern "C" int32_t ug_pack_profile(ug_pack* hp, int32_t enable) {
+    if (!hp)
+        return fail(UG_EINVAL, "ug_pack_profile: NULL handle");
+    Pack* h = (Pack*)hp;
+    if (enable && !h->ev0) {
+        if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+            return fail(UG_EHIP, "ug_pack_profile: event creation failed");
+        if (hipMalloc((void**)&h->d_ctr, 16) != hipSuccess)
+            return fail(UG_ENOMEM, "ug_pack_profile: counter allocation failed");
+        if (hipMemset(h->d_ctr, 0, 16) != hipSuccess)
+            return fail(UG_EHIP, "ug_pack_profile: counter clear failed");
+    }
+    h->profile = enable != 0;
+    return UG_OK;
+}
+
+extern "C" int32_t ug_pack_kernel_stats(const ug_pack* hp, int32_t which, ug_kernel_stats* out) {
+    if (!hp || !out || which < 0 || which > 1)
+        return fail(UG_EINVAL, "ug_pack_kernel_stats: invalid argument");
+    *out = ((const Pack*)hp)->kstats[which];
+    return UG_OK;
 }
 
 extern "C" int32_t ug_pack_run_sub(ug_pack* hp, int32_t sub_terms, int64_t count, ug_pack_status* st) {

@@ -1,8 +1,519 @@
-// ug_fast.hip -- split-BF16 prefilter (placeholder until the MFMA path lands).
+// ug_fast.hip -- split-BF16 prefilter of the packer (UG_PREC_SPLIT_BF16).
+//
+// Evaluation is the dense contraction X[c][t] = a1[c] + (1/W_c) sum_r A[c][r] B[r][t]:
+// A holds, per frequency pair (k even, k+1 odd), the unnormalized coefficients
+//   (a~_k cos f_k, a~_k sin f_k, a~_{k+1}, 0),   a~ = +-e/k  (e the exponential spacing),
+// and B the canonical basis rows (S_k, C_k, S_{k+1}, 0) at the timesteps.  The reference
+// evaluates the same sum term by term with FP64 sin() (cuda/braid_cuda.cu:418-441) and
+// its --sparse mode from FP32 tables (cuda/braid_cuda3d.cu:630-717).
+//
+// One wave owns a tile of 32 candidates: lanes generate their A fragments directly in
+// registers (Philox + FP32 log / sincospi), split each value into bf16 hi + lo, and run
+// the 3-term product hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_bf16 with FP32
+// accumulation against the basis of the first 32 endpoint-first timesteps, staged in LDS.
+// The lookup then rejects a candidate only on a CERTAIN contact (every axis within
+// r - m_t, m_t a bound on the FP32/BF16 position error, like the one-sided margin of
+// cuda/braid_cuda3d.cu:938-948); everything else -- gray contacts, candidates without a
+// certain contact among those timesteps -- is flagged for the exact FP64 test.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "ug_arith.h"
 #include "ug_internal.h"
-namespace ug {
-int32_t fast_setup(Pack* h) { (void)h; return fail(UG_EINVAL, "split-BF16 form not built"); }
-int32_t launch_fast(Pack* h, uint64_t, int64_t, uint8_t*) { (void)h; return fail(UG_EINVAL, "split-BF16 form not built"); }
-int32_t launch_fast_eval(Pack* h, const int64_t*, int64_t, double*) { (void)h; return fail(UG_EINVAL, "split-BF16 form not built"); }
-void fast_release(Pack* h) { (void)h; }
+
+using namespace ug;
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+namespace {
+
+constexpr int TILE = 32;        // candidates per wave tile (MFMA M)
+constexpr int NCOL = 32;        // timesteps per N block (MFMA N)
+constexpr int FAST_WAVES = 4;   // waves per workgroup
+
+// FP32 sin(pi r), cos(pi r) for |r| <= 1/4 (Taylor in r^2).
+__device__ __forceinline__ void sincospi_f32_small(float r, float& s, float& c) {
+    const float x = r * r;
+    float ps = 0.0821458866f;  // pi^9/9!
+    ps = fmaf(ps, x, -0.599264529f);
+    ps = fmaf(ps, x, 2.55016404f);
+    ps = fmaf(ps, x, -5.16771278f);
+    ps = fmaf(ps, x, 3.14159265f);
+    s = r * ps;
+    float pc = 0.235330630f;   // pi^8/8!
+    pc = fmaf(pc, x, -1.33526277f);
+    pc = fmaf(pc, x, 4.05871213f);
+    pc = fmaf(pc, x, -4.93480220f);
+    pc = fmaf(pc, x, 1.0f);
+    c = pc;
 }
+
+// cos(pi u), sin(pi u) for u = w 2^-32 in [0, 1).
+__device__ __forceinline__ void phase_cs(uint32_t w, float& cf, float& sf) {
+    // q = nearest integer to 2u, r = u - q/2 computed exactly in integers
+    const uint32_t q = (uint32_t)(((uint64_t)w + 0x40000000ull) >> 31);  // 0, 1 or 2
+    const int64_t num = (int64_t)w - (int64_t)q * 0x80000000ll;          // |num| <= 2^30
+    const float r = (float)num * 0x1p-32f;
+    float s, c;
+    sincospi_f32_small(r, s, c);
+    if (q == 0) {
+        cf = c;
+        sf = s;
+    } else if (q == 1) {
+        cf = -s;
+        sf = c;
+    } else {
+        cf = -c;
+        sf = -s;
+    }
+}
+
+__device__ __forceinline__ float expo_f32(uint32_t w) {
+    const float v = (float)(0x80000000u - (w >> 1)) * 0x1p-31f;
+    return -__log2f(v) * 0.693147182f;
+}
+
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+    hi = (__bf16)x;
+    lo = (__bf16)(x - (float)hi);
+}
+
+// A fragments of one K step (pairs p0 = 4q + 2h, p0 + 1) for axis a, candidate idx.
+__device__ __forceinline__ void gen_kstep(const DevModel& m, uint64_t idx, bool valid, int a,
+                                          int q, int h, bf16x8& Ahi, bf16x8& Alo, float& Wpart,
+                                          const float* invk) {
+    const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
+#pragma unroll
+    for (int pp = 0; pp < 2; pp++) {
+        const int p = 4 * q + 2 * h + pp;
+        float v0 = 0.f, v1 = 0.f, v2 = 0.f;
+        if (valid && p < m.npairs) {
+            const u32x4 r = philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, m.key0, m.key1);
+            const int j0 = 2 * p;
+            const float e0 = expo_f32(r.x);
+            float ae = e0 * invk[j0 + 2];
+            if (r.x & 1u)
+                ae = -ae;
+            float cf, sf;
+            phase_cs(r.z, cf, sf);
+            v0 = ae * cf;
+            v1 = ae * sf;
+            Wpart += e0;
+            if (j0 + 1 < m.nw) {
+                const float e1 = expo_f32(r.y);
+                float ao = e1 * invk[j0 + 3];
+                if (r.y & 1u)
+                    ao = -ao;
+                v2 = ao;
+                Wpart += e1;
+            }
+        }
+        __bf16 h0, l0, h1, l1, h2, l2;
+        split_bf16(v0, h0, l0);
+        split_bf16(v1, h1, l1);
+        split_bf16(v2, h2, l2);
+        Ahi[4 * pp + 0] = h0;
+        Ahi[4 * pp + 1] = h1;
+        Ahi[4 * pp + 2] = h2;
+        Ahi[4 * pp + 3] = (__bf16)0.f;
+        Alo[4 * pp + 0] = l0;
+        Alo[4 * pp + 1] = l1;
+        Alo[4 * pp + 2] = l2;
+        Alo[4 * pp + 3] = (__bf16)0.f;
+    }
+}
+
+struct FastArgs {
+    DevModel m;
+    int ks;                 // K steps (4 pairs each)
+    int ncols;              // ordered timesteps examined (min(T, 32))
+    const bf16x8* bfrag;    // [nb][ks][2][64] basis fragments, columns in endpoint-first order
+    const float* invk;      // [T+2] 1/k as float
+    const int* colt;        // [nb*32] timestep of each column (-1 = padding)
+};
+
+// Positions of the wave's 32 candidates at the 32 columns of N block nb into
+// pos[D][32][32] (LDS), FP32, wrapped.  Returns per-candidate W-check flags in lane bits.
+template <int D>
+__device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, bool valid,
+                                               int lane, int nb, const bf16x8* bsrc,
+                                               float* pos, float* sa1, float* sinvw) {
+    const DevModel& m = f.m;
+    const int mrow = lane & 31, h = lane >> 5;
+    const u32x4 w0 = philox((uint32_t)idx, (uint32_t)(idx >> 32), 0u, 0u, m.key0, m.key1);
+    const uint32_t wa[3] = {w0.x, w0.y, w0.z};
+    bool wbad = false;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        f32x16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; v++)
+            acc[v] = 0.f;
+        float Wpart = 0.f;
+        for (int q = 0; q < f.ks; q++) {
+            bf16x8 Ahi, Alo;
+            gen_kstep(m, idx, valid, a, q, h, Ahi, Alo, Wpart, f.invk);
+            const bf16x8 Bhi = bsrc[((size_t)(nb * f.ks + q) * 2 + 0) * 64 + lane];
+            const bf16x8 Blo = bsrc[((size_t)(nb * f.ks + q) * 2 + 1) * 64 + lane];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Bhi, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Blo, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo, Bhi, acc, 0, 0, 0);
+        }
+        const float W = Wpart + __shfl_xor(Wpart, 32, 64);
+        wbad = wbad || (valid && W < 0.25f * (float)m.nw);
+        if (h == 0) {
+            sa1[mrow] = m.pin ? 0.f : (float)a1_of_word(wa[a]);
+            sinvw[mrow] = 1.0f / W;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // D layout of 32x32 MFMA: lane holds column n = lane & 31, rows 8(v/4) + 4h + (v%4)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int row = 8 * (v >> 2) + 4 * h + (v & 3);
+            float X = fmaf(acc[v], sinvw[row], sa1[row]);
+            X = X - 2.0f * floorf((X + 1.0f) * 0.5f);
+            pos[(a * TILE + row) * NCOL + mrow] = X;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    return wbad;
+}
+
+// Prefilter: flags[i] = 0 when candidate i certainly collides, 1 when the FP64 test must decide.
+template <int D>
+__global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
+                                                        uint8_t* flags, unsigned long long* ctr) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const DevModel& m = f.m;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // basis fragments of N block 0, shared by the workgroup
+    bf16x8* bl = (bf16x8*)lds;
+    const int nfrag = f.ks * 2 * 64;
+    for (int i = threadIdx.x; i < nfrag; i += blockDim.x)
+        bl[i] = f.bfrag[i];
+    float* pos = (float*)(lds + (size_t)nfrag * 16) + (size_t)wv * (D * TILE * NCOL + 2 * TILE);
+    float* sa1 = pos + D * TILE * NCOL;
+    float* sinvw = sa1 + TILE;
+    __syncthreads();
+    const int mrow = lane & 31, h = lane >> 5;
+    const int64_t ntiles = (count + TILE - 1) / TILE;
+    unsigned long long nvis = 0, npts = 0;
+    const float r = (float)m.r;
+    for (int64_t tile = (int64_t)blockIdx.x * FAST_WAVES + wv; tile < ntiles;
+         tile += (int64_t)gridDim.x * FAST_WAVES) {
+        const int64_t i = tile * TILE + mrow;
+        const bool valid = i < count;
+        const uint64_t idx = base + (uint64_t)(valid ? i : 0);
+        const bool wbad = tile_positions<D>(f, idx, valid, lane, 0, bl, pos, sa1, sinvw);
+        // lookup: lanes m and m+32 take alternate columns of candidate m.  A candidate whose
+        // exponential spacings sum below nw/4 (wbad) is outside the error bound's premise and
+        // is never rejected here.
+        bool rej = wbad;
+        for (int k = 0; 2 * k < f.ncols; k++) {
+            const int col = 2 * k + h;
+            const bool active = valid && !rej && col < f.ncols;
+            bool cert = false;
+            if (active) {
+                const int t = f.colt[col];
+                const float mt = m.marg[col];
+                float X[D];
+                int c[D];
+#pragma unroll
+                for (int a = 0; a < D; a++) {
+                    X[a] = pos[(a * TILE + mrow) * NCOL + col];
+                    int cc = (int)floorf((X[a] + 1.0f) * (float)m.halfT);
+                    cc = cc < 0 ? cc + m.T : (cc >= m.T ? cc - m.T : cc);
+                    c[a] = cc;
+                }
+                nvis++;
+                constexpr int ROWS = D == 2 ? 3 : 9;
+                for (int rr = 0; rr < ROWS && !cert; rr++) {
+                    int64_t row;
+                    if (D == 2) {
+                        row = ((int64_t)t * m.T + (c[0] + rr - 1 + m.T) % m.T) * m.rowlen;
+                    } else {
+                        const int gx = (c[0] + rr / 3 - 1 + m.T) % m.T;
+                        const int gy = (c[1] + rr % 3 - 1 + m.T) % m.T;
+                        row = (((int64_t)t * m.T + gx) * m.T + gy) * m.rowlen;
+                    }
+                    const int* tr = m.tab + row + c[D - 1];
+                    const int hd[3] = {tr[0], tr[1], tr[2]};
+                    for (int kk = 0; kk < 3 && !cert; kk++) {
+                        for (int n = hd[kk]; n >= 0 && !cert;) {
+                            const float4 P = m.pos32[(size_t)n * m.T + t];
+                            const float p3[3] = {P.x, P.y, P.z};
+                            float dmax = 0.f;
+#pragma unroll
+                            for (int a = 0; a < D; a++) {
+                                float d = X[a] - p3[a];
+                                d = d > 1.f ? d - 2.f : (d < -1.f ? d + 2.f : d);
+                                dmax = fmaxf(dmax, fabsf(d));
+                            }
+                            npts++;
+                            cert = dmax <= r - mt;
+                            n = __float_as_int(P.w);
+                        }
+                    }
+                }
+            }
+            rej = rej || cert;
+            rej = rej || __shfl_xor((int)rej, 32, 64);
+            if (__ballot(valid && !rej && (2 * k + 2 < f.ncols)) == 0)
+                break;
+        }
+        // every candidate without a certain contact goes to the FP64 test
+        if (valid && h == 0)
+            flags[i] = (rej && !wbad) ? 0 : 1;
+    }
+    if (ctr) {
+        unsigned long long a = nvis, b = npts;
+        for (int s = 32; s >= 1; s >>= 1) {
+            a += __shfl_xor(a, s, 64);
+            b += __shfl_xor(b, s, 64);
+        }
+        if (lane == 0 && a) {
+            atomicAdd(&ctr[0], a);
+            atomicAdd(&ctr[1], b);
+        }
+    }
+}
+
+// Split-BF16 positions of listed candidates at every timestep (error reporting, tests).
+template <int D>
+__global__ __launch_bounds__(256) void k_fast_eval(FastArgs f, int nb_count, const int64_t* list,
+                                                   int64_t count, double* out) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float* pos = (float*)lds + (size_t)wv * (D * TILE * NCOL + 2 * TILE);
+    float* sa1 = pos + D * TILE * NCOL;
+    float* sinvw = sa1 + TILE;
+    const int mrow = lane & 31, h = lane >> 5;
+    const int64_t ntiles = (count + TILE - 1) / TILE;
+    for (int64_t tile = (int64_t)blockIdx.x * FAST_WAVES + wv; tile < ntiles;
+         tile += (int64_t)gridDim.x * FAST_WAVES) {
+        const int64_t i = tile * TILE + mrow;
+        const bool valid = i < count;
+        const uint64_t idx = valid ? (uint64_t)list[i] : 0ull;
+        for (int nb = 0; nb < nb_count; nb++) {
+            (void)tile_positions<D>(f, idx, valid, lane, nb, f.bfrag, pos, sa1, sinvw);
+            for (int col = h; col < NCOL; col += 2) {
+                const int t = f.colt[nb * NCOL + col];
+                if (valid && t >= 0)
+                    for (int a = 0; a < D; a++)
+                        out[((size_t)i * f.m.T + t) * D + a] = (double)pos[(a * TILE + mrow) * NCOL + col];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ host side
+uint16_t bf16_bits_rne(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float bf16_to_float(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float x;
+    memcpy(&x, &u, 4);
+    return x;
+}
+
+struct FastState {
+    bf16x8* d_bfrag = nullptr;
+    float* d_invk = nullptr;
+    int* d_colt = nullptr;
+    int ks = 0, nb = 0, ncols = 0;
+    size_t lds_prefilter = 0;
+};
+
+}  // namespace
+
+namespace ug {
+
+int32_t fast_setup(Pack* h) {
+    if (!h->full)
+        return UG_OK;  // partial spectra run the FP64 candidate test (see launch_fast)
+    const int T = h->T, W = T + 1, npairs = h->npairs;
+    FastState* fs = new FastState();
+    fs->ks = (npairs + 3) / 4;
+    fs->nb = (T + NCOL - 1) / NCOL;
+    fs->ncols = std::min(T, NCOL);
+    // host copies of the canonical tables (same arithmetic as ug_pack_create)
+    std::vector<double> S((size_t)T * W), C((size_t)T * W);
+    int32_t rc = hip_check(hipMemcpy(S.data(), h->d_S, S.size() * 8, hipMemcpyDeviceToHost), "fast S");
+    if (!rc)
+        rc = hip_check(hipMemcpy(C.data(), h->d_C, C.size() * 8, hipMemcpyDeviceToHost), "fast C");
+    std::vector<int> order(T);
+    if (!rc)
+        rc = hip_check(hipMemcpy(order.data(), h->d_order, (size_t)T * 4, hipMemcpyDeviceToHost), "fast order");
+    if (rc) {
+        delete fs;
+        return rc;
+    }
+    std::vector<int> colt((size_t)fs->nb * NCOL, -1);
+    for (int oi = 0; oi < T; oi++)
+        colt[oi] = order[oi];
+    // basis fragments: row 16q + 8hh + i of K step q is basis row r = 4 p + s, p = r / 4
+    std::vector<uint16_t> frag((size_t)fs->nb * fs->ks * 2 * 64 * 8, 0);
+    auto basis = [&](int r, int t) -> double {
+        const int p = r / 4, s = r % 4;
+        const int ke = 2 * p + 2, ko = 2 * p + 3;
+        if (t < 0 || p >= npairs)
+            return 0.0;
+        if (s == 0)
+            return S[(size_t)t * W + ke];
+        if (s == 1)
+            return C[(size_t)t * W + ke];
+        if (s == 2)
+            return ko <= T ? S[(size_t)t * W + ko] : 0.0;
+        return 0.0;
+    };
+    for (int nb = 0; nb < fs->nb; nb++)
+        for (int q = 0; q < fs->ks; q++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int i = 0; i < 8; i++) {
+                    const int r = 16 * q + 8 * (lane >> 5) + i;
+                    const int t = colt[(size_t)nb * NCOL + (lane & 31)];
+                    const double b = basis(r, t);
+                    const uint16_t hb = bf16_bits_rne((float)b);
+                    const double rem = b - (double)bf16_to_float(hb);
+                    const uint16_t lb = bf16_bits_rne((float)rem);
+                    const size_t base = (((size_t)(nb * fs->ks + q) * 2) * 64 + lane) * 8 + i;
+                    frag[base] = hb;
+                    frag[base + (size_t)64 * 8] = lb;
+                }
+    std::vector<float> invk(T + 2, 0.f);
+    for (int k = 1; k <= T + 1; k++)
+        invk[k] = (float)(1.0 / (double)k);
+    // certainty margin per examined column: B_t bounds sum |coef * basis| for unit budget
+    const double n_add = 48.0 * fs->ks;
+    const double eps = 3.1 * 0x1p-16 + n_add * 0x1p-24 + 0x1p-20;
+    h->marg_host.assign(fs->ncols, 0.f);
+    double mmax = 0.0;
+    for (int col = 0; col < fs->ncols; col++) {
+        const int t = colt[col];
+        double Bt = 0.0;
+        for (int k = 2; k <= T; k++) {
+            double v = std::fabs(S[(size_t)t * W + k]);
+            if ((k & 1) == 0)
+                v += std::fabs(C[(size_t)t * W + k]);
+            Bt = std::max(Bt, v / k);
+        }
+        double mt = Bt * eps + 2e-6;
+        mt = std::max(mt, h->cfg.margin);
+        h->marg_host[col] = (float)mt;
+        mmax = std::max(mmax, mt);
+    }
+    h->margin = mmax;
+    if (hipMalloc((void**)&fs->d_bfrag, frag.size() * 2) != hipSuccess ||
+        hipMalloc((void**)&fs->d_invk, invk.size() * 4) != hipSuccess ||
+        hipMalloc((void**)&fs->d_colt, colt.size() * 4) != hipSuccess ||
+        hipMalloc((void**)&h->d_marg, h->marg_host.size() * 4) != hipSuccess) {
+        h->d_basis16 = fs;
+        return fail(UG_ENOMEM, "split-BF16 table allocation failed");
+    }
+    rc = hip_check(hipMemcpy(fs->d_bfrag, frag.data(), frag.size() * 2, hipMemcpyHostToDevice), "frag upload");
+    if (!rc)
+        rc = hip_check(hipMemcpy(fs->d_invk, invk.data(), invk.size() * 4, hipMemcpyHostToDevice), "invk upload");
+    if (!rc)
+        rc = hip_check(hipMemcpy(fs->d_colt, colt.data(), colt.size() * 4, hipMemcpyHostToDevice), "colt upload");
+    if (!rc)
+        rc = hip_check(hipMemcpy(h->d_marg, h->marg_host.data(), h->marg_host.size() * 4, hipMemcpyHostToDevice),
+                       "margin upload");
+    h->dm.marg = h->d_marg;
+    fs->lds_prefilter = (size_t)fs->ks * 2 * 64 * 16 + (size_t)FAST_WAVES * (h->d * TILE * NCOL + 2 * TILE) * 4;
+    h->d_basis16 = fs;
+    if (!rc && fs->lds_prefilter > 160 * 1024)
+        rc = fail(UG_EINVAL, "split-BF16 tile needs %zu bytes of LDS", fs->lds_prefilter);
+    if (!rc) {
+        if (h->d == 2)
+            rc = hip_check(hipFuncSetAttribute((const void*)k_fast_prefilter<2>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)fs->lds_prefilter),
+                           "lds attribute");
+        else
+            rc = hip_check(hipFuncSetAttribute((const void*)k_fast_prefilter<3>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)fs->lds_prefilter),
+                           "lds attribute");
+    }
+    return rc;
+}
+
+static FastArgs fast_args(Pack* h) {
+    FastState* fs = (FastState*)h->d_basis16;
+    FastArgs f;
+    f.m = h->dm;
+    f.ks = fs->ks;
+    f.ncols = fs->ncols;
+    f.bfrag = fs->d_bfrag;
+    f.invk = fs->d_invk;
+    f.colt = fs->d_colt;
+    return f;
+}
+
+int32_t launch_fast(Pack* h, uint64_t base, int64_t count, uint8_t* flags) {
+    if (!h->full || !h->d_basis16)
+        return launch_cand64(h, base, count, nullptr, flags, nullptr, false, h->profile ? h->d_ctr : nullptr);
+    FastState* fs = (FastState*)h->d_basis16;
+    const int64_t ntiles = (count + TILE - 1) / TILE;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, 1024));
+    const FastArgs f = fast_args(h);
+    unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
+    if (h->d == 2)
+        hipLaunchKernelGGL(k_fast_prefilter<2>, dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base, count,
+                           flags, ctr);
+    else
+        hipLaunchKernelGGL(k_fast_prefilter<3>, dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base, count,
+                           flags, ctr);
+    return hip_check(hipGetLastError(), "k_fast_prefilter");
+}
+
+int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* out) {
+    if (!h->full)
+        return fail(UG_EINVAL, "split-BF16 evaluation covers full-spectrum models only (terms = T)");
+    if (!h->d_basis16) {
+        int32_t rc = fast_setup(h);
+        if (rc)
+            return rc;
+    }
+    if (count <= 0)
+        return UG_OK;
+    FastState* fs = (FastState*)h->d_basis16;
+    const int64_t ntiles = (count + TILE - 1) / TILE;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, 1024));
+    const size_t lds = (size_t)FAST_WAVES * (h->d * TILE * NCOL + 2 * TILE) * 4;
+    const FastArgs f = fast_args(h);
+    if (h->d == 2)
+        hipLaunchKernelGGL(k_fast_eval<2>, dim3(blocks), dim3(256), lds, h->stream, f, fs->nb, list, count, out);
+    else
+        hipLaunchKernelGGL(k_fast_eval<3>, dim3(blocks), dim3(256), lds, h->stream, f, fs->nb, list, count, out);
+    return hip_check(hipGetLastError(), "k_fast_eval");
+}
+
+void fast_release(Pack* h) {
+    FastState* fs = (FastState*)h->d_basis16;
+    if (!fs)
+        return;
+    if (fs->d_bfrag)
+        (void)hipFree(fs->d_bfrag);
+    if (fs->d_invk)
+        (void)hipFree(fs->d_invk);
+    if (fs->d_colt)
+        (void)hipFree(fs->d_colt);
+    delete fs;
+    h->d_basis16 = nullptr;
+}
+
+}  // namespace ug

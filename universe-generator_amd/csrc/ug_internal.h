@@ -79,6 +79,11 @@ struct Pack {
     bool done = false;
     int64_t survivors = 0, rechecked = 0;
     double seconds = 0.0;
+    // measurement (ug_pack_profile)
+    bool profile = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long* d_ctr = nullptr;  // [0] visits, [1] points
+    ug_kernel_stats kstats[2] = {};
     std::deque<std::pair<int64_t, int64_t>> win;
     int64_t win_att = 0, win_adm = 0;
 };
@@ -87,7 +92,9 @@ int32_t fail(int32_t code, const char* fmt, ...);
 int32_t hip_check(hipError_t e, const char* what);
 int grid_for(int64_t items, int per_block);
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
-                      double* out, bool full);
+                      double* out, bool full, unsigned long long* ctr = nullptr);
+int32_t timed_begin(Pack* h);
+int32_t timed_end(Pack* h, int which, int64_t items);
 int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, const int64_t* list,
                 int64_t* out, int64_t* S);
 int32_t ensure_paths(Pack* h, int64_t need);

@@ -60,12 +60,12 @@ __device__ __forceinline__ double wave_sum_canonical(double v) {
 
 // =====================================================================================
 // per-wave candidate coefficients (FP64, canonical)
-template <int D>
+template <int D, int PL>
 struct WaveCoef {
     double a1[D];
-    double c[D][MAXPL][2];
-    double s[D][MAXPL][2];
-    int b[D][MAXPL][2];  // frequency of the slot, 0 = no slot
+    double c[D][PL][2];
+    double s[D][PL][2];
+    int b[D][PL][2];  // frequency of the slot, 0 = no slot
 };
 
 // Frequencies of a partial spectrum (nw < modmax): lane 0 replays the unique-frequency
@@ -108,8 +108,8 @@ __host__ __device__ inline int partial_lds_words(const DevModel& m) {
     return m.full ? 0 : D * m.nw + (m.modmax + 2 + 31) / 32;
 }
 
-template <int D>
-__device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D>& w, int* sb,
+template <int D, int PL>
+__device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D, PL>& w, int* sb,
                            uint32_t* used) {
     const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
     const u32x4 w0 = philox(lo, hi, 0u, 0u, m.key0, m.key1);
@@ -125,11 +125,11 @@ __device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D
     }
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        double e[MAXPL][2];
-        uint32_t neg[MAXPL][2], ph[MAXPL][2];
+        double e[PL][2];
+        uint32_t neg[PL][2], ph[PL][2];
         double q = 0.0;
 #pragma unroll
-        for (int i = 0; i < MAXPL; i++) {
+        for (int i = 0; i < PL; i++) {
             const int p = lane + 64 * i;
 #pragma unroll
             for (int k = 0; k < 2; k++) {
@@ -157,7 +157,7 @@ __device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D
         const double W = wave_sum_canonical(q);
         const double invW = 1.0 / W;
 #pragma unroll
-        for (int i = 0; i < MAXPL; i++)
+        for (int i = 0; i < PL; i++)
 #pragma unroll
             for (int k = 0; k < 2; k++) {
                 const int bj = w.b[a][i][k];
@@ -182,8 +182,8 @@ __device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D
 }
 
 // Canonical wrapped position of the wave's candidate at timestep t (all lanes).
-template <int D>
-__device__ __forceinline__ void eval_wave64(const DevModel& m, const WaveCoef<D>& w, int t,
+template <int D, int PL>
+__device__ __forceinline__ void eval_wave64(const DevModel& m, const WaveCoef<D, PL>& w, int t,
                                             double* X) {
     const double* St = m.S + (size_t)t * m.ldb;
     const double* Ct = m.C + (size_t)t * m.ldb;
@@ -191,7 +191,7 @@ __device__ __forceinline__ void eval_wave64(const DevModel& m, const WaveCoef<D>
     for (int a = 0; a < D; a++) {
         double part = 0.0;
 #pragma unroll
-        for (int i = 0; i < MAXPL; i++)
+        for (int i = 0; i < PL; i++)
 #pragma unroll
             for (int k = 0; k < 2; k++) {
                 const int bj = w.b[a][i][k];
@@ -209,7 +209,7 @@ __device__ __forceinline__ void eval_wave64(const DevModel& m, const WaveCoef<D>
 // Lanes < 3^(D-1) each scan one 3-cell row of the neighbourhood (cuda/braid_cuda.cu:442-462).
 template <int D>
 __device__ __forceinline__ bool lookup_wave64(const DevModel& m, int t, const double* X,
-                                              int lane) {
+                                              int lane, unsigned& npts) {
     int c[D];
 #pragma unroll
     for (int a = 0; a < D; a++)
@@ -237,6 +237,7 @@ __device__ __forceinline__ bool lookup_wave64(const DevModel& m, int t, const do
                 for (int a = 0; a < D; a++)
                     h = h && fabs(torus_delta(X[a] - P[a])) <= m.r;
                 hit = h;
+                npts++;
                 n = __float_as_int(m.pos32[o].w);
             }
         }
@@ -247,10 +248,10 @@ __device__ __forceinline__ bool lookup_wave64(const DevModel& m, int t, const do
 // One wave per candidate.  FULL = false: flags[i] = 1 iff candidate survives the cell
 // table (early exit on the first contact).  FULL = true: write the canonical trajectory
 // [i][T][D] to out (no lookup).
-template <int D, bool FULL>
+template <int D, bool FULL, int PL>
 __global__ __launch_bounds__(256) void k_cand64(DevModel m, uint64_t base, int64_t count,
                                                 const int64_t* list, uint8_t* flags,
-                                                double* out) {
+                                                double* out, unsigned long long* ctr) {
     // Dynamic LDS (partial spectra only): per wave D*nw slot frequencies + a bitmask.
     extern __shared__ int dyn_lds[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -258,14 +259,16 @@ __global__ __launch_bounds__(256) void k_cand64(DevModel m, uint64_t base, int64
     int* sb = dyn_lds + wv * per_wave;
     uint32_t* used = (uint32_t*)(sb + D * m.nw);
     const int64_t nwaves = (int64_t)gridDim.x * 4;
+    unsigned long long nvis = 0;
+    unsigned npts = 0;
     for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < count; i += nwaves) {
         const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
-        WaveCoef<D> w;
-        gen_wave64<D>(m, idx, lane, w, sb, used);
+        WaveCoef<D, PL> w;
+        gen_wave64<D, PL>(m, idx, lane, w, sb, used);
         if (FULL) {
             for (int t = 0; t < m.T; t++) {
                 double X[D];
-                eval_wave64<D>(m, w, t, X);
+                eval_wave64<D, PL>(m, w, t, X);
                 if (lane < D)
                     out[((size_t)i * m.T + t) * D + lane] = X[lane];
             }
@@ -274,11 +277,22 @@ __global__ __launch_bounds__(256) void k_cand64(DevModel m, uint64_t base, int64
             for (int oi = 0; oi < m.T && !hit; oi++) {
                 const int t = m.order[oi];
                 double X[D];
-                eval_wave64<D>(m, w, t, X);
-                hit = lookup_wave64<D>(m, t, X, lane);
+                eval_wave64<D, PL>(m, w, t, X);
+                hit = lookup_wave64<D>(m, t, X, lane, npts);
+                nvis++;
             }
             if (lane == 0)
                 flags[i] = hit ? 0 : 1;
+        }
+    }
+    if (!FULL && ctr) {
+        // exact counts for the algorithmic-bytes figure (one atomic per wave)
+        unsigned long long pw = npts;
+        for (int h = 32; h >= 1; h >>= 1)
+            pw += __shfl_xor(pw, h, 64);
+        if (lane == 0 && nvis) {
+            atomicAdd(&ctr[0], nvis);
+            atomicAdd(&ctr[1], pw);
         }
     }
 }
@@ -494,23 +508,71 @@ int grid_for(int64_t items, int per_block) {
 }
 
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
-                      double* out, bool full) {
+                      double* out, bool full, unsigned long long* ctr) {
     if (count <= 0)
         return UG_OK;
     const int blocks = (int)std::min<int64_t>((count + 3) / 4, 256 * 16);
     const size_t lds = (size_t)4 * 4 * (h->d == 2 ? partial_lds_words<2>(h->dm) : partial_lds_words<3>(h->dm));
+    // slot pairs per lane: 1 up to 64 pairs (T <= 129), 2 up to 128, else 4
+    const int pl = h->npairs <= 64 ? 1 : (h->npairs <= 128 ? 2 : 4);
+#define UG_CAND64(DD, FF, PP)                                                                         \
+    hipLaunchKernelGGL((k_cand64<DD, FF, PP>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, \
+                       list, flags, out, ctr)
+#define UG_CAND64_PL(DD, FF)       \
+    do {                           \
+        if (pl == 1)               \
+            UG_CAND64(DD, FF, 1);  \
+        else if (pl == 2)          \
+            UG_CAND64(DD, FF, 2);  \
+        else                       \
+            UG_CAND64(DD, FF, 4);  \
+    } while (0)
     if (h->d == 2) {
         if (full)
-            hipLaunchKernelGGL((k_cand64<2, true>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, list, flags, out);
+            UG_CAND64_PL(2, true);
         else
-            hipLaunchKernelGGL((k_cand64<2, false>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, list, flags, out);
+            UG_CAND64_PL(2, false);
     } else {
         if (full)
-            hipLaunchKernelGGL((k_cand64<3, true>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, list, flags, out);
+            UG_CAND64_PL(3, true);
         else
-            hipLaunchKernelGGL((k_cand64<3, false>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, list, flags, out);
+            UG_CAND64_PL(3, false);
     }
+#undef UG_CAND64_PL
+#undef UG_CAND64
     return hip_check(hipGetLastError(), "k_cand64");
+}
+
+int32_t timed_begin(Pack* h) {
+    if (!h->profile)
+        return UG_OK;
+    return hip_check(hipEventRecord(h->ev0, h->stream), "event record");
+}
+
+// Records the end event, waits for it and accumulates the launch into kstats[which].
+int32_t timed_end(Pack* h, int which, int64_t items) {
+    if (!h->profile)
+        return UG_OK;
+    int32_t rc = hip_check(hipEventRecord(h->ev1, h->stream), "event record");
+    if (!rc)
+        rc = hip_check(hipEventSynchronize(h->ev1), "event sync");
+    float ms = 0.f;
+    if (!rc)
+        rc = hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time");
+    unsigned long long c[2] = {0, 0};
+    if (!rc)
+        rc = hip_check(hipMemcpy(c, h->d_ctr, 16, hipMemcpyDeviceToHost), "counter copy");
+    if (!rc)
+        rc = hip_check(hipMemset(h->d_ctr, 0, 16), "counter clear");
+    if (rc)
+        return rc;
+    ug_kernel_stats& k = h->kstats[which];
+    k.launches++;
+    k.items += items;
+    k.ms += ms;
+    k.visits += (int64_t)c[0];
+    k.points += (int64_t)c[1];
+    return UG_OK;
 }
 
 int32_t hip_check(hipError_t e, const char* what) {

@@ -47,6 +47,13 @@ class PackStatus(C.Structure):
     ]
 
 
+class KernelStats(C.Structure):
+    _fields_ = [("launches", C.c_int64), ("items", C.c_int64), ("ms", C.c_double),
+                ("visits", C.c_int64), ("points", C.c_int64)]
+
+
+KERNEL_PREFILTER, KERNEL_RECHECK = 0, 1
+
 _lib = None
 
 
@@ -71,6 +78,8 @@ def lib() -> C.CDLL:
         "ug_pack_positions": ([vp, vp, i64], i32),
         "ug_pack_curve": ([vp, vp, i64, C.POINTER(i64)], i32),
         "ug_eval_candidates": ([vp, vp, i64, i32, vp], i32),
+        "ug_pack_profile": ([vp, i32], i32),
+        "ug_pack_kernel_stats": ([vp, i32, C.POINTER(KernelStats)], i32),
         "ug_pack_destroy": ([vp], None),
     }
     for name, (args, res) in sigs.items():
@@ -87,7 +96,8 @@ def exported_symbols() -> list[str]:
     return [
         "ug_abi_version", "ug_last_error", "ug_pack_create", "ug_pack_run", "ug_pack_run_sub",
         "ug_pack_get_status", "ug_pack_accepted", "ug_pack_params", "ug_pack_positions",
-        "ug_pack_curve", "ug_eval_candidates", "ug_pack_destroy",
+        "ug_pack_curve", "ug_eval_candidates", "ug_pack_profile", "ug_pack_kernel_stats",
+        "ug_pack_destroy",
     ]
 
 
@@ -186,6 +196,15 @@ class Packer:
         out = torch.empty((max(n, 1), self.T, self.dim), dtype=torch.float64, device="cuda")
         check(lib().ug_pack_positions(self._h, out.data_ptr(), out.shape[0]))
         return out[:n]
+
+    def profile(self, enable: bool = True) -> None:
+        check(lib().ug_pack_profile(self._h, int(enable)))
+
+    def kernel_stats(self, which: int = KERNEL_PREFILTER) -> dict:
+        ks = KernelStats()
+        check(lib().ug_pack_kernel_stats(self._h, which, C.byref(ks)))
+        return {"launches": ks.launches, "items": ks.items, "ms": ks.ms, "visits": ks.visits,
+                "points": ks.points}
 
     def curve(self) -> np.ndarray:
         n = C.c_int64()
