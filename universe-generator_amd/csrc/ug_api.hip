@@ -103,6 +103,8 @@ void release(Pack* h) {
     fast_release(h);
     free_dev(h->d_S);
     free_dev(h->d_C);
+    free_dev(h->d_ST);
+    free_dev(h->d_CT);
     free_dev(h->d_order);
     free_dev(h->d_tab);
     free_dev(h->d_marg);
@@ -114,6 +116,7 @@ void release(Pack* h) {
     free_dev(h->d_flags);
     free_dev(h->d_list);
     free_dev(h->d_list2);
+    free_dev(h->d_list3);
     free_dev(h->d_bcount);
     free_dev(h->d_count);
     free_dev(h->d_spos);
@@ -138,10 +141,7 @@ __global__ void k_fill_tab(int* p, int64_t n) {
 
 int32_t record_curve(Pack* h) {
     while (h->attempts >= h->next_ms) {
-        unsigned long long filled = 0;
-        if (hipMemcpy(&filled, h->d_filled, 8, hipMemcpyDeviceToHost) != hipSuccess)
-            return fail(UG_EHIP, "curve: filled-count copy failed");
-        h->curve.push_back({h->attempts, h->n, h->nsub, (int64_t)filled});
+        h->curve.push_back({h->attempts, h->n, h->nsub, (int64_t)h->filled});
         h->next_ms = (int64_t)((double)h->next_ms * 1.15) + 1;
     }
     return UG_OK;
@@ -207,6 +207,12 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
             C[(size_t)t * W + k] = (-2.0 * (hk * hk)) / s1;
         }
     }
+    std::vector<double> ST((size_t)W * T), CT((size_t)W * T);
+    for (int t = 0; t < T; t++)
+        for (int k = 0; k <= T; k++) {
+            ST[(size_t)k * T + t] = S[(size_t)t * W + k];
+            CT[(size_t)k * T + t] = C[(size_t)t * W + k];
+        }
     std::vector<int> order(T);
     for (int i = 0, o = 0; o < T; i++) {
         order[o++] = i;
@@ -227,26 +233,33 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     } while (0)
     UG_ALLOC(h->d_S, S.size() * 8);
     UG_ALLOC(h->d_C, C.size() * 8);
+    UG_ALLOC(h->d_ST, ST.size() * 8);
+    UG_ALLOC(h->d_CT, CT.size() * 8);
     UG_ALLOC(h->d_order, (size_t)T * 4);
     UG_ALLOC(h->d_tab, (size_t)tab_len * 4);
     UG_ALLOC(h->d_filled, 8);
     UG_ALLOC(h->d_flags, (size_t)UG_SUB_MAX);
     UG_ALLOC(h->d_list, (size_t)UG_SUB_MAX * 8);
     UG_ALLOC(h->d_list2, (size_t)UG_SUB_MAX * 8);
+    UG_ALLOC(h->d_list3, (size_t)UG_SUB_MAX * 8);
     h->cmp_blocks_cap = UG_SUB_MAX / 4096 + 1;
     UG_ALLOC(h->d_bcount, (size_t)h->cmp_blocks_cap * 4);
-    UG_ALLOC(h->d_count, 4);
+    UG_ALLOC(h->d_count, 16);
     UG_ALLOC(h->d_spos, (size_t)UG_S_MAX * T * h->d * 8);
     UG_ALLOC(h->d_mask, (size_t)UG_S_MAX * UG_MASK_WORDS * 8);
     UG_ALLOC(h->d_acc_list, (size_t)UG_S_MAX * 4);
     UG_ALLOC(h->d_nacc, 4);
 #undef UG_ALLOC
-    if (rc == UG_OK && hipHostMalloc(&h->h_pinned, 64) != hipSuccess)
+    if (rc == UG_OK && hipHostMalloc(&h->h_pinned, 64 + 8 * UG_S_MAX) != hipSuccess)
         rc = fail(UG_ENOMEM, "pinned host allocation failed");
     if (rc == UG_OK) {
         rc = hip_check(hipMemcpyAsync(h->d_S, S.data(), S.size() * 8, hipMemcpyHostToDevice, h->stream), "upload S");
         if (!rc)
             rc = hip_check(hipMemcpyAsync(h->d_C, C.data(), C.size() * 8, hipMemcpyHostToDevice, h->stream), "upload C");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(h->d_ST, ST.data(), ST.size() * 8, hipMemcpyHostToDevice, h->stream), "upload ST");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(h->d_CT, CT.data(), CT.size() * 8, hipMemcpyHostToDevice, h->stream), "upload CT");
         if (!rc)
             rc = hip_check(hipMemcpyAsync(h->d_order, order.data(), (size_t)T * 4, hipMemcpyHostToDevice, h->stream),
                            "upload order");
@@ -270,6 +283,8 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     m.rowlen = T + 2;
     m.S = h->d_S;
     m.C = h->d_C;
+    m.ST = h->d_ST;
+    m.CT = h->d_CT;
     m.order = h->d_order;
     m.tab = h->d_tab;
     m.pos64 = nullptr;
@@ -365,37 +380,11 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
             sub = std::min(sub, nb - h->attempts);
         }
         const uint64_t base = (uint64_t)h->attempts;
-        rc = timed_begin(h);
-        if (rc)
-            break;
-        if (cfg.precision == UG_PREC_SPLIT_BF16)
-            rc = launch_fast(h, base, sub, h->d_flags);
-        else
-            rc = launch_cand64(h, base, sub, nullptr, h->d_flags, nullptr, false, h->profile ? h->d_ctr : nullptr);
-        if (!rc)
-            rc = timed_end(h, UG_KERNEL_PREFILTER, sub);
-        if (rc)
-            break;
         int64_t S = 0;
-        rc = compact(h, h->d_flags, sub, base, nullptr, h->d_list, &S);
+        const int64_t rechecked_before = h->rechecked;
+        rc = filter_candidates(h, base, sub, nullptr, h->d_list, &S);
         if (rc)
             break;
-        if (cfg.precision == UG_PREC_SPLIT_BF16 && S > 0) {
-            // FP64 recheck of every candidate the split-BF16 prefilter could not reject
-            h->rechecked += S;
-            rc = timed_begin(h);
-            if (!rc)
-                rc = launch_cand64(h, 0, S, h->d_list, h->d_flags, nullptr, false, h->profile ? h->d_ctr : nullptr);
-            if (!rc)
-                rc = timed_end(h, UG_KERNEL_RECHECK, S);
-            if (!rc)
-                rc = compact(h, h->d_flags, S, 0, h->d_list, h->d_list2, &S);
-            if (!rc && S > 0)
-                rc = hip_check(hipMemcpyAsync(h->d_list, h->d_list2, (size_t)S * 8, hipMemcpyDeviceToDevice, h->stream),
-                               "list copy");
-            if (rc)
-                break;
-        }
         h->survivors += S;
         int64_t acc = 0;
         const int64_t n_before = h->n;
@@ -408,11 +397,18 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         } else {
             h->attempts += sub;
         }
-        // sub-batch adaptation toward ~256 survivors (cuda/braid_cuda.cu:920-924)
-        if (S > 512 && h->sub > 4096)
-            h->sub /= 2;
-        else if (S < 64 && h->sub < sub_cap)
-            h->sub *= 2;
+        // Sub-batch size from the measured survivor rate, aiming at ~1024 candidates reaching
+        // the FP64 stages per sub-batch (the reference adapts toward ~256 survivors,
+        // cuda/braid_cuda.cu:920-924).  Results do not depend on it.
+        {
+            const int64_t seen = std::max<int64_t>(1, std::max(S, h->rechecked - rechecked_before));
+            double want = 1024.0 * (double)sub / (double)seen;
+            want = std::min(want, 8.0 * (double)sub);
+            int64_t next = 4096;
+            while (next * 2 <= (int64_t)want && next * 2 <= sub_cap)
+                next *= 2;
+            h->sub = next;
+        }
         if (win_target) {
             const int64_t adm = h->n - n_before;
             h->win.push_back({sub, adm});

@@ -191,7 +191,8 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 // Prefilter: flags[i] = 0 when candidate i certainly collides, 1 when the FP64 test must decide.
 template <int D>
 __global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
-                                                        uint8_t* flags, unsigned long long* ctr) {
+                                                        const int64_t* list, uint8_t* flags,
+                                                        unsigned long long* ctr) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevModel& m = f.m;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t bas
     const int nfrag = f.ks * 2 * 64;
     for (int i = threadIdx.x; i < nfrag; i += blockDim.x)
         bl[i] = f.bfrag[i];
-    float* pos = (float*)(lds + (size_t)nfrag * 16) + (size_t)wv * (D * TILE * NCOL + 2 * TILE);
+    float* pos = (float*)(lds + (size_t)nfrag * 16) + (size_t)wv * (D * TILE * NCOL + 3 * TILE);
     float* sa1 = pos + D * TILE * NCOL;
     float* sinvw = sa1 + TILE;
     __syncthreads();
@@ -212,24 +213,37 @@ __global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t bas
          tile += (int64_t)gridDim.x * FAST_WAVES) {
         const int64_t i = tile * TILE + mrow;
         const bool valid = i < count;
-        const uint64_t idx = base + (uint64_t)(valid ? i : 0);
+        const uint64_t idx = list ? (valid ? (uint64_t)list[i] : 0ull) : base + (uint64_t)(valid ? i : 0);
         const bool wbad = tile_positions<D>(f, idx, valid, lane, 0, bl, pos, sa1, sinvw);
-        // lookup: lanes m and m+32 take alternate columns of candidate m.  A candidate whose
-        // exponential spacings sum below nw/4 (wbad) is outside the error bound's premise and
-        // is never rejected here.
-        bool rej = wbad;
-        for (int k = 0; 2 * k < f.ncols; k++) {
-            const int col = 2 * k + h;
-            const bool active = valid && !rej && col < f.ncols;
+        // Adaptive cooperative lookup: every round hands all 64 lanes to the still
+        // unresolved candidates of the tile (64/U consecutive columns each), so the tile
+        // finishes in a few dependent memory round trips.  A candidate whose exponential
+        // spacings sum below nw/4 (wbad) is outside the error bound's premise and goes
+        // straight to the FP64 test.
+        uint32_t unres = (uint32_t)__ballot(lane < 32 && valid && !wbad);
+        uint32_t rejm = 0;
+        int ncol = 0;  // lanes < 32: next column of candidate `lane`
+        int* ulist = (int*)(sinvw + TILE);
+        while (unres) {
+            const int U = __popc(unres);
+            const int per = 64 / U;
+            if (lane < 32 && ((unres >> lane) & 1u))
+                ulist[__popc(unres & ((1u << lane) - 1u))] = lane;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int rk = lane / per, off = lane % per;
+            const int cm = rk < U ? ulist[rk] : 0;
+            const int col = __shfl(ncol, cm, 64) + off;
             bool cert = false;
-            if (active) {
+            if (rk < U && col < f.ncols) {
                 const int t = f.colt[col];
                 const float mt = m.marg[col];
                 float X[D];
                 int c[D];
 #pragma unroll
                 for (int a = 0; a < D; a++) {
-                    X[a] = pos[(a * TILE + mrow) * NCOL + col];
+                    X[a] = pos[(a * TILE + cm) * NCOL + col];
                     int cc = (int)floorf((X[a] + 1.0f) * (float)m.halfT);
                     cc = cc < 0 ? cc + m.T : (cc >= m.T ? cc - m.T : cc);
                     c[a] = cc;
@@ -265,14 +279,21 @@ __global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t bas
                     }
                 }
             }
-            rej = rej || cert;
-            rej = rej || __shfl_xor((int)rej, 32, 64);
-            if (__ballot(valid && !rej && (2 * k + 2 < f.ncols)) == 0)
-                break;
+            const uint64_t hits = __ballot(cert);
+            if (lane < 32 && ((unres >> lane) & 1u)) {
+                const int myrk = __popc(unres & ((1u << lane) - 1u));
+                const uint64_t segmask = per >= 64 ? ~0ull : ((1ull << per) - 1ull);
+                if ((hits >> (myrk * per)) & segmask)
+                    rejm |= 1u << lane;
+                ncol += per;
+            }
+            const uint32_t rej_all = (uint32_t)__ballot(lane < 32 && ((rejm >> lane) & 1u));
+            const uint32_t done = (uint32_t)__ballot(lane < 32 && ncol >= f.ncols);
+            unres &= ~(rej_all | done);
+            rejm = rej_all;
         }
-        // every candidate without a certain contact goes to the FP64 test
         if (valid && h == 0)
-            flags[i] = (rej && !wbad) ? 0 : 1;
+            flags[i] = ((rejm >> lane) & 1u) ? 0 : 1;
     }
     if (ctr) {
         unsigned long long a = nvis, b = npts;
@@ -432,7 +453,7 @@ int32_t fast_setup(Pack* h) {
         rc = hip_check(hipMemcpy(h->d_marg, h->marg_host.data(), h->marg_host.size() * 4, hipMemcpyHostToDevice),
                        "margin upload");
     h->dm.marg = h->d_marg;
-    fs->lds_prefilter = (size_t)fs->ks * 2 * 64 * 16 + (size_t)FAST_WAVES * (h->d * TILE * NCOL + 2 * TILE) * 4;
+    fs->lds_prefilter = (size_t)fs->ks * 2 * 64 * 16 + (size_t)FAST_WAVES * (h->d * TILE * NCOL + 3 * TILE) * 4;
     h->d_basis16 = fs;
     if (!rc && fs->lds_prefilter > 160 * 1024)
         rc = fail(UG_EINVAL, "split-BF16 tile needs %zu bytes of LDS", fs->lds_prefilter);
@@ -463,9 +484,11 @@ static FastArgs fast_args(Pack* h) {
     return f;
 }
 
-int32_t launch_fast(Pack* h, uint64_t base, int64_t count, uint8_t* flags) {
+int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags) {
+    if (count <= 0)
+        return UG_OK;
     if (!h->full || !h->d_basis16)
-        return launch_cand64(h, base, count, nullptr, flags, nullptr, false, h->profile ? h->d_ctr : nullptr);
+        return fail(UG_ESTATE, "split-BF16 prefilter not set up for this model");
     FastState* fs = (FastState*)h->d_basis16;
     const int64_t ntiles = (count + TILE - 1) / TILE;
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, 1024));
@@ -473,10 +496,10 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, uint8_t* flags) {
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
     if (h->d == 2)
         hipLaunchKernelGGL(k_fast_prefilter<2>, dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base, count,
-                           flags, ctr);
+                           list, flags, ctr);
     else
         hipLaunchKernelGGL(k_fast_prefilter<3>, dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base, count,
-                           flags, ctr);
+                           list, flags, ctr);
     return hip_check(hipGetLastError(), "k_fast_prefilter");
 }
 

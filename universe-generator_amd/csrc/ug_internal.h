@@ -27,6 +27,8 @@ struct DevModel {
     int rowlen;       // cell-table row length along the last axis (T + 2: one halo cell each side)
     const double* S;  // [T][T+1]  sin(k z_t)/sin z_t
     const double* C;  // [T][T+1]  (cos(k z_t) - 1)/sin z_t
+    const double* ST; // [T+1][T]  S transposed (lanes over timesteps read contiguously)
+    const double* CT; // [T+1][T]  C transposed
     const int* order; // [T] endpoint-first scan order
     int* tab;         // cell table: head path of each (t, cell) chain, -1 = empty
     const double* pos64;  // [n][T][D] canonical positions of accepted paths
@@ -43,7 +45,7 @@ struct Pack {
     bool full;
     DevModel dm;
     // tables
-    double *d_S = nullptr, *d_C = nullptr;
+    double *d_S = nullptr, *d_C = nullptr, *d_ST = nullptr, *d_CT = nullptr;
     int* d_order = nullptr;
     int* d_tab = nullptr;
     int64_t tab_len = 0;
@@ -57,10 +59,12 @@ struct Pack {
     int32_t* d_acc_gid = nullptr;
     unsigned long long* d_filled = nullptr;
     int64_t cap = 0, n = 0, nsub = 0;
+    unsigned long long filled = 0;  // occupied (t, cell) buckets, as of the last insert
     // batch scratch
     uint8_t* d_flags = nullptr;
     int64_t* d_list = nullptr;
     int64_t* d_list2 = nullptr;
+    int64_t* d_list3 = nullptr;
     int* d_bcount = nullptr;
     int cmp_blocks_cap = 0;
     int* d_count = nullptr;
@@ -92,16 +96,19 @@ int32_t fail(int32_t code, const char* fmt, ...);
 int32_t hip_check(hipError_t e, const char* what);
 int grid_for(int64_t items, int per_block);
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
-                      double* out, bool full, unsigned long long* ctr = nullptr);
+                      double* out, bool full, unsigned long long* ctr = nullptr,
+                      const int* dcount = nullptr, int head = -1);
 int32_t timed_begin(Pack* h);
 int32_t timed_end(Pack* h, int which, int64_t items);
 int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, const int64_t* list,
-                int64_t* out, int64_t* S);
+                int64_t* out, int64_t* S, const int* dcount = nullptr, int slot = 1, bool sync = true);
+int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out,
+                          int64_t* S);
 int32_t ensure_paths(Pack* h, int64_t need);
 int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc);
 // split-BF16 prefilter (ug_fast.hip)
 int32_t fast_setup(Pack* h);
-int32_t launch_fast(Pack* h, uint64_t base, int64_t count, uint8_t* flags);
+int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags);
 int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* out);
 void fast_release(Pack* h);
 

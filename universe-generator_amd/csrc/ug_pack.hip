@@ -103,11 +103,6 @@ __device__ void draw_freqs_partial(const DevModel& m, uint64_t idx, int* sb, uin
     }
 }
 
-template <int D>
-__host__ __device__ inline int partial_lds_words(const DevModel& m) {
-    return m.full ? 0 : D * m.nw + (m.modmax + 2 + 31) / 32;
-}
-
 template <int D, int PL>
 __device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D, PL>& w, int* sb,
                            uint32_t* used) {
@@ -148,7 +143,7 @@ __device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D
                         e[i][k] = expo_of_word(dw[k]);
                         neg[i][k] = dw[k] & 1u;
                         ph[i][k] = pw[k];
-                        w.b[a][i][k] = m.full ? j + 2 : sb[a * m.nw + j];
+                        w.b[a][i][k] = m.full ? j + 2 : sb[a * m.nw + j];  // read before the caller reuses sb
                         q = q + e[i][k];
                     }
                 }
@@ -181,7 +176,8 @@ __device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D
     }
 }
 
-// Canonical wrapped position of the wave's candidate at timestep t (all lanes).
+// Canonical wrapped position of the wave's candidate at timestep t (all lanes): lane
+// partials over its slot pairs, then the halving tree across lanes.
 template <int D, int PL>
 __device__ __forceinline__ void eval_wave64(const DevModel& m, const WaveCoef<D, PL>& w, int t,
                                             double* X) {
@@ -205,85 +201,229 @@ __device__ __forceinline__ void eval_wave64(const DevModel& m, const WaveCoef<D,
     }
 }
 
-// Does position X (canonical FP64) collide with a path of the cell table at timestep t?
-// Lanes < 3^(D-1) each scan one 3-cell row of the neighbourhood (cuda/braid_cuda.cu:442-462).
-template <int D>
-__device__ __forceinline__ bool lookup_wave64(const DevModel& m, int t, const double* X,
-                                              int lane, unsigned& npts) {
-    int c[D];
+__host__ __device__ constexpr int bitrev3(int i) {
+    return ((i & 1) << 2) | (i & 2) | ((i & 4) >> 2);
+}
+
+// Slot partial P_s at timestep t from the LDS coefficients: fma chain over the pairs
+// p = s + 64 i (ascending), even frequencies adding their (cos - 1) basis term.
+template <int PL>
+__device__ __forceinline__ double slot_partial(const DevModel& m, const double2* ca, const int* ba,
+                                               int sl, int t) {
+    double acc = 0.0;
 #pragma unroll
-    for (int a = 0; a < D; a++)
-        c[a] = cell_of(X[a], m.halfT, m.T);
-    bool hit = false;
-    constexpr int ROWS = D == 2 ? 3 : 9;
-    if (lane < ROWS) {
-        const int T = m.T;
-        int64_t row;
-        if (D == 2) {
-            const int gx = (c[0] + lane - 1 + T) % T;
-            row = ((int64_t)t * T + gx) * m.rowlen;
-        } else {
-            const int gx = (c[0] + lane / 3 - 1 + T) % T;
-            const int gy = (c[1] + lane % 3 - 1 + T) % T;
-            row = (((int64_t)t * T + gx) * T + gy) * m.rowlen;
-        }
-        const int* tr = m.tab + row + c[D - 1];  // padded cells c-1..c+1 sit at c..c+2
-        for (int k = 0; k < 3 && !hit; k++) {
-            for (int n = tr[k]; n >= 0 && !hit;) {
-                const size_t o = (size_t)n * T + t;
-                const double* P = m.pos64 + o * D;
-                bool h = true;
+    for (int ii = 0; ii < PL; ii++) {
+        const int p = sl + 64 * ii;
 #pragma unroll
-                for (int a = 0; a < D; a++)
-                    h = h && fabs(torus_delta(X[a] - P[a])) <= m.r;
-                hit = h;
-                npts++;
-                n = __float_as_int(m.pos32[o].w);
+        for (int k = 0; k < 2; k++) {
+            const int j = 2 * p + k;
+            if (j < m.nw) {
+                const int b = ba[j];
+                const double2 c = ca[j];
+                acc = fma(c.x, m.ST[(size_t)b * m.T + t], acc);
+                if ((b & 1) == 0)
+                    acc = fma(c.y, m.CT[(size_t)b * m.T + t], acc);
             }
         }
     }
-    return __ballot(hit) != 0;
+    return acc;
 }
 
-// One wave per candidate.  FULL = false: flags[i] = 1 iff candidate survives the cell
-// table (early exit on the first contact).  FULL = true: write the canonical trajectory
-// [i][T][D] to out (no lookup).
+// The same canonical position, evaluated by ONE lane (lanes take different timesteps).
+// The halving tree over the 64 slot partials equals the adjacent pairwise tree over the
+// slots in bit-reversed order; element 8g + k of that order is slot 8 bitrev3(k) + bitrev3(g),
+// so each group of 8 is a fixed 3-level tree and the 8 group sums merge as a binary counter.
+template <int D, int PL>
+__device__ __forceinline__ void eval_lane(const DevModel& m, const double2* cs, const int* bb,
+                                          const double* a1, int t, double* X) {
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const double2* ca = cs + a * m.nw;
+        const int* ba = bb + a * m.nw;
+        double L0 = 0.0, L1 = 0.0, L2 = 0.0, total = 0.0;
+#pragma unroll 1
+        for (int g = 0; g < 8; g++) {
+            const int rg = bitrev3(g);
+            double e[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                e[k] = slot_partial<PL>(m, ca, ba, 8 * bitrev3(k) + rg, t);
+            double G = ((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7]));
+            if ((g & 1) == 0) {
+                L0 = G;
+                continue;
+            }
+            G = L0 + G;
+            if ((g & 2) == 0) {
+                L1 = G;
+                continue;
+            }
+            G = L1 + G;
+            if ((g & 4) == 0) {
+                L2 = G;
+                continue;
+            }
+            total = L2 + G;
+        }
+        X[a] = torus_wrap(a1[a] + total);
+    }
+}
+
+// Per-wave LDS of k_cand64: partial-spectrum draws (slot frequencies + bitmask), the slot
+// coefficients (c, s) of every axis, and positions of up to 64 timesteps.
+template <int D>
+__host__ __device__ inline int cand64_lds_words(const DevModel& m) {
+    const int head = (D * m.nw + (m.modmax + 2 + 31) / 32 + 3) / 4 * 4;
+    return head + D * m.nw * 4 + 64 * D * 2;
+}
+
+// Lookup round: lanes take (timestep slot sl, 3-cell row rr) items of the timesteps whose
+// positions are in pos[]; returns true when any lane finds a contact (canonical predicate).
+template <int D>
+__device__ __forceinline__ bool lookup_round(const DevModel& m, const double* pos, const int* tsl,
+                                             int G, int lane, unsigned long long& npts) {
+    constexpr int R = D == 2 ? 3 : 9;
+    const int T = m.T;
+    const int sl = lane / R, rr = lane % R;
+    bool h = false;
+    if (sl < G) {
+        const int t = tsl[sl];
+        double x[D];
+        int c[D];
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            x[a] = pos[sl * D + a];
+            c[a] = cell_of(x[a], m.halfT, T);
+        }
+        int64_t row;
+        if (D == 2) {
+            row = ((int64_t)t * T + (c[0] + rr - 1 + T) % T) * m.rowlen;
+        } else {
+            const int gx = (c[0] + rr / 3 - 1 + T) % T;
+            const int gy = (c[1] + rr % 3 - 1 + T) % T;
+            row = (((int64_t)t * T + gx) * T + gy) * m.rowlen;
+        }
+        const int* tr = m.tab + row + c[D - 1];  // padded cells c-1..c+1 at c..c+2
+        const int hd[3] = {tr[0], tr[1], tr[2]};
+        for (int k = 0; k < 3 && !h; k++)
+            for (int n = hd[k]; n >= 0 && !h;) {
+                const size_t q = (size_t)n * T + t;
+                const double* P = m.pos64 + q * D;
+                bool hh = true;
+#pragma unroll
+                for (int a = 0; a < D; a++)
+                    hh = hh && fabs(torus_delta(x[a] - P[a])) <= m.r;
+                h = hh;
+                npts++;
+                n = __float_as_int(m.pos32[q].w);
+            }
+    }
+    return __ballot(h) != 0;
+}
+
+// One wave per candidate.  Lanes generate the coefficients over slot pairs.  The first
+// `head` endpoint-first timesteps (where most candidates meet a contact) are evaluated across
+// the wave (canonical lane tree) in chunks of 2, 4, 8, ... (at most 64 / 3^(D-1)), each
+// chunk looked up in one round; a candidate still clear after them has every remaining
+// timestep evaluated in parallel, one lane per timestep, and looked up 64 / 3^(D-1)
+// timesteps per round (cuda/braid_cuda.cu:418-462).  Bulk tests of fresh candidates use
+// head = T; lists of likely survivors use a short head.
+// FULL = false: flags[i] = 1 iff candidate i survives the cell table.
+// FULL = true: out[i][T][D] = canonical trajectory (no lookup).
 template <int D, bool FULL, int PL>
-__global__ __launch_bounds__(256) void k_cand64(DevModel m, uint64_t base, int64_t count,
-                                                const int64_t* list, uint8_t* flags,
-                                                double* out, unsigned long long* ctr) {
-    // Dynamic LDS (partial spectra only): per wave D*nw slot frequencies + a bitmask.
-    extern __shared__ int dyn_lds[];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int per_wave = partial_lds_words<D>(m);
-    int* sb = dyn_lds + wv * per_wave;
-    uint32_t* used = (uint32_t*)(sb + D * m.nw);
-    const int64_t nwaves = (int64_t)gridDim.x * 4;
-    unsigned long long nvis = 0;
-    unsigned npts = 0;
-    for (int64_t i = (int64_t)blockIdx.x * 4 + wv; i < count; i += nwaves) {
+__global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_t count,
+                                               const int64_t* list, const int* dcount,
+                                               uint8_t* flags, double* out, int head,
+                                               unsigned long long* ctr) {
+    if (dcount)
+        count = min(count, (int64_t)*dcount);
+    extern __shared__ __align__(16) int dyn_lds[];
+    const int lane = threadIdx.x;
+    const int T = m.T, nw = m.nw;
+    int* sb = dyn_lds;
+    uint32_t* used = (uint32_t*)(sb + D * nw);
+    const int lds_head = (D * nw + (m.modmax + 2 + 31) / 32 + 3) / 4 * 4;
+    double2* cs = (double2*)(dyn_lds + lds_head);
+    double* pos = (double*)(cs + D * nw);
+    __shared__ int tsl[64];
+    constexpr int R = D == 2 ? 3 : 9;
+    constexpr int PER_ROUND = 64 / R;
+    unsigned long long nvis = 0, npts = 0;
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
         const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
         WaveCoef<D, PL> w;
         gen_wave64<D, PL>(m, idx, lane, w, sb, used);
-        if (FULL) {
-            for (int t = 0; t < m.T; t++) {
-                double X[D];
-                eval_wave64<D, PL>(m, w, t, X);
-                if (lane < D)
-                    out[((size_t)i * m.T + t) * D + lane] = X[lane];
+        bool hit = false;
+        int o0 = 0;
+        if (!FULL) {
+            // head: chunks of 2, 4, 8 timesteps evaluated across the wave, one lookup round each
+            for (int chunk = 2; !hit && o0 < min(head, T); chunk *= 2) {
+                const int G = min(min(chunk, PER_ROUND), T - o0);
+                for (int g = 0; g < G; g++) {
+                    double X[D];
+                    const int t = m.order[o0 + g];
+                    eval_wave64<D, PL>(m, w, t, X);
+                    if (lane == g) {
+                        tsl[g] = t;
+#pragma unroll
+                        for (int a = 0; a < D; a++)
+                            pos[g * D + a] = X[a];
+                    }
+                }
+                __syncthreads();
+                hit = lookup_round<D>(m, pos, tsl, G, lane, npts);
+                nvis += (unsigned long long)G;
+                o0 += G;
+                __syncthreads();
             }
-        } else {
-            bool hit = false;
-            for (int oi = 0; oi < m.T && !hit; oi++) {
-                const int t = m.order[oi];
-                double X[D];
-                eval_wave64<D, PL>(m, w, t, X);
-                hit = lookup_wave64<D>(m, t, X, lane, npts);
-                nvis++;
-            }
-            if (lane == 0)
-                flags[i] = hit ? 0 : 1;
         }
+        if (!hit && o0 < T) {
+            // remaining timesteps: coefficients to LDS, one lane per timestep
+#pragma unroll
+            for (int a = 0; a < D; a++)
+#pragma unroll
+                for (int ii = 0; ii < PL; ii++)
+#pragma unroll
+                    for (int k = 0; k < 2; k++) {
+                        const int j = 2 * (lane + 64 * ii) + k;
+                        if (j < nw) {
+                            cs[a * nw + j] = make_double2(w.c[a][ii][k], w.s[a][ii][k]);
+                            sb[a * nw + j] = w.b[a][ii][k];
+                        }
+                    }
+            __syncthreads();
+            for (int ps = o0; ps < T && !hit; ps += 64) {
+                const int oi = ps + lane;
+                const int pend = min(64, T - ps);
+                if (oi < T) {
+                    const int t = FULL ? oi : m.order[oi];
+                    double X[D];
+                    eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
+                    if (FULL) {
+#pragma unroll
+                        for (int a = 0; a < D; a++)
+                            out[((size_t)i * T + t) * D + a] = X[a];
+                    } else {
+                        tsl[lane] = t;
+#pragma unroll
+                        for (int a = 0; a < D; a++)
+                            pos[lane * D + a] = X[a];
+                    }
+                }
+                if (FULL)
+                    continue;
+                __syncthreads();
+                for (int r0 = 0; r0 < pend && !hit; r0 += PER_ROUND) {
+                    const int G = min(PER_ROUND, pend - r0);
+                    hit = lookup_round<D>(m, pos + r0 * D, tsl + r0, G, lane, npts);
+                    nvis += (unsigned long long)G;
+                }
+                __syncthreads();
+            }
+        }
+        if (!FULL && lane == 0)
+            flags[i] = hit ? 0 : 1;
+        __syncthreads();
     }
     if (!FULL && ctr) {
         // exact counts for the algorithmic-bytes figure (one atomic per wave)
@@ -302,8 +442,10 @@ __global__ __launch_bounds__(256) void k_cand64(DevModel m, uint64_t base, int64
 constexpr int CMP_ITEMS = 4096;  // items per block (256 threads x 16)
 
 __global__ __launch_bounds__(256) void k_compact_count(const uint8_t* flags, int64_t count,
-                                                       int* bcount) {
+                                                       const int* dcount, int* bcount) {
     __shared__ int red[256];
+    if (dcount)
+        count = min(count, (int64_t)*dcount);
     const int64_t b0 = (int64_t)blockIdx.x * CMP_ITEMS;
     int n = 0;
     for (int k = threadIdx.x; k < CMP_ITEMS; k += 256) {
@@ -323,10 +465,12 @@ __global__ __launch_bounds__(256) void k_compact_count(const uint8_t* flags, int
 }
 
 __global__ __launch_bounds__(256) void k_compact_scatter(const uint8_t* flags, int64_t count,
-                                                         uint64_t base, const int64_t* list,
-                                                         const int* bcount, int nblocks,
-                                                         int64_t* out, int* total) {
+                                                         const int* dcount, uint64_t base,
+                                                         const int64_t* list, const int* bcount,
+                                                         int nblocks, int64_t* out, int* total) {
     __shared__ int red[256];
+    if (dcount)
+        count = min(count, (int64_t)*dcount);
     __shared__ int scan[256];
     // offset of this block = sum of earlier block counts
     int s0 = 0;
@@ -440,53 +584,73 @@ __global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, int*
         *nacc = na;
 }
 
-// Insert the chunk's accepted survivors (admission order) into the cell table; one
-// thread per timestep, so chain order equals admission order (cuda/braid_cuda.cu:865-879).
+// Insert the chunk's accepted survivors into the cell table, one thread per (path, timestep):
+// path n becomes the head of its (t, cell) chain and links to the previous head
+// (cuda/braid_cuda.cu:865-879).  Accepted paths of one chunk never collide, so each cell
+// receives at most one of them except at rounding ties, where the atomic exchange keeps the
+// chain intact.  Row-edge cells are mirrored into the halo cells afterwards (k_halo).
 template <int D>
-__global__ __launch_bounds__(64) void k_insert(DevModel m, const double* pos, const int* acc_list,
-                                               const int* nacc, const int64_t* surv_idx,
-                                               int64_t n0, int64_t* acc_idx, int32_t* acc_gid,
-                                               double* pos64, float4* pos32,
-                                               unsigned long long* filled) {
-    const int t = blockIdx.x * 64 + threadIdx.x;
-    const int na = *nacc;
-    if (t < m.T) {
-        unsigned long long fresh = 0;
-        for (int k = 0; k < na; k++) {
-            const int s = acc_list[k];
-            const int64_t n = n0 + k;
-            const double* X = pos + ((size_t)s * m.T + t) * D;
-            int c[D];
-            float f[3] = {0.f, 0.f, 0.f};
-            const size_t o = (size_t)n * m.T + t;
+__global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, const int* acc_list,
+                                                const int* nacc, int limit, const int64_t* surv_idx,
+                                                int64_t n0, int64_t* acc_idx, int32_t* acc_gid,
+                                                double* pos64, float4* pos32,
+                                                unsigned long long* filled) {
+    const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int na = min(*nacc, limit);
+    const int k = (int)(item / m.T), t = (int)(item % m.T);
+    bool fresh = false;
+    if (k < na) {
+        const int s = acc_list[k];
+        const int64_t n = n0 + k;
+        const double* X = pos + ((size_t)s * m.T + t) * D;
+        int c[D];
+        float f[3] = {0.f, 0.f, 0.f};
+        const size_t o = (size_t)n * m.T + t;
 #pragma unroll
-            for (int a = 0; a < D; a++) {
-                c[a] = cell_of(X[a], m.halfT, m.T);
-                pos64[o * D + a] = X[a];
-                f[a] = (float)X[a];
-            }
-            int64_t row = (int64_t)t * m.T + c[0];
-            if (D == 3)
-                row = row * m.T + c[1];
-            row *= m.rowlen;
-            const int cl = c[D - 1];
-            const int head = m.tab[row + cl + 1];
-            fresh += head < 0 ? 1ull : 0ull;
-            pos32[o] = make_float4(f[0], f[1], f[2], __int_as_float(head));
-            m.tab[row + cl + 1] = (int)n;
-            if (cl == 0)
-                m.tab[row + m.T + 1] = (int)n;
-            if (cl == m.T - 1)
-                m.tab[row] = (int)n;
+        for (int a = 0; a < D; a++) {
+            c[a] = cell_of(X[a], m.halfT, m.T);
+            pos64[o * D + a] = X[a];
+            f[a] = (float)X[a];
         }
-        if (fresh)
-            atomicAdd(filled, fresh);
+        int64_t row = (int64_t)t * m.T + c[0];
+        if (D == 3)
+            row = row * m.T + c[1];
+        row *= m.rowlen;
+        const int prev = atomicExch(&m.tab[row + c[D - 1] + 1], (int)n);
+        fresh = prev < 0;
+        pos32[o] = make_float4(f[0], f[1], f[2], __int_as_float(prev));
+        if (t == 0) {
+            acc_idx[n] = surv_idx[s];
+            acc_gid[n] = (int32_t)n;
+        }
     }
-    if (t == 0)
-        for (int k = 0; k < na; k++) {
-            acc_idx[n0 + k] = surv_idx[acc_list[k]];
-            acc_gid[n0 + k] = (int32_t)(n0 + k);
-        }
+    const uint64_t fm = __ballot(fresh);
+    if ((threadIdx.x & 63) == 0 && fm)
+        atomicAdd(filled, (unsigned long long)__popcll(fm));
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_halo(DevModel m, const double* pos, const int* acc_list,
+                                              const int* nacc, int limit) {
+    const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int na = min(*nacc, limit);
+    const int k = (int)(item / m.T), t = (int)(item % m.T);
+    if (k >= na)
+        return;
+    const double* X = pos + ((size_t)acc_list[k] * m.T + t) * D;
+    int c[D];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        c[a] = cell_of(X[a], m.halfT, m.T);
+    int64_t row = (int64_t)t * m.T + c[0];
+    if (D == 3)
+        row = row * m.T + c[1];
+    row *= m.rowlen;
+    const int cl = c[D - 1];
+    if (cl == 0)
+        m.tab[row + m.T + 1] = m.tab[row + 1];
+    if (cl == m.T - 1)
+        m.tab[row] = m.tab[row + m.T];
 }
 
 __global__ void k_fill_i32(int* p, int64_t n, int v) {
@@ -508,16 +672,18 @@ int grid_for(int64_t items, int per_block) {
 }
 
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
-                      double* out, bool full, unsigned long long* ctr) {
+                      double* out, bool full, unsigned long long* ctr, const int* dcount, int head) {
     if (count <= 0)
         return UG_OK;
-    const int blocks = (int)std::min<int64_t>((count + 3) / 4, 256 * 16);
-    const size_t lds = (size_t)4 * 4 * (h->d == 2 ? partial_lds_words<2>(h->dm) : partial_lds_words<3>(h->dm));
+    if (head < 0)
+        head = list ? (h->d == 2 ? 14 : 13) : h->T;
+    const int blocks = (int)std::min<int64_t>(count, 256 * 32);
+    const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
     // slot pairs per lane: 1 up to 64 pairs (T <= 129), 2 up to 128, else 4
     const int pl = h->npairs <= 64 ? 1 : (h->npairs <= 128 ? 2 : 4);
 #define UG_CAND64(DD, FF, PP)                                                                         \
-    hipLaunchKernelGGL((k_cand64<DD, FF, PP>), dim3(blocks), dim3(256), lds, h->stream, h->dm, base, count, \
-                       list, flags, out, ctr)
+    hipLaunchKernelGGL((k_cand64<DD, FF, PP>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, \
+                       list, dcount, flags, out, head, ctr)
 #define UG_CAND64_PL(DD, FF)       \
     do {                           \
         if (pl == 1)               \
@@ -581,28 +747,75 @@ int32_t hip_check(hipError_t e, const char* what) {
     return UG_OK;
 }
 
-// Ordered compaction of flags[count] into h->list_out; returns the count via *S (syncs).
+// Ordered compaction of flags[count] (count bounded by *dcount when given) into out; the
+// number kept lands in h->d_count[slot] and, when sync is set, in *S.
 int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, const int64_t* list,
-                int64_t* out, int64_t* S) {
+                int64_t* out, int64_t* S, const int* dcount, int slot, bool sync) {
     const int nb = (int)((count + CMP_ITEMS - 1) / CMP_ITEMS);
     if (nb > h->cmp_blocks_cap)
         return fail(UG_EOVERFLOW, "compaction of %lld items exceeds the batch capacity",
                     (long long)count);
-    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(256), 0, h->stream, flags, count, h->d_bcount);
-    hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(256), 0, h->stream, flags, count, base, list,
-                       h->d_bcount, nb, out, h->d_count);
+    if (count <= 0) {
+        if (S)
+            *S = 0;
+        return hip_check(hipMemsetAsync(h->d_count + slot, 0, 4, h->stream), "count clear");
+    }
+    hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(256), 0, h->stream, flags, count, dcount, h->d_bcount);
+    hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(256), 0, h->stream, flags, count, dcount, base, list,
+                       h->d_bcount, nb, out, h->d_count + slot);
     int32_t rc = hip_check(hipGetLastError(), "compact");
-    if (rc)
+    if (rc || !sync)
         return rc;
-    rc = hip_check(hipMemcpyAsync(h->h_pinned, h->d_count, sizeof(int), hipMemcpyDeviceToHost, h->stream),
+    rc = hip_check(hipMemcpyAsync(h->h_pinned, h->d_count, 8, hipMemcpyDeviceToHost, h->stream),
                    "compact count copy");
-    if (rc)
+    if (!rc)
+        rc = hip_check(hipStreamSynchronize(h->stream), "compact sync");
+    if (!rc)
+        *S = ((int*)h->h_pinned)[slot];
+    return rc;
+}
+
+// Candidates [base, base+count) or list[0..count) -> survivors of the exact FP64 test
+// against the cell table, in index order, into out (*S of them).  The split-BF16 form
+// prefilters with k_fast_prefilter and rechecks only the candidates it could not reject,
+// chained through a device-side count.  Profiled runs time both stages.
+int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out,
+                          int64_t* S) {
+    int32_t rc;
+    unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
+    const bool split = h->cfg.precision == UG_PREC_SPLIT_BF16 && h->full;
+    if (!split) {
+        rc = timed_begin(h);
+        if (!rc)
+            rc = launch_cand64(h, base, count, list, h->d_flags, nullptr, false, ctr, nullptr, -1);
+        if (!rc && !list)
+            rc = timed_end(h, UG_KERNEL_PREFILTER, count);
+        if (!rc)
+            rc = compact(h, h->d_flags, count, base, list, out, S, nullptr, 1, true);
         return rc;
-    rc = hip_check(hipStreamSynchronize(h->stream), "compact sync");
-    if (rc)
-        return rc;
-    *S = *(int*)h->h_pinned;
-    return UG_OK;
+    }
+    rc = timed_begin(h);
+    if (!rc)
+        rc = launch_fast(h, base, count, list, h->d_flags);
+    if (!rc && !list)
+        rc = timed_end(h, UG_KERNEL_PREFILTER, count);
+    if (!rc)
+        rc = compact(h, h->d_flags, count, base, list, h->d_list3, nullptr, nullptr, 0, false);
+    if (!rc)
+        rc = timed_begin(h);
+    if (!rc)
+        rc = launch_cand64(h, 0, count, h->d_list3, h->d_flags, nullptr, false, ctr, h->d_count, -1);
+    if (!rc && h->profile) {
+        int32_t n1 = 0;
+        rc = hip_check(hipMemcpy(&n1, h->d_count, 4, hipMemcpyDeviceToHost), "recheck count");
+        if (!rc)
+            rc = timed_end(h, UG_KERNEL_RECHECK, n1);
+    }
+    if (!rc)
+        rc = compact(h, h->d_flags, count, 0, h->d_list3, out, S, h->d_count, 1, true);
+    if (!rc)
+        h->rechecked += ((int*)h->h_pinned)[0];
+    return rc;
 }
 
 int32_t ensure_paths(Pack* h, int64_t need) {
@@ -646,13 +859,16 @@ int32_t ensure_paths(Pack* h, int64_t need) {
 
 // Resolve the survivors list[0..S) (already clear of the cell table) in index order and
 // insert the accepted ones.  Survivors beyond one chunk are retested against the updated
-// table before their turn.  Returns the number accepted via *acc.
+// table before their turn.  Returns the number accepted via *acc_total.
 int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total) {
     int32_t rc;
     *acc_total = 0;
     while (S > 0) {
         const int s = (int)std::min<int64_t>(S, UG_S_MAX);
-        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true);
+        rc = ensure_paths(h, h->n + s);
+        if (rc)
+            return rc;
+        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, nullptr, 0);
         if (rc)
             return rc;
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((int64_t)s * UG_MASK_WORDS, 256)), dim3(256), 0,
@@ -668,60 +884,49 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         }
         hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, h->d_acc_list,
                            h->d_nacc);
-        rc = hip_check(hipMemcpyAsync(h->h_pinned, h->d_nacc, sizeof(int), hipMemcpyDeviceToHost, h->stream),
-                       "nacc copy");
-        if (rc)
-            return rc;
-        rc = hip_check(hipStreamSynchronize(h->stream), "resolve sync");
-        if (rc)
-            return rc;
-        int na = *(int*)h->h_pinned;
-        if (h->max_n > 0 && h->n + na > h->max_n) {
-            na = (int)(h->max_n - h->n);
-            rc = hip_check(hipMemcpyAsync(h->d_nacc, &na, sizeof(int), hipMemcpyHostToDevice, h->stream),
-                           "nacc clamp");
-            if (rc)
-                return rc;
+        const int limit = h->max_n > 0 ? (int)std::min<int64_t>(h->max_n - h->n, INT32_MAX) : INT32_MAX;
+        const int ib = grid_for((int64_t)s * h->T, 256);
+        if (h->d == 2) {
+            hipLaunchKernelGGL(k_insert<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, limit, list, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
+                               h->d_pos32, h->d_filled);
+            hipLaunchKernelGGL(k_halo<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, limit);
+        } else {
+            hipLaunchKernelGGL(k_insert<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, limit, list, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
+                               h->d_pos32, h->d_filled);
+            hipLaunchKernelGGL(k_halo<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, limit);
         }
-        rc = ensure_paths(h, h->n + na);
+        rc = hip_check(hipGetLastError(), "resolve kernels");
         if (rc)
             return rc;
-        const int tb = (h->T + 63) / 64;
-        if (h->d == 2)
-            hipLaunchKernelGGL(k_insert<2>, dim3(tb), dim3(64), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
-                               h->d_nacc, list, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64, h->d_pos32,
-                               h->d_filled);
-        else
-            hipLaunchKernelGGL(k_insert<3>, dim3(tb), dim3(64), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
-                               h->d_nacc, list, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64, h->d_pos32,
-                               h->d_filled);
-        rc = hip_check(hipGetLastError(), "k_insert");
-        if (rc)
-            return rc;
-        // host mirror of the accepted indices (needed by the stop rule)
-        if (na > 0) {
-            h->acc_host.resize(h->n + na);
-            rc = hip_check(hipMemcpyAsync(h->acc_host.data() + h->n, h->d_acc_idx + h->n, (size_t)na * 8,
-                                          hipMemcpyDeviceToHost, h->stream),
+        // one round trip: accepted count, their candidate indices and the filled-cell count
+        char* pin = (char*)h->h_pinned;
+        rc = hip_check(hipMemcpyAsync(pin + 8, h->d_nacc, 4, hipMemcpyDeviceToHost, h->stream), "nacc copy");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(pin + 16, h->d_filled, 8, hipMemcpyDeviceToHost, h->stream), "filled copy");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(pin + 64, h->d_acc_idx + h->n, (size_t)s * 8, hipMemcpyDeviceToHost,
+                                          h->stream),
                            "accepted copy");
-            if (rc)
-                return rc;
-            rc = hip_check(hipStreamSynchronize(h->stream), "accepted sync");
-            if (rc)
-                return rc;
-        }
+        if (!rc)
+            rc = hip_check(hipStreamSynchronize(h->stream), "resolve sync");
+        if (rc)
+            return rc;
+        const int na = std::min(*(int*)(pin + 8), limit);
+        h->filled = *(unsigned long long*)(pin + 16);
+        const int64_t* ai = (const int64_t*)(pin + 64);
+        h->acc_host.insert(h->acc_host.end(), ai, ai + na);
         h->n += na;
         *acc_total += na;
         if (h->max_n > 0 && h->n >= h->max_n)
             return UG_OK;
         if (S > s) {
             // retest the rest against the updated table
-            int64_t rest = S - s;
-            rc = launch_cand64(h, 0, rest, list + s, h->d_flags, nullptr, false);
-            if (rc)
-                return rc;
             int64_t S2 = 0;
-            rc = compact(h, h->d_flags, rest, 0, list + s, h->d_list2, &S2);
+            rc = filter_candidates(h, 0, S - s, list + s, h->d_list2, &S2);
             if (rc)
                 return rc;
             rc = hip_check(hipMemcpyAsync(list, h->d_list2, (size_t)S2 * 8, hipMemcpyDeviceToDevice, h->stream),
