@@ -24,10 +24,23 @@ HBM_PEAK_GBS = 8000.0
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def pack_once(ugpack, seed: int, precision: int, profile: bool = False):
-    p = ugpack.Packer(2, 64, seed=seed, precision=precision,
+WORKLOADS = {
+    # headline: 2+1, F = T = 64, packed to jamming
+    "cfg2": dict(dim=2, T=64, max_attempts=0.0,
+                 text="2+1 pack to jamming, F=T=64, batches of 2^20, stop when accepts in the "
+                      "last 2^22 candidates < 1e-4*2^22"),
+    # 3+1, F = T = 128, packed to jamming
+    "cfg3": dict(dim=3, T=128, max_attempts=0.0,
+                 text="3+1 pack to jamming, F=T=128, batches of 2^20, stop when accepts in the "
+                      "last 2^22 candidates < 1e-4*2^22"),
+}
+
+
+def pack_once(ugpack, seed: int, precision: int, profile: bool = False, workload: str = "cfg2"):
+    w = WORKLOADS[workload]
+    p = ugpack.Packer(w["dim"], w["T"], seed=seed, precision=precision,
                       stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4,
-                      stop_batch=1 << 20, stop_window=1 << 22)
+                      stop_batch=1 << 20, stop_window=1 << 22, max_attempts=w["max_attempts"])
     if profile:
         p.profile(True)
     st = p.run()
@@ -56,6 +69,23 @@ def roofline_of(stats: dict, dim: int, T: int, precision_split: bool) -> dict:
     }
 
 
+def split_error(ugpack, torch, workload: str) -> dict:
+    """Max |X_split-bf16 - X_fp64| over 65,536 candidates, on the timesteps the prefilter
+    examines and on all timesteps, beside the certainty margin in use."""
+    w = WORKLOADS[workload]
+    idx = torch.arange(0, 1 << 16, dtype=torch.int64) * 104729 + 7
+    with ugpack.Packer(w["dim"], w["T"], seed=1, precision=ugpack.PREC_SPLIT_BF16) as p:
+        margin = p.status().margin
+        x16 = p.eval_candidates(idx, ugpack.PREC_SPLIT_BF16)
+        x64 = p.eval_candidates(idx, ugpack.PREC_FP64)
+    d = (x16 - x64).abs()
+    d = torch.minimum(d, 2.0 - d)
+    T = w["T"]
+    order = [i // 2 if i % 2 == 0 else T - 1 - i // 2 for i in range(T)][: min(T, 32)]
+    return {"max_err_examined": float(d[:, order, :].max()), "max_err_all_t": float(d.max()),
+            "margin_max": margin, "candidates": int(idx.numel())}
+
+
 def cpu_baseline(seconds_target: float = 12.0) -> dict:
     """The CPU oracle (serial RSA restatement, 1 thread) on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -76,8 +106,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp64", choices=["split", "fp64", "both"])
+    ap.add_argument("--precision", default="both", choices=["split", "fp64", "both"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the packer runs one universe per "
@@ -95,20 +126,22 @@ def main() -> int:
     golden = {}
     for s in range(1, 9):
         f = os.path.join(ROOT, "tests", "golden", f"cfg2_d2_T64_s{s}_jam.json")
+        if args.workload != "cfg2":
+            break
         if os.path.exists(f):
             golden[s] = json.load(open(f))["idx"]
 
     results = {}
     for prec in forms:
         for w in range(args.warmup):
-            p, _ = pack_once(ugpack, 1001 + w, prec)
+            p, _ = pack_once(ugpack, 1001 + w, prec, workload=args.workload)
             p.close()
         torch.cuda.synchronize()
         cands, accepted, per_pack, parity = 0, [], [], []
         t_start = time.perf_counter()
         for k in range(args.steps):
             t0 = time.perf_counter()
-            p, st = pack_once(ugpack, k + 1, prec)
+            p, st = pack_once(ugpack, k + 1, prec, workload=args.workload)
             torch.cuda.synchronize()
             per_pack.append(time.perf_counter() - t0)
             cands += st.attempts
@@ -120,7 +153,7 @@ def main() -> int:
         torch.cuda.synchronize()
         wall = time.perf_counter() - t_start
         # separate profiled pack (events + exact counters) for the roofline figures
-        p, st = pack_once(ugpack, 1, prec, profile=True)
+        p, st = pack_once(ugpack, 1, prec, profile=True, workload=args.workload)
         stats = p.kernel_stats(ugpack.abi.KERNEL_PREFILTER)
         rstats = p.kernel_stats(ugpack.abi.KERNEL_RECHECK)
         p.close()
@@ -143,16 +176,17 @@ def main() -> int:
         "dtype": "bf16x3+f32acc, f64 recheck" if split else "f64",
         "data": "synthetic (Philox candidate stream, seeds 1..steps)",
         "config": {
-            "workload": "2+1 pack to jamming, F=T=64, batches of 2^20, stop when accepts in the "
-                        "last 2^22 candidates < 1e-4*2^22",
-            "dim": 2, "T": 64, "seeds": list(range(1, args.steps + 1)),
+            "workload": WORKLOADS[args.workload]["text"],
+            "dim": WORKLOADS[args.workload]["dim"], "T": WORKLOADS[args.workload]["T"],
+            "seeds": list(range(1, args.steps + 1)),
             "precision": "split-bf16" if split else "fp64",
         },
         "s_to_jam_mean": round(main_r["wall"] / args.steps, 5),
         "candidates_per_pack_mean": main_r["cands"] / args.steps,
         "accepted_per_pack": main_r["accepted"],
         "parity_vs_oracle_fixtures": main_r["parity"],
-        "roofline": roofline_of(main_r["stats"], 2, 64, split),
+        "roofline": roofline_of(main_r["stats"], WORKLOADS[args.workload]["dim"],
+                                WORKLOADS[args.workload]["T"], split),
     }
     if ugpack.PREC_FP64 in results and split:
         f = results[ugpack.PREC_FP64]
@@ -162,6 +196,7 @@ def main() -> int:
         line["speedup_split_vs_fp64"] = round(main_r["value"] / f["value"], 3)
     if split:
         line["recheck"] = main_r["rstats"]
+        line["split_error"] = split_error(ugpack, torch, args.workload)
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(line))
     return 0

@@ -191,3 +191,19 @@ def test_split_config2_pack_to_jam_equals_golden(gpu, seed):
     assert st.done and st.attempts == g["candidates"]
     assert list(idx) == g["idx"] and list(gid) == g["gid"]
     assert st.rechecked < st.attempts
+
+
+# ------------------------------------------------------------------ 3+1 at config-3 scale
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_config3_prefix_equals_golden(gpu, precision):
+    """3+1, F = T = 128: the first 2^20 candidates of seed 1 (N = 39,818 accepts)."""
+    import ugpack
+
+    print(f"cfg3 prefix {precision}", flush=True)
+    g = _gold("cfg3_d3_T128_s1_prefix1M")
+    prec = ugpack.PREC_SPLIT_BF16 if precision == "split" else ugpack.PREC_FP64
+    with ugpack.Packer(3, 128, seed=1, max_attempts=g["candidates"], precision=prec) as p:
+        st = p.run()
+        idx, gid = p.accepted()
+    assert st.attempts == g["candidates"] and st.n == g["n"]
+    assert list(idx) == g["idx"] and list(gid) == g["gid"]
