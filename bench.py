@@ -47,7 +47,19 @@ def pack_once(ugpack, seed: int, precision: int, profile: bool = False, workload
     return p, st
 
 
-def roofline_of(stats: dict, dim: int, T: int, precision_split: bool) -> dict:
+def measured_traffic(workload: str, split: bool):
+    """HBM bytes per prefilter launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of the same workload (None when no summary exists)."""
+    f = os.path.join(ROOT, "profiles", f"{workload}_prefilter_hbm_traffic.md")
+    if not split or not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def roofline_of(stats: dict, dim: int, T: int, precision_split: bool, traffic=None) -> dict:
     """Algorithmic bytes of the candidate-test kernel per launch over its average duration.
 
     Per visited (candidate, timestep): 3^(d-1) rows of 3 int32 cell heads (12 B each);
@@ -61,7 +73,7 @@ def roofline_of(stats: dict, dim: int, T: int, precision_split: bool) -> dict:
     achieved = (bytes_total / launches) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return {
         "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
         "kernel": "k_fast_prefilter" if precision_split else "k_cand64<test>",
         "avg_launch_ms": round(avg_ms, 5), "launches": stats["launches"],
         "algorithmic_bytes_per_launch": round(bytes_total / launches, 1),
@@ -109,6 +121,8 @@ def main() -> int:
     ap.add_argument("--precision", default="both", choices=["split", "fp64", "both"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--calib", action="store_true",
+                    help="first copy 256 MiB device-to-device (counter calibration for rocprof --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the packer runs one universe per "
@@ -120,6 +134,13 @@ def main() -> int:
     import ugpack
 
     torch.cuda.set_device(0)
+    if args.calib:
+        src = torch.ones(1 << 26, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     prec_main = ugpack.PREC_SPLIT_BF16 if args.precision in ("split", "both") else ugpack.PREC_FP64
     forms = [prec_main] + ([ugpack.PREC_FP64] if args.precision == "both" else [])
 
@@ -186,7 +207,8 @@ def main() -> int:
         "accepted_per_pack": main_r["accepted"],
         "parity_vs_oracle_fixtures": main_r["parity"],
         "roofline": roofline_of(main_r["stats"], WORKLOADS[args.workload]["dim"],
-                                WORKLOADS[args.workload]["T"], split),
+                                WORKLOADS[args.workload]["T"], split,
+                                measured_traffic(args.workload, split)),
     }
     if ugpack.PREC_FP64 in results and split:
         f = results[ugpack.PREC_FP64]
