@@ -154,6 +154,65 @@ int32_t ug_pack_kernel_stats(const ug_pack* h, int32_t which, ug_kernel_stats* o
 /* Releases the handle and its device memory. */
 void ug_pack_destroy(ug_pack* h);
 
+/* ------------------------------------------------------------------ frame baker
+ * Replaces the frame_bake executable (cuda/frame_bake.cu:398-635), whose BRF1 output
+ * braidlab/frames.py:88-144 reads. */
+
+/* A device-resident worldline set.  Host arrays in dump order: a2[n][dim] (sin1
+ * amplitudes), amp/freq/phase[n][dim][nw] (the dump's a_j, b_j, f_j columns; freq must be
+ * integers; phase may be NULL for phase-free dumps), as load_dump reads them
+ * (cuda/frame_bake.cu:109-190) or ug_pack_params returns them. */
+typedef struct ug_paths ug_paths;
+int32_t ug_paths_create(int32_t dim, int64_t n, int32_t nw, const double* a2, const double* amp,
+                        const double* freq, const double* phase, void* hip_stream, ug_paths** out);
+void ug_paths_destroy(ug_paths* p);
+
+/* Front of one observation (the --front/--metric/--speed/--zmin/--wraps flags,
+ * cuda/frame_bake.cu:409-453). */
+typedef struct ug_front {
+    int32_t kind;   /* 0 speed, 1 fit, 2 budget */
+    int32_t cheb;   /* 1 square (Chebyshev) metric, 0 circle; budget is always square */
+    double speed;   /* conformal front speed for kind 0 */
+    double z_min;   /* rewind floor (radians) */
+    int32_t wraps;  /* torus image cap K, -1 = auto ceil((chi_max + 4)/2) */
+    int32_t reserved;
+} ug_front;
+
+typedef struct ug_observer {
+    int32_t type;  /* 0 path, 1 comoving point */
+    int32_t path;  /* path index for type 0 */
+    double px, py; /* point for type 1 */
+} ug_observer;
+
+/* One image: the BRF1 hit record (u32 src, f32 dx, f32 dy, f32 chi, f32 z_emit). */
+typedef struct ug_hit {
+    uint32_t src;
+    float dx, dy, chi, z_emit;
+} ug_hit;
+
+/* Per (instant, observer) header fields of BRF1. */
+typedef struct ug_frame_info {
+    double z_obs, chi_max;
+    int32_t K, reserved;
+    double ox, oy, betax, betay;
+    uint64_t n_hits;
+} ug_frame_info;
+
+/* Past-light-cone hits of every worldline (and torus image) for one observation, as
+ * hits_kernel + its host driver (cuda/frame_bake.cu:281-371,522-623), into caller device
+ * memory dev_hits[cap], ordered by (path, image, crossing).  When the hits exceed cap,
+ * returns UG_EOVERFLOW with info->n_hits set (cap = 0 is a size query).  2+1 only. */
+int32_t ug_frame_bake_lightcone(ug_paths* p, const ug_front* front, double z_obs,
+                                const ug_observer* obs, ug_hit* dev_hits, uint64_t cap,
+                                ug_frame_info* info);
+
+/* Equal-time slices (no reference counterpart; the frame step of the large 3+1 workload):
+ * wrapped positions of every path at z[0..nz) into dev_pos[nz][n][dim] (float, may be
+ * NULL) and raster_px^2 float density rasters of the (x, y) projection over [-1, 1)^2
+ * into dev_raster[nz][raster_px][raster_px] (may be NULL; row = y bin, column = x bin). */
+int32_t ug_frame_bake_slices(ug_paths* p, const double* z, int32_t nz, float* dev_pos,
+                             float* dev_raster, int32_t raster_px);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,117 @@
+"""GPU frame baker against the reference's bake_reference hit lists (golden fixtures) and
+the CPU oracle; BRF1 file layout; equal-time slices and rasters.
+
+Tolerance: hits are float32 records (cuda/frame_bake.cu:261-264); the GPU evaluates the
+worldlines with a rotation recurrence instead of per-term libm sin, so values agree to
+float32 resolution: atol 2e-6 on dx, dy, chi, z_emit, identical hit counts and sources."""
+
+from __future__ import annotations
+
+import math
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import bake_oracle
+from test_frames import CASES, load_case, match_hits
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_bake_equals_reference_hits(gpu, name):
+    from ugpack import frames
+
+    print(f"frames {name}", flush=True)
+    g, axes, kw = load_case(name)
+    got = frames.bake(axes, g["z_obs"], g["z_min"], **kw)
+    match_hits(got, [tuple(h) for h in g["hits"]], 2e-6)
+
+
+def test_gpu_bake_matches_oracle_on_a_packing(gpu):
+    """Worldlines of a real packing (2+1, T=32, 60 paths), fit and budget fronts."""
+    import ugpack
+    from ugpack import frames
+
+    with ugpack.Packer(2, 32, seed=3, max_n=60) as p:
+        p.run()
+        paths = frames.Paths.from_packer(p)
+        par = p.params()
+    axes = [{"a": par[:, a, 1::3], "b": par[:, a, 2::3], "f": par[:, a, 3::3], "a2": par[:, a, 0]}
+            for a in range(2)]
+    for front, kw in (("fit", dict(observer_path=5)), ("budget", dict(observer_point=(0.2, -0.4)))):
+        got = frames.bake(paths, 0.55 * math.pi, 0.1 * math.pi, front=front, **kw)
+        want = bake_oracle.bake(axes, 0.55 * math.pi, 0.1 * math.pi, front=front, **kw)
+        assert len(want) > 20
+        match_hits(got, want, 2e-6)
+
+
+def test_gpu_hits_are_ordered_deterministically(gpu):
+    from ugpack import frames
+
+    g, axes, kw = load_case("demo50_fit_path0")
+    a = frames.bake(axes, g["z_obs"], g["z_min"], **kw)
+    b = frames.bake(axes, g["z_obs"], g["z_min"], **kw)
+    assert a == b
+    assert [h[0] for h in a] == sorted(h[0] for h in a)
+
+
+def test_bake_file_writes_brf1(gpu, tmp_path):
+    from ugpack import frames
+
+    dump = tmp_path / "d.csv"
+    from test_frames import GOLD  # noqa: F401
+    dump.write_text(
+        "ax2,ay2,ax_1,bx_1,fx_1,ay_1,by_1,fy_1,ax_2,bx_2,fx_2,ay_2,by_2,fy_2,gid\n"
+        "0.31,-0.62,0.20,3,0,-0.13,2,0.8,0.10,4,1.1,0.24,5,0,0\n"
+        "-0.55,0.12,-0.33,2,0.4,0.25,4,2.1,0.08,5,0,-0.15,3,0,1\n"
+        "0.05,0.77,0.5,2,2.2,-0.5,2,0.3,0.0,3,0,0.0,4,0,1\n")
+    out = tmp_path / "f.frames"
+    n = frames.bake_file(str(dump), str(out), [("path", 0), ("point", (0.1, 0.3))],
+                         from_pi=0.3, to_pi=0.6, frames=3, front="fit")
+    raw = out.read_bytes()
+    assert raw[:4] == b"BRF1"
+    n_paths, n_obs, n_inst, front, cheb = struct.unpack_from("<5I", raw, 4)
+    assert (n_paths, n_obs, n_inst, front, cheb) == (3, 2, 3, 1, 1)
+    fs = frames.load_frames(str(out))
+    assert len(fs["frames"]) == 3 and sum(len(o["hits"]) for r in fs["frames"] for o in r) == n
+    assert fs["observers"][1][:2] == (1, -1)
+    # every instant matches a direct bake with the same settings
+    z0 = fs["frames"][0][0]["z_obs"]
+    assert z0 == pytest.approx(0.3 * math.pi)
+    hits = frames.bake(frames.Paths.from_dump(str(dump)), z0, 0.02 * math.pi, observer_path=0)
+    assert len(hits) == len(fs["frames"][0][0]["hits"])
+    # file size equals the layout's sum
+    size = 4 + 20 + 16 + 4 + 24 * 2 + 3 * (20 + 2 * 40) + 20 * n
+    assert len(raw) == size
+
+
+def test_slices_and_rasters(gpu):
+    import torch
+    import ugpack
+    from ugpack import frames
+
+    with ugpack.Packer(3, 24, seed=2, max_attempts=20000) as p:
+        p.run()
+        paths = frames.Paths.from_packer(p)
+        par = p.params()
+    n = par.shape[0]
+    z = (np.arange(16) + 0.5) * math.pi / 16
+    pos, ras = frames.slices(paths, z, raster_px=256)
+    pos, ras = pos.cpu().numpy(), ras.cpu().numpy()
+    axes = [{"a": par[:, a, 1::3], "b": par[:, a, 2::3], "f": par[:, a, 3::3], "a2": par[:, a, 0]}
+            for a in range(3)]
+    for j, zj in enumerate(z):
+        want = np.stack([np.asarray(bake_oracle._comov(axes, i, zj)) for i in range(n)])
+        want = want - 2.0 * np.floor((want + 1.0) * 0.5)
+        d = np.abs(pos[j] - want)
+        assert np.minimum(d, 2.0 - d).max() < 1e-5
+        assert ras[j].sum() == n
+        # raster = histogram of the GPU's own positions with the same bin rule
+        cx = np.clip(np.floor((pos[j][:, 0].astype(np.float64) + 1.0) * 128).astype(int), 0, 255)
+        cy = np.clip(np.floor((pos[j][:, 1].astype(np.float64) + 1.0) * 128).astype(int), 0, 255)
+        h = np.zeros((256, 256))
+        np.add.at(h, (cy, cx), 1.0)
+        assert np.abs(h - ras[j]).sum() <= 2 * max(1, int(0.002 * n))

@@ -1,0 +1,564 @@
+// ug_frames.hip -- frame baker of the packer's worldlines (2+1 past light cone, BRF1 hits,
+// and equal-time slices with density rasters).
+//
+// Reference: cuda/frame_bake.cu (grid_kernel :376-386, hits_kernel :281-371, host driver
+// :398-635) and its Python oracle braidlab/frames.py:246-362 (bake_reference).  An
+// observation (observer, z_obs) freezes a pulse centre at the observer's comoving position,
+// rewinds to z_min while a front expands (conformal chi = c (eta(z_obs) - eta(z)),
+// eta = ln tan(z/2), or the wiggle-budget reach), and images every worldline (and every
+// torus copy shifted by (2nx, 2ny), |n| <= K) where it crosses the front: a sign change of
+// dist - chi on the rewind grid refined by 24 bisection steps with fresh evaluations.
+//
+// Here worldlines are held once as dense per-frequency coefficients,
+//   X(z) = a2 + [sum_k A_k sin(kz) + B_k (cos(kz) - 1)] / sin z,
+//   A_k = sum_{b_j = k} a_j cos f_j,  B_k = sum_{b_j = k} a_j sin f_j,
+// evaluated with a half-angle rotation recurrence (sin kz = 2 s_k c_k, cos kz - 1 = -2 s_k^2,
+// (s_k, c_k) = (sin ku, cos ku), u = z/2), so no per-term sin() is needed.  The hit search
+// runs one wave per worldline with its rewind rows staged in LDS and one lane per torus
+// image; a count pass and a prefix scan give every worldline a fixed output range, so hits
+// come out in (path, image, row) order rather than atomic order.
+#include <math.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ug_internal.h"
+
+using namespace ug;
+
+namespace {
+
+struct DevPaths {
+    int n, dim, kmax;    // kmax: largest frequency; coefficient rows k = 0..kmax
+    const double* a2;    // [n][dim]
+    const double* A;     // [n][dim][kmax+1]
+    const double* B;     // [n][dim][kmax+1]
+};
+
+// Unwrapped comoving position of path p at z (x[0..dim)).
+__device__ __forceinline__ void comov(const DevPaths& P, int p, double z, double* x) {
+    const double u = 0.5 * z;
+    double s1, c1;
+    sincos(u, &s1, &c1);
+    const double sz = 2.0 * s1 * c1;
+    double acc[3] = {0.0, 0.0, 0.0};
+    double sk = 0.0, ck = 1.0;
+    const size_t base = (size_t)p * P.dim * (P.kmax + 1);
+    for (int k = 1; k <= P.kmax; k++) {
+        const double sn = sk * c1 + ck * s1;
+        const double cn = ck * c1 - sk * s1;
+        sk = sn;
+        ck = cn;
+        const double sinkz = 2.0 * sk * ck;
+        const double cosm1 = -2.0 * sk * sk;
+        for (int a = 0; a < P.dim; a++) {
+            const size_t o = base + (size_t)a * (P.kmax + 1) + k;
+            acc[a] = fma(P.A[o], sinkz, fma(P.B[o], cosm1, acc[a]));
+        }
+    }
+    for (int a = 0; a < P.dim; a++)
+        x[a] = P.a2[(size_t)p * P.dim + a] + acc[a] / sz;
+}
+
+struct FrontDev {
+    int budget, cheb, M, bmax;
+    double spd, zobs, etaobs, chimax;
+    const double* zg;     // [M+1]
+    const double* chig;   // [M+1]
+    const double* robs;   // [2 (bmax+1)]: (cos(b zobs)-1)/(b sobs), sin(b zobs)/(b sobs)
+};
+
+__device__ __forceinline__ double dist_of(double dx, double dy, int cheb) {
+    return cheb ? fmax(fabs(dx), fabs(dy)) : sqrt(dx * dx + dy * dy);
+}
+
+__device__ __forceinline__ double z_at_chi(const FrontDev& F, double chi) {
+    return 2.0 * atan(exp(F.etaobs - chi / F.spd));
+}
+
+// Wiggle-budget reach (cuda/frame_bake.cu:240-251): the largest two-time displacement of a
+// unit-budget term over frequencies 2..bmax.
+__device__ double reach_dev(const FrontDev& F, double z) {
+    double s1, c1;
+    sincos(z, &s1, &c1);
+    double sk = s1, ck = c1, best = 0.0;
+    for (int b = 2; b <= F.bmax; b++) {
+        const double sn = sk * c1 + ck * s1;
+        const double cn = ck * c1 - sk * s1;
+        sk = sn;
+        ck = cn;
+        const double da = (ck - 1.0) / (b * s1) - F.robs[2 * b];
+        const double db = sk / (b * s1) - F.robs[2 * b + 1];
+        const double d = (b % 2 == 0) ? sqrt(da * da + db * db) : fabs(db);
+        best = fmax(best, d);
+    }
+    return best;
+}
+
+// Comoving x, y of every path at every rewind row (cuda/frame_bake.cu:376-386).
+__global__ __launch_bounds__(256) void k_frame_grid(DevPaths P, int M, const double* zg,
+                                                    double2* rows) {
+    const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (tid >= (int64_t)P.n * (M + 1))
+        return;
+    const int p = (int)(tid / (M + 1)), i = (int)(tid % (M + 1));
+    double x[3];
+    comov(P, p, zg[i], x);
+    rows[tid] = make_double2(x[0], x[1]);
+}
+
+// One wave per path; lanes over torus images (nx, ny) in [-K, K]^2.  COUNT: per-path
+// number of crossings into counts[p].  Otherwise: bisect each crossing and write hits at
+// offs[p] in (image, row) order.
+template <bool COUNT>
+__global__ __launch_bounds__(64) void k_frame_hits(DevPaths P, FrontDev F, double ox, double oy,
+                                                   int K, int obs_idx, const double2* rows,
+                                                   uint32_t* counts, const uint64_t* offs,
+                                                   ug_hit* hits) {
+    extern __shared__ double2 lrows[];
+    const int p = blockIdx.x, lane = threadIdx.x;
+    const int M = F.M;
+    for (int i = lane; i <= M; i += 64)
+        lrows[i] = rows[(size_t)p * (M + 1) + i];
+    __syncthreads();
+    const int side = 2 * K + 1, nimg = side * side;
+    uint32_t cnt = 0;
+    uint64_t out = COUNT ? 0 : offs[p];
+    // lanes write in image order: each image's output position is the prefix of earlier
+    // images' counts, so the kernel loops images in rounds and scans per round
+    for (int r0 = 0; r0 < nimg; r0 += 64) {
+        const int im = r0 + lane;
+        uint32_t mine = 0;
+        uint64_t wpos = 0;
+        if (!COUNT) {
+            // count this lane's crossings first (cheap walk) to place its output
+            if (im < nimg) {
+                const int nx = im % side - K, ny = im / side - K;
+                const double bx = 2.0 * nx - ox, by = 2.0 * ny - oy;
+                const double dmin = fmax(0.0, 2.0 * fmax(fabs((double)nx), fabs((double)ny)) - 4.0);
+                const int i0 = F.budget ? 1 : max(1, (int)ceil(dmin * M / F.chimax));
+                if (dmin < F.chimax && i0 <= M) {
+                    double prev = dist_of(lrows[i0 - 1].x + bx, lrows[i0 - 1].y + by, F.cheb) - F.chig[i0 - 1];
+                    if (p == obs_idx && nx == 0 && ny == 0 && i0 == 1 && fabs(prev) < 1e-9)
+                        prev = -1e-12;
+                    else if (prev == 0.0)
+                        prev = 1e-12;
+                    for (int i = i0; i <= M; i++) {
+                        const double g = dist_of(lrows[i].x + bx, lrows[i].y + by, F.cheb) - F.chig[i];
+                        mine += ((prev < 0.0) != (g < 0.0)) ? 1u : 0u;
+                        prev = g;
+                    }
+                }
+            }
+            // exclusive prefix over lanes
+            uint32_t incl = mine;
+            for (int s = 1; s < 64; s <<= 1) {
+                const uint32_t v = __shfl_up(incl, s, 64);
+                if (lane >= s)
+                    incl += v;
+            }
+            wpos = out + (incl - mine);
+            out += __shfl(incl, 63, 64);
+        }
+        if (im < nimg) {
+            const int nx = im % side - K, ny = im / side - K;
+            const double bx = 2.0 * nx - ox, by = 2.0 * ny - oy;
+            const double dmin = fmax(0.0, 2.0 * fmax(fabs((double)nx), fabs((double)ny)) - 4.0);
+            const int i0 = F.budget ? 1 : max(1, (int)ceil(dmin * M / F.chimax));
+            if (dmin < F.chimax && i0 <= M) {
+                double prev = dist_of(lrows[i0 - 1].x + bx, lrows[i0 - 1].y + by, F.cheb) - F.chig[i0 - 1];
+                if (p == obs_idx && nx == 0 && ny == 0 && i0 == 1 && fabs(prev) < 1e-9)
+                    prev = -1e-12;
+                else if (prev == 0.0)
+                    prev = 1e-12;
+                for (int i = i0; i <= M; i++) {
+                    const double g = dist_of(lrows[i].x + bx, lrows[i].y + by, F.cheb) - F.chig[i];
+                    if ((prev < 0.0) != (g < 0.0)) {
+                        if (COUNT) {
+                            cnt++;
+                        } else {
+                            const bool inside_prev = prev < 0.0;
+                            double chi, zemit, q[3];
+                            if (F.budget) {
+                                double lo = F.zg[i - 1], hi = F.zg[i];
+                                for (int it = 0; it < 24; it++) {
+                                    const double mid = 0.5 * (lo + hi);
+                                    comov(P, p, mid, q);
+                                    const double gm = dist_of(q[0] + bx, q[1] + by, F.cheb) - reach_dev(F, mid);
+                                    if ((gm < 0.0) == inside_prev)
+                                        lo = mid;
+                                    else
+                                        hi = mid;
+                                }
+                                zemit = 0.5 * (lo + hi);
+                                chi = reach_dev(F, zemit);
+                            } else {
+                                double lo = F.chig[i - 1], hi = F.chig[i];
+                                for (int it = 0; it < 24; it++) {
+                                    const double mid = 0.5 * (lo + hi);
+                                    comov(P, p, z_at_chi(F, mid), q);
+                                    const double gm = dist_of(q[0] + bx, q[1] + by, F.cheb) - mid;
+                                    if ((gm < 0.0) == inside_prev)
+                                        lo = mid;
+                                    else
+                                        hi = mid;
+                                }
+                                chi = 0.5 * (lo + hi);
+                                zemit = z_at_chi(F, chi);
+                            }
+                            comov(P, p, zemit, q);
+                            ug_hit hrec;
+                            hrec.src = (uint32_t)p;
+                            hrec.dx = (float)(q[0] + bx);
+                            hrec.dy = (float)(q[1] + by);
+                            hrec.chi = (float)chi;
+                            hrec.z_emit = (float)zemit;
+                            hits[wpos++] = hrec;
+                        }
+                    }
+                    prev = g;
+                }
+            }
+        }
+    }
+    if (COUNT) {
+        for (int s = 32; s >= 1; s >>= 1)
+            cnt += __shfl_xor(cnt, s, 64);
+        if (lane == 0)
+            counts[p] = cnt;
+    }
+}
+
+// Equal-time slices: positions (wrapped onto [-1, 1)) of every path at the given z values,
+// optionally binned into raster_px^2 density rasters of the (x, y) projection.
+__global__ __launch_bounds__(256) void k_frame_slices(DevPaths P, const double* zs, int nz,
+                                                      float* pos, float* raster, int rpx) {
+    const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (tid >= (int64_t)nz * P.n)
+        return;
+    const int j = (int)(tid / P.n), p = (int)(tid % P.n);
+    double x[3];
+    comov(P, p, zs[j], x);
+    for (int a = 0; a < P.dim; a++)
+        x[a] = x[a] - 2.0 * floor((x[a] + 1.0) * 0.5);
+    if (pos)
+        for (int a = 0; a < P.dim; a++)
+            pos[((size_t)j * P.n + p) * P.dim + a] = (float)x[a];
+    if (raster) {
+        int cx = (int)floor((x[0] + 1.0) * 0.5 * rpx), cy = (int)floor((x[1] + 1.0) * 0.5 * rpx);
+        cx = cx < 0 ? 0 : (cx >= rpx ? rpx - 1 : cx);
+        cy = cy < 0 ? 0 : (cy >= rpx ? rpx - 1 : cy);
+        atomicAdd(&raster[((size_t)j * rpx + cy) * rpx + cx], 1.0f);
+    }
+}
+
+__global__ void k_scan_u32(const uint32_t* counts, int n, uint64_t* offs, uint64_t* total) {
+    // single block exclusive scan (n up to a few million; sequential chunks per thread)
+    __shared__ uint64_t part[256];
+    const int per = (n + 255) / 256;
+    const int b = threadIdx.x * per, e = min(n, b + per);
+    uint64_t s = 0;
+    for (int i = b; i < e; i++)
+        s += counts[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t acc = 0;
+        for (int i = 0; i < 256; i++) {
+            const uint64_t v = part[i];
+            part[i] = acc;
+            acc += v;
+        }
+        *total = acc;
+    }
+    __syncthreads();
+    uint64_t acc = part[threadIdx.x];
+    for (int i = b; i < e; i++) {
+        offs[i] = acc;
+        acc += counts[i];
+    }
+}
+
+// host-side reference formulas (cuda/frame_bake.cu:201-251), libm
+double eta_host(double z) { return std::log(std::tan(z / 2.0)); }
+
+}  // namespace
+
+struct ug_paths {
+    int dim, kmax;
+    int64_t n;
+    hipStream_t stream;
+    double *d_a2 = nullptr, *d_A = nullptr, *d_B = nullptr;
+    std::vector<double> a2, A, B;  // host copies (observer evaluation)
+    int bmax_terms = 2;            // largest frequency carried by a term (reach band)
+};
+
+namespace {
+
+void host_comov(const ug_paths* P, int p, double z, double* x, double* v) {
+    // Same dense form as the device, with libm sin/cos per frequency.
+    const double s = std::sin(z), c = std::cos(z);
+    for (int a = 0; a < P->dim; a++) {
+        double xx = 0.0, dd = 0.0;
+        for (int k = 1; k <= P->kmax; k++) {
+            const size_t o = ((size_t)p * P->dim + a) * (P->kmax + 1) + k;
+            const double skz = std::sin(k * z), ckz = std::cos(k * z);
+            xx += P->A[o] * skz + P->B[o] * (ckz - 1.0);
+            dd += k * (P->A[o] * ckz - P->B[o] * skz);
+        }
+        const double a2 = P->a2[(size_t)p * P->dim + a];
+        x[a] = a2 + xx / s;
+        // d/dz [a2 sin z + xx] / sin z, as comov_vel_xy (cuda/frame_bake.cu:215-228)
+        const double num = a2 * s + xx, dnum = a2 * c + dd;
+        if (v)
+            v[a] = (dnum * s - num * c) / (s * s);
+    }
+}
+
+}  // namespace
+
+extern "C" int32_t ug_paths_create(int32_t dim, int64_t n, int32_t nw, const double* a2, const double* amp,
+                                   const double* freq, const double* phase, void* hip_stream, ug_paths** out) {
+    if (!out || (n > 0 && (!a2 || !amp || !freq)))
+        return fail(UG_EINVAL, "ug_paths_create: NULL argument");
+    if (dim < 2 || dim > 3 || n < 0 || nw < 1)
+        return fail(UG_EINVAL, "ug_paths_create: invalid dim %d / n %lld / nw %d", dim, (long long)n, nw);
+    int kmax = 1;
+    for (int64_t i = 0; i < n * dim * nw; i++) {
+        const double b = freq[i];
+        if (!(b >= 0.0 && b <= 8192.0) || b != std::floor(b))
+            return fail(UG_EINVAL, "ug_paths_create: frequency %g is not an integer in [0, 8192]", b);
+        kmax = std::max(kmax, (int)b);
+    }
+    ug_paths* P = new ug_paths();
+    P->dim = dim;
+    P->n = n;
+    P->kmax = kmax;
+    P->stream = (hipStream_t)hip_stream;
+    const size_t W = (size_t)kmax + 1;
+    P->a2.assign(a2, a2 + (size_t)n * dim);
+    P->A.assign((size_t)n * dim * W, 0.0);
+    P->B.assign((size_t)n * dim * W, 0.0);
+    for (int64_t p = 0; p < n; p++)
+        for (int a = 0; a < dim; a++)
+            for (int j = 0; j < nw; j++) {
+                const size_t t = ((size_t)p * dim + a) * nw + j;
+                const int b = (int)freq[t];
+                const double f = phase ? phase[t] : 0.0;
+                if (b == 0 || amp[t] == 0.0)
+                    continue;
+                P->bmax_terms = std::max(P->bmax_terms, b);
+                P->A[((size_t)p * dim + a) * W + b] += amp[t] * std::cos(f);
+                P->B[((size_t)p * dim + a) * W + b] += amp[t] * std::sin(f);
+            }
+    int32_t rc = UG_OK;
+    if (hipMalloc((void**)&P->d_a2, std::max<size_t>(8, P->a2.size() * 8)) != hipSuccess ||
+        hipMalloc((void**)&P->d_A, std::max<size_t>(8, P->A.size() * 8)) != hipSuccess ||
+        hipMalloc((void**)&P->d_B, std::max<size_t>(8, P->B.size() * 8)) != hipSuccess)
+        rc = fail(UG_ENOMEM, "ug_paths_create: device allocation failed");
+    if (!rc && n > 0) {
+        rc = hip_check(hipMemcpyAsync(P->d_a2, P->a2.data(), P->a2.size() * 8, hipMemcpyHostToDevice, P->stream), "a2");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(P->d_A, P->A.data(), P->A.size() * 8, hipMemcpyHostToDevice, P->stream), "A");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(P->d_B, P->B.data(), P->B.size() * 8, hipMemcpyHostToDevice, P->stream), "B");
+        if (!rc)
+            rc = hip_check(hipStreamSynchronize(P->stream), "paths sync");
+    }
+    if (rc) {
+        ug_paths_destroy(P);
+        return rc;
+    }
+    *out = P;
+    return UG_OK;
+}
+
+extern "C" void ug_paths_destroy(ug_paths* P) {
+    if (!P)
+        return;
+    if (P->d_a2)
+        (void)hipFree(P->d_a2);
+    if (P->d_A)
+        (void)hipFree(P->d_A);
+    if (P->d_B)
+        (void)hipFree(P->d_B);
+    delete P;
+}
+
+static DevPaths dev_of(const ug_paths* P) {
+    DevPaths D;
+    D.n = (int)P->n;
+    D.dim = P->dim;
+    D.kmax = P->kmax;
+    D.a2 = P->d_a2;
+    D.A = P->d_A;
+    D.B = P->d_B;
+    return D;
+}
+
+extern "C" int32_t ug_frame_bake_lightcone(ug_paths* P, const ug_front* fr, double z_obs, const ug_observer* obs,
+                                           ug_hit* dev_hits, uint64_t cap, ug_frame_info* info) {
+    if (!P || !fr || !obs || !info)
+        return fail(UG_EINVAL, "ug_frame_bake_lightcone: NULL argument");
+    if (P->dim != 2)
+        return fail(UG_EINVAL, "error: the light-cone baker is 2+1 only (cuda/frame_bake.cu:124-127)");
+    if (fr->kind < 0 || fr->kind > 2)
+        return fail(UG_EINVAL, "error: front must be speed|fit|budget");
+    if (obs->type == 0 && (obs->path < 0 || obs->path >= P->n))
+        return fail(UG_EINVAL, "error: observer path %d out of range", obs->path);
+    const double z_min = fr->z_min;
+    if (!(z_min < z_obs))
+        return fail(UG_EINVAL, "error: rewind floor %.4f >= z_obs %.4f", z_min, z_obs);
+    const bool budget = fr->kind == 2;
+    const int bmax = P->bmax_terms;
+    FrontDev F;
+    F.budget = budget ? 1 : 0;
+    F.cheb = (budget || fr->cheb) ? 1 : 0;
+    F.zobs = z_obs;
+    F.etaobs = eta_host(z_obs);
+    F.spd = fr->kind == 0 ? std::max(1e-3, fr->speed) : -1.0 / eta_host(z_min);
+    F.bmax = bmax;
+    F.M = budget ? std::min(4000, std::max(420, 2 * bmax)) : 420;
+    const int M = F.M;
+    std::vector<double> zg(M + 1), chig(M + 1), robs(2 * (size_t)(bmax + 1), 0.0);
+    const double sobs = std::sin(z_obs);
+    for (int b = 2; b <= bmax; b++) {
+        robs[2 * b] = (std::cos(b * z_obs) - 1.0) / (b * sobs);
+        robs[2 * b + 1] = std::sin(b * z_obs) / (b * sobs);
+    }
+    auto reach_host = [&](double z) {
+        const double s = std::sin(z);
+        double best = 0.0;
+        for (int b = 2; b <= bmax; b++) {
+            const double da = (std::cos(b * z) - 1.0) / (b * s) - robs[2 * b];
+            const double db = std::sin(b * z) / (b * s) - robs[2 * b + 1];
+            const double d = (b % 2 == 0) ? std::sqrt(da * da + db * db) : std::fabs(db);
+            best = std::max(best, d);
+        }
+        return best;
+    };
+    if (budget) {
+        F.chimax = 0.0;
+        for (int i = 0; i <= M; i++) {
+            zg[i] = z_obs - (z_obs - z_min) * i / M;
+            chig[i] = reach_host(zg[i]);
+            F.chimax = std::max(F.chimax, chig[i]);
+        }
+    } else {
+        F.chimax = F.spd * (F.etaobs - eta_host(z_min));
+        for (int i = 0; i <= M; i++) {
+            chig[i] = F.chimax * i / M;
+            zg[i] = 2.0 * std::atan(std::exp(F.etaobs - chig[i] / F.spd));
+        }
+    }
+    const int K = fr->wraps >= 0 ? fr->wraps : (int)std::ceil((F.chimax + 4.0) / 2.0);
+    double ox, oy, bx = 0.0, by = 0.0;
+    if (obs->type == 0) {
+        double x[3], v[3];
+        host_comov(P, obs->path, z_obs, x, v);
+        ox = x[0];
+        oy = x[1];
+        bx = std::sin(z_obs) * v[0] / F.spd;
+        by = std::sin(z_obs) * v[1] / F.spd;
+    } else {
+        ox = obs->px;
+        oy = obs->py;
+    }
+    info->z_obs = z_obs;
+    info->chi_max = F.chimax;
+    info->K = K;
+    info->ox = ox;
+    info->oy = oy;
+    info->betax = bx;
+    info->betay = by;
+    info->n_hits = 0;
+    if (P->n == 0)
+        return UG_OK;
+    hipStream_t st = P->stream;
+    double *dzg = nullptr, *dchig = nullptr, *drobs = nullptr;
+    double2* drows = nullptr;
+    uint32_t* dcnt = nullptr;
+    uint64_t *doff = nullptr, *dtot = nullptr;
+    int32_t rc = UG_OK;
+    if (hipMalloc((void**)&dzg, (M + 1) * 8) != hipSuccess || hipMalloc((void**)&dchig, (M + 1) * 8) != hipSuccess ||
+        hipMalloc((void**)&drobs, robs.size() * 8) != hipSuccess ||
+        hipMalloc((void**)&drows, (size_t)P->n * (M + 1) * 16) != hipSuccess ||
+        hipMalloc((void**)&dcnt, (size_t)P->n * 4) != hipSuccess ||
+        hipMalloc((void**)&doff, (size_t)P->n * 8) != hipSuccess || hipMalloc((void**)&dtot, 8) != hipSuccess)
+        rc = fail(UG_ENOMEM, "ug_frame_bake_lightcone: device allocation failed");
+    if (!rc)
+        rc = hip_check(hipMemcpyAsync(dzg, zg.data(), (M + 1) * 8, hipMemcpyHostToDevice, st), "zg");
+    if (!rc)
+        rc = hip_check(hipMemcpyAsync(dchig, chig.data(), (M + 1) * 8, hipMemcpyHostToDevice, st), "chig");
+    if (!rc)
+        rc = hip_check(hipMemcpyAsync(drobs, robs.data(), robs.size() * 8, hipMemcpyHostToDevice, st), "robs");
+    F.zg = dzg;
+    F.chig = dchig;
+    F.robs = drobs;
+    const DevPaths D = dev_of(P);
+    const int obs_idx = obs->type == 0 ? obs->path : -1;
+    uint64_t total = 0;
+    if (!rc) {
+        const int64_t nr = P->n * (int64_t)(M + 1);
+        hipLaunchKernelGGL(k_frame_grid, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, D, M, dzg, drows);
+        const size_t lds = (size_t)(M + 1) * 16;
+        hipLaunchKernelGGL(k_frame_hits<true>, dim3((unsigned)P->n), dim3(64), lds, st, D, F, ox, oy, K, obs_idx,
+                           drows, dcnt, doff, dev_hits);
+        hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(256), 0, st, dcnt, (int)P->n, doff, dtot);
+        rc = hip_check(hipGetLastError(), "frame count");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(&total, dtot, 8, hipMemcpyDeviceToHost, st), "total");
+        if (!rc)
+            rc = hip_check(hipStreamSynchronize(st), "frame sync");
+        info->n_hits = total;
+        if (!rc && total > cap)
+            rc = fail(UG_EOVERFLOW, "ug_frame_bake_lightcone: %llu hits exceed the capacity %llu",
+                      (unsigned long long)total, (unsigned long long)cap);
+        if (!rc && total > 0) {
+            hipLaunchKernelGGL(k_frame_hits<false>, dim3((unsigned)P->n), dim3(64), lds, st, D, F, ox, oy, K,
+                               obs_idx, drows, dcnt, doff, dev_hits);
+            rc = hip_check(hipGetLastError(), "frame hits");
+            if (!rc)
+                rc = hip_check(hipStreamSynchronize(st), "frame sync");
+        }
+    }
+    (void)hipFree(dzg);
+    (void)hipFree(dchig);
+    (void)hipFree(drobs);
+    (void)hipFree(drows);
+    (void)hipFree(dcnt);
+    (void)hipFree(doff);
+    (void)hipFree(dtot);
+    return rc;
+}
+
+extern "C" int32_t ug_frame_bake_slices(ug_paths* P, const double* z, int32_t nz, float* dev_pos, float* dev_raster,
+                                        int32_t raster_px) {
+    if (!P || !z || nz < 0 || (dev_raster && raster_px <= 0))
+        return fail(UG_EINVAL, "ug_frame_bake_slices: invalid argument");
+    if (nz == 0 || P->n == 0)
+        return UG_OK;
+    for (int j = 0; j < nz; j++)
+        if (!(z[j] > 0.0 && z[j] < 3.141592653589793))
+            return fail(UG_EINVAL, "ug_frame_bake_slices: z[%d] = %g outside (0, pi)", j, z[j]);
+    hipStream_t st = P->stream;
+    double* dz = nullptr;
+    int32_t rc = UG_OK;
+    if (hipMalloc((void**)&dz, (size_t)nz * 8) != hipSuccess)
+        return fail(UG_ENOMEM, "ug_frame_bake_slices: allocation failed");
+    rc = hip_check(hipMemcpyAsync(dz, z, (size_t)nz * 8, hipMemcpyHostToDevice, st), "z upload");
+    if (!rc && dev_raster)
+        rc = hip_check(hipMemsetAsync(dev_raster, 0, (size_t)nz * raster_px * raster_px * 4, st), "raster clear");
+    if (!rc) {
+        const int64_t tot = (int64_t)nz * P->n;
+        hipLaunchKernelGGL(k_frame_slices, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, dev_of(P), dz, nz,
+                           dev_pos, dev_raster, raster_px);
+        rc = hip_check(hipGetLastError(), "k_frame_slices");
+    }
+    if (!rc)
+        rc = hip_check(hipStreamSynchronize(st), "slices sync");
+    (void)hipFree(dz);
+    return rc;
+}

@@ -54,6 +54,21 @@ class KernelStats(C.Structure):
 
 KERNEL_PREFILTER, KERNEL_RECHECK = 0, 1
 
+
+class Front(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("cheb", C.c_int32), ("speed", C.c_double),
+                ("z_min", C.c_double), ("wraps", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Observer(C.Structure):
+    _fields_ = [("type", C.c_int32), ("path", C.c_int32), ("px", C.c_double), ("py", C.c_double)]
+
+
+class FrameInfo(C.Structure):
+    _fields_ = [("z_obs", C.c_double), ("chi_max", C.c_double), ("K", C.c_int32),
+                ("reserved", C.c_int32), ("ox", C.c_double), ("oy", C.c_double),
+                ("betax", C.c_double), ("betay", C.c_double), ("n_hits", C.c_uint64)]
+
 _lib = None
 
 
@@ -81,6 +96,11 @@ def lib() -> C.CDLL:
         "ug_pack_profile": ([vp, i32], i32),
         "ug_pack_kernel_stats": ([vp, i32, C.POINTER(KernelStats)], i32),
         "ug_pack_destroy": ([vp], None),
+        "ug_paths_create": ([i32, i64, i32, vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
+        "ug_paths_destroy": ([vp], None),
+        "ug_frame_bake_lightcone": ([vp, C.POINTER(Front), C.c_double, C.POINTER(Observer), vp,
+                                     C.c_uint64, C.POINTER(FrameInfo)], i32),
+        "ug_frame_bake_slices": ([vp, vp, i32, vp, vp, i32], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -97,7 +117,8 @@ def exported_symbols() -> list[str]:
         "ug_abi_version", "ug_last_error", "ug_pack_create", "ug_pack_run", "ug_pack_run_sub",
         "ug_pack_get_status", "ug_pack_accepted", "ug_pack_params", "ug_pack_positions",
         "ug_pack_curve", "ug_eval_candidates", "ug_pack_profile", "ug_pack_kernel_stats",
-        "ug_pack_destroy",
+        "ug_pack_destroy", "ug_paths_create", "ug_paths_destroy", "ug_frame_bake_lightcone",
+        "ug_frame_bake_slices",
     ]
 
 
