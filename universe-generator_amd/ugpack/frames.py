@@ -77,11 +77,16 @@ class Paths:
 
     @classmethod
     def from_axes(cls, axes) -> "Paths":
-        """From per-axis objects with .a/.b/.f (n, nw) and .a2 (n,) (braidlab AxisTerms)."""
-        a2 = np.stack([np.asarray(ax.a2, float) for ax in axes], 1)
-        amp = np.stack([np.asarray(ax.a, float) for ax in axes], 1)
-        freq = np.stack([np.asarray(ax.b, float) for ax in axes], 1)
-        phase = np.stack([np.asarray(ax.f, float) for ax in axes], 1)
+        """From per-axis term arrays a/b/f (n, nw) and a2 (n,): braidlab AxisTerms objects
+        (braidlab/corrdim.py:49-63) or dicts with those keys."""
+
+        def get(ax, k):
+            return np.asarray(ax[k] if isinstance(ax, dict) else getattr(ax, k), float)
+
+        a2 = np.stack([get(ax, "a2") for ax in axes], 1)
+        amp = np.stack([get(ax, "a") for ax in axes], 1)
+        freq = np.stack([get(ax, "b") for ax in axes], 1)
+        phase = np.stack([get(ax, "f") for ax in axes], 1)
         return cls(a2, amp, freq, phase)
 
     @classmethod
