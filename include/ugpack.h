@@ -75,7 +75,7 @@ typedef struct ug_pack_status {
     int64_t n;         /* accepted unique worldlines (the done: line's N) */
     int64_t nsub;      /* accepted subpaths */
     int32_t done;      /* 1 once the stop rule (or budget / max_n) has fired */
-    int32_t reserved;
+    int32_t sub_done;  /* the subpath phase met its stop rule */
     int64_t survivors; /* candidates that reached the in-batch accept */
     int64_t rechecked; /* candidates settled by the FP64 recheck (split-BF16 form) */
     double margin;     /* certainty margin in use (split-BF16 form) */
@@ -99,10 +99,22 @@ int32_t ug_pack_create(const ug_pack_config* cfg, void* hip_stream, ug_pack** ou
  * max_candidates more candidates have been consumed (0 = no limit).  st may be NULL. */
 int32_t ug_pack_run(ug_pack* h, int64_t max_candidates, ug_pack_status* st);
 
-/* Subpath phase (cuda/braid_cuda.cu:949-1043, --subpaths): tests candidates continuing
- * the index stream for count more candidates with the adoption rule of host_sub_gid
- * (:726-750); dim 2 only, like the reference (braidlab/config.py:56-57). */
-int32_t ug_pack_run_sub(ug_pack* h, int32_t sub_terms, int64_t count, ug_pack_status* st);
+/* Subpath phase flags (cuda/braid_cuda.cu:572-579). */
+typedef struct ug_sub_config {
+    int32_t sub_terms;   /* --sub-terms (0 = --terms; other values are rejected) */
+    int32_t reserved;
+    double max_attempts; /* --sub-attempts (0 = the --attempts budget again) */
+    double accept_rate;  /* --sub-until-accept-rate (0 = off) */
+    double fill_rate;    /* --sub-until-fill-rate (0 = off) */
+} ug_sub_config;
+
+/* Subpath phase (cuda/braid_cuda.cu:949-1043, --subpaths): continues the candidate index
+ * stream after the unique phase; a candidate joins the single group it touches (contacts
+ * with the table and with earlier subpaths, host_sub_gid :726-750), touching none or two
+ * groups rejects.  Ends the unique phase.  Stops on the budget or the accept-rate /
+ * fill-rate windows (:956-1037), or after max_candidates more candidates (0 = no limit). */
+int32_t ug_pack_run_sub(ug_pack* h, const ug_sub_config* cfg, int64_t max_candidates,
+                        ug_pack_status* st);
 
 /* Current status without running. */
 int32_t ug_pack_get_status(const ug_pack* h, ug_pack_status* st);

@@ -52,8 +52,8 @@ def lib() -> C.CDLL:
                               C.POINTER(C.c_int64)]
         L.or_pack.restype = C.c_int64
         L.or_pack_sub.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
-                                  C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
-                                  C.POINTER(C.c_int64)]
+                                  C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                  C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.or_pack_sub.restype = C.c_int64
         L.or_check_rsa.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
         L.or_check_rsa.restype = C.c_int64
@@ -181,19 +181,23 @@ class Model:
             out = out + (pos[:n].copy(),)
         return out
 
-    def pack_sub(self, pos0: np.ndarray, gid0: np.ndarray, first: int, count: int,
-                 cap: int = 1 << 20):
+    def pack_sub(self, pos0: np.ndarray, gid0: np.ndarray, first: int, budget: float,
+                 accept_rate: float = 0.0, fill_rate: float = 0.0, cap: int = 1 << 20):
+        """Subpath phase after the uniques pos0 (n0,T,d) with groups gid0: returns
+        (idx int64[Nsub], gid int32[Nsub], subpath candidates drawn, newly filled cells)."""
         pos0 = np.ascontiguousarray(pos0, np.float64)
         gid0 = np.ascontiguousarray(gid0, np.int32)
         idx = np.zeros(cap, np.int64)
         gid = np.zeros(cap, np.int32)
+        att = C.c_int64(0)
         filled = C.c_int64(0)
-        n = lib().or_pack_sub(self.ref, _p(pos0), _p(gid0), len(gid0), first, count, _p(idx),
-                              _p(gid), cap, C.byref(filled))
+        n = lib().or_pack_sub(self.ref, _p(pos0), _p(gid0), len(gid0), first, float(budget),
+                              float(accept_rate), float(fill_rate), _p(idx), _p(gid), cap,
+                              C.byref(att), C.byref(filled))
         if n < 0:
             raise RuntimeError(f"or_pack_sub failed ({n})")
         n = min(n, cap)
-        return idx[:n].copy(), gid[:n].copy(), filled.value
+        return idx[:n].copy(), gid[:n].copy(), att.value, filled.value
 
     def check_rsa(self, M: int, acc: np.ndarray) -> int:
         acc = np.ascontiguousarray(acc, np.int64)

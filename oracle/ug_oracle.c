@@ -584,8 +584,71 @@ int64_t or_pack(const or_model* m, int64_t max_candidates, int stop_kind, int64_
     return ret;
 }
 
+/* Touched group of candidate c against the packing P: -1 none, -2 two or more. */
+static int32_t sub_group(const or_model* m, const packing* P, cand_buf* cb, uint64_t c) {
+    or_candidate(m, c, cb->a1, cb->b, cb->c, cb->s, NULL, NULL);
+    int32_t g = -1;
+    for (int oi = 0; oi < m->T; oi++) {
+        int t = m->order[oi];
+        or_eval_t(m, t, cb->a1, cb->b, cb->c, cb->s, cb->X + (size_t)t * m->d);
+        if (pk_scan(P, t, cb->X + (size_t)t * m->d, 1, &g))
+            return -2;
+    }
+    return g;
+}
+
+typedef struct {
+    int64_t* att;
+    int64_t* val;
+    int64_t n, head, cap, sum_att, sum_val, target;
+} rate_window;
+
+/* One windowed-rate stop (cuda/braid_cuda.cu:1008-1037); returns 1 when it fires. */
+static int window_push(rate_window* w, int64_t att, int64_t val, int64_t sub_att, double rate) {
+    if (!w->target)
+        return 0;
+    if (w->n == w->cap) {
+        int64_t nc = w->cap ? 2 * w->cap : 64;
+        int64_t* a = (int64_t*)malloc((size_t)nc * 8);
+        int64_t* v = (int64_t*)malloc((size_t)nc * 8);
+        for (int64_t k = 0; k < w->n; k++) {
+            a[k] = w->att[(w->head + k) % w->cap];
+            v[k] = w->val[(w->head + k) % w->cap];
+        }
+        free(w->att);
+        free(w->val);
+        w->att = a;
+        w->val = v;
+        w->head = 0;
+        w->cap = nc;
+    }
+    int64_t tail = (w->head + w->n) % w->cap;
+    w->att[tail] = att;
+    w->val[tail] = val;
+    w->n++;
+    w->sum_att += att;
+    w->sum_val += val;
+    while (w->n > 1 && w->sum_att - w->att[w->head] >= w->target) {
+        w->sum_att -= w->att[w->head];
+        w->sum_val -= w->val[w->head];
+        w->head = (w->head + 1) % w->cap;
+        w->n--;
+    }
+    return w->sum_att >= w->target && sub_att > 1000000 &&
+           (double)w->sum_val / (double)w->sum_att < rate;
+}
+
+static int64_t window_target(double rate) {
+    if (rate <= 0)
+        return 0;
+    double t = 30.0 / rate;
+    t = t < 2e7 ? 2e7 : (t > 5e9 ? 5e9 : t);
+    return (int64_t)t;
+}
+
 int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, int64_t n0,
-                    uint64_t first, int64_t count, int64_t* idx, int32_t* gid, int64_t cap,
+                    uint64_t first, double budget, double accept_rate, double fill_rate,
+                    int64_t* idx, int32_t* gid, int64_t cap, int64_t* attempts,
                     int64_t* filled_new) {
     packing P;
     cand_buf cb;
@@ -594,39 +657,100 @@ int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, 
         cb_free(&cb);
         return -2;
     }
-    const int T = m->T, d = m->d;
-    int64_t ret = 0, nsub = 0, fresh = 0;
+    const int64_t surv_cap = 1 << 16; /* the reference's survivor buffer at KMAX_WIGGLE > 16 */
+    int64_t ret = 0, nsub = 0, fresh = 0, att = 0, batch = 1 << 16;
+    uint64_t* surv = NULL;
+    int64_t surv_n = 0, surv_alloc = 0, flag_alloc = 0;
+    char* flag = NULL;
+    rate_window aw = {0}, fw = {0};
+    aw.target = window_target(accept_rate);
+    fw.target = window_target(fill_rate);
     for (int64_t i = 0; i < n0; i++)
-        if (pk_insert(&P, pos0 + (size_t)i * T * d, gid0[i]) < 0)
+        if (pk_insert(&P, pos0 + (size_t)i * m->T * m->d, gid0[i]) < 0)
             ret = -2;
-    for (int64_t k = 0; k < count && ret == 0; k++) {
-        uint64_t c = first + (uint64_t)k;
-        or_candidate(m, c, cb.a1, cb.b, cb.c, cb.s, NULL, NULL);
-        int32_t g = -1;
-        int bad = 0;
-        for (int oi = 0; oi < T && !bad; oi++) {
-            int t = m->order[oi];
-            or_eval_t(m, t, cb.a1, cb.b, cb.c, cb.s, cb.X + (size_t)t * d);
-            bad = pk_scan(&P, t, cb.X + (size_t)t * d, 1, &g);
+    while (ret == 0 && (double)att < budget) {
+        /* round-start test: exactly one group touched in the packing as it stands (the
+         * packing is read-only here, so candidates are tested in parallel) */
+        if (batch > flag_alloc) {
+            free(flag);
+            flag = (char*)malloc((size_t)batch);
+            flag_alloc = batch;
+            if (!flag) {
+                ret = -2;
+                break;
+            }
         }
-        if (bad || g < 0)
-            continue;
-        int64_t f = pk_insert(&P, cb.X, g);
-        if (f < 0) {
+        int par_err = 0;
+#pragma omp parallel reduction(| : par_err)
+        {
+            cand_buf tb;
+            if (cb_init(&tb, m)) {
+                par_err = 1;
+            } else {
+#pragma omp for schedule(dynamic, 256)
+                for (int64_t k = 0; k < batch; k++)
+                    flag[k] = sub_group(m, &P, &tb, first + (uint64_t)(att + k)) >= 0;
+                cb_free(&tb);
+            }
+        }
+        if (par_err) {
             ret = -2;
             break;
         }
-        fresh += f;
-        if (nsub < cap) {
-            if (idx)
-                idx[nsub] = (int64_t)c;
-            if (gid)
-                gid[nsub] = g;
+        surv_n = 0;
+        for (int64_t k = 0; k < batch; k++) {
+            if (!flag[k])
+                continue;
+            if (surv_n == surv_alloc) {
+                surv_alloc = surv_alloc ? 2 * surv_alloc : 1024;
+                surv = (uint64_t*)realloc(surv, (size_t)surv_alloc * 8);
+            }
+            surv[surv_n++] = first + (uint64_t)(att + k);
         }
-        nsub++;
+        att += batch;
+        /* admission in index order against the up-to-date packing */
+        int64_t admitted = 0, round_new = 0;
+        for (int64_t s = 0; s < surv_n; s++) {
+            int32_t g = sub_group(m, &P, &cb, surv[s]);
+            if (g < 0)
+                continue;
+            int64_t f = pk_insert(&P, cb.X, g);
+            if (f < 0) {
+                ret = -2;
+                break;
+            }
+            round_new += f;
+            if (nsub < cap) {
+                if (idx)
+                    idx[nsub] = (int64_t)surv[s];
+                if (gid)
+                    gid[nsub] = g;
+            }
+            nsub++;
+            admitted++;
+        }
+        fresh += round_new;
+        int64_t got = surv_n < surv_cap ? surv_n : surv_cap;
+        if (got > 512 && batch > 4096)
+            batch /= 2;
+        else if (got < 64 && batch < ((int64_t)1 << 26))
+            batch *= 2;
+        /* the windows record the adapted batch, as the reference does */
+        if (window_push(&aw, batch, admitted, att, accept_rate))
+            break;
+        if (window_push(&fw, batch, round_new, att, fill_rate))
+            break;
     }
+    if (attempts)
+        *attempts = att;
     if (filled_new)
         *filled_new = fresh;
+    free(surv);
+    free(flag);
+    free(aw.att);
+    free(aw.val);
+    free(fw.att);
+    free(fw.val);
     pk_free(&P);
     cb_free(&cb);
     return ret ? ret : nsub;

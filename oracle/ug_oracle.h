@@ -70,11 +70,18 @@ int64_t or_pack(const or_model* m, int64_t max_candidates, int stop_kind, int64_
                 int64_t window, double stop_rate, int64_t max_n, int64_t* idx, int32_t* gid,
                 double* pos, int64_t cap, int64_t* attempts);
 
-/* Serial RSA continuing an existing packing: subpath phase.  Candidates
- * [first, first+count) join the single group they touch (host_sub_gid,
- * cuda/braid_cuda.cu:726-750); uniques are given by pos0[n0][T][d], gid0[n0]. */
+/* Subpath phase continuing an existing packing (cuda/braid_cuda.cu:949-1043).  Rounds of
+ * `batch` candidates from index `first` on: a candidate survives the round when, against
+ * the packing as it stood at the round's start, it touches exactly one group (collides_dev
+ * sub mode, :463-464); survivors are then taken in index order and admitted into the
+ * single group they touch in the up-to-date packing (host_sub_gid, :726-750).  batch
+ * starts at 2^16 and adapts on min(survivors, 2^16) (:1000-1003); rounds run while fewer
+ * than `budget` subpath candidates were drawn; accept_rate / fill_rate > 0 arm the
+ * windowed stops of :1008-1037.  Uniques are pos0[n0][T][d] with groups gid0[n0].
+ * Returns the number of admitted subpaths (idx/gid hold the first cap), or < 0. */
 int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, int64_t n0,
-                    uint64_t first, int64_t count, int64_t* idx, int32_t* gid, int64_t cap,
+                    uint64_t first, double budget, double accept_rate, double fill_rate,
+                    int64_t* idx, int32_t* gid, int64_t cap, int64_t* attempts,
                     int64_t* filled_new);
 
 /* Brute-force check that acc[0..na) is exactly the serial RSA output over candidate

@@ -33,6 +33,52 @@ CASES = {
     "cfg3_d3_T128_s1_prefix1M": (3, 128, 128, 1, False, 1 << 20, 0),
 }
 
+# Subpath phase with a windowed stop (these need >= 2e7 subpath candidates, minutes of
+# oracle time): name -> (d, T, seed, unique-phase candidates, accept_rate, fill_rate).
+# Stored as counts, a digest of the lists and their first/last entries.
+SUB_CASES = {
+    "sub_d2_T16_s1_accept2e-3": (2, 16, 1, 20000, 2e-3, 0.0),
+    "sub_d2_T16_s2_fill3e-5": (2, 16, 2, 20000, 0.0, 3e-5),
+}
+
+
+def list_digest(idx, gid) -> str:
+    import hashlib
+
+    import numpy as np
+
+    h = hashlib.sha256()
+    h.update(np.asarray(idx, "<i8").tobytes())
+    h.update(np.asarray(gid, "<i4").tobytes())
+    return h.hexdigest()
+
+
+def make_sub(name: str) -> dict:
+    import numpy as np
+
+    d, T, seed, uniq, ar, fr = SUB_CASES[name]
+    m = pyoracle.Model(d, T, seed=seed)
+    idx, gid, att = m.pack(uniq)
+    pos = np.stack([m.positions(int(i)) for i in idx])
+    t0 = time.time()
+    sidx, sgid, satt, fresh = m.pack_sub(pos, gid, first=att, budget=1e15, accept_rate=ar, fill_rate=fr)
+    dt = time.time() - t0
+    all_idx = list(idx) + list(sidx)
+    all_gid = list(gid) + list(sgid)
+    rec = {
+        "name": name, "dim": d, "T": T, "seed": seed, "unique_candidates": uniq,
+        "accept_rate": ar, "fill_rate": fr, "n": int(len(idx)), "nsub": int(len(sidx)),
+        "attempts": int(att + satt), "sub_attempts": int(satt), "filled_new": int(fresh),
+        "digest": list_digest(all_idx, all_gid),
+        "head_idx": [int(x) for x in sidx[:64]], "head_gid": [int(x) for x in sgid[:64]],
+        "tail_idx": [int(x) for x in sidx[-64:]], "tail_gid": [int(x) for x in sgid[-64:]],
+        "oracle_seconds": round(dt, 2),
+    }
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(rec, f)
+    print(f"{name}: N={len(idx)} Nsub={len(sidx)} sub candidates={satt} ({dt:.1f} s)")
+    return rec
+
 
 def make(name: str) -> dict:
     d, T, terms, seed, pin, cand, stop = CASES[name]
@@ -53,5 +99,5 @@ def make(name: str) -> dict:
 
 
 if __name__ == "__main__":
-    for n in sys.argv[1:] or list(CASES):
-        make(n)
+    for n in sys.argv[1:] or list(CASES) + list(SUB_CASES):
+        make_sub(n) if n in SUB_CASES else make(n)

@@ -207,3 +207,93 @@ def test_config3_prefix_equals_golden(gpu, precision):
         idx, gid = p.accepted()
     assert st.attempts == g["candidates"] and st.n == g["n"]
     assert list(idx) == g["idx"] and list(gid) == g["gid"]
+
+
+def _sub_reference(d, T, seed, unique_cands, sub_budget, terms=0, accept_rate=0.0, fill_rate=0.0):
+    """Unique phase then subpath phase on the oracle."""
+    m = po.Model(d, T, terms=terms or None, seed=seed)
+    idx, gid, att = m.pack(unique_cands)
+    pos = np.stack([m.positions(int(i)) for i in idx])
+    sidx, sgid, satt, fresh = m.pack_sub(pos, gid, first=att, budget=sub_budget,
+                                         accept_rate=accept_rate, fill_rate=fill_rate)
+    return list(idx) + list(sidx), list(gid) + list(sgid), att, satt
+
+
+@pytest.mark.parametrize("d,T,seed,unique,sub", [
+    (2, 16, 1, 20000, 200000), (2, 16, 4, 3000, 400000), (2, 32, 2, 30000, 300000),
+    (3, 12, 1, 5000, 100000), (2, 64, 3, 200000, 200000),
+])
+def test_subpath_phase_equals_oracle(gpu, d, T, seed, unique, sub):
+    import ugpack
+
+    print(f"subpaths d={d} T={T} seed={seed} unique={unique} sub={sub}", flush=True)
+    want_idx, want_gid, att, satt = _sub_reference(d, T, seed, unique, sub)
+    with ugpack.Packer(d, T, seed=seed, max_attempts=unique) as p:
+        st = p.run()
+        assert st.attempts == att
+        st = p.run_sub(max_attempts=sub)
+        idx, gid = p.accepted()
+    assert st.sub_done and st.attempts == att + satt
+    assert st.n + st.nsub == len(want_idx)
+    assert list(idx) == want_idx and list(gid) == want_gid
+
+
+def test_subpath_phase_in_pieces(gpu):
+    import ugpack
+
+    want_idx, want_gid, att, satt = _sub_reference(2, 16, 3, 8000, 500000)
+    with ugpack.Packer(2, 16, seed=3, max_attempts=8000) as p:
+        p.run()
+        for _ in range(5):
+            p.run_sub(max_attempts=500000, max_candidates=40000)
+        st = p.run_sub(max_attempts=500000)
+        idx, gid = p.accepted()
+    assert st.sub_done and st.attempts == att + satt
+    assert list(idx) == want_idx and list(gid) == want_gid
+
+
+def test_subpath_phase_split_precision(gpu):
+    import ugpack
+
+    want_idx, want_gid, _, _ = _sub_reference(2, 32, 5, 20000, 200000)
+    with ugpack.Packer(2, 32, seed=5, max_attempts=20000, precision=ugpack.PREC_SPLIT_BF16) as p:
+        p.run()
+        p.run_sub(max_attempts=200000)
+        idx, gid = p.accepted()
+    assert list(idx) == want_idx and list(gid) == want_gid
+
+
+def test_subpath_phase_rules(gpu):
+    import ugpack
+
+    with ugpack.Packer(2, 16, seed=1, max_attempts=2000) as p:
+        p.run()
+        with pytest.raises(ugpack.UgError):
+            p.run_sub(sub_terms=3)  # --sub-terms other than --terms is unsupported
+        st = p.run_sub(max_attempts=1000)
+        assert st.sub_done and st.attempts == 2000 + (1 << 16)
+        with pytest.raises(ugpack.UgError):
+            p.run()  # the unique phase ended where the subpath phase started
+
+
+@pytest.mark.parametrize("name", ["sub_d2_T16_s1_accept2e-3", "sub_d2_T16_s2_fill3e-5"])
+def test_subpath_windowed_stop_equals_golden(gpu, name):
+    import hashlib
+
+    import ugpack
+
+    print(f"subpath window {name}", flush=True)
+    g = _gold(name)
+    with ugpack.Packer(g["dim"], g["T"], seed=g["seed"], max_attempts=g["unique_candidates"]) as p:
+        p.run()
+        st = p.run_sub(max_attempts=1e15, accept_rate=g["accept_rate"], fill_rate=g["fill_rate"])
+        idx, gid = p.accepted()
+    assert st.sub_done and st.n == g["n"] and st.nsub == g["nsub"]
+    assert st.attempts == g["attempts"]
+    sidx, sgid = list(idx[st.n:]), list(gid[st.n:])
+    assert sidx[:64] == g["head_idx"] and sgid[:64] == g["head_gid"]
+    assert sidx[-64:] == g["tail_idx"] and sgid[-64:] == g["tail_gid"]
+    h = hashlib.sha256()
+    h.update(np.asarray(idx, "<i8").tobytes())
+    h.update(np.asarray(gid, "<i4").tobytes())
+    assert h.hexdigest() == g["digest"]

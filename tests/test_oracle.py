@@ -329,25 +329,83 @@ def test_max_n_stops_at_the_nth_accept():
     assert len(idx) == 10 and att == idx[-1] + 1
 
 
-def test_subpath_rule_adopts_a_single_group():
+def _touch_groups(X, others, groups, r):
+    """Groups of the paths in others (n, T, d) that X (T, d) touches at some timestep."""
+    if len(others) == 0:
+        return set()
+    d = np.abs(X[None] - others)
+    d = np.minimum(d, 2.0 - d)
+    hit = np.any(np.all(d <= r, axis=2), axis=1)
+    return {int(g) for g in np.asarray(groups)[hit]}
+
+
+def test_subpath_first_round_by_brute_force():
+    # Round 1 of the subpath phase (2^16 candidates) restated with dense numpy distances:
+    # survivors touch exactly one group among the uniques; then, in index order, a survivor
+    # joins iff it touches exactly one group among uniques + subpaths admitted so far.
     m = po.Model(2, 16, seed=1)
-    idx, gid, att, pos = m.pack(20000, want_pos=True)
-    sidx, sgid, filled = m.pack_sub(pos, gid, first=att, count=20000)
-    assert len(sidx) > 0 and np.all(sgid >= 0) and np.all(sgid < len(idx))
-    # each subpath touches only its adopted group among the paths before it
-    X0 = {int(i): m.positions(int(i)) for i in idx}
+    idx, gid, att = m.pack(20000)
+    pos = np.stack([m.positions(int(i)) for i in idx])
+    sidx, sgid, satt, _ = m.pack_sub(pos, gid, first=att, budget=1)
+    assert satt == 1 << 16  # one full round however small the budget
     r = m.r
-    for k in range(min(len(sidx), 15)):
-        Xs = m.positions(int(sidx[k]))
-        groups = set()
-        for i, g in zip(idx, gid):
-            d = np.abs(Xs - X0[int(i)])
-            d = np.minimum(d, 2.0 - d)
-            if np.any(np.all(d <= r, axis=1)):
-                groups.add(int(g))
-        for kk in range(k):
-            d = np.abs(Xs - m.positions(int(sidx[kk])))
-            d = np.minimum(d, 2.0 - d)
-            if np.any(np.all(d <= r, axis=1)):
-                groups.add(int(sgid[kk]))
-        assert groups == {int(sgid[k])}
+    surv = []
+    for c in range(att, att + (1 << 16)):
+        X = m.positions(c)
+        g = _touch_groups(X, pos, gid, r)
+        if len(g) == 1:
+            surv.append((c, X))
+    want_idx, want_gid, apos, agid = [], [], list(pos), list(gid)
+    for c, X in surv:
+        g = _touch_groups(X, np.array(apos), agid, r)
+        if len(g) == 1:
+            (gg,) = g
+            want_idx.append(c)
+            want_gid.append(gg)
+            apos.append(X)
+            agid.append(gg)
+    assert len(want_idx) > 10
+    assert list(sidx) == want_idx and list(sgid) == want_gid
+
+
+def test_subpath_round_start_rule_differs_from_serial():
+    # A candidate whose only contact is a subpath admitted earlier in the same round is
+    # dropped (the round-start test sees only the uniques), so the subpath list is not a
+    # serial RSA over single-group contacts; check that at least one such candidate exists
+    # in the first round and that it is absent from the output.
+    m = po.Model(2, 16, seed=1)
+    idx, gid, att = m.pack(20000)
+    pos = np.stack([m.positions(int(i)) for i in idx])
+    sidx, sgid, _, _ = m.pack_sub(pos, gid, first=att, budget=1)
+    spos = np.stack([m.positions(int(i)) for i in sidx])
+    first_round = set(range(att, att + (1 << 16)))
+    found = 0
+    for c in range(att, att + (1 << 16)):
+        if c in set(sidx.tolist()):
+            continue
+        X = m.positions(c)
+        if _touch_groups(X, pos, gid, m.r):
+            continue
+        earlier = sidx < c
+        if len(_touch_groups(X, spos[earlier], sgid[earlier], m.r)) == 1:
+            found += 1
+            assert c in first_round
+            if found >= 3:
+                break
+    assert found >= 1
+
+
+def test_subpath_round_schedule_and_budget():
+    # rounds run while fewer than budget candidates were drawn; round sizes follow the
+    # survivor-driven halving/doubling from 2^16, so attempts land on a sum of round sizes
+    m = po.Model(2, 16, seed=2)
+    idx, gid, att = m.pack(5000)
+    pos = np.stack([m.positions(int(i)) for i in idx])
+    _, _, a1, _ = m.pack_sub(pos, gid, first=att, budget=1)
+    _, _, a2, _ = m.pack_sub(pos, gid, first=att, budget=(1 << 16) + 1)
+    assert a1 == 1 << 16
+    assert a2 - a1 in (1 << 15, 1 << 16, 1 << 17)
+    sidx, sgid, a3, filled = m.pack_sub(pos, gid, first=att, budget=a2)
+    assert a3 == a2 and filled > 0 and np.all(np.diff(sidx) > 0)
+    assert np.all(sidx >= att) and np.all(sidx < att + a3)
+

@@ -124,6 +124,11 @@ void release(Pack* h) {
     free_dev(h->d_acc_list);
     free_dev(h->d_nacc);
     free_dev(h->d_ctr);
+    free_dev(h->d_gflag);
+    free_dev(h->d_glist);
+    free_dev(h->d_glist2);
+    free_dev(h->d_gsurv);
+    free_dev(h->d_rscount);
     if (h->ev0)
         (void)hipEventDestroy(h->ev0);
     if (h->ev1)
@@ -321,7 +326,7 @@ static void fill_status(const Pack* h, ug_pack_status* st) {
     st->n = h->n;
     st->nsub = h->nsub;
     st->done = h->done ? 1 : 0;
-    st->reserved = 0;
+    st->sub_done = h->sub_done ? 1 : 0;
     st->survivors = h->survivors;
     st->rechecked = h->rechecked;
     st->margin = h->margin;
@@ -352,6 +357,8 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
     if (!hp)
         return fail(UG_EINVAL, "ug_pack_run: NULL handle");
     Pack* h = (Pack*)hp;
+    if (h->d_gflag)
+        return fail(UG_ESTATE, "ug_pack_run: the subpath phase has started");
     const auto t0 = std::chrono::steady_clock::now();
     const ug_pack_config& cfg = h->cfg;
     const int64_t budget = (int64_t)std::min(cfg.max_attempts, 9.2e18);
@@ -546,11 +553,103 @@ extern "C" int32_t ug_pack_kernel_stats(const ug_pack* hp, int32_t which, ug_ker
     return UG_OK;
 }
 
-extern "C" int32_t ug_pack_run_sub(ug_pack* hp, int32_t sub_terms, int64_t count, ug_pack_status* st) {
-    (void)sub_terms;
-    (void)count;
-    if (!hp)
-        return fail(UG_EINVAL, "ug_pack_run_sub: NULL handle");
-    fill_status((Pack*)hp, st);
-    return fail(UG_ESTATE, "ug_pack_run_sub: subpath phase not available in this build");
+extern "C" int32_t ug_pack_run_sub(ug_pack* hp, const ug_sub_config* sc, int64_t max_candidates,
+                                   ug_pack_status* st) {
+    if (!hp || !sc)
+        return fail(UG_EINVAL, "ug_pack_run_sub: NULL argument");
+    Pack* h = (Pack*)hp;
+    if (sc->sub_terms != 0 && sc->sub_terms != h->cfg.terms)
+        return fail(UG_EINVAL, "error: --sub-terms %d differs from --terms %d (not supported)", sc->sub_terms,
+                    h->cfg.terms);
+    if (h->d_gflag == nullptr) {
+        if (hipMalloc((void**)&h->d_gflag, (size_t)UG_SUB_MAX * 4) != hipSuccess ||
+            hipMalloc((void**)&h->d_glist, (size_t)UG_SUB_MAX * 4) != hipSuccess ||
+            hipMalloc((void**)&h->d_glist2, (size_t)UG_SUB_MAX * 4) != hipSuccess ||
+            hipMalloc((void**)&h->d_gsurv, (size_t)UG_S_MAX * 4) != hipSuccess ||
+            hipMalloc((void**)&h->d_rscount, 16) != hipSuccess)
+            return fail(UG_ENOMEM, "ug_pack_run_sub: allocation failed");
+        h->done = true;  // the unique phase ends where the subpath phase starts
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const double budget = sc->max_attempts > 0 ? sc->max_attempts : h->cfg.max_attempts;
+    const int64_t call_end = max_candidates > 0 ? h->sub_attempts + max_candidates : INT64_MAX;
+    // windowed stops of cuda/braid_cuda.cu:956-1037 (30 events, clamped to [2e7, 5e9] attempts)
+    const int64_t a_target = sc->accept_rate > 0 ? (int64_t)std::min(5e9, std::max(2e7, 30.0 / sc->accept_rate)) : 0;
+    const int64_t f_target = sc->fill_rate > 0 ? (int64_t)std::min(5e9, std::max(2e7, 30.0 / sc->fill_rate)) : 0;
+    int32_t rc = UG_OK;
+    int* rs_host = (int*)((char*)h->h_pinned + 24);
+    // One round = sub_batch candidates tested against the packing as it stood when the round
+    // began (rows < sub_nlimit), survivors admitted in index order.  Rounds are never split
+    // between calls, so results do not depend on max_candidates.
+    while (!h->sub_done && rc == UG_OK) {
+        if ((double)h->sub_attempts >= budget) {
+            h->sub_done = true;
+            break;
+        }
+        if (h->sub_attempts >= call_end)
+            break;
+        const int64_t round = h->sub_batch;
+        h->sub_nlimit = h->n + h->nsub;
+        const unsigned long long filled0 = h->filled;
+        rc = hip_check(hipMemsetAsync(h->d_rscount, 0, 4, h->stream), "rscount clear");
+        int64_t adm = 0;
+        for (int64_t off = 0; off < round && rc == UG_OK; off += UG_SUB_MAX) {
+            const int64_t piece = std::min<int64_t>(UG_SUB_MAX, round - off);
+            int64_t S = 0, a = 0;
+            rc = sub_filter(h, (uint64_t)(h->attempts + off), piece, nullptr, h->d_list, h->d_glist, &S,
+                            h->d_rscount);
+            if (rc == UG_OK)
+                rc = resolve_sub(h, h->d_list, h->d_glist, S, &a);
+            adm += a;
+        }
+        if (rc == UG_OK)
+            rc = hip_check(hipMemcpyAsync(rs_host, h->d_rscount, 4, hipMemcpyDeviceToHost, h->stream),
+                           "rscount copy");
+        if (rc == UG_OK)
+            rc = hip_check(hipStreamSynchronize(h->stream), "round sync");
+        if (rc)
+            break;
+        const int64_t got = std::min<int64_t>(*rs_host, 1 << 16);  // survCap at KMAX_WIGGLE > 16
+        h->survivors += *rs_host;
+        h->attempts += round;
+        h->sub_attempts += round;
+        const int64_t fresh = (int64_t)(h->filled - filled0);
+        if (got > 512 && h->sub_batch > 4096)
+            h->sub_batch /= 2;
+        else if (got < 64 && h->sub_batch < (int64_t(1) << 26))
+            h->sub_batch *= 2;
+        // the windows record the adapted round size, as cuda/braid_cuda.cu:1009-1010 does
+        if (a_target) {
+            h->awin.push_back({h->sub_batch, adm});
+            h->a_att += h->sub_batch;
+            h->a_adm += adm;
+            while (h->awin.size() > 1 && h->a_att - h->awin.front().first >= a_target) {
+                h->a_att -= h->awin.front().first;
+                h->a_adm -= h->awin.front().second;
+                h->awin.pop_front();
+            }
+            if (h->a_att >= a_target && h->sub_attempts > 1000000 &&
+                (double)h->a_adm / (double)h->a_att < sc->accept_rate)
+                h->sub_done = true;
+        }
+        if (f_target && !h->sub_done) {
+            h->fwin.push_back({h->sub_batch, fresh});
+            h->f_att += h->sub_batch;
+            h->f_new += fresh;
+            while (h->fwin.size() > 1 && h->f_att - h->fwin.front().first >= f_target) {
+                h->f_att -= h->fwin.front().first;
+                h->f_new -= h->fwin.front().second;
+                h->fwin.pop_front();
+            }
+            if (h->f_att >= f_target && h->sub_attempts > 1000000 &&
+                (double)h->f_new / (double)h->f_att < sc->fill_rate)
+                h->sub_done = true;
+        }
+        rc = record_curve(h);
+    }
+    if (rc == UG_OK)
+        rc = hip_check(hipStreamSynchronize(h->stream), "run_sub sync");
+    h->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    fill_status(h, st);
+    return rc;
 }

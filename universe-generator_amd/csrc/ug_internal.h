@@ -72,6 +72,18 @@ struct Pack {
     uint64_t* d_mask = nullptr;
     int* d_acc_list = nullptr;
     int* d_nacc = nullptr;
+    // subpath phase
+    int* d_gflag = nullptr;   // per-candidate table group of a sub-batch
+    int* d_glist = nullptr;   // groups of the survivors list
+    int* d_glist2 = nullptr;
+    int* d_gsurv = nullptr;     // adopted groups of a resolve chunk
+    int* d_rscount = nullptr;   // survivors of a round against its round-start packing
+    bool sub_done = false;
+    // sub_batch: the round size, adapted as cuda/braid_cuda.cu:952,1000-1003;
+    // sub_nlimit: rows stored before the current round started
+    int64_t sub_attempts = 0, sub_batch = 1 << 16, sub_nlimit = 0;
+    std::deque<std::pair<int64_t, int64_t>> awin, fwin;
+    int64_t a_att = 0, a_adm = 0, f_att = 0, f_new = 0;
     void* h_pinned = nullptr;
     // split-BF16 scratch
     void* d_basis16 = nullptr;
@@ -101,7 +113,12 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
 int32_t timed_begin(Pack* h);
 int32_t timed_end(Pack* h, int which, int64_t items);
 int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, const int64_t* list,
-                int64_t* out, int64_t* S, const int* dcount = nullptr, int slot = 1, bool sync = true);
+                int64_t* out, int64_t* S, const int* dcount = nullptr, int slot = 1, bool sync = true,
+                const int* vin = nullptr, int* vout = nullptr);
+// subpath phase (ug_pack.hip)
+int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out, int* gout,
+                   int64_t* S, int* rs_count);
+int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_total);
 int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out,
                           int64_t* S);
 int32_t ensure_paths(Pack* h, int64_t need);

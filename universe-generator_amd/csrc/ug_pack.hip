@@ -437,6 +437,189 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
     }
 }
 
+// Subpath test (cuda/braid_cuda.cu:452-465 sub branch, host_sub_gid :726-750): every
+// timestep is scanned; the contacted groups are reduced across lanes with min/max, and the
+// candidate fails as soon as two distinct groups appear.  Paths stored at rows < nlimit are
+// the packing at the start of the current round.  flags[i] = 1 iff the candidate touches
+// exactly one group in the table and at least one of its contacts is a round-start path
+// (the reference's device test sees only the round-start grid and drops candidates that
+// touch nothing there, :463-464); gid_out[i] = the touched group.  When rs_count is given,
+// it counts the candidates the round-start grid alone lets through (the reference's
+// survivor count, which drives its batch size).
+template <int D, int PL>
+__global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int64_t count,
+                                                 const int64_t* list, const int32_t* acc_gid,
+                                                 int64_t nlimit, int* rs_count, uint8_t* flags,
+                                                 int* gid_out) {
+    extern __shared__ __align__(16) int dyn_lds[];
+    const int lane = threadIdx.x;
+    const int T = m.T, nw = m.nw;
+    int* sb = dyn_lds;
+    uint32_t* used = (uint32_t*)(sb + D * nw);
+    const int lds_head = (D * nw + (m.modmax + 2 + 31) / 32 + 3) / 4 * 4;
+    double2* cs = (double2*)(dyn_lds + lds_head);
+    double* pos = (double*)(cs + D * nw);
+    __shared__ int tsl[64];
+    constexpr int R = D == 2 ? 3 : 9;
+    constexpr int PER_ROUND = 64 / R;
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
+        WaveCoef<D, PL> w;
+        gen_wave64<D, PL>(m, idx, lane, w, sb, used);
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int ii = 0; ii < PL; ii++)
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const int j = 2 * (lane + 64 * ii) + k;
+                    if (j < nw) {
+                        cs[a * nw + j] = make_double2(w.c[a][ii][k], w.s[a][ii][k]);
+                        sb[a * nw + j] = w.b[a][ii][k];
+                    }
+                }
+        __syncthreads();
+        // all contacts: (gcur, bad); contacts with round-start paths: (gcur0, bad0)
+        int gcur = -1, gcur0 = -1;
+        bool bad = false, bad0 = false;
+        for (int ps = 0; ps < T && !bad0 && !(bad && !rs_count); ps += 64) {
+            const int oi = ps + lane;
+            const int pend = min(64, T - ps);
+            if (oi < T) {
+                const int t = m.order[oi];
+                double X[D];
+                eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
+                tsl[lane] = t;
+#pragma unroll
+                for (int a = 0; a < D; a++)
+                    pos[lane * D + a] = X[a];
+            }
+            __syncthreads();
+            for (int r0 = 0; r0 < pend && !bad0 && !(bad && !rs_count); r0 += PER_ROUND) {
+                const int G = min(PER_ROUND, pend - r0);
+                const int sl = lane / R, rr = lane % R;
+                int gmin = INT32_MAX, gmax = -1, gmin0 = INT32_MAX, gmax0 = -1;
+                if (sl < G) {
+                    const int t = tsl[r0 + sl];
+                    double x[D];
+                    int c[D];
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                        x[a] = pos[(r0 + sl) * D + a];
+                        c[a] = cell_of(x[a], m.halfT, T);
+                    }
+                    int64_t row;
+                    if (D == 2) {
+                        row = ((int64_t)t * T + (c[0] + rr - 1 + T) % T) * m.rowlen;
+                    } else {
+                        const int gx = (c[0] + rr / 3 - 1 + T) % T;
+                        const int gy = (c[1] + rr % 3 - 1 + T) % T;
+                        row = (((int64_t)t * T + gx) * T + gy) * m.rowlen;
+                    }
+                    const int* tr = m.tab + row + c[D - 1];
+                    const int hd[3] = {tr[0], tr[1], tr[2]};
+                    for (int k = 0; k < 3; k++)
+                        for (int n = hd[k]; n >= 0;) {
+                            const size_t q = (size_t)n * T + t;
+                            const double* P = m.pos64 + q * D;
+                            bool hh = true;
+#pragma unroll
+                            for (int a = 0; a < D; a++)
+                                hh = hh && fabs(torus_delta(x[a] - P[a])) <= m.r;
+                            if (hh) {
+                                const int g = acc_gid[n];
+                                gmin = min(gmin, g);
+                                gmax = max(gmax, g);
+                                if (n < nlimit) {
+                                    gmin0 = min(gmin0, g);
+                                    gmax0 = max(gmax0, g);
+                                }
+                            }
+                            n = __float_as_int(m.pos32[q].w);
+                        }
+                }
+                for (int s = 32; s >= 1; s >>= 1) {
+                    gmin = min(gmin, __shfl_xor(gmin, s, 64));
+                    gmax = max(gmax, __shfl_xor(gmax, s, 64));
+                    gmin0 = min(gmin0, __shfl_xor(gmin0, s, 64));
+                    gmax0 = max(gmax0, __shfl_xor(gmax0, s, 64));
+                }
+                if (gmax >= 0) {
+                    if (gmin != gmax || (gcur >= 0 && gcur != gmin))
+                        bad = true;
+                    else
+                        gcur = gmin;
+                }
+                if (gmax0 >= 0) {
+                    if (gmin0 != gmax0 || (gcur0 >= 0 && gcur0 != gmin0))
+                        bad0 = true;
+                    else
+                        gcur0 = gmin0;
+                }
+            }
+            __syncthreads();
+        }
+        if (lane == 0) {
+            flags[i] = (bad || gcur0 < 0) ? 0 : 1;
+            gid_out[i] = gcur;
+            if (rs_count && !bad0 && gcur0 >= 0)
+                atomicAdd(rs_count, 1);
+        }
+        __syncthreads();
+    }
+}
+
+// Sequential adoption over the chunk's survivors in index order (one wave): survivor j's
+// groups are its table group (gin[j], -1 if none) plus the groups of earlier accepted
+// survivors it collides with (conflict bits); it is accepted iff exactly one group results,
+// and carries that group (cuda/braid_cuda.cu:984-999).
+__global__ __launch_bounds__(64) void k_greedy_sub(const uint64_t* mask, int S, const int* gin,
+                                                   int* gsurv, int* acc_list, int* nacc) {
+    __shared__ uint64_t accw[UG_MASK_WORDS];
+    __shared__ int gacc[UG_S_MAX];
+    const int lane = threadIdx.x;
+    if (lane < UG_MASK_WORDS)
+        accw[lane] = 0ull;
+    __syncthreads();
+    int na = 0;
+    for (int j = 0; j < S; j++) {
+        int gmin = INT32_MAX, gmax = -1;
+        const int nwd = (j + 63) / 64;
+        if (lane < nwd) {
+            uint64_t bits = mask[(size_t)j * UG_MASK_WORDS + lane] & accw[lane];
+            while (bits) {
+                const int b = __ffsll((long long)bits) - 1;
+                bits &= bits - 1;
+                const int g = gacc[lane * 64 + b];
+                gmin = min(gmin, g);
+                gmax = max(gmax, g);
+            }
+        }
+        for (int s = 32; s >= 1; s >>= 1) {
+            gmin = min(gmin, __shfl_xor(gmin, s, 64));
+            gmax = max(gmax, __shfl_xor(gmax, s, 64));
+        }
+        const int g0 = gin[j];
+        if (g0 >= 0) {
+            gmin = min(gmin, g0);
+            gmax = max(gmax, g0);
+        }
+        const bool ok = gmax >= 0 && gmin == gmax;
+        if (lane == 0) {
+            gacc[j] = ok ? gmax : -1;
+            gsurv[j] = ok ? gmax : -1;
+            if (ok) {
+                accw[j >> 6] |= 1ull << (j & 63);
+                acc_list[na] = j;
+            }
+        }
+        na += ok ? 1 : 0;
+        __syncthreads();
+    }
+    if (lane == 0)
+        *nacc = na;
+}
+
 // =====================================================================================
 // ordered compaction: out[k] = candidate index of the k-th flagged item (stable)
 constexpr int CMP_ITEMS = 4096;  // items per block (256 threads x 16)
@@ -467,7 +650,8 @@ __global__ __launch_bounds__(256) void k_compact_count(const uint8_t* flags, int
 __global__ __launch_bounds__(256) void k_compact_scatter(const uint8_t* flags, int64_t count,
                                                          const int* dcount, uint64_t base,
                                                          const int64_t* list, const int* bcount,
-                                                         int nblocks, int64_t* out, int* total) {
+                                                         int nblocks, int64_t* out, int* total,
+                                                         const int* vin, int* vout) {
     __shared__ int red[256];
     if (dcount)
         count = min(count, (int64_t)*dcount);
@@ -501,8 +685,11 @@ __global__ __launch_bounds__(256) void k_compact_scatter(const uint8_t* flags, i
     int o = boff + scan[threadIdx.x] - n;
     for (int k = 0; k < 16; k++) {
         const int64_t i = i0 + k;
-        if (i < count && flags[i])
+        if (i < count && flags[i]) {
+            if (vin)
+                vout[o] = vin[i];
             out[o++] = list ? list[i] : (int64_t)(base + (uint64_t)i);
+        }
     }
     if (blockIdx.x == nblocks - 1 && threadIdx.x == 255)
         *total = boff + scan[255];
@@ -592,8 +779,8 @@ __global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, int*
 template <int D>
 __global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, const int* acc_list,
                                                 const int* nacc, int limit, const int64_t* surv_idx,
-                                                int64_t n0, int64_t* acc_idx, int32_t* acc_gid,
-                                                double* pos64, float4* pos32,
+                                                const int* surv_gid, int64_t n0, int64_t* acc_idx,
+                                                int32_t* acc_gid, double* pos64, float4* pos32,
                                                 unsigned long long* filled) {
     const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int na = min(*nacc, limit);
@@ -621,7 +808,7 @@ __global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, c
         pos32[o] = make_float4(f[0], f[1], f[2], __int_as_float(prev));
         if (t == 0) {
             acc_idx[n] = surv_idx[s];
-            acc_gid[n] = (int32_t)n;
+            acc_gid[n] = surv_gid ? surv_gid[s] : (int32_t)n;  // a unique's group is its admission index
         }
     }
     const uint64_t fm = __ballot(fresh);
@@ -750,7 +937,8 @@ int32_t hip_check(hipError_t e, const char* what) {
 // Ordered compaction of flags[count] (count bounded by *dcount when given) into out; the
 // number kept lands in h->d_count[slot] and, when sync is set, in *S.
 int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, const int64_t* list,
-                int64_t* out, int64_t* S, const int* dcount, int slot, bool sync) {
+                int64_t* out, int64_t* S, const int* dcount, int slot, bool sync, const int* vin,
+                int* vout) {
     const int nb = (int)((count + CMP_ITEMS - 1) / CMP_ITEMS);
     if (nb > h->cmp_blocks_cap)
         return fail(UG_EOVERFLOW, "compaction of %lld items exceeds the batch capacity",
@@ -762,7 +950,7 @@ int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, con
     }
     hipLaunchKernelGGL(k_compact_count, dim3(nb), dim3(256), 0, h->stream, flags, count, dcount, h->d_bcount);
     hipLaunchKernelGGL(k_compact_scatter, dim3(nb), dim3(256), 0, h->stream, flags, count, dcount, base, list,
-                       h->d_bcount, nb, out, h->d_count + slot);
+                       h->d_bcount, nb, out, h->d_count + slot, vin, vout);
     int32_t rc = hip_check(hipGetLastError(), "compact");
     if (rc || !sync)
         return rc;
@@ -836,11 +1024,12 @@ int32_t ensure_paths(Pack* h, int64_t need) {
         (void)hipFree(ag);
         return fail(UG_ENOMEM, "cannot grow path storage to %lld paths", (long long)nc);
     }
-    if (h->n > 0) {
-        (void)hipMemcpyAsync(p64, h->d_pos64, (size_t)h->n * T * D * 8, hipMemcpyDeviceToDevice, h->stream);
-        (void)hipMemcpyAsync(p32, h->d_pos32, (size_t)h->n * T * 16, hipMemcpyDeviceToDevice, h->stream);
-        (void)hipMemcpyAsync(ai, h->d_acc_idx, (size_t)h->n * 8, hipMemcpyDeviceToDevice, h->stream);
-        (void)hipMemcpyAsync(ag, h->d_acc_gid, (size_t)h->n * 4, hipMemcpyDeviceToDevice, h->stream);
+    const size_t rows = (size_t)(h->n + h->nsub);  // uniques, then subpaths
+    if (rows > 0) {
+        (void)hipMemcpyAsync(p64, h->d_pos64, rows * T * D * 8, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(p32, h->d_pos32, rows * T * 16, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(ai, h->d_acc_idx, rows * 8, hipMemcpyDeviceToDevice, h->stream);
+        (void)hipMemcpyAsync(ag, h->d_acc_gid, rows * 4, hipMemcpyDeviceToDevice, h->stream);
     }
     int32_t rc = hip_check(hipStreamSynchronize(h->stream), "grow copy");
     (void)hipFree(h->d_pos64);
@@ -888,13 +1077,13 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         const int ib = grid_for((int64_t)s * h->T, 256);
         if (h->d == 2) {
             hipLaunchKernelGGL(k_insert<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
-                               h->d_nacc, limit, list, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
+                               h->d_nacc, limit, list, nullptr, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
                                h->d_pos32, h->d_filled);
             hipLaunchKernelGGL(k_halo<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
                                h->d_nacc, limit);
         } else {
             hipLaunchKernelGGL(k_insert<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
-                               h->d_nacc, limit, list, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
+                               h->d_nacc, limit, list, nullptr, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
                                h->d_pos32, h->d_filled);
             hipLaunchKernelGGL(k_halo<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
                                h->d_nacc, limit);
@@ -931,6 +1120,115 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
                 return rc;
             rc = hip_check(hipMemcpyAsync(list, h->d_list2, (size_t)S2 * 8, hipMemcpyDeviceToDevice, h->stream),
                            "list copy");
+            if (rc)
+                return rc;
+            S = S2;
+        } else {
+            S = 0;
+        }
+    }
+    return UG_OK;
+}
+
+// Candidates -> survivors of the subpath test, with their table groups (gout).
+int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out, int* gout,
+                   int64_t* S, int* rs_count) {
+    if (count <= 0) {
+        *S = 0;
+        return UG_OK;
+    }
+    const int blocks = (int)std::min<int64_t>(count, 256 * 32);
+    const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
+    const int pl = h->npairs <= 64 ? 1 : (h->npairs <= 128 ? 2 : 4);
+#define UG_SUBT(DD, PP)                                                                                      \
+    hipLaunchKernelGGL((k_sub_test<DD, PP>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, list, \
+                       h->d_acc_gid, h->sub_nlimit, rs_count, h->d_flags, h->d_gflag)
+    if (h->d == 2) {
+        if (pl == 1)
+            UG_SUBT(2, 1);
+        else if (pl == 2)
+            UG_SUBT(2, 2);
+        else
+            UG_SUBT(2, 4);
+    } else {
+        if (pl == 1)
+            UG_SUBT(3, 1);
+        else if (pl == 2)
+            UG_SUBT(3, 2);
+        else
+            UG_SUBT(3, 4);
+    }
+#undef UG_SUBT
+    int32_t rc = hip_check(hipGetLastError(), "k_sub_test");
+    if (!rc)
+        rc = compact(h, h->d_flags, count, base, list, out, S, nullptr, 1, true, h->d_gflag, gout);
+    return rc;
+}
+
+// Subpath survivors list[0..S) with table groups gl[]: adoption in index order, chunk by
+// chunk, later survivors retested against the table after each chunk's insertion.
+int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_total) {
+    int32_t rc;
+    *acc_total = 0;
+    while (S > 0) {
+        const int s = (int)std::min<int64_t>(S, UG_S_MAX);
+        const int64_t n0 = h->n + h->nsub;
+        rc = ensure_paths(h, n0 + s);
+        if (rc)
+            return rc;
+        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, nullptr, 0);
+        if (rc)
+            return rc;
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((int64_t)s * UG_MASK_WORDS, 256)), dim3(256), 0, h->stream,
+                           h->d_mask, (int64_t)s * UG_MASK_WORDS);
+        const int64_t npairs = (int64_t)s * (s - 1) / 2;
+        if (npairs > 0) {
+            if (h->d == 2)
+                hipLaunchKernelGGL(k_conflicts<2>, dim3(grid_for(npairs, 256)), dim3(256), 0, h->stream, h->dm,
+                                   h->d_spos, s, h->d_mask);
+            else
+                hipLaunchKernelGGL(k_conflicts<3>, dim3(grid_for(npairs, 256)), dim3(256), 0, h->stream, h->dm,
+                                   h->d_spos, s, h->d_mask);
+        }
+        hipLaunchKernelGGL(k_greedy_sub, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, gl, h->d_gsurv,
+                           h->d_acc_list, h->d_nacc);
+        const int ib = grid_for((int64_t)s * h->T, 256);
+        if (h->d == 2) {
+            hipLaunchKernelGGL(k_insert<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, INT32_MAX, list, h->d_gsurv, n0, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
+                               h->d_pos32, h->d_filled);
+            hipLaunchKernelGGL(k_halo<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, INT32_MAX);
+        } else {
+            hipLaunchKernelGGL(k_insert<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, INT32_MAX, list, h->d_gsurv, n0, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
+                               h->d_pos32, h->d_filled);
+            hipLaunchKernelGGL(k_halo<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
+                               h->d_nacc, INT32_MAX);
+        }
+        rc = hip_check(hipGetLastError(), "sub resolve kernels");
+        char* pin = (char*)h->h_pinned;
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(pin + 8, h->d_nacc, 4, hipMemcpyDeviceToHost, h->stream), "nacc copy");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(pin + 16, h->d_filled, 8, hipMemcpyDeviceToHost, h->stream), "filled copy");
+        if (!rc)
+            rc = hip_check(hipStreamSynchronize(h->stream), "sub resolve sync");
+        if (rc)
+            return rc;
+        const int na = *(int*)(pin + 8);
+        h->filled = *(unsigned long long*)(pin + 16);
+        h->nsub += na;
+        *acc_total += na;
+        if (S > s) {
+            int64_t S2 = 0;
+            rc = sub_filter(h, 0, S - s, list + s, h->d_list2, h->d_glist2, &S2, nullptr);
+            if (!rc && S2 > 0)
+                rc = hip_check(hipMemcpyAsync(list, h->d_list2, (size_t)S2 * 8, hipMemcpyDeviceToDevice, h->stream),
+                               "list copy");
+            if (!rc && S2 > 0)
+                rc = hip_check(hipMemcpyAsync(gl, h->d_glist2, (size_t)S2 * 4, hipMemcpyDeviceToDevice, h->stream),
+                               "gid copy");
             if (rc)
                 return rc;
             S = S2;

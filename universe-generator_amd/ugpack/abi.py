@@ -42,9 +42,14 @@ class PackConfig(C.Structure):
 class PackStatus(C.Structure):
     _fields_ = [
         ("attempts", C.c_int64), ("n", C.c_int64), ("nsub", C.c_int64), ("done", C.c_int32),
-        ("reserved", C.c_int32), ("survivors", C.c_int64), ("rechecked", C.c_int64),
+        ("sub_done", C.c_int32), ("survivors", C.c_int64), ("rechecked", C.c_int64),
         ("margin", C.c_double), ("seconds", C.c_double),
     ]
+
+
+class SubConfig(C.Structure):
+    _fields_ = [("sub_terms", C.c_int32), ("reserved", C.c_int32), ("max_attempts", C.c_double),
+                ("accept_rate", C.c_double), ("fill_rate", C.c_double)]
 
 
 class KernelStats(C.Structure):
@@ -86,7 +91,7 @@ def lib() -> C.CDLL:
         "ug_last_error": ([], C.c_char_p),
         "ug_pack_create": ([C.POINTER(PackConfig), vp, C.POINTER(vp)], i32),
         "ug_pack_run": ([vp, i64, C.POINTER(PackStatus)], i32),
-        "ug_pack_run_sub": ([vp, i32, i64, C.POINTER(PackStatus)], i32),
+        "ug_pack_run_sub": ([vp, C.POINTER(SubConfig), i64, C.POINTER(PackStatus)], i32),
         "ug_pack_get_status": ([vp, C.POINTER(PackStatus)], i32),
         "ug_pack_accepted": ([vp, vp, vp, i64, C.POINTER(i64)], i32),
         "ug_pack_params": ([vp, i64, i64, vp], i32),
@@ -137,11 +142,12 @@ class Status:
     rechecked: int
     margin: float
     seconds: float
+    sub_done: bool = False
 
     @staticmethod
     def of(st: PackStatus) -> "Status":
         return Status(st.attempts, st.n, st.nsub, bool(st.done), st.survivors, st.rechecked,
-                      st.margin, st.seconds)
+                      st.margin, st.seconds, bool(st.sub_done))
 
 
 class Packer:
@@ -183,6 +189,14 @@ class Packer:
     def run(self, max_candidates: int = 0) -> Status:
         st = PackStatus()
         check(lib().ug_pack_run(self._h, max_candidates, C.byref(st)))
+        return Status.of(st)
+
+    def run_sub(self, max_attempts: float = 0.0, accept_rate: float = 0.0, fill_rate: float = 0.0,
+                max_candidates: int = 0, sub_terms: int = 0) -> Status:
+        """Subpath phase (--subpaths): candidates continue the index stream."""
+        sc = SubConfig(sub_terms, 0, float(max_attempts), float(accept_rate), float(fill_rate))
+        st = PackStatus()
+        check(lib().ug_pack_run_sub(self._h, C.byref(sc), max_candidates, C.byref(st)))
         return Status.of(st)
 
     def status(self) -> Status:
