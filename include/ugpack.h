@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define UG_ABI_VERSION 1
+#define UG_ABI_VERSION 2
 
 enum {
     UG_OK = 0,
@@ -68,6 +68,9 @@ typedef struct ug_pack_config {
     int64_t max_n;       /* --max-n: stop after this many uniques (0 = off) */
     double margin;       /* split-BF16 certainty margin; 0 = derived error bound */
     int64_t max_paths;   /* initial path capacity (0 = automatic; grows on demand) */
+    int32_t euclid;      /* --euclid-collision: L2 ball sum(d_a^2) <= r^2 instead of the
+                            Chebyshev cube (cuda/braid_cuda3d.cu:605-608) */
+    int32_t reserved;
 } ug_pack_config;
 
 typedef struct ug_pack_status {

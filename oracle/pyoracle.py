@@ -39,6 +39,10 @@ def lib() -> C.CDLL:
         L.or_log.restype = C.c_double
         L.or_model_init.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int]
         L.or_model_init.restype = C.c_int
+        L.or_model_set_euclid.argtypes = [C.c_void_p, C.c_int]
+        L.or_model_set_euclid.restype = None
+        L.or_touches.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_touches.restype = C.c_int
         L.or_model_free.argtypes = [C.c_void_p]
         L.or_model_free.restype = None
         L.or_candidate.argtypes = [C.c_void_p, C.c_uint64] + [C.c_void_p] * 6
@@ -66,7 +70,7 @@ class _Model(C.Structure):
         ("d", C.c_int), ("T", C.c_int), ("nw", C.c_int), ("modmax", C.c_int), ("pin", C.c_int),
         ("seed", C.c_uint64), ("r", C.c_double), ("halfT", C.c_double),
         ("S", C.POINTER(C.c_double)), ("C", C.POINTER(C.c_double)),
-        ("sz", C.POINTER(C.c_double)), ("order", C.POINTER(C.c_int)),
+        ("sz", C.POINTER(C.c_double)), ("order", C.POINTER(C.c_int)), ("euclid", C.c_int),
     ]
 
 
@@ -104,13 +108,17 @@ class Candidate:
 class Model:
     """Oracle model for (d, T, terms, seed, pin)."""
 
-    def __init__(self, d: int, T: int, terms: int | None = None, seed: int = 1, pin: bool = False):
+    def __init__(self, d: int, T: int, terms: int | None = None, seed: int = 1, pin: bool = False,
+                 euclid: bool = False):
         self.d, self.T = d, T
         self.terms = T if terms is None else terms
         self._m = _Model()
         rc = lib().or_model_init(C.byref(self._m), d, T, self.terms, seed, int(pin))
         if rc != 0:
             raise ValueError(f"or_model_init failed ({rc}) for d={d} T={T} terms={self.terms}")
+        if euclid:
+            lib().or_model_set_euclid(C.byref(self._m), 1)
+        self.euclid = bool(euclid)
         self.nw = self._m.nw
         self.r = self._m.r
 
@@ -198,6 +206,11 @@ class Model:
             raise RuntimeError(f"or_pack_sub failed ({n})")
         n = min(n, cap)
         return idx[:n].copy(), gid[:n].copy(), att.value, filled.value
+
+    def touches(self, x, y) -> bool:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        return bool(lib().or_touches(self.ref, _p(x), _p(y)))
 
     def check_rsa(self, M: int, acc: np.ndarray) -> int:
         acc = np.ascontiguousarray(acc, np.int64)

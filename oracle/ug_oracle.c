@@ -182,6 +182,12 @@ int or_model_init(or_model* m, int d, int T, int terms, uint64_t seed, int pin) 
     return 0;
 }
 
+static int touches(const or_model* m, const double* x, const double* y);
+
+void or_model_set_euclid(or_model* m, int euclid) { m->euclid = euclid ? 1 : 0; }
+
+int or_touches(const or_model* m, const double* x, const double* y) { return touches(m, x, y); }
+
 void or_model_free(or_model* m) {
     free(m->S);
     free(m->C);
@@ -459,6 +465,24 @@ static int64_t pk_insert(packing* P, const double* X, int32_t g) {
     P->n++;
     return fresh;
 }
+/* Contact of two points at one timestep (torus min-image per axis): Chebyshev cube
+ * (every |d| <= r, cuda/braid_cuda.cu:451) or the L2 ball of --euclid-collision. */
+static int touches(const or_model* m, const double* x, const double* y) {
+    double dd[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < m->d; a++)
+        dd[a] = tdelta(x[a] - y[a]);
+    if (m->euclid) {
+        double q = dd[0] * dd[0];
+        for (int a = 1; a < m->d; a++)
+            q = fma(dd[a], dd[a], q);
+        return q <= m->r * m->r;
+    }
+    for (int a = 0; a < m->d; a++)
+        if (fabs(dd[a]) > m->r)
+            return 0;
+    return 1;
+}
+
 /* Scans the 3^d neighbour cells of x at timestep t.  mode 0: returns 1 on the first
  * contact.  mode 1 (subpath): tracks the single contacted group in *g, returns 1 on a
  * second distinct group. */
@@ -481,11 +505,7 @@ static int pk_scan(const packing* P, int t, const double* x, int mode, int32_t* 
             continue;
         for (int32_t n = *head; n >= 0; n = P->nxt[(size_t)n * T + t]) {
             const double* p = P->pos + ((size_t)n * T + t) * d;
-            int hit = 1;
-            for (int a = 0; a < d; a++)
-                if (fabs(tdelta(x[a] - p[a])) > m->r)
-                    hit = 0;
-            if (!hit)
+            if (!touches(m, x, p))
                 continue;
             if (mode == 0)
                 return 1;
@@ -765,10 +785,8 @@ static int pair_hit(const or_model* m, const double* x, const double* y) {
             dc = -dc;
         if (dc > 1 && dc != m->T - 1)
             return 0;
-        if (fabs(tdelta(x[a] - y[a])) > m->r)
-            return 0;
     }
-    return 1;
+    return touches(m, x, y);
 }
 
 int64_t or_check_rsa(const or_model* m, int64_t M, const int64_t* acc, int64_t na) {

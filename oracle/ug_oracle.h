@@ -41,10 +41,18 @@ typedef struct or_model {
     double* C;      /* [T][T+1]: (cos(k z_t) - 1)/sin z_t */
     double* sz;     /* [T]: sin z_t */
     int* order;     /* [T]: endpoint-first scan order */
+    int euclid;     /* contact: 0 = Chebyshev cube, 1 = L2 ball (--euclid-collision) */
 } or_model;
 
 int or_model_init(or_model* m, int d, int T, int terms, uint64_t seed, int pin);
 void or_model_free(or_model* m);
+
+/* --euclid-collision (cuda/braid_cuda3d.cu:605-608): contact iff
+ * dx*dx + dy*dy [+ dw*dw] <= r*r, summed in that order with fma. */
+void or_model_set_euclid(or_model* m, int euclid);
+
+/* The contact predicate of the model at one timestep (points x[d], y[d]). */
+int or_touches(const or_model* m, const double* x, const double* y);
 
 /* Candidate idx: a1[d], b[d*nw], c[d*nw], s[d*nw] (per-slot coefficients),
  * and optionally the dump form amp[d*nw], phase[d*nw] (may be NULL). */

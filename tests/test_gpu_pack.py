@@ -297,3 +297,19 @@ def test_subpath_windowed_stop_equals_golden(gpu, name):
     h.update(np.asarray(idx, "<i8").tobytes())
     h.update(np.asarray(gid, "<i4").tobytes())
     assert h.hexdigest() == g["digest"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+@pytest.mark.parametrize("d,T,seed,cands", [(3, 12, 1, 20000), (2, 16, 3, 20000), (3, 32, 2, 60000),
+                                            (2, 64, 4, 400000)])
+def test_euclid_collision_equals_oracle(gpu, precision, d, T, seed, cands):
+    import ugpack
+
+    print(f"euclid {precision} d={d} T={T} seed={seed}", flush=True)
+    want_idx, _, att = po.Model(d, T, seed=seed, euclid=True).pack(cands)
+    prec = ugpack.PREC_FP64 if precision == "fp64" else ugpack.PREC_SPLIT_BF16
+    with ugpack.Packer(d, T, seed=seed, max_attempts=cands, precision=prec, euclid=True) as p:
+        st = p.run()
+        idx, gid = p.accepted()
+    assert st.attempts == att
+    assert list(idx) == list(want_idx) and list(gid) == list(range(len(idx)))

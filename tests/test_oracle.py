@@ -309,6 +309,45 @@ def test_characterization_rejects_planted_mistakes():
     assert m.check_rsa(6000, bad) != 0                 # an extra (colliding) accept
 
 
+def test_euclid_predicate_cases():
+    # --euclid-collision: the L2 ball of radius r replaces the Chebyshev cube
+    for d in (2, 3):
+        cube, ball = po.Model(d, 16), po.Model(d, 16, euclid=True)
+        r = cube.r
+        o = np.zeros(d)
+        on_axis = np.zeros(d)
+        on_axis[0] = r                      # distance exactly r: contact in both
+        diag = np.full(d, 0.9 * r)          # inside the cube, outside the ball
+        seam = np.zeros(d)
+        seam[0], o2 = 0.999, np.zeros(d)
+        o2[0] = -0.999                      # 0.002 apart across the seam
+        for m in (cube, ball):
+            assert m.touches(o, on_axis) and m.touches(seam, o2)
+        assert cube.touches(o, diag) and not ball.touches(o, diag)
+        inner = np.full(d, 0.99 * r / np.sqrt(d))
+        assert ball.touches(o, inner)
+
+
+@pytest.mark.parametrize("d,T,seed", [(3, 12, 1), (2, 16, 3)])
+def test_euclid_rsa_by_numpy_brute_force(d, T, seed):
+    # serial RSA under the L2 ball, restated with numpy norms over all candidate pairs
+    m = po.Model(d, T, seed=seed, euclid=True)
+    M = 2500
+    idx, _, _ = m.pack(M)
+    X = np.stack([m.positions(c) for c in range(M)])       # (M, T, d)
+    acc = []
+    for c in range(M):
+        if acc:
+            dd = np.abs(X[c][None] - X[acc])
+            dd = np.minimum(dd, 2.0 - dd)
+            if np.any(np.sum(dd * dd, axis=2) <= m.r * m.r):
+                continue
+        acc.append(c)
+    assert list(idx) == acc
+    cube_idx, _, _ = po.Model(d, T, seed=seed).pack(M)
+    assert list(cube_idx) != acc
+
+
 def test_fixed_window_stop_rule_by_hand():
     # tiny batches: the rule fires at the first boundary whose window holds too few accepts
     m = po.Model(2, 16, seed=1)

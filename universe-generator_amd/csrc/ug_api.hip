@@ -283,6 +283,8 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     m.key0 = (uint32_t)cfg.seed;
     m.key1 = (uint32_t)(cfg.seed >> 32);
     m.r = 2.0 / (double)T;
+    m.euclid = cfg.euclid ? 1 : 0;
+    m.r2 = m.r * m.r;
     m.halfT = 0.5 * (double)T;
     m.ldb = W;
     m.rowlen = T + 2;

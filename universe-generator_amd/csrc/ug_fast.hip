@@ -239,6 +239,10 @@ __global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t bas
             if (rk < U && col < f.ncols) {
                 const int t = f.colt[col];
                 const float mt = m.marg[col];
+                // L2 ball: the FP64 distance exceeds the FP32 one by at most sqrt(D)*mt, and
+                // the FP32 sum of squares is within (D+1)*2^-24 of its exact value
+                const float rb = r - sqrtf((float)D) * mt;
+                const float rb2 = rb > 0.f ? rb * rb * (1.0f - 0x1p-19f) : -1.0f;
                 float X[D];
                 int c[D];
 #pragma unroll
@@ -265,15 +269,16 @@ __global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t bas
                         for (int n = hd[kk]; n >= 0 && !cert;) {
                             const float4 P = m.pos32[(size_t)n * m.T + t];
                             const float p3[3] = {P.x, P.y, P.z};
-                            float dmax = 0.f;
+                            float dmax = 0.f, q = 0.f;
 #pragma unroll
                             for (int a = 0; a < D; a++) {
                                 float d = X[a] - p3[a];
                                 d = d > 1.f ? d - 2.f : (d < -1.f ? d + 2.f : d);
                                 dmax = fmaxf(dmax, fabsf(d));
+                                q = fmaf(d, d, q);
                             }
                             npts++;
-                            cert = dmax <= r - mt;
+                            cert = m.euclid ? q <= rb2 : dmax <= r - mt;
                             n = __float_as_int(P.w);
                         }
                     }

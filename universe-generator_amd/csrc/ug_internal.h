@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/ugpack.h"
+#include "ug_arith.h"
 
 #define UG_MAX_T 512      // basis pairs per lane per axis <= 4 (T <= 512)
 #define MAXPL 4
@@ -23,6 +24,8 @@ struct DevModel {
     int T, nw, npairs, modmax, pin, full;
     uint32_t key0, key1;
     double r, halfT;
+    int euclid;       // contact predicate: 0 = Chebyshev cube |d_a| <= r, 1 = L2 ball
+    double r2;        // r * r (L2 ball)
     int ldb;          // basis row length (T + 1)
     int rowlen;       // cell-table row length along the last axis (T + 2: one halo cell each side)
     const double* S;  // [T][T+1]  sin(k z_t)/sin z_t
@@ -36,6 +39,29 @@ struct DevModel {
     // split-BF16 form
     const float* marg;    // [T] certainty margin per timestep
 };
+
+// Canonical FP64 contact predicate between two points at one timestep (torus min-image
+// per axis).  Cube: every |d_a| <= r.  Ball: d0*d0, then fma(d1, d1, .), fma(d2, d2, .)
+// <= r*r, the order fixed so host and device round alike.
+template <int D>
+__host__ __device__ inline bool contact(const DevModel& m, const double* x, const double* p) {
+    double dd[D];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        dd[a] = torus_delta(x[a] - p[a]);
+    if (m.euclid) {
+        double q = dd[0] * dd[0];
+#pragma unroll
+        for (int a = 1; a < D; a++)
+            q = fma(dd[a], dd[a], q);
+        return q <= m.r2;
+    }
+    bool h = true;
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        h = h && fabs(dd[a]) <= m.r;
+    return h;
+}
 
 struct Pack {
     ug_pack_config cfg;

@@ -309,11 +309,7 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
             for (int n = hd[k]; n >= 0 && !h;) {
                 const size_t q = (size_t)n * T + t;
                 const double* P = m.pos64 + q * D;
-                bool hh = true;
-#pragma unroll
-                for (int a = 0; a < D; a++)
-                    hh = hh && fabs(torus_delta(x[a] - P[a])) <= m.r;
-                h = hh;
+                h = contact<D>(m, x, P);
                 npts++;
                 n = __float_as_int(m.pos32[q].w);
             }
@@ -522,10 +518,7 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
                         for (int n = hd[k]; n >= 0;) {
                             const size_t q = (size_t)n * T + t;
                             const double* P = m.pos64 + q * D;
-                            bool hh = true;
-#pragma unroll
-                            for (int a = 0; a < D; a++)
-                                hh = hh && fabs(torus_delta(x[a] - P[a])) <= m.r;
+                            const bool hh = contact<D>(m, x, P);
                             if (hh) {
                                 const int g = acc_gid[n];
                                 gmin = min(gmin, g);
@@ -697,8 +690,8 @@ __global__ __launch_bounds__(256) void k_compact_scatter(const uint8_t* flags, i
 
 // =====================================================================================
 // in-batch accept
-// Canonical pair predicate at one timestep: neighbouring cells on every axis and
-// |torus_delta| <= r on every axis -- exactly what a cell-table lookup would find.
+// Canonical pair predicate at one timestep: neighbouring cells on every axis and the
+// contact predicate -- exactly what a cell-table lookup would find.
 template <int D>
 __device__ __forceinline__ bool pair_hit(const DevModel& m, const double* x, const double* y) {
     bool h = true;
@@ -706,9 +699,9 @@ __device__ __forceinline__ bool pair_hit(const DevModel& m, const double* x, con
     for (int a = 0; a < D; a++) {
         int dc = cell_of(x[a], m.halfT, m.T) - cell_of(y[a], m.halfT, m.T);
         dc = dc < 0 ? -dc : dc;
-        h = h && (dc <= 1 || dc == m.T - 1) && fabs(torus_delta(x[a] - y[a])) <= m.r;
+        h = h && (dc <= 1 || dc == m.T - 1);
     }
-    return h;
+    return h && contact<D>(m, x, y);
 }
 
 // Conflict bits among the first S survivors: mask[i][j/64] bit j%64 set iff i > j collide.

@@ -18,7 +18,7 @@ from ._build import LIB_PATH
 UG_OK, UG_EINVAL, UG_ENOMEM, UG_EHIP, UG_EOVERFLOW, UG_ESTATE = 0, -1, -2, -3, -4, -5
 PREC_FP64, PREC_SPLIT_BF16 = 0, 1
 STOP_BUDGET, STOP_FIXED_WINDOW, STOP_REF_WINDOW = 0, 1, 2
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class UgError(RuntimeError):
@@ -35,7 +35,7 @@ class PackConfig(C.Structure):
         ("seed", C.c_uint64), ("pin_sin1", C.c_int32), ("stop_kind", C.c_int32),
         ("max_attempts", C.c_double), ("stop_rate", C.c_double), ("stop_batch", C.c_int64),
         ("stop_window", C.c_int64), ("max_n", C.c_int64), ("margin", C.c_double),
-        ("max_paths", C.c_int64),
+        ("max_paths", C.c_int64), ("euclid", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -157,11 +157,12 @@ class Packer:
                  precision: int = PREC_FP64, pin_sin1: bool = False,
                  stop_kind: int = STOP_BUDGET, max_attempts: float = 0.0,
                  stop_rate: float = 0.0, stop_batch: int = 1 << 20, stop_window: int = 1 << 22,
-                 max_n: int = 0, margin: float = 0.0, max_paths: int = 0, stream: int = 0):
+                 max_n: int = 0, margin: float = 0.0, max_paths: int = 0, stream: int = 0,
+                 euclid: bool = False):
         self._h = None
         cfg = PackConfig(dim, T, terms, precision, seed, int(pin_sin1), stop_kind,
                          float(max_attempts), float(stop_rate), stop_batch, stop_window, max_n,
-                         float(margin), max_paths)
+                         float(margin), max_paths, int(euclid), 0)
         h = C.c_void_p()
         check(lib().ug_pack_create(C.byref(cfg), C.c_void_p(stream), C.byref(h)))
         self._h = h
