@@ -78,12 +78,20 @@ typedef struct ug_pack_status {
     int64_t n;         /* accepted unique worldlines (the done: line's N) */
     int64_t nsub;      /* accepted subpaths */
     int32_t done;      /* 1 once the stop rule (or budget / max_n) has fired */
-    int32_t sub_done;  /* the subpath phase met its stop rule */
+    int32_t sub_done;  /* subpath phase: 0 running, else its stop rule: UG_SUB_BUDGET,
+                          UG_SUB_ACCEPT_RATE or UG_SUB_FILL_RATE */
     int64_t survivors; /* candidates that reached the in-batch accept */
     int64_t rechecked; /* candidates settled by the FP64 recheck (split-BF16 form) */
     double margin;     /* certainty margin in use (split-BF16 form) */
-    double seconds;    /* wall time spent inside ug_pack_run */
+    double seconds;    /* wall time spent inside ug_pack_run / ug_pack_run_sub */
+    int64_t filled;    /* occupied (timestep, cell) pairs (the done: line's filled) */
+    int64_t sub_attempts; /* candidates consumed by the subpath phase */
 } ug_pack_status;
+
+/* ug_pack_status.sub_done: why the subpath phase stopped (cuda/braid_cuda.cu:968,1020,1035). */
+#define UG_SUB_BUDGET 1
+#define UG_SUB_ACCEPT_RATE 2
+#define UG_SUB_FILL_RATE 3
 
 typedef struct ug_pack ug_pack;
 

@@ -37,4 +37,4 @@ def test_abi_version_and_error_string():
 def test_config_struct_layout_matches_header():
     # ug_pack_config: 4 i32, u64, 2 i32, 2 f64, 3 i64, f64, i64, 2 i32
     assert ctypes.sizeof(abi.PackConfig) == 4 * 4 + 8 + 2 * 4 + 2 * 8 + 3 * 8 + 8 + 8 + 2 * 4
-    assert ctypes.sizeof(abi.PackStatus) == 3 * 8 + 2 * 4 + 2 * 8 + 2 * 8
+    assert ctypes.sizeof(abi.PackStatus) == 3 * 8 + 2 * 4 + 2 * 8 + 2 * 8 + 2 * 8

@@ -328,11 +328,13 @@ static void fill_status(const Pack* h, ug_pack_status* st) {
     st->n = h->n;
     st->nsub = h->nsub;
     st->done = h->done ? 1 : 0;
-    st->sub_done = h->sub_done ? 1 : 0;
+    st->sub_done = h->sub_done;
     st->survivors = h->survivors;
     st->rechecked = h->rechecked;
     st->margin = h->margin;
     st->seconds = h->seconds;
+    st->filled = (int64_t)h->filled;
+    st->sub_attempts = h->sub_attempts;
 }
 
 extern "C" int32_t ug_pack_get_status(const ug_pack* hp, ug_pack_status* st) {
@@ -585,7 +587,7 @@ extern "C" int32_t ug_pack_run_sub(ug_pack* hp, const ug_sub_config* sc, int64_t
     // between calls, so results do not depend on max_candidates.
     while (!h->sub_done && rc == UG_OK) {
         if ((double)h->sub_attempts >= budget) {
-            h->sub_done = true;
+            h->sub_done = UG_SUB_BUDGET;
             break;
         }
         if (h->sub_attempts >= call_end)
@@ -632,7 +634,7 @@ extern "C" int32_t ug_pack_run_sub(ug_pack* hp, const ug_sub_config* sc, int64_t
             }
             if (h->a_att >= a_target && h->sub_attempts > 1000000 &&
                 (double)h->a_adm / (double)h->a_att < sc->accept_rate)
-                h->sub_done = true;
+                h->sub_done = UG_SUB_ACCEPT_RATE;
         }
         if (f_target && !h->sub_done) {
             h->fwin.push_back({h->sub_batch, fresh});
@@ -645,7 +647,7 @@ extern "C" int32_t ug_pack_run_sub(ug_pack* hp, const ug_sub_config* sc, int64_t
             }
             if (h->f_att >= f_target && h->sub_attempts > 1000000 &&
                 (double)h->f_new / (double)h->f_att < sc->fill_rate)
-                h->sub_done = true;
+                h->sub_done = UG_SUB_FILL_RATE;
         }
         rc = record_curve(h);
     }

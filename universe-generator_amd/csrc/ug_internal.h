@@ -104,7 +104,7 @@ struct Pack {
     int* d_glist2 = nullptr;
     int* d_gsurv = nullptr;     // adopted groups of a resolve chunk
     int* d_rscount = nullptr;   // survivors of a round against its round-start packing
-    bool sub_done = false;
+    int sub_done = 0;  // UG_SUB_* once the subpath phase stopped
     // sub_batch: the round size, adapted as cuda/braid_cuda.cu:952,1000-1003;
     // sub_nlimit: rows stored before the current round started
     int64_t sub_attempts = 0, sub_batch = 1 << 16, sub_nlimit = 0;

@@ -8,5 +8,6 @@ braidlab/engine.py call surface.
 from ._build import build, LIB_PATH  # noqa: F401
 from .abi import (  # noqa: F401
     PREC_FP64, PREC_SPLIT_BF16, STOP_BUDGET, STOP_FIXED_WINDOW, STOP_REF_WINDOW,
-    Packer, Status, UgError, lib,
+    SUB_ACCEPT_RATE, SUB_BUDGET, SUB_FILL_RATE, Packer, Status, UgError, lib,
 )
+from .engine import BIN_DIR, PackResult, command, pack  # noqa: F401

@@ -27,15 +27,52 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the ugpack HIP library cannot be built")
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every HIP source into ugpack/libugpack.so (always rebuilds)."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs]
+BIN_DIR = os.path.join(PKG_DIR, "bin")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(PKG_DIR)), "include")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+# Command-line front ends over the C ABI: the packer engines (2+1 and 3+1 builds of one
+# source) and the frame baker.  Host-only C++; they link libugpack.so by rpath.
+CLIS = {
+    "braid_ug": ("ug_cli_pack.cpp", ["-DUG_CLI_DIM=2"]),
+    "braid_ug3d": ("ug_cli_pack.cpp", ["-DUG_CLI_DIM=3"]),
+    "frame_bake_ug": ("ug_cli_bake.cpp", ["-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
+                                          f"-L{ROCM}/lib", "-lamdhip64"]),
+}
+
+
+def _run(cmd: list[str], what: str, verbose: bool) -> None:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"{what} failed:\n" + r.stdout + r.stderr)
     if verbose and r.stderr:
         print(r.stderr)
+
+
+def build_clis(verbose: bool = False) -> list[str]:
+    """Compile the command-line front ends into ugpack/bin/ (always rebuilds)."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    out = []
+    for name, (src, extra) in CLIS.items():
+        exe = os.path.join(BIN_DIR, name)
+        srcp = os.path.join(CSRC, src)
+        libs = [x for x in extra if x.startswith("-l") or x.startswith("-L")]
+        flags = [x for x in extra if x not in libs]
+        _run([cxx, "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", *flags, "-o", exe + ".tmp", srcp,
+              f"-L{PKG_DIR}", "-lugpack", *libs, "-Wl,-rpath,$ORIGIN/..",
+              f"-Wl,-rpath-link,{ROCM}/lib"], f"{name} build", verbose)
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every HIP source into ugpack/libugpack.so, then the command-line front ends
+    (always rebuilds)."""
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB_PATH + ".tmp"
+    _run([hipcc(), *HIPCC_FLAGS, "-o", tmp, *srcs], "hipcc", verbose)
     os.replace(tmp, LIB_PATH)
+    build_clis(verbose)
     return LIB_PATH

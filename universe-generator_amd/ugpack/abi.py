@@ -43,7 +43,8 @@ class PackStatus(C.Structure):
     _fields_ = [
         ("attempts", C.c_int64), ("n", C.c_int64), ("nsub", C.c_int64), ("done", C.c_int32),
         ("sub_done", C.c_int32), ("survivors", C.c_int64), ("rechecked", C.c_int64),
-        ("margin", C.c_double), ("seconds", C.c_double),
+        ("margin", C.c_double), ("seconds", C.c_double), ("filled", C.c_int64),
+        ("sub_attempts", C.c_int64),
     ]
 
 
@@ -58,6 +59,7 @@ class KernelStats(C.Structure):
 
 
 KERNEL_PREFILTER, KERNEL_RECHECK = 0, 1
+SUB_BUDGET, SUB_ACCEPT_RATE, SUB_FILL_RATE = 1, 2, 3
 
 
 class Front(C.Structure):
@@ -142,12 +144,14 @@ class Status:
     rechecked: int
     margin: float
     seconds: float
-    sub_done: bool = False
+    sub_done: int = 0       # 0 running, else SUB_BUDGET / SUB_ACCEPT_RATE / SUB_FILL_RATE
+    filled: int = 0
+    sub_attempts: int = 0
 
     @staticmethod
     def of(st: PackStatus) -> "Status":
         return Status(st.attempts, st.n, st.nsub, bool(st.done), st.survivors, st.rechecked,
-                      st.margin, st.seconds, bool(st.sub_done))
+                      st.margin, st.seconds, st.sub_done, st.filled, st.sub_attempts)
 
 
 class Packer:
