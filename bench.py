@@ -98,19 +98,40 @@ def split_error(ugpack, torch, workload: str) -> dict:
             "margin_max": margin, "candidates": int(idx.numel())}
 
 
-def cpu_baseline(seconds_target: float = 12.0) -> dict:
-    """The CPU oracle (serial RSA restatement, 1 thread) on a bounded sample."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(threads: int = 0) -> dict:
+    """The CPU oracle (serial RSA restatement, oracle/) packing config 2 seed 1 to jam with
+    the same fixed-window rule: once on all host cores (parallel prefilter per block, serial
+    in-order admission, the reference's multi-core pattern) and once single-threaded."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
+    nproc = os.cpu_count() or 1
+    threads = threads or min(nproc, 16)
     m = pyoracle.Model(2, 64, seed=1)
-    n = 1 << 20
     t0 = time.perf_counter()
-    m.pack(n)
-    dt = time.perf_counter() - t0
-    return {"value": round(n / dt, 1), "unit": "candidates/s", "cores": 1, "kind": "port",
-            "sample": "first 2^20 candidates of the seed-1 2+1 F=64 pack (serial RSA, oracle/)",
-            "seconds": round(dt, 2)}
+    idx_mt, att_mt = m.pack_mt(1 << 40, threads, stop_kind=1)
+    dt_mt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    idx_1, _, att_1 = m.pack(1 << 40, stop_kind=1)
+    dt_1 = time.perf_counter() - t0
+    assert att_1 == att_mt and list(idx_1) == list(idx_mt)
+    return {"value": round(att_mt / dt_mt, 1), "unit": "candidates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"config 2 seed 1 packed to jam ({att_mt} candidates, N={len(idx_mt)}), "
+                      "same stop rule as the GPU",
+            "s_to_jam": round(dt_mt, 3),
+            "single_thread": {"value": round(att_1 / dt_1, 1), "s_to_jam": round(dt_1, 3)},
+            "nproc": nproc, "cpu_model": _cpu_model()}
 
 
 def main() -> int:

@@ -55,6 +55,9 @@ def lib() -> C.CDLL:
                               C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                               C.POINTER(C.c_int64)]
         L.or_pack.restype = C.c_int64
+        L.or_pack_mt.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_int64, C.c_int64, C.c_double,
+                                 C.c_int, C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
+        L.or_pack_mt.restype = C.c_int64
         L.or_pack_sub.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
                                   C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                   C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
@@ -188,6 +191,18 @@ class Model:
         if want_pos:
             out = out + (pos[:n].copy(),)
         return out
+
+    def pack_mt(self, max_candidates: int, threads: int, stop_kind: int = 0,
+                stop_batch: int = 1 << 20, window: int = 1 << 22, stop_rate: float = 1e-4,
+                cap: int = 1 << 22):
+        """or_pack with a parallel per-block prefilter: returns (idx int64[N], attempts)."""
+        idx = np.zeros(cap, np.int64)
+        att = C.c_int64(0)
+        n = lib().or_pack_mt(self.ref, max_candidates, stop_kind, stop_batch, window, stop_rate,
+                             threads, _p(idx), cap, C.byref(att))
+        if n < 0:
+            raise RuntimeError(f"or_pack_mt failed ({n})")
+        return idx[:min(n, cap)].copy(), att.value
 
     def pack_sub(self, pos0: np.ndarray, gid0: np.ndarray, first: int, budget: float,
                  accept_rate: float = 0.0, fill_rate: float = 0.0, cap: int = 1 << 20):

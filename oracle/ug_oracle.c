@@ -604,6 +604,108 @@ int64_t or_pack(const or_model* m, int64_t max_candidates, int stop_kind, int64_
     return ret;
 }
 
+/* Serial RSA with the reference's multi-core pattern (cuda/braid_cuda3d.cu:190-212,
+ * 1472-1535): blocks of candidates are first tested in parallel against the packing as it
+ * stands (read-only), then the survivors are re-tested and admitted serially in index order.
+ * A candidate rejected in the parallel pass collides with a path the serial run also holds,
+ * so the output equals or_pack's for every block size and thread count. */
+int64_t or_pack_mt(const or_model* m, int64_t max_candidates, int stop_kind, int64_t stop_batch,
+                   int64_t window, double stop_rate, int threads, int64_t* idx, int64_t cap,
+                   int64_t* attempts) {
+    packing P;
+    cand_buf cb;
+    if (pk_init(&P, m) || cb_init(&cb, m)) {
+        pk_free(&P);
+        cb_free(&cb);
+        return -2;
+    }
+    const int T = m->T, d = m->d;
+    const int64_t blk = 8192;
+    char* flag = (char*)malloc((size_t)blk);
+    int64_t nacc_cap = 1024, nacc = 0, lo = 0;
+    int64_t* acc = (int64_t*)malloc((size_t)nacc_cap * 8);
+    int64_t c = 0, ret = 0;
+    int stop = 0;
+    while (!stop && ret == 0) {
+        if (stop_kind == 1 && stop_batch > 0 && c > 0 && c % stop_batch == 0 && c >= window) {
+            while (lo < nacc && acc[lo] < c - window)
+                lo++;
+            if ((double)(nacc - lo) < stop_rate * (double)window)
+                break;
+        }
+        if (c >= max_candidates)
+            break;
+        int64_t e = c + blk < max_candidates ? c + blk : max_candidates;
+        if (stop_kind == 1 && stop_batch > 0) {
+            int64_t nb = (c / stop_batch + 1) * stop_batch;
+            e = e < nb ? e : nb;
+        }
+        const int64_t nblk = e - c;
+        int par_err = 0;
+#pragma omp parallel num_threads(threads) reduction(| : par_err)
+        {
+            cand_buf tb;
+            if (cb_init(&tb, m)) {
+                par_err = 1;
+            } else {
+#pragma omp for schedule(dynamic, 64)
+                for (int64_t k = 0; k < nblk; k++) {
+                    or_candidate(m, (uint64_t)(c + k), tb.a1, tb.b, tb.c, tb.s, NULL, NULL);
+                    int hit = 0;
+                    for (int oi = 0; oi < T && !hit; oi++) {
+                        int t = m->order[oi];
+                        or_eval_t(m, t, tb.a1, tb.b, tb.c, tb.s, tb.X + (size_t)t * d);
+                        hit = pk_scan(&P, t, tb.X + (size_t)t * d, 0, NULL);
+                    }
+                    flag[k] = (char)!hit;
+                }
+                cb_free(&tb);
+            }
+        }
+        if (par_err) {
+            ret = -2;
+            break;
+        }
+        for (int64_t k = 0; k < nblk; k++) {
+            if (!flag[k])
+                continue;
+            const int64_t cc = c + k;
+            or_candidate(m, (uint64_t)cc, cb.a1, cb.b, cb.c, cb.s, NULL, NULL);
+            int hit = 0;
+            for (int oi = 0; oi < T && !hit; oi++) {
+                int t = m->order[oi];
+                or_eval_t(m, t, cb.a1, cb.b, cb.c, cb.s, cb.X + (size_t)t * d);
+                hit = pk_scan(&P, t, cb.X + (size_t)t * d, 0, NULL);
+            }
+            if (hit)
+                continue;
+            if (pk_insert(&P, cb.X, (int32_t)P.n) < 0) {
+                ret = -2;
+                break;
+            }
+            if (nacc == nacc_cap) {
+                nacc_cap *= 2;
+                acc = (int64_t*)realloc(acc, (size_t)nacc_cap * 8);
+            }
+            acc[nacc++] = cc;
+        }
+        c = e;
+    }
+    if (ret == 0) {
+        for (int64_t i = 0; i < nacc && i < cap; i++)
+            if (idx)
+                idx[i] = acc[i];
+        ret = nacc;
+        if (attempts)
+            *attempts = c;
+    }
+    free(flag);
+    free(acc);
+    pk_free(&P);
+    cb_free(&cb);
+    return ret;
+}
+
 /* Touched group of candidate c against the packing P: -1 none, -2 two or more. */
 static int32_t sub_group(const or_model* m, const packing* P, cand_buf* cb, uint64_t c) {
     or_candidate(m, c, cb->a1, cb->b, cb->c, cb->s, NULL, NULL);

@@ -78,6 +78,12 @@ int64_t or_pack(const or_model* m, int64_t max_candidates, int stop_kind, int64_
                 int64_t window, double stop_rate, int64_t max_n, int64_t* idx, int32_t* gid,
                 double* pos, int64_t cap, int64_t* attempts);
 
+/* or_pack (stop_kind 0/1, no max_n) with a parallel prefilter per block of candidates and
+ * serial in-order admission on `threads` OpenMP threads; same output as or_pack. */
+int64_t or_pack_mt(const or_model* m, int64_t max_candidates, int stop_kind, int64_t stop_batch,
+                   int64_t window, double stop_rate, int threads, int64_t* idx, int64_t cap,
+                   int64_t* attempts);
+
 /* Subpath phase continuing an existing packing (cuda/braid_cuda.cu:949-1043).  Rounds of
  * `batch` candidates from index `first` on: a candidate survives the round when, against
  * the packing as it stood at the round's start, it touches exactly one group (collides_dev

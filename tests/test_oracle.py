@@ -309,6 +309,21 @@ def test_characterization_rejects_planted_mistakes():
     assert m.check_rsa(6000, bad) != 0                 # an extra (colliding) accept
 
 
+@pytest.mark.parametrize("d,T,seed,cands,threads", [(2, 16, 1, 20000, 3), (3, 12, 1, 20000, 8),
+                                                    (2, 24, 3, 30000, 2)])
+def test_multicore_pack_equals_serial(d, T, seed, cands, threads):
+    m = po.Model(d, T, seed=seed)
+    want, _, att = m.pack(cands)
+    got, att2 = m.pack_mt(cands, threads)
+    assert list(got) == list(want) and att2 == att == cands
+
+
+def test_multicore_pack_to_jam_equals_golden():
+    g = json.load(open(os.path.join(GOLD, "cfg2_d2_T64_s2_jam.json")))
+    got, att = po.Model(2, 64, seed=2).pack_mt(1 << 40, 8, stop_kind=1)
+    assert att == g["candidates"] and list(got) == g["idx"]
+
+
 def test_euclid_predicate_cases():
     # --euclid-collision: the L2 ball of radius r replaces the Chebyshev cube
     for d in (2, 3):

@@ -148,23 +148,32 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
     const u32x4 w0 = philox((uint32_t)idx, (uint32_t)(idx >> 32), 0u, 0u, m.key0, m.key1);
     const uint32_t wa[3] = {w0.x, w0.y, w0.z};
     bool wbad = false;
+    // K steps outer, axes inner: each basis fragment is loaded once for all axes; every
+    // axis accumulates its own K steps in the same order as before
+    f32x16 acc[D];
+    float Wpart[D];
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        f32x16 acc;
+        Wpart[a] = 0.f;
 #pragma unroll
         for (int v = 0; v < 16; v++)
-            acc[v] = 0.f;
-        float Wpart = 0.f;
-        for (int q = 0; q < f.ks; q++) {
+            acc[a][v] = 0.f;
+    }
+    for (int q = 0; q < f.ks; q++) {
+        const bf16x8 Bhi = bsrc[((size_t)(nb * f.ks + q) * 2 + 0) * 64 + lane];
+        const bf16x8 Blo = bsrc[((size_t)(nb * f.ks + q) * 2 + 1) * 64 + lane];
+#pragma unroll
+        for (int a = 0; a < D; a++) {
             bf16x8 Ahi, Alo;
-            gen_kstep(m, idx, valid, a, q, h, Ahi, Alo, Wpart, f.invk);
-            const bf16x8 Bhi = bsrc[((size_t)(nb * f.ks + q) * 2 + 0) * 64 + lane];
-            const bf16x8 Blo = bsrc[((size_t)(nb * f.ks + q) * 2 + 1) * 64 + lane];
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Bhi, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Blo, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo, Bhi, acc, 0, 0, 0);
+            gen_kstep(m, idx, valid, a, q, h, Ahi, Alo, Wpart[a], f.invk);
+            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Bhi, acc[a], 0, 0, 0);
+            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Blo, acc[a], 0, 0, 0);
+            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo, Bhi, acc[a], 0, 0, 0);
         }
-        const float W = Wpart + __shfl_xor(Wpart, 32, 64);
+    }
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        const float W = Wpart[a] + __shfl_xor(Wpart[a], 32, 64);
         wbad = wbad || (valid && W < 0.25f * (float)m.nw);
         if (h == 0) {
             sa1[mrow] = m.pin ? 0.f : (float)a1_of_word(wa[a]);
@@ -177,7 +186,7 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const int row = 8 * (v >> 2) + 4 * h + (v & 3);
-            float X = fmaf(acc[v], sinvw[row], sa1[row]);
+            float X = fmaf(acc[a][v], sinvw[row], sa1[row]);
             X = X - 2.0f * floorf((X + 1.0f) * 0.5f);
             pos[(a * TILE + row) * NCOL + mrow] = X;
         }
@@ -196,15 +205,12 @@ __global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t bas
     extern __shared__ __align__(16) unsigned char lds[];
     const DevModel& m = f.m;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // basis fragments of N block 0, shared by the workgroup
-    bf16x8* bl = (bf16x8*)lds;
-    const int nfrag = f.ks * 2 * 64;
-    for (int i = threadIdx.x; i < nfrag; i += blockDim.x)
-        bl[i] = f.bfrag[i];
-    float* pos = (float*)(lds + (size_t)nfrag * 16) + (size_t)wv * (D * TILE * NCOL + 3 * TILE);
+    // basis fragments of N block 0 are read from global memory (a few tens of KB, cache
+    // resident); LDS holds only each wave's position tile, which keeps occupancy up
+    const bf16x8* bl = f.bfrag;
+    float* pos = (float*)lds + (size_t)wv * (D * TILE * NCOL + 3 * TILE);
     float* sa1 = pos + D * TILE * NCOL;
     float* sinvw = sa1 + TILE;
-    __syncthreads();
     const int mrow = lane & 31, h = lane >> 5;
     const int64_t ntiles = (count + TILE - 1) / TILE;
     unsigned long long nvis = 0, npts = 0;
@@ -458,7 +464,7 @@ int32_t fast_setup(Pack* h) {
         rc = hip_check(hipMemcpy(h->d_marg, h->marg_host.data(), h->marg_host.size() * 4, hipMemcpyHostToDevice),
                        "margin upload");
     h->dm.marg = h->d_marg;
-    fs->lds_prefilter = (size_t)fs->ks * 2 * 64 * 16 + (size_t)FAST_WAVES * (h->d * TILE * NCOL + 3 * TILE) * 4;
+    fs->lds_prefilter = (size_t)FAST_WAVES * (h->d * TILE * NCOL + 3 * TILE) * 4;
     h->d_basis16 = fs;
     if (!rc && fs->lds_prefilter > 160 * 1024)
         rc = fail(UG_EINVAL, "split-BF16 tile needs %zu bytes of LDS", fs->lds_prefilter);
@@ -496,7 +502,7 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
         return fail(UG_ESTATE, "split-BF16 prefilter not set up for this model");
     FastState* fs = (FastState*)h->d_basis16;
     const int64_t ntiles = (count + TILE - 1) / TILE;
-    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, 1024));
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, 2048));
     const FastArgs f = fast_args(h);
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
     if (h->d == 2)

@@ -172,6 +172,11 @@ __device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D
                 }
                 w.c[a][i][k] = cj;
                 w.s[a][i][k] = sj;
+                // unused slots: zero coefficients on a valid basis row, so evaluation needs no
+                // branches (an accumulator that starts at +0 is never -0, hence
+                // fma(0, v, acc) == acc exactly)
+                if (!bj)
+                    w.b[a][i][k] = 2;
             }
     }
 }
@@ -190,12 +195,11 @@ __device__ __forceinline__ void eval_wave64(const DevModel& m, const WaveCoef<D,
         for (int i = 0; i < PL; i++)
 #pragma unroll
             for (int k = 0; k < 2; k++) {
+                // odd frequencies and unused slots carry s = 0 (and unused c = 0): adding
+                // their zero products leaves the sum unchanged, bit for bit
                 const int bj = w.b[a][i][k];
-                if (bj) {
-                    part = fma(w.c[a][i][k], St[bj], part);
-                    if ((bj & 1) == 0)
-                        part = fma(w.s[a][i][k], Ct[bj], part);
-                }
+                part = fma(w.c[a][i][k], St[bj], part);
+                part = fma(w.s[a][i][k], Ct[bj], part);
             }
         X[a] = torus_wrap(w.a1[a] + wave_sum_canonical(part));
     }
@@ -205,31 +209,10 @@ __host__ __device__ constexpr int bitrev3(int i) {
     return ((i & 1) << 2) | (i & 2) | ((i & 4) >> 2);
 }
 
-// Slot partial P_s at timestep t from the LDS coefficients: fma chain over the pairs
-// p = s + 64 i (ascending), even frequencies adding their (cos - 1) basis term.
-template <int PL>
-__device__ __forceinline__ double slot_partial(const DevModel& m, const double2* ca, const int* ba,
-                                               int sl, int t) {
-    double acc = 0.0;
-#pragma unroll
-    for (int ii = 0; ii < PL; ii++) {
-        const int p = sl + 64 * ii;
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const int j = 2 * p + k;
-            if (j < m.nw) {
-                const int b = ba[j];
-                const double2 c = ca[j];
-                acc = fma(c.x, m.ST[(size_t)b * m.T + t], acc);
-                if ((b & 1) == 0)
-                    acc = fma(c.y, m.CT[(size_t)b * m.T + t], acc);
-            }
-        }
-    }
-    return acc;
-}
-
 // The same canonical position, evaluated by ONE lane (lanes take different timesteps).
+// Slot partial P_s is the fma chain over the pairs p = s + 64 i (ascending), each adding
+// c S_b then s C_b (s = 0 for odd frequencies and padding slots, whose zero products leave
+// the chain unchanged bit for bit, so no branches are needed).
 // The halving tree over the 64 slot partials equals the adjacent pairwise tree over the
 // slots in bit-reversed order; element 8g + k of that order is slot 8 bitrev3(k) + bitrev3(g),
 // so each group of 8 is a fixed 3-level tree and the 8 group sums merge as a binary counter.
@@ -238,8 +221,8 @@ __device__ __forceinline__ void eval_lane(const DevModel& m, const double2* cs, 
                                           const double* a1, int t, double* X) {
 #pragma unroll
     for (int a = 0; a < D; a++) {
-        const double2* ca = cs + a * m.nw;
-        const int* ba = bb + a * m.nw;
+        const double2* ca = cs + a * (128 * PL);
+        const int* ba = bb + a * (128 * PL);
         double L0 = 0.0, L1 = 0.0, L2 = 0.0, total = 0.0;
 #pragma unroll 1
         for (int g = 0; g < 8; g++) {
@@ -247,7 +230,32 @@ __device__ __forceinline__ void eval_lane(const DevModel& m, const double2* cs, 
             double e[8];
 #pragma unroll
             for (int k = 0; k < 8; k++)
-                e[k] = slot_partial<PL>(m, ca, ba, 8 * bitrev3(k) + rg, t);
+                e[k] = 0.0;
+#pragma unroll
+            for (int ii = 0; ii < PL; ii++) {
+                // the 32 basis values of this group's slot pairs are loaded in one batch (the
+                // scheduling barrier keeps the loads ahead of the fma chains that use them)
+                double sv[8][2], cv[8][2];
+                double2 cc[8][2];
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+#pragma unroll
+                    for (int kk = 0; kk < 2; kk++) {
+                        const int j = 2 * (8 * bitrev3(k) + rg + 64 * ii) + kk;
+                        const int b = ba[j];
+                        cc[k][kk] = ca[j];
+                        sv[k][kk] = m.ST[(size_t)b * m.T + t];
+                        cv[k][kk] = m.CT[(size_t)b * m.T + t];
+                    }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+#pragma unroll
+                    for (int kk = 0; kk < 2; kk++) {
+                        e[k] = fma(cc[k][kk].x, sv[k][kk], e[k]);
+                        e[k] = fma(cc[k][kk].y, cv[k][kk], e[k]);
+                    }
+            }
             double G = ((e[0] + e[1]) + (e[2] + e[3])) + ((e[4] + e[5]) + (e[6] + e[7]));
             if ((g & 1) == 0) {
                 L0 = G;
@@ -269,12 +277,21 @@ __device__ __forceinline__ void eval_lane(const DevModel& m, const double2* cs, 
     }
 }
 
-// Per-wave LDS of k_cand64: partial-spectrum draws (slot frequencies + bitmask), the slot
-// coefficients (c, s) of every axis, and positions of up to 64 timesteps.
+// The slot arrays of the lane-per-timestep evaluation use a padded stride of 128 PL slots per
+// axis (PL = slot pairs per lane: 1 up to 64 pairs, 2 up to 128, else 4).
+__host__ __device__ inline int slot_pl(int npairs) { return npairs <= 64 ? 1 : (npairs <= 128 ? 2 : 4); }
+
+template <int D>
+__host__ __device__ inline int cand64_lds_head(const DevModel& m, int pl) {
+    const int draws = D * m.nw + (m.modmax + 2 + 31) / 32;
+    const int slots = D * 128 * pl;
+    return ((draws > slots ? draws : slots) + 3) / 4 * 4;
+}
+
 template <int D>
 __host__ __device__ inline int cand64_lds_words(const DevModel& m) {
-    const int head = (D * m.nw + (m.modmax + 2 + 31) / 32 + 3) / 4 * 4;
-    return head + D * m.nw * 4 + 64 * D * 2;
+    const int pl = slot_pl(m.npairs);
+    return cand64_lds_head<D>(m, pl) + D * 128 * pl * 4 + 64 * D * 2;
 }
 
 // Lookup round: lanes take (timestep slot sl, 3-cell row rr) items of the timesteps whose
@@ -338,9 +355,10 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
     const int T = m.T, nw = m.nw;
     int* sb = dyn_lds;
     uint32_t* used = (uint32_t*)(sb + D * nw);
-    const int lds_head = (D * nw + (m.modmax + 2 + 31) / 32 + 3) / 4 * 4;
+    constexpr int NWP = 128 * PL;  // padded slots per axis
+    const int lds_head = cand64_lds_head<D>(m, PL);
     double2* cs = (double2*)(dyn_lds + lds_head);
-    double* pos = (double*)(cs + D * nw);
+    double* pos = (double*)(cs + D * NWP);
     __shared__ int tsl[64];
     constexpr int R = D == 2 ? 3 : 9;
     constexpr int PER_ROUND = 64 / R;
@@ -382,10 +400,8 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
 #pragma unroll
                     for (int k = 0; k < 2; k++) {
                         const int j = 2 * (lane + 64 * ii) + k;
-                        if (j < nw) {
-                            cs[a * nw + j] = make_double2(w.c[a][ii][k], w.s[a][ii][k]);
-                            sb[a * nw + j] = w.b[a][ii][k];
-                        }
+                        cs[a * NWP + j] = make_double2(w.c[a][ii][k], w.s[a][ii][k]);
+                        sb[a * NWP + j] = w.b[a][ii][k];
                     }
             __syncthreads();
             for (int ps = o0; ps < T && !hit; ps += 64) {
@@ -452,9 +468,10 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
     const int T = m.T, nw = m.nw;
     int* sb = dyn_lds;
     uint32_t* used = (uint32_t*)(sb + D * nw);
-    const int lds_head = (D * nw + (m.modmax + 2 + 31) / 32 + 3) / 4 * 4;
+    constexpr int NWP = 128 * PL;  // padded slots per axis
+    const int lds_head = cand64_lds_head<D>(m, PL);
     double2* cs = (double2*)(dyn_lds + lds_head);
-    double* pos = (double*)(cs + D * nw);
+    double* pos = (double*)(cs + D * NWP);
     __shared__ int tsl[64];
     constexpr int R = D == 2 ? 3 : 9;
     constexpr int PER_ROUND = 64 / R;
@@ -469,10 +486,8 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
 #pragma unroll
                 for (int k = 0; k < 2; k++) {
                     const int j = 2 * (lane + 64 * ii) + k;
-                    if (j < nw) {
-                        cs[a * nw + j] = make_double2(w.c[a][ii][k], w.s[a][ii][k]);
-                        sb[a * nw + j] = w.b[a][ii][k];
-                    }
+                    cs[a * NWP + j] = make_double2(w.c[a][ii][k], w.s[a][ii][k]);
+                    sb[a * NWP + j] = w.b[a][ii][k];
                 }
         __syncthreads();
         // all contacts: (gcur, bad); contacts with round-start paths: (gcur0, bad0)
@@ -860,7 +875,7 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
     const int blocks = (int)std::min<int64_t>(count, 256 * 32);
     const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
     // slot pairs per lane: 1 up to 64 pairs (T <= 129), 2 up to 128, else 4
-    const int pl = h->npairs <= 64 ? 1 : (h->npairs <= 128 ? 2 : 4);
+    const int pl = slot_pl(h->npairs);
 #define UG_CAND64(DD, FF, PP)                                                                         \
     hipLaunchKernelGGL((k_cand64<DD, FF, PP>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, \
                        list, dcount, flags, out, head, ctr)
@@ -1132,7 +1147,7 @@ int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, i
     }
     const int blocks = (int)std::min<int64_t>(count, 256 * 32);
     const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
-    const int pl = h->npairs <= 64 ? 1 : (h->npairs <= 128 ? 2 : 4);
+    const int pl = slot_pl(h->npairs);
 #define UG_SUBT(DD, PP)                                                                                      \
     hipLaunchKernelGGL((k_sub_test<DD, PP>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, list, \
                        h->d_acc_gid, h->sub_nlimit, rs_count, h->d_flags, h->d_gflag)
