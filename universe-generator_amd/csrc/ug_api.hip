@@ -408,12 +408,14 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         } else {
             h->attempts += sub;
         }
-        // Sub-batch size from the measured survivor rate, aiming at ~1024 candidates reaching
-        // the FP64 stages per sub-batch (the reference adapts toward ~256 survivors,
-        // cuda/braid_cuda.cu:920-924).  Results do not depend on it.
+        // Sub-batch size from the measured rates, aiming at ~1024 survivors (one in-batch
+        // resolve chunk) and at most ~16k candidates for the FP64 recheck per sub-batch (the
+        // reference adapts toward ~256 survivors, cuda/braid_cuda.cu:920-924).  Results do
+        // not depend on it.
         {
-            const int64_t seen = std::max<int64_t>(1, std::max(S, h->rechecked - rechecked_before));
-            double want = 1024.0 * (double)sub / (double)seen;
+            const double surv = (double)std::max<int64_t>(1, S);
+            const double rech = (double)std::max<int64_t>(1, h->rechecked - rechecked_before);
+            double want = std::min(1024.0 * (double)sub / surv, 16384.0 * (double)sub / rech);
             want = std::min(want, 8.0 * (double)sub);
             int64_t next = 4096;
             while (next * 2 <= (int64_t)want && next * 2 <= sub_cap)
