@@ -107,6 +107,7 @@ void release(Pack* h) {
     free_dev(h->d_CT);
     free_dev(h->d_order);
     free_dev(h->d_tab);
+    free_dev(h->d_occ);
     free_dev(h->d_marg);
     free_dev(h->d_pos64);
     free_dev(h->d_pos32);
@@ -228,6 +229,8 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     if (h->d == 3)
         tab_len *= T;
     h->tab_len = tab_len;
+    const int occw = (T + 2 + 31) / 32 + 1;  // one spare word: a 3-bit window may cross a word
+    h->occ_len = tab_len / (T + 2) * occw;
     int64_t cap0 = cfg.max_paths > 0 ? cfg.max_paths : 4096;
 
     int32_t rc = UG_OK;
@@ -242,6 +245,7 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     UG_ALLOC(h->d_CT, CT.size() * 8);
     UG_ALLOC(h->d_order, (size_t)T * 4);
     UG_ALLOC(h->d_tab, (size_t)tab_len * 4);
+    UG_ALLOC(h->d_occ, (size_t)h->occ_len * 4);
     UG_ALLOC(h->d_filled, 8);
     UG_ALLOC(h->d_flags, (size_t)UG_SUB_MAX);
     UG_ALLOC(h->d_list, (size_t)UG_SUB_MAX * 8);
@@ -271,6 +275,8 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
         if (!rc) {
             hipLaunchKernelGGL(k_fill_tab, dim3(grid_for(tab_len, 256 * 8)), dim3(256), 0, h->stream, h->d_tab, tab_len);
             rc = hip_check(hipMemsetAsync(h->d_filled, 0, 8, h->stream), "clear filled");
+            if (!rc)
+                rc = hip_check(hipMemsetAsync(h->d_occ, 0, (size_t)h->occ_len * 4, h->stream), "clear occupancy");
         }
     }
     DevModel& m = h->dm;
@@ -294,6 +300,8 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     m.CT = h->d_CT;
     m.order = h->d_order;
     m.tab = h->d_tab;
+    m.occ = h->d_occ;
+    m.occw = occw;
     m.pos64 = nullptr;
     m.pos32 = nullptr;
     m.marg = nullptr;

@@ -199,7 +199,7 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 
 // Prefilter: flags[i] = 0 when candidate i certainly collides, 1 when the FP64 test must decide.
 template <int D>
-__global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
+__global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
                                                         const int64_t* list, uint8_t* flags,
                                                         unsigned long long* ctr) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -260,21 +260,45 @@ __global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t bas
                 }
                 nvis++;
                 constexpr int ROWS = D == 2 ? 3 : 9;
-                for (int rr = 0; rr < ROWS && !cert; rr++) {
-                    int64_t row;
-                    if (D == 2) {
-                        row = ((int64_t)t * m.T + (c[0] + rr - 1 + m.T) % m.T) * m.rowlen;
-                    } else {
-                        const int gx = (c[0] + rr / 3 - 1 + m.T) % m.T;
-                        const int gy = (c[1] + rr % 3 - 1 + m.T) % m.T;
-                        row = (((int64_t)t * m.T + gx) * m.T + gy) * m.rowlen;
+                // table row of neighbour row rr (x offset rr/3 - 1, y offset rr%3 - 1 in 3+1)
+                auto row_of = [&](int rr) -> int64_t {
+                    if (D == 2)
+                        return (int64_t)t * m.T + (c[0] + rr - 1 + m.T) % m.T;
+                    const int gx = (c[0] + rr / 3 - 1 + m.T) % m.T;
+                    const int gy = (c[1] + rr % 3 - 1 + m.T) % m.T;
+                    return ((int64_t)t * m.T + gx) * m.T + gy;
+                };
+                // occupancy of all neighbour cells first (independent loads, issued together)
+                // as one mask; then up to 2 occupied cells at a time load their heads together
+                // and walk their chains together, so a visit costs about three dependent
+                // memory steps (bits, heads, points)
+                uint32_t cells = 0;
+#pragma unroll
+                for (int rr = 0; rr < ROWS; rr++)
+                    cells |= occ_bits3(m, row_of(rr), c[D - 1]) << (3 * rr);
+                while (cells && !cert) {
+                    constexpr int NG = 2;
+                    int node[NG];
+#pragma unroll
+                    for (int g = 0; g < NG; g++) {
+                        node[g] = -1;
+                        if (cells) {
+                            const int b = __builtin_ctz(cells);
+                            cells &= cells - 1u;
+                            node[g] = m.tab[row_of(b / 3) * m.rowlen + c[D - 1] + b % 3];
+                        }
                     }
-                    const int* tr = m.tab + row + c[D - 1];
-                    const int hd[3] = {tr[0], tr[1], tr[2]};
-                    for (int kk = 0; kk < 3 && !cert; kk++) {
-                        for (int n = hd[kk]; n >= 0 && !cert;) {
-                            const float4 P = m.pos32[(size_t)n * m.T + t];
-                            const float p3[3] = {P.x, P.y, P.z};
+                    while (!cert && (node[0] >= 0 || node[1] >= 0)) {
+                        float4 P[NG];
+#pragma unroll
+                        for (int g = 0; g < NG; g++)
+                            if (node[g] >= 0)
+                                P[g] = m.pos32[(size_t)node[g] * m.T + t];
+#pragma unroll
+                        for (int g = 0; g < NG; g++) {
+                            if (node[g] < 0)
+                                continue;
+                            const float p3[3] = {P[g].x, P[g].y, P[g].z};
                             float dmax = 0.f, q = 0.f;
 #pragma unroll
                             for (int a = 0; a < D; a++) {
@@ -284,8 +308,8 @@ __global__ __launch_bounds__(256) void k_fast_prefilter(FastArgs f, uint64_t bas
                                 q = fmaf(d, d, q);
                             }
                             npts++;
-                            cert = m.euclid ? q <= rb2 : dmax <= r - mt;
-                            n = __float_as_int(P.w);
+                            cert = cert || (m.euclid ? q <= rb2 : dmax <= r - mt);
+                            node[g] = __float_as_int(P[g].w);
                         }
                     }
                 }

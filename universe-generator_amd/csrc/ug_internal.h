@@ -34,6 +34,8 @@ struct DevModel {
     const double* CT; // [T+1][T]  C transposed
     const int* order; // [T] endpoint-first scan order
     int* tab;         // cell table: head path of each (t, cell) chain, -1 = empty
+    uint32_t* occ;    // occupancy bits of the same padded rows (bit i <-> head i), occw words a row
+    int occw;
     const double* pos64;  // [n][T][D] canonical positions of accepted paths
     const float4* pos32;  // [n][T] (x, y, w, next path in the same cell as int bits)
     // split-BF16 form
@@ -63,6 +65,16 @@ __host__ __device__ inline bool contact(const DevModel& m, const double* x, cons
     return h;
 }
 
+// Occupancy of the padded cells c..c+2 (cells c-1..c+1 of the unpadded row) of table row
+// `row` (row = t*T + x in 2+1, (t*T + x)*T + y in 3+1) as bits 0..2.  The bit rows are a
+// 32nd of the head table, small enough to stay in the last-level cache, so empty
+// neighbour cells cost no head-table traffic.
+__device__ __forceinline__ uint32_t occ_bits3(const DevModel& m, int64_t row, int c) {
+    const uint32_t* w = m.occ + row * m.occw + (c >> 5);
+    const uint64_t v = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+    return (uint32_t)(v >> (c & 31)) & 7u;
+}
+
 struct Pack {
     ug_pack_config cfg;
     hipStream_t stream;
@@ -75,6 +87,8 @@ struct Pack {
     int* d_order = nullptr;
     int* d_tab = nullptr;
     int64_t tab_len = 0;
+    uint32_t* d_occ = nullptr;
+    int64_t occ_len = 0;
     float* d_marg = nullptr;
     std::vector<float> marg_host;
     double margin = 0.0;

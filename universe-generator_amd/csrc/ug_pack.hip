@@ -312,18 +312,18 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
             x[a] = pos[sl * D + a];
             c[a] = cell_of(x[a], m.halfT, T);
         }
-        int64_t row;
+        int64_t rid;
         if (D == 2) {
-            row = ((int64_t)t * T + (c[0] + rr - 1 + T) % T) * m.rowlen;
+            rid = (int64_t)t * T + (c[0] + rr - 1 + T) % T;
         } else {
             const int gx = (c[0] + rr / 3 - 1 + T) % T;
             const int gy = (c[1] + rr % 3 - 1 + T) % T;
-            row = (((int64_t)t * T + gx) * T + gy) * m.rowlen;
+            rid = ((int64_t)t * T + gx) * T + gy;
         }
-        const int* tr = m.tab + row + c[D - 1];  // padded cells c-1..c+1 at c..c+2
-        const int hd[3] = {tr[0], tr[1], tr[2]};
+        const uint32_t bits = occ_bits3(m, rid, c[D - 1]);
+        const int* tr = m.tab + rid * m.rowlen + c[D - 1];  // padded cells c-1..c+1 at c..c+2
         for (int k = 0; k < 3 && !h; k++)
-            for (int n = hd[k]; n >= 0 && !h;) {
+            for (int n = ((bits >> k) & 1u) ? tr[k] : -1; n >= 0 && !h;) {
                 const size_t q = (size_t)n * T + t;
                 const double* P = m.pos64 + q * D;
                 h = contact<D>(m, x, P);
@@ -519,18 +519,18 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
                         x[a] = pos[(r0 + sl) * D + a];
                         c[a] = cell_of(x[a], m.halfT, T);
                     }
-                    int64_t row;
+                    int64_t rid;
                     if (D == 2) {
-                        row = ((int64_t)t * T + (c[0] + rr - 1 + T) % T) * m.rowlen;
+                        rid = (int64_t)t * T + (c[0] + rr - 1 + T) % T;
                     } else {
                         const int gx = (c[0] + rr / 3 - 1 + T) % T;
                         const int gy = (c[1] + rr % 3 - 1 + T) % T;
-                        row = (((int64_t)t * T + gx) * T + gy) * m.rowlen;
+                        rid = ((int64_t)t * T + gx) * T + gy;
                     }
-                    const int* tr = m.tab + row + c[D - 1];
-                    const int hd[3] = {tr[0], tr[1], tr[2]};
+                    const uint32_t bits = occ_bits3(m, rid, c[D - 1]);
+                    const int* tr = m.tab + rid * m.rowlen + c[D - 1];
                     for (int k = 0; k < 3; k++)
-                        for (int n = hd[k]; n >= 0;) {
+                        for (int n = ((bits >> k) & 1u) ? tr[k] : -1; n >= 0;) {
                             const size_t q = (size_t)n * T + t;
                             const double* P = m.pos64 + q * D;
                             const bool hh = contact<D>(m, x, P);
@@ -807,11 +807,19 @@ __global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, c
             pos64[o * D + a] = X[a];
             f[a] = (float)X[a];
         }
-        int64_t row = (int64_t)t * m.T + c[0];
+        int64_t rid = (int64_t)t * m.T + c[0];
         if (D == 3)
-            row = row * m.T + c[1];
-        row *= m.rowlen;
-        const int prev = atomicExch(&m.tab[row + c[D - 1] + 1], (int)n);
+            rid = rid * m.T + c[1];
+        const int64_t row = rid * m.rowlen;
+        const int cl = c[D - 1];
+        const int prev = atomicExch(&m.tab[row + cl + 1], (int)n);
+        // occupancy bit of the padded cell, and of its halo copy at the row ends
+        uint32_t* ob = m.occ + rid * m.occw;
+        atomicOr(&ob[(cl + 1) >> 5], 1u << ((cl + 1) & 31));
+        if (cl == 0)
+            atomicOr(&ob[(m.T + 1) >> 5], 1u << ((m.T + 1) & 31));
+        if (cl == m.T - 1)
+            atomicOr(&ob[0], 1u);
         fresh = prev < 0;
         pos32[o] = make_float4(f[0], f[1], f[2], __int_as_float(prev));
         if (t == 0) {
