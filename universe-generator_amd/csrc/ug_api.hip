@@ -121,6 +121,7 @@ void release(Pack* h) {
     free_dev(h->d_bcount);
     free_dev(h->d_count);
     free_dev(h->d_spos);
+    free_dev(h->d_scell);
     free_dev(h->d_mask);
     free_dev(h->d_acc_list);
     free_dev(h->d_nacc);
@@ -255,6 +256,7 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     UG_ALLOC(h->d_bcount, (size_t)h->cmp_blocks_cap * 4);
     UG_ALLOC(h->d_count, 16);
     UG_ALLOC(h->d_spos, (size_t)UG_S_MAX * T * h->d * 8);
+    UG_ALLOC(h->d_scell, (size_t)UG_S_MAX * T * 4);
     UG_ALLOC(h->d_mask, (size_t)UG_S_MAX * UG_MASK_WORDS * 8);
     UG_ALLOC(h->d_acc_list, (size_t)UG_S_MAX * 4);
     UG_ALLOC(h->d_nacc, 4);

@@ -109,6 +109,7 @@ struct Pack {
     int cmp_blocks_cap = 0;
     int* d_count = nullptr;
     double* d_spos = nullptr;
+    int* d_scell = nullptr;  // packed cells of a resolve chunk's survivors [UG_S_MAX][T]
     uint64_t* d_mask = nullptr;
     int* d_acc_list = nullptr;
     int* d_nacc = nullptr;

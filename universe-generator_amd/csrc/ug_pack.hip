@@ -322,14 +322,33 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
         }
         const uint32_t bits = occ_bits3(m, rid, c[D - 1]);
         const int* tr = m.tab + rid * m.rowlen + c[D - 1];  // padded cells c-1..c+1 at c..c+2
-        for (int k = 0; k < 3 && !h; k++)
-            for (int n = ((bits >> k) & 1u) ? tr[k] : -1; n >= 0 && !h;) {
-                const size_t q = (size_t)n * T + t;
-                const double* P = m.pos64 + q * D;
-                h = contact<D>(m, x, P);
-                npts++;
-                n = __float_as_int(m.pos32[q].w);
+        // the occupied cells' heads load together and their chains are walked together
+        int node[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            node[k] = ((bits >> k) & 1u) ? tr[k] : -1;
+        while (!h && (node[0] >= 0 || node[1] >= 0 || node[2] >= 0)) {
+            double P[3][D];
+            int nxt[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                nxt[k] = -1;
+                if (node[k] >= 0) {
+                    const size_t q = (size_t)node[k] * T + t;
+#pragma unroll
+                    for (int a = 0; a < D; a++)
+                        P[k][a] = m.pos64[q * D + a];
+                    nxt[k] = __float_as_int(m.pos32[q].w);
+                }
             }
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (node[k] >= 0) {
+                    h = h || contact<D>(m, x, P[k]);
+                    npts++;
+                    node[k] = nxt[k];
+                }
+        }
     }
     return __ballot(h) != 0;
 }
@@ -719,27 +738,74 @@ __device__ __forceinline__ bool pair_hit(const DevModel& m, const double* x, con
     return h && contact<D>(m, x, y);
 }
 
-// Conflict bits among the first S survivors: mask[i][j/64] bit j%64 set iff i > j collide.
+// Packed cells (10 bits per axis) of the first S survivors at every timestep.
 template <int D>
-__global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos, int S,
+__global__ __launch_bounds__(256) void k_cells(DevModel m, const double* pos, int S, int* cells) {
+    const int64_t n = (int64_t)S * m.T;
+    for (int64_t it = (int64_t)blockIdx.x * 256 + threadIdx.x; it < n; it += (int64_t)gridDim.x * 256) {
+        int v = 0;
+#pragma unroll
+        for (int a = 0; a < D; a++)
+            v |= cell_of(pos[(size_t)it * D + a], m.halfT, m.T) << (10 * a);
+        cells[it] = v;
+    }
+}
+
+// Neighbouring cells on every axis (torus), from packed cells.
+template <int D>
+__device__ __forceinline__ bool cells_adjacent(int u, int v, int T) {
+    bool adj = true;
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        int dc = ((u >> (10 * a)) & 1023) - ((v >> (10 * a)) & 1023);
+        dc = dc < 0 ? -dc : dc;
+        adj = adj && (dc <= 1 || dc == T - 1);
+    }
+    return adj;
+}
+
+// Conflict bits among the first S survivors: mask[i][j/64] bit j%64 set iff i > j collide.
+// One thread per (pair, block of CONF_T timesteps): packed cells first (two ints per
+// timestep), the FP64 contact test only where the cells neighbour -- the canonical pair
+// predicate.
+constexpr int CONF_T = 16;
+
+template <int D>
+__global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos, const int* cells, int S,
                                                    uint64_t* mask) {
-    const int64_t npairs = (int64_t)S * (S - 1) / 2;
-    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < npairs;
-         k += (int64_t)gridDim.x * 256) {
+    const int nblk = (m.T + CONF_T - 1) / CONF_T;
+    const int64_t items = (int64_t)S * (S - 1) / 2 * nblk;
+    for (int64_t it = (int64_t)blockIdx.x * 256 + threadIdx.x; it < items;
+         it += (int64_t)gridDim.x * 256) {
+        const int64_t k = it / nblk;
+        const int t0 = (int)(it % nblk) * CONF_T;
         int64_t i = (int64_t)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
         while (i * (i - 1) / 2 > k)
             i--;
         while ((i + 1) * i / 2 <= k)
             i++;
         const int64_t j = k - i * (i - 1) / 2;
-        const double* xi = pos + (size_t)i * m.T * D;
-        const double* xj = pos + (size_t)j * m.T * D;
-        for (int t = 0; t < m.T; t++)
-            if (pair_hit<D>(m, xi + (size_t)t * D, xj + (size_t)t * D)) {
-                atomicOr((unsigned long long*)&mask[(size_t)i * UG_MASK_WORDS + (j >> 6)],
-                         1ull << (j & 63));
-                break;
+        const int* ci = cells + (size_t)i * m.T;
+        const int* cj = cells + (size_t)j * m.T;
+        const int t1 = min(t0 + CONF_T, m.T);
+        bool hit = false;
+        for (int t = t0; t < t1 && !hit; t += 4) {
+            int a[4], b[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int tt = min(t + u, t1 - 1);
+                a[u] = ci[tt];
+                b[u] = cj[tt];
             }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int tt = min(t + u, t1 - 1);
+                if (!hit && cells_adjacent<D>(a[u], b[u], m.T))
+                    hit = contact<D>(m, pos + ((size_t)i * m.T + tt) * D, pos + ((size_t)j * m.T + tt) * D);
+            }
+        }
+        if (hit)
+            atomicOr((unsigned long long*)&mask[(size_t)i * UG_MASK_WORDS + (j >> 6)], 1ull << (j & 63));
     }
 }
 
@@ -1062,6 +1128,24 @@ int32_t ensure_paths(Pack* h, int64_t need) {
     return rc;
 }
 
+// Conflict bits among the chunk's s survivors (trajectories in d_spos, mask cleared).
+void launch_conflicts(Pack* h, int s) {
+    const int64_t npairs = (int64_t)s * (s - 1) / 2;
+    if (npairs <= 0)
+        return;
+    const int64_t items = npairs * ((h->T + CONF_T - 1) / CONF_T);
+    const int cb = grid_for((int64_t)s * h->T, 256);
+    if (h->d == 2) {
+        hipLaunchKernelGGL(k_cells<2>, dim3(cb), dim3(256), 0, h->stream, h->dm, h->d_spos, s, h->d_scell);
+        hipLaunchKernelGGL(k_conflicts<2>, dim3(grid_for(items, 256)), dim3(256), 0, h->stream, h->dm, h->d_spos,
+                           h->d_scell, s, h->d_mask);
+    } else {
+        hipLaunchKernelGGL(k_cells<3>, dim3(cb), dim3(256), 0, h->stream, h->dm, h->d_spos, s, h->d_scell);
+        hipLaunchKernelGGL(k_conflicts<3>, dim3(grid_for(items, 256)), dim3(256), 0, h->stream, h->dm, h->d_spos,
+                           h->d_scell, s, h->d_mask);
+    }
+}
+
 // Resolve the survivors list[0..S) (already clear of the cell table) in index order and
 // insert the accepted ones.  Survivors beyond one chunk are retested against the updated
 // table before their turn.  Returns the number accepted via *acc_total.
@@ -1078,15 +1162,7 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
             return rc;
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((int64_t)s * UG_MASK_WORDS, 256)), dim3(256), 0,
                            h->stream, h->d_mask, (int64_t)s * UG_MASK_WORDS);
-        const int64_t npairs = (int64_t)s * (s - 1) / 2;
-        if (npairs > 0) {
-            if (h->d == 2)
-                hipLaunchKernelGGL(k_conflicts<2>, dim3(grid_for(npairs, 256)), dim3(256), 0, h->stream,
-                                   h->dm, h->d_spos, s, h->d_mask);
-            else
-                hipLaunchKernelGGL(k_conflicts<3>, dim3(grid_for(npairs, 256)), dim3(256), 0, h->stream,
-                                   h->dm, h->d_spos, s, h->d_mask);
-        }
+        launch_conflicts(h, s);
         hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, h->d_acc_list,
                            h->d_nacc);
         const int limit = h->max_n > 0 ? (int)std::min<int64_t>(h->max_n - h->n, INT32_MAX) : INT32_MAX;
@@ -1197,15 +1273,7 @@ int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_tot
             return rc;
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((int64_t)s * UG_MASK_WORDS, 256)), dim3(256), 0, h->stream,
                            h->d_mask, (int64_t)s * UG_MASK_WORDS);
-        const int64_t npairs = (int64_t)s * (s - 1) / 2;
-        if (npairs > 0) {
-            if (h->d == 2)
-                hipLaunchKernelGGL(k_conflicts<2>, dim3(grid_for(npairs, 256)), dim3(256), 0, h->stream, h->dm,
-                                   h->d_spos, s, h->d_mask);
-            else
-                hipLaunchKernelGGL(k_conflicts<3>, dim3(grid_for(npairs, 256)), dim3(256), 0, h->stream, h->dm,
-                                   h->d_spos, s, h->d_mask);
-        }
+        launch_conflicts(h, s);
         hipLaunchKernelGGL(k_greedy_sub, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, gl, h->d_gsurv,
                            h->d_acc_list, h->d_nacc);
         const int ib = grid_for((int64_t)s * h->T, 256);
