@@ -33,6 +33,11 @@ WORKLOADS = {
     "cfg3": dict(dim=3, T=128, max_attempts=0.0,
                  text="3+1 pack to jamming, F=T=128, batches of 2^20, stop when accepts in the "
                       "last 2^22 candidates < 1e-4*2^22"),
+    # 3+1, F = T = 256, packed to jamming, then 512 equal-time frames with 2048^2 rasters
+    "cfg5": dict(dim=3, T=256, max_attempts=0.0, frames=512, raster=2048,
+                 text="3+1 pack to jamming, F=T=256 (same stop rule), then 512 frames at "
+                      "z_j=(j+1/2)pi/512: per-frame positions of every worldline and 2048^2 "
+                      "float32 density rasters of the (x,y) torus projection"),
 }
 
 
@@ -108,6 +113,31 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def bake_frames(ugpack, torch, packer, nframes: int, px: int) -> dict:
+    """Per-frame positions and density rasters of every accepted worldline (64 frames per
+    call), timed from the device-built worldline set to the last raster."""
+    import math
+
+    import numpy as np
+    from ugpack import frames
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    paths = frames.Paths.from_packer(packer)
+    z = (np.arange(nframes) + 0.5) * math.pi / nframes
+    mass = 0.0
+    for b in range(0, nframes, 64):
+        pos, ras = frames.slices(paths, z[b:b + 64], raster_px=px)
+        mass += float(ras.sum())
+        del pos, ras
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    paths.close()
+    return {"frames": nframes, "raster_px": px, "paths": paths.n, "seconds": round(dt, 4),
+            "ms_per_frame": round(1e3 * dt / nframes, 4),
+            "raster_mass_ok": abs(mass - nframes * paths.n) < 0.5}
+
+
 def cpu_baseline(threads: int = 0) -> dict:
     """The CPU oracle (serial RSA restatement, oracle/) packing config 2 seed 1 to jam with
     the same fixed-window rule: once on all host cores (parallel prefilter per block, serial
@@ -174,6 +204,7 @@ def main() -> int:
             golden[s] = json.load(open(f))["idx"]
 
     results = {}
+    frame_stats = None
     for prec in forms:
         for w in range(args.warmup):
             p, _ = pack_once(ugpack, 1001 + w, prec, workload=args.workload)
@@ -198,6 +229,9 @@ def main() -> int:
         p, st = pack_once(ugpack, 1, prec, profile=True, workload=args.workload)
         stats = p.kernel_stats(ugpack.abi.KERNEL_PREFILTER)
         rstats = p.kernel_stats(ugpack.abi.KERNEL_RECHECK)
+        wl = WORKLOADS[args.workload]
+        if wl.get("frames") and prec == prec_main:
+            frame_stats = bake_frames(ugpack, torch, p, wl["frames"], wl["raster"])
         p.close()
         results[prec] = dict(value=cands / wall, wall=wall, per_pack=per_pack, accepted=accepted,
                              parity=parity, stats=stats, rstats=rstats, cands=cands)
@@ -240,6 +274,8 @@ def main() -> int:
     if split:
         line["recheck"] = main_r["rstats"]
         line["split_error"] = split_error(ugpack, torch, args.workload)
+    if frame_stats:
+        line["frames"] = frame_stats
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
     print(json.dumps(line))
     return 0

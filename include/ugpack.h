@@ -190,6 +190,11 @@ int32_t ug_paths_create(int32_t dim, int64_t n, int32_t nw, const double* a2, co
                         const double* freq, const double* phase, void* hip_stream, ug_paths** out);
 void ug_paths_destroy(ug_paths* p);
 
+/* The worldline set of a packing (uniques, then subpaths), built on the device from the
+ * accepted candidate indices: no host round trip of the dump parameters (no reference
+ * counterpart; the reference bakes from the dump CSV, cuda/frame_bake.cu:109-190). */
+int32_t ug_paths_from_pack(const ug_pack* h, void* hip_stream, ug_paths** out);
+
 /* Front of one observation (the --front/--metric/--speed/--zmin/--wraps flags,
  * cuda/frame_bake.cu:409-453). */
 typedef struct ug_front {

@@ -115,3 +115,28 @@ def test_slices_and_rasters(gpu):
         h = np.zeros((256, 256))
         np.add.at(h, (cy, cx), 1.0)
         assert np.abs(h - ras[j]).sum() <= 2 * max(1, int(0.002 * n))
+
+
+@pytest.mark.parametrize("d,T,terms,seed", [(2, 32, 0, 3), (3, 16, 0, 1), (2, 24, 5, 2)])
+def test_paths_from_pack_equals_dump_parameters(gpu, d, T, terms, seed):
+    """The device-built worldline set equals the one built from the dump parameters."""
+    import torch
+
+    import ugpack
+    from ugpack import frames
+
+    with ugpack.Packer(d, T, terms=terms, seed=seed, max_attempts=20000) as p:
+        p.run()
+        dev = frames.Paths.from_packer(p)
+        host = frames.Paths.from_params(p)
+        z = np.linspace(0.05, math.pi - 0.05, 17)
+        a, _ = frames.slices(dev, z, raster_px=64)
+        b, _ = frames.slices(host, z, raster_px=64)
+    assert dev.n == host.n > 5
+    diff = (a - b).abs()
+    diff = torch.minimum(diff, 2.0 - diff)
+    assert float(diff.max()) < 1e-6
+    if d == 2:
+        got = frames.bake(dev, 0.5 * math.pi, 0.1 * math.pi, observer_path=1, front="fit")
+        want = frames.bake(host, 0.5 * math.pi, 0.1 * math.pi, observer_path=1, front="fit")
+        match_hits(got, want, 1e-5)

@@ -297,24 +297,45 @@ struct ug_paths {
 
 namespace {
 
-void host_comov(const ug_paths* P, int p, double z, double* x, double* v) {
-    // Same dense form as the device, with libm sin/cos per frequency.
+int32_t host_comov(const ug_paths* P, int p, double z, double* x, double* v) {
+    // Same dense form as the device, with libm sin/cos per frequency.  Sets built on the
+    // device (ug_paths_from_pack) keep no host copy: the one path is fetched.
+    const size_t W = (size_t)P->kmax + 1, row = (size_t)P->dim * W;
+    std::vector<double> a2v, Av, Bv;
+    const double *A = P->A.data(), *B = P->B.data(), *A2 = P->a2.data();
+    if (P->A.empty()) {
+        a2v.resize(P->dim);
+        Av.resize(row);
+        Bv.resize(row);
+        int32_t rc = hip_check(hipMemcpy(a2v.data(), P->d_a2 + (size_t)p * P->dim, P->dim * 8, hipMemcpyDeviceToHost),
+                               "observer a2");
+        if (!rc)
+            rc = hip_check(hipMemcpy(Av.data(), P->d_A + (size_t)p * row, row * 8, hipMemcpyDeviceToHost), "observer A");
+        if (!rc)
+            rc = hip_check(hipMemcpy(Bv.data(), P->d_B + (size_t)p * row, row * 8, hipMemcpyDeviceToHost), "observer B");
+        if (rc)
+            return rc;
+        A = Av.data() - (size_t)p * row;
+        B = Bv.data() - (size_t)p * row;
+        A2 = a2v.data() - (size_t)p * P->dim;
+    }
     const double s = std::sin(z), c = std::cos(z);
     for (int a = 0; a < P->dim; a++) {
         double xx = 0.0, dd = 0.0;
         for (int k = 1; k <= P->kmax; k++) {
-            const size_t o = ((size_t)p * P->dim + a) * (P->kmax + 1) + k;
+            const size_t o = ((size_t)p * P->dim + a) * W + k;
             const double skz = std::sin(k * z), ckz = std::cos(k * z);
-            xx += P->A[o] * skz + P->B[o] * (ckz - 1.0);
-            dd += k * (P->A[o] * ckz - P->B[o] * skz);
+            xx += A[o] * skz + B[o] * (ckz - 1.0);
+            dd += k * (A[o] * ckz - B[o] * skz);
         }
-        const double a2 = P->a2[(size_t)p * P->dim + a];
+        const double a2 = A2[(size_t)p * P->dim + a];
         x[a] = a2 + xx / s;
         // d/dz [a2 sin z + xx] / sin z, as comov_vel_xy (cuda/frame_bake.cu:215-228)
         const double num = a2 * s + xx, dnum = a2 * c + dd;
         if (v)
             v[a] = (dnum * s - num * c) / (s * s);
     }
+    return UG_OK;
 }
 
 }  // namespace
@@ -387,6 +408,43 @@ extern "C" void ug_paths_destroy(ug_paths* P) {
     delete P;
 }
 
+extern "C" int32_t ug_paths_from_pack(const ug_pack* hp, void* hip_stream, ug_paths** out) {
+    if (!hp || !out)
+        return fail(UG_EINVAL, "ug_paths_from_pack: NULL argument");
+    const ug::Pack* h = (const ug::Pack*)hp;
+    const int64_t n = h->n + h->nsub;
+    ug_paths* P = new ug_paths();
+    P->dim = h->d;
+    P->n = n;
+    P->kmax = h->T;
+    P->stream = (hipStream_t)hip_stream;
+    const size_t W = (size_t)P->kmax + 1;
+    int32_t rc = UG_OK;
+    int* d_bmax = nullptr;
+    if (hipMalloc((void**)&P->d_a2, std::max<size_t>(8, (size_t)n * P->dim * 8)) != hipSuccess ||
+        hipMalloc((void**)&P->d_A, std::max<size_t>(8, (size_t)n * P->dim * W * 8)) != hipSuccess ||
+        hipMalloc((void**)&P->d_B, std::max<size_t>(8, (size_t)n * P->dim * W * 8)) != hipSuccess ||
+        hipMalloc((void**)&d_bmax, 4) != hipSuccess)
+        rc = fail(UG_ENOMEM, "ug_paths_from_pack: device allocation failed");
+    if (!rc && n > 0) {
+        rc = ug::launch_paths_from_pack(const_cast<ug::Pack*>(h), P->stream, P->d_a2, P->d_A, P->d_B, P->kmax, d_bmax);
+        int bmax = 2;
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(&bmax, d_bmax, 4, hipMemcpyDeviceToHost, P->stream), "bmax copy");
+        if (!rc)
+            rc = hip_check(hipStreamSynchronize(P->stream), "paths sync");
+        P->bmax_terms = std::max(2, bmax);
+    }
+    if (d_bmax)
+        (void)hipFree(d_bmax);
+    if (rc) {
+        ug_paths_destroy(P);
+        return rc;
+    }
+    *out = P;
+    return UG_OK;
+}
+
 static DevPaths dev_of(const ug_paths* P) {
     DevPaths D;
     D.n = (int)P->n;
@@ -457,7 +515,11 @@ extern "C" int32_t ug_frame_bake_lightcone(ug_paths* P, const ug_front* fr, doub
     double ox, oy, bx = 0.0, by = 0.0;
     if (obs->type == 0) {
         double x[3], v[3];
-        host_comov(P, obs->path, z_obs, x, v);
+        {
+            const int32_t hc = host_comov(P, obs->path, z_obs, x, v);
+            if (hc)
+                return hc;
+        }
         ox = x[0];
         oy = x[1];
         bx = std::sin(z_obs) * v[0] / F.spd;

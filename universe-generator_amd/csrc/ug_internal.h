@@ -157,6 +157,11 @@ int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, con
                 int64_t* out, int64_t* S, const int* dcount = nullptr, int slot = 1, bool sync = true,
                 const int* vin = nullptr, int* vout = nullptr);
 // subpath phase (ug_pack.hip)
+// Baker coefficients of every stored path (uniques, then subpaths) from its candidate index:
+// a2[p][d], A/B[p][d][kmax+1] (A_k = a_k cos f_k, B_k = a_k sin f_k), largest frequency into
+// *d_bmax; enqueued on `stream` after the handle's own work.
+int32_t launch_paths_from_pack(Pack* h, hipStream_t stream, double* a2, double* A, double* B, int kmax,
+                               int* d_bmax);
 int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out, int* gout,
                    int64_t* S, int* rs_count);
 int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_total);

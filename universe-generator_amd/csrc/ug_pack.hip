@@ -468,6 +468,46 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
     }
 }
 
+// Baker coefficients of stored path p (one wave per path): the canonical per-slot
+// (a cos f, a sin f) of its candidate, scattered by frequency into dense rows.
+template <int D, int PL>
+__global__ __launch_bounds__(64) void k_paths_from_pack(DevModel m, const int64_t* acc_idx, int64_t n,
+                                                        double* a2, double* A, double* B, int kmax,
+                                                        int* bmax) {
+    extern __shared__ __align__(16) int dyn_lds[];
+    const int lane = threadIdx.x;
+    int* sb = dyn_lds;
+    uint32_t* used = (uint32_t*)(sb + D * m.nw);
+    const size_t W = (size_t)kmax + 1;
+    int bm = 0;
+    for (int64_t p = blockIdx.x; p < n; p += gridDim.x) {
+        WaveCoef<D, PL> w;
+        gen_wave64<D, PL>(m, (uint64_t)acc_idx[p], lane, w, sb, used);
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            if (lane == 0)
+                a2[(size_t)p * D + a] = w.a1[a];
+#pragma unroll
+            for (int ii = 0; ii < PL; ii++)
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const int j = 2 * (lane + 64 * ii) + k;
+                    if (j < m.nw) {
+                        const int b = w.b[a][ii][k];
+                        A[((size_t)p * D + a) * W + b] = w.c[a][ii][k];
+                        B[((size_t)p * D + a) * W + b] = w.s[a][ii][k];
+                        bm = max(bm, b);
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    for (int s = 32; s >= 1; s >>= 1)
+        bm = max(bm, __shfl_xor(bm, s, 64));
+    if (lane == 0)
+        atomicMax(bmax, bm);
+}
+
 // Subpath test (cuda/braid_cuda.cu:452-465 sub branch, host_sub_gid :726-750): every
 // timestep is scanned; the contacted groups are reduced across lanes with min/max, and the
 // candidate fails as soon as two distinct groups appear.  Paths stored at rows < nlimit are
@@ -1321,6 +1361,45 @@ int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_tot
         }
     }
     return UG_OK;
+}
+
+int32_t launch_paths_from_pack(Pack* h, hipStream_t stream, double* a2, double* A, double* B, int kmax,
+                               int* d_bmax) {
+    const int64_t n = h->n + h->nsub;
+    const size_t W = (size_t)kmax + 1;
+    // the coefficient rows are filled after the handle's queued work, on the given stream
+    int32_t rc = hip_check(hipStreamSynchronize(h->stream), "paths: handle sync");
+    if (!rc)
+        rc = hip_check(hipMemsetAsync(A, 0, (size_t)n * h->d * W * 8, stream), "paths A clear");
+    if (!rc)
+        rc = hip_check(hipMemsetAsync(B, 0, (size_t)n * h->d * W * 8, stream), "paths B clear");
+    if (!rc)
+        rc = hip_check(hipMemsetAsync(d_bmax, 0, 4, stream), "paths bmax clear");
+    if (rc)
+        return rc;
+    const int blocks = (int)std::min<int64_t>(n, 256 * 32);
+    const size_t lds = (size_t)4 * (h->d * h->nw + (h->modmax + 2 + 31) / 32);
+    const int pl = slot_pl(h->npairs);
+#define UG_PFP(DD, PP)                                                                                   \
+    hipLaunchKernelGGL((k_paths_from_pack<DD, PP>), dim3(blocks), dim3(64), lds, stream, h->dm, h->d_acc_idx, n, \
+                       a2, A, B, kmax, d_bmax)
+    if (h->d == 2) {
+        if (pl == 1)
+            UG_PFP(2, 1);
+        else if (pl == 2)
+            UG_PFP(2, 2);
+        else
+            UG_PFP(2, 4);
+    } else {
+        if (pl == 1)
+            UG_PFP(3, 1);
+        else if (pl == 2)
+            UG_PFP(3, 2);
+        else
+            UG_PFP(3, 4);
+    }
+#undef UG_PFP
+    return hip_check(hipGetLastError(), "k_paths_from_pack");
 }
 
 }  // namespace ug

@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
         "ug_pack_destroy": ([vp], None),
         "ug_paths_create": ([i32, i64, i32, vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
         "ug_paths_destroy": ([vp], None),
+        "ug_paths_from_pack": ([vp, vp, C.POINTER(vp)], i32),
         "ug_frame_bake_lightcone": ([vp, C.POINTER(Front), C.c_double, C.POINTER(Observer), vp,
                                      C.c_uint64, C.POINTER(FrameInfo)], i32),
         "ug_frame_bake_slices": ([vp, vp, i32, vp, vp, i32], i32),
@@ -124,7 +125,8 @@ def exported_symbols() -> list[str]:
         "ug_abi_version", "ug_last_error", "ug_pack_create", "ug_pack_run", "ug_pack_run_sub",
         "ug_pack_get_status", "ug_pack_accepted", "ug_pack_params", "ug_pack_positions",
         "ug_pack_curve", "ug_eval_candidates", "ug_pack_profile", "ug_pack_kernel_stats",
-        "ug_pack_destroy", "ug_paths_create", "ug_paths_destroy", "ug_frame_bake_lightcone",
+        "ug_pack_destroy", "ug_paths_create", "ug_paths_destroy", "ug_paths_from_pack",
+        "ug_frame_bake_lightcone",
         "ug_frame_bake_slices",
     ]
 

@@ -90,8 +90,19 @@ class Paths:
         return cls(a2, amp, freq, phase)
 
     @classmethod
-    def from_packer(cls, packer) -> "Paths":
-        """The accepted worldlines of a ugpack.Packer, in admission order."""
+    def from_packer(cls, packer, stream: int = 0) -> "Paths":
+        """The accepted worldlines of a ugpack.Packer, in admission order, built on the
+        device from the accepted candidate indices (no host copy of the parameters)."""
+        h = C.c_void_p()
+        check(lib().ug_paths_from_pack(packer._h, C.c_void_p(stream), C.byref(h)))
+        self = cls.__new__(cls)
+        st = packer.status()
+        self._h, self.n, self.dim, self.nw = h, st.n + st.nsub, packer.dim, packer.nw
+        return self
+
+    @classmethod
+    def from_params(cls, packer) -> "Paths":
+        """The same set through the host dump parameters (ug_pack_params)."""
         par = packer.params()  # (n, dim, 1 + 3 nw)
         return cls(par[:, :, 0], par[:, :, 1::3], par[:, :, 2::3], par[:, :, 3::3])
 
