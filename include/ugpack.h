@@ -177,6 +177,17 @@ int32_t ug_pack_kernel_stats(const ug_pack* h, int32_t which, ug_kernel_stats* o
 /* Releases the handle and its device memory. */
 void ug_pack_destroy(ug_pack* h);
 
+/* Packs n independent universes (cfgs[i], each run as ug_pack_run with max_candidates 0)
+ * on the current device, `threads` at a time: one host worker thread and one HIP stream
+ * each, universes taken in index order from a shared counter (no reference counterpart:
+ * the reference runs one universe per engine process, braidlab/orchestrator.py).
+ * status[i] receives universe i's final status.  keep == NULL: every handle is destroyed
+ * after its run; else keep[i] receives universe i's handle (on the null stream) for the
+ * caller to query and destroy.  On error the remaining universes are not started and the
+ * first failure's message is in ug_last_error(). */
+int32_t ug_ensemble_run(const ug_pack_config* cfgs, int32_t n, int32_t threads, ug_pack** keep,
+                        ug_pack_status* status);
+
 /* ------------------------------------------------------------------ frame baker
  * Replaces the frame_bake executable (cuda/frame_bake.cu:398-635), whose BRF1 output
  * braidlab/frames.py:88-144 reads. */

@@ -1,0 +1,87 @@
+// ug_ensemble.hip -- many independent universes on one GPU (the batched-ensemble workload:
+// uneven T and dimension, distinct seeds).
+//
+// A universe's pack is a chain of dependent launches with a host round trip per
+// sub-batch, so one universe leaves most of the GPU idle.  The ensemble runner keeps
+// `threads` universes in flight: each worker thread owns one HIP stream (default priority)
+// and takes the next universe from a shared counter, so the launches of different
+// universes overlap on the device while each worker waits on its own stream.  Results do
+// not depend on the thread count: every universe is the deterministic pack of its config.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "ug_internal.h"
+
+extern "C" int32_t ug_ensemble_run(const ug_pack_config* cfgs, int32_t n, int32_t threads,
+                                   ug_pack** keep, ug_pack_status* status) {
+    if (!cfgs || !status || n < 0)
+        return ug::fail(UG_EINVAL, "ug_ensemble_run: NULL argument or negative count");
+    if (n == 0)
+        return UG_OK;
+    if (threads < 1)
+        threads = 1;
+    threads = std::min(threads, n);
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess)
+        return ug::fail(UG_EHIP, "ug_ensemble_run: no current device");
+    std::atomic<int> next{0};
+    std::atomic<bool> stop{false};
+    std::mutex err_mu;
+    int32_t err_rc = UG_OK;
+    std::string err_msg;
+    auto record = [&](int32_t rc, const std::string& msg) {
+        std::lock_guard<std::mutex> g(err_mu);
+        if (err_rc == UG_OK) {
+            err_rc = rc;
+            err_msg = msg;
+        }
+        stop = true;
+    };
+    auto worker = [&]() {
+        if (hipSetDevice(device) != hipSuccess) {
+            record(UG_EHIP, "ug_ensemble_run: hipSetDevice failed in a worker");
+            return;
+        }
+        // kept handles outlive the call, so they use the device's null stream
+        hipStream_t s = nullptr;
+        if (!keep && hipStreamCreate(&s) != hipSuccess) {
+            record(UG_EHIP, "ug_ensemble_run: hipStreamCreate failed");
+            return;
+        }
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= n || stop)
+                break;
+            ug_pack* h = nullptr;
+            int32_t rc = ug_pack_create(&cfgs[i], (void*)s, &h);
+            if (rc == UG_OK)
+                rc = ug_pack_run(h, 0, &status[i]);
+            if (rc != UG_OK) {
+                record(rc, std::string("universe ") + std::to_string(i) + ": " + ug_last_error());
+                if (h)
+                    ug_pack_destroy(h);
+                break;
+            }
+            if (keep)
+                keep[i] = h;
+            else
+                ug_pack_destroy(h);
+        }
+        if (s)
+            (void)hipStreamDestroy(s);
+    };
+    std::vector<std::thread> pool;
+    pool.reserve((size_t)threads);
+    for (int t = 0; t < threads; t++)
+        pool.emplace_back(worker);
+    for (auto& th : pool)
+        th.join();
+    if (err_rc != UG_OK)
+        return ug::fail(err_rc, "%s", err_msg.c_str());
+    return UG_OK;
+}

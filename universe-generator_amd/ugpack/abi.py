@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
         "ug_pack_profile": ([vp, i32], i32),
         "ug_pack_kernel_stats": ([vp, i32, C.POINTER(KernelStats)], i32),
         "ug_pack_destroy": ([vp], None),
+        "ug_ensemble_run": ([C.POINTER(PackConfig), i32, i32, C.POINTER(vp), C.POINTER(PackStatus)], i32),
         "ug_paths_create": ([i32, i64, i32, vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
         "ug_paths_destroy": ([vp], None),
         "ug_paths_from_pack": ([vp, vp, C.POINTER(vp)], i32),
@@ -125,7 +126,7 @@ def exported_symbols() -> list[str]:
         "ug_abi_version", "ug_last_error", "ug_pack_create", "ug_pack_run", "ug_pack_run_sub",
         "ug_pack_get_status", "ug_pack_accepted", "ug_pack_params", "ug_pack_positions",
         "ug_pack_curve", "ug_eval_candidates", "ug_pack_profile", "ug_pack_kernel_stats",
-        "ug_pack_destroy", "ug_paths_create", "ug_paths_destroy", "ug_paths_from_pack",
+        "ug_pack_destroy", "ug_ensemble_run", "ug_paths_create", "ug_paths_destroy", "ug_paths_from_pack",
         "ug_frame_bake_lightcone",
         "ug_frame_bake_slices",
     ]
@@ -175,6 +176,16 @@ class Packer:
         self.dim, self.T = dim, T
         self.terms = terms if terms else T
         self.nw = self.terms - 1
+
+    @classmethod
+    def _adopt(cls, h, dim: int, T: int, terms: int) -> "Packer":
+        """Wrap a handle the library created (ensemble runs); the wrapper destroys it."""
+        self = cls.__new__(cls)
+        self._h = h
+        self.dim, self.T = dim, T
+        self.terms = terms if terms else T
+        self.nw = self.terms - 1
+        return self
 
     def close(self) -> None:
         if self._h is not None:
@@ -265,3 +276,28 @@ class Packer:
         check(lib().ug_eval_candidates(self._h, idx.data_ptr(), idx.numel(), precision,
                                        out.data_ptr()))
         return out
+
+
+def pack_config(dim: int, T: int, terms: int = 0, seed: int = 1, precision: int = PREC_FP64,
+                pin_sin1: bool = False, stop_kind: int = STOP_BUDGET, max_attempts: float = 0.0,
+                stop_rate: float = 0.0, stop_batch: int = 1 << 20, stop_window: int = 1 << 22,
+                max_n: int = 0, margin: float = 0.0, max_paths: int = 0,
+                euclid: bool = False) -> PackConfig:
+    return PackConfig(dim, T, terms, precision, seed, int(pin_sin1), stop_kind, float(max_attempts),
+                      float(stop_rate), stop_batch, stop_window, max_n, float(margin), max_paths,
+                      int(euclid), 0)
+
+
+def ensemble(configs: list[PackConfig], threads: int = 16, keep: bool = False):
+    """Pack independent universes `threads` at a time (ug_ensemble_run).  Returns the list
+    of final Status, and with keep=True also the list of Packer handles."""
+    n = len(configs)
+    arr = (PackConfig * n)(*configs)
+    st = (PackStatus * n)()
+    handles = (C.c_void_p * n)() if keep else None
+    check(lib().ug_ensemble_run(arr, n, threads, handles, st))
+    status = [Status.of(st[i]) for i in range(n)]
+    if not keep:
+        return status
+    packers = [Packer._adopt(C.c_void_p(handles[i]), c.dim, c.T, c.terms) for i, c in enumerate(configs)]
+    return status, packers
