@@ -33,6 +33,11 @@ WORKLOADS = {
     "cfg3": dict(dim=3, T=128, max_attempts=0.0,
                  text="3+1 pack to jamming, F=T=128, batches of 2^20, stop when accepts in the "
                       "last 2^22 candidates < 1e-4*2^22"),
+    # 512 independent universes: T uniform in {16, 32, 64, 128}, dim 2 or 3 (p = 1/2), seeds
+    # 1..512, each packed to 10^7 candidates, 16 in flight (one stream each)
+    "cfg4": dict(dim=0, T=0, max_attempts=1e7, universes=512,
+                 text="ensemble of 512 universes, T uniform in {16,32,64,128}, dim 2|3 with "
+                      "p=1/2, seeds 1..512, each packed to 10^7 candidates"),
     # 3+1, F = T = 256, packed to jamming, then 512 equal-time frames with 2048^2 rasters
     "cfg5": dict(dim=3, T=256, max_attempts=0.0, frames=512, raster=2048,
                  text="3+1 pack to jamming, F=T=256 (same stop rule), then 512 frames at "
@@ -138,6 +143,67 @@ def bake_frames(ugpack, torch, packer, nframes: int, px: int) -> dict:
             "raster_mass_ok": abs(mass - nframes * paths.n) < 0.5}
 
 
+def ensemble_configs(ugpack, n: int, cands: float, precision: int):
+    """The ensemble's universes: T and dim drawn from a fixed-seed generator."""
+    import numpy as np
+
+    rng = np.random.default_rng(20260709)
+    Ts = rng.choice([16, 32, 64, 128], size=n)
+    ds = rng.choice([2, 3], size=n)
+    return [ugpack.pack_config(int(d), int(T), seed=i + 1, max_attempts=cands, precision=precision)
+            for i, (d, T) in enumerate(zip(ds, Ts))]
+
+
+def run_ensemble(args, ugpack, torch, forms, prec_main) -> int:
+    """One step = the whole ensemble; value = all universes' candidates / wall time."""
+    w = WORKLOADS["cfg4"]
+    n, cands, threads = w["universes"], w["max_attempts"], args.threads
+    res = {}
+    for prec in forms:
+        cfgs = ensemble_configs(ugpack, n, cands, prec)
+        ugpack.ensemble(ensemble_configs(ugpack, 32, 1e6, prec), threads=threads)  # warmup
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = ugpack.ensemble(cfgs, threads=threads)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / args.steps
+        res[prec] = dict(wall=wall, value=n * cands / wall, n=[s.n for s in st],
+                         attempts=sum(s.attempts for s in st))
+    # roofline of the dominant universe class (T=128, 3+1) from a profiled pack at 10^7
+    p = ugpack.Packer(3, 128, seed=1, precision=prec_main, max_attempts=cands)
+    p.profile(True)
+    p.run()
+    stats = p.kernel_stats(ugpack.abi.KERNEL_PREFILTER)
+    p.close()
+    main_r = res[prec_main]
+    cfgs = ensemble_configs(ugpack, n, cands, prec_main)
+    split = prec_main == ugpack.PREC_SPLIT_BF16
+    line = {
+        "metric": "candidates tested/s & s to jam, 2+1 F=64; split-BF16 vs FP64 GPU, same accepts",
+        "value": round(main_r["value"], 1), "unit": "candidates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": round(1e3 * main_r["wall"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f32acc, f64 recheck" if split else "f64",
+        "data": "synthetic (Philox candidate streams, seeds 1..512)",
+        "config": {"workload": w["text"], "universes": n, "candidates_per_universe": cands,
+                   "threads": threads,
+                   "T_counts": {str(T): sum(1 for c in cfgs if c.T == T) for T in (16, 32, 64, 128)},
+                   "dim3_count": sum(1 for c in cfgs if c.dim == 3),
+                   "precision": "split-bf16" if split else "fp64"},
+        "accepted_total": sum(main_r["n"]),
+        "roofline": dict(roofline_of(stats, 3, 128, split), class_profiled="T=128 3+1 universe, 10^7 candidates"),
+    }
+    if ugpack.PREC_FP64 in res and split:
+        f = res[ugpack.PREC_FP64]
+        line["fp64_form"] = {"value": round(f["value"], 1), "ms_per_step": round(1e3 * f["wall"], 3),
+                             "same_accepts_as_split": f["n"] == main_r["n"]}
+        line["speedup_split_vs_fp64"] = round(main_r["value"] / f["value"], 3)
+    line["cpu_baseline"] = None
+    print(json.dumps(line))
+    return 0
+
+
 def cpu_baseline(threads: int = 0) -> dict:
     """The CPU oracle (serial RSA restatement, oracle/) packing config 2 seed 1 to jam with
     the same fixed-window rule: once on all host cores (parallel prefilter per block, serial
@@ -172,6 +238,7 @@ def main() -> int:
     ap.add_argument("--precision", default="both", choices=["split", "fp64", "both"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--threads", type=int, default=16, help="ensemble workers (cfg4)")
     ap.add_argument("--calib", action="store_true",
                     help="first copy 256 MiB device-to-device (counter calibration for rocprof --pmc)")
     args = ap.parse_args()
@@ -194,6 +261,8 @@ def main() -> int:
         del src, dst
     prec_main = ugpack.PREC_SPLIT_BF16 if args.precision in ("split", "both") else ugpack.PREC_FP64
     forms = [prec_main] + ([ugpack.PREC_FP64] if args.precision == "both" else [])
+    if args.workload == "cfg4":
+        return run_ensemble(args, ugpack, torch, forms, prec_main)
 
     golden = {}
     for s in range(1, 9):
