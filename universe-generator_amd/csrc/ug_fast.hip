@@ -260,13 +260,11 @@ __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t 
                 }
                 nvis++;
                 constexpr int ROWS = D == 2 ? 3 : 9;
-                // table row of neighbour row rr (x offset rr/3 - 1, y offset rr%3 - 1 in 3+1)
+                // neighbour row rr: x offset rr/3 - 1 and y offset rr%3 - 1 (3+1), x offset rr - 1 (2+1)
+                auto gx_of = [&](int rr) { return wrap_cell(c[0], (D == 2 ? rr : rr / 3) - 1, m.T); };
+                auto gy_of = [&](int rr) { return D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, m.T); };
                 auto row_of = [&](int rr) -> int64_t {
-                    if (D == 2)
-                        return (int64_t)t * m.T + (c[0] + rr - 1 + m.T) % m.T;
-                    const int gx = (c[0] + rr / 3 - 1 + m.T) % m.T;
-                    const int gy = (c[1] + rr % 3 - 1 + m.T) % m.T;
-                    return ((int64_t)t * m.T + gx) * m.T + gy;
+                    return D == 2 ? (int64_t)t * m.T + gx_of(rr) : ((int64_t)t * m.T + gx_of(rr)) * m.T + gy_of(rr);
                 };
                 // occupancy of all neighbour cells first (independent loads, issued together)
                 // as one mask; then up to 2 occupied cells at a time load their heads together

@@ -65,10 +65,16 @@ __host__ __device__ inline bool contact(const DevModel& m, const double* x, cons
     return h;
 }
 
+// Cell index c + d (d in {-1, 0, 1}) wrapped onto [0, T).
+__host__ __device__ inline int wrap_cell(int c, int d, int T) {
+    const int v = c + d;
+    return v < 0 ? v + T : (v >= T ? v - T : v);
+}
+
 // Occupancy of the padded cells c..c+2 (cells c-1..c+1 of the unpadded row) of table row
-// `row` (row = t*T + x in 2+1, (t*T + x)*T + y in 3+1) as bits 0..2.  The bit rows are a
-// 32nd of the head table, small enough to stay in the last-level cache, so empty
-// neighbour cells cost no head-table traffic.
+// `row` (row = t*T + x in 2+1, (t*T + x)*T + y in 3+1) as bits 0..2: one two-word window, no
+// branches.  The bit rows are a 32nd of the head table, small enough to stay in the
+// last-level cache at 3+1 T=128, so empty neighbour cells cost no head-table traffic.
 __device__ __forceinline__ uint32_t occ_bits3(const DevModel& m, int64_t row, int c) {
     const uint32_t* w = m.occ + row * m.occw + (c >> 5);
     const uint64_t v = (uint64_t)w[0] | ((uint64_t)w[1] << 32);

@@ -312,14 +312,9 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
             x[a] = pos[sl * D + a];
             c[a] = cell_of(x[a], m.halfT, T);
         }
-        int64_t rid;
-        if (D == 2) {
-            rid = (int64_t)t * T + (c[0] + rr - 1 + T) % T;
-        } else {
-            const int gx = (c[0] + rr / 3 - 1 + T) % T;
-            const int gy = (c[1] + rr % 3 - 1 + T) % T;
-            rid = ((int64_t)t * T + gx) * T + gy;
-        }
+        const int gx = wrap_cell(c[0], (D == 2 ? rr : rr / 3) - 1, T);
+        const int gy = D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, T);
+        const int64_t rid = D == 2 ? (int64_t)t * T + gx : ((int64_t)t * T + gx) * T + gy;
         const uint32_t bits = occ_bits3(m, rid, c[D - 1]);
         const int* tr = m.tab + rid * m.rowlen + c[D - 1];  // padded cells c-1..c+1 at c..c+2
         // the occupied cells' heads load together and their chains are walked together
@@ -578,14 +573,9 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
                         x[a] = pos[(r0 + sl) * D + a];
                         c[a] = cell_of(x[a], m.halfT, T);
                     }
-                    int64_t rid;
-                    if (D == 2) {
-                        rid = (int64_t)t * T + (c[0] + rr - 1 + T) % T;
-                    } else {
-                        const int gx = (c[0] + rr / 3 - 1 + T) % T;
-                        const int gy = (c[1] + rr % 3 - 1 + T) % T;
-                        rid = ((int64_t)t * T + gx) * T + gy;
-                    }
+                    const int gx = wrap_cell(c[0], (D == 2 ? rr : rr / 3) - 1, T);
+                    const int gy = D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, T);
+                    const int64_t rid = D == 2 ? (int64_t)t * T + gx : ((int64_t)t * T + gx) * T + gy;
                     const uint32_t bits = occ_bits3(m, rid, c[D - 1]);
                     const int* tr = m.tab + rid * m.rowlen + c[D - 1];
                     for (int k = 0; k < 3; k++)
