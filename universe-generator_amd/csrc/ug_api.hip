@@ -403,15 +403,18 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         const uint64_t base = (uint64_t)h->attempts;
         int64_t S = 0;
         const int64_t rechecked_before = h->rechecked;
-        rc = filter_candidates(h, base, sub, nullptr, h->d_list, &S);
+        // survivors stay on the device: the resolve reads their count there, and it comes
+        // back with the resolve's results (one host round trip per sub-batch)
+        rc = filter_candidates(h, base, sub, nullptr, h->d_list, &S, false);
         if (rc)
             break;
-        h->survivors += S;
         int64_t acc = 0;
         const int64_t n_before = h->n;
-        rc = resolve_and_insert(h, h->d_list, S, &acc);
+        rc = resolve_and_insert(h, h->d_list, 0, &acc, h->d_count + 1);
         if (rc)
             break;
+        S = h->last_S;
+        h->survivors += S;
         if (h->max_n > 0 && h->n >= h->max_n) {
             h->attempts = h->acc_host[(size_t)h->max_n - 1] + 1;
             h->done = true;

@@ -126,6 +126,7 @@ struct Pack {
     int* d_gsurv = nullptr;     // adopted groups of a resolve chunk
     int* d_rscount = nullptr;   // survivors of a round against its round-start packing
     int sub_done = 0;  // UG_SUB_* once the subpath phase stopped
+    int64_t last_S = 0;  // survivors of the last filter whose count stayed on the device
     // sub_batch: the round size, adapted as cuda/braid_cuda.cu:952,1000-1003;
     // sub_nlimit: rows stored before the current round started
     int64_t sub_attempts = 0, sub_batch = 1 << 16, sub_nlimit = 0;
@@ -172,9 +173,9 @@ int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, i
                    int64_t* S, int* rs_count);
 int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_total);
 int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out,
-                          int64_t* S);
+                          int64_t* S, bool sync = true);
 int32_t ensure_paths(Pack* h, int64_t need);
-int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc);
+int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total, const int* dS = nullptr);
 // split-BF16 prefilter (ug_fast.hip)
 int32_t fast_setup(Pack* h);
 int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags);

@@ -770,7 +770,9 @@ __device__ __forceinline__ bool pair_hit(const DevModel& m, const double* x, con
 
 // Packed cells (10 bits per axis) of the first S survivors at every timestep.
 template <int D>
-__global__ __launch_bounds__(256) void k_cells(DevModel m, const double* pos, int S, int* cells) {
+__global__ __launch_bounds__(256) void k_cells(DevModel m, const double* pos, int S, const int* dS, int* cells) {
+    if (dS)
+        S = min(S, *dS);
     const int64_t n = (int64_t)S * m.T;
     for (int64_t it = (int64_t)blockIdx.x * 256 + threadIdx.x; it < n; it += (int64_t)gridDim.x * 256) {
         int v = 0;
@@ -802,7 +804,9 @@ constexpr int CONF_T = 16;
 
 template <int D>
 __global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos, const int* cells, int S,
-                                                   uint64_t* mask) {
+                                                   const int* dS, uint64_t* mask) {
+    if (dS)
+        S = min(S, *dS);
     const int nblk = (m.T + CONF_T - 1) / CONF_T;
     const int64_t items = (int64_t)S * (S - 1) / 2 * nblk;
     for (int64_t it = (int64_t)blockIdx.x * 256 + threadIdx.x; it < items;
@@ -841,8 +845,10 @@ __global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos
 
 // Greedy in index order over the conflict bits (one wave): survivor i is accepted iff it
 // conflicts with no earlier accepted survivor -- serial RSA restricted to the chunk.
-__global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, int* acc_list,
+__global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, const int* dS, int* acc_list,
                                                int* nacc) {
+    if (dS)
+        S = min(S, *dS);
     __shared__ uint64_t accw[UG_MASK_WORDS];
     const int lane = threadIdx.x;
     int na = 0;
@@ -1076,11 +1082,12 @@ int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, con
 }
 
 // Candidates [base, base+count) or list[0..count) -> survivors of the exact FP64 test
-// against the cell table, in index order, into out (*S of them).  The split-BF16 form
+// against the cell table, in index order, into out (*S of them; without sync the count
+// stays in d_count[1] and *S is not set).  The split-BF16 form
 // prefilters with k_fast_prefilter and rechecks only the candidates it could not reject,
 // chained through a device-side count.  Profiled runs time both stages.
 int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out,
-                          int64_t* S) {
+                          int64_t* S, bool sync) {
     int32_t rc;
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
     const bool split = h->cfg.precision == UG_PREC_SPLIT_BF16 && h->full;
@@ -1091,7 +1098,7 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
         if (!rc && !list)
             rc = timed_end(h, UG_KERNEL_PREFILTER, count);
         if (!rc)
-            rc = compact(h, h->d_flags, count, base, list, out, S, nullptr, 1, true);
+            rc = compact(h, h->d_flags, count, base, list, out, S, nullptr, 1, sync);
         return rc;
     }
     rc = timed_begin(h);
@@ -1112,8 +1119,8 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
             rc = timed_end(h, UG_KERNEL_RECHECK, n1);
     }
     if (!rc)
-        rc = compact(h, h->d_flags, count, 0, h->d_list3, out, S, h->d_count, 1, true);
-    if (!rc)
+        rc = compact(h, h->d_flags, count, 0, h->d_list3, out, S, h->d_count, 1, sync);
+    if (!rc && sync)
         h->rechecked += ((int*)h->h_pinned)[0];
     return rc;
 }
@@ -1158,42 +1165,48 @@ int32_t ensure_paths(Pack* h, int64_t need) {
     return rc;
 }
 
-// Conflict bits among the chunk's s survivors (trajectories in d_spos, mask cleared).
-void launch_conflicts(Pack* h, int s) {
+// Conflict bits among the chunk's s survivors (trajectories in d_spos, mask cleared); with
+// dS the count is min(s, *dS), known only on the device (grid-stride launches).
+void launch_conflicts(Pack* h, int s, const int* dS = nullptr) {
     const int64_t npairs = (int64_t)s * (s - 1) / 2;
     if (npairs <= 0)
         return;
     const int64_t items = npairs * ((h->T + CONF_T - 1) / CONF_T);
     const int cb = grid_for((int64_t)s * h->T, 256);
+    const int kb = dS ? std::min(grid_for(items, 256), 2048) : grid_for(items, 256);
     if (h->d == 2) {
-        hipLaunchKernelGGL(k_cells<2>, dim3(cb), dim3(256), 0, h->stream, h->dm, h->d_spos, s, h->d_scell);
-        hipLaunchKernelGGL(k_conflicts<2>, dim3(grid_for(items, 256)), dim3(256), 0, h->stream, h->dm, h->d_spos,
-                           h->d_scell, s, h->d_mask);
+        hipLaunchKernelGGL(k_cells<2>, dim3(cb), dim3(256), 0, h->stream, h->dm, h->d_spos, s, dS, h->d_scell);
+        hipLaunchKernelGGL(k_conflicts<2>, dim3(kb), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell, s, dS,
+                           h->d_mask);
     } else {
-        hipLaunchKernelGGL(k_cells<3>, dim3(cb), dim3(256), 0, h->stream, h->dm, h->d_spos, s, h->d_scell);
-        hipLaunchKernelGGL(k_conflicts<3>, dim3(grid_for(items, 256)), dim3(256), 0, h->stream, h->dm, h->d_spos,
-                           h->d_scell, s, h->d_mask);
+        hipLaunchKernelGGL(k_cells<3>, dim3(cb), dim3(256), 0, h->stream, h->dm, h->d_spos, s, dS, h->d_scell);
+        hipLaunchKernelGGL(k_conflicts<3>, dim3(kb), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell, s, dS,
+                           h->d_mask);
     }
 }
 
 // Resolve the survivors list[0..S) (already clear of the cell table) in index order and
 // insert the accepted ones.  Survivors beyond one chunk are retested against the updated
 // table before their turn.  Returns the number accepted via *acc_total.
-int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total) {
+int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total, const int* dS) {
     int32_t rc;
     *acc_total = 0;
-    while (S > 0) {
-        const int s = (int)std::min<int64_t>(S, UG_S_MAX);
+    // dS: the survivor count is still on the device (filter without a host round trip); the
+    // first chunk runs on min(*dS, UG_S_MAX) and the count comes back with its results
+    bool dev = dS != nullptr;
+    while (dev || S > 0) {
+        const int s = dev ? UG_S_MAX : (int)std::min<int64_t>(S, UG_S_MAX);
+        const int* cs = dev ? dS : nullptr;
         rc = ensure_paths(h, h->n + s);
         if (rc)
             return rc;
-        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, nullptr, 0);
+        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, cs, 0);
         if (rc)
             return rc;
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((int64_t)s * UG_MASK_WORDS, 256)), dim3(256), 0,
                            h->stream, h->d_mask, (int64_t)s * UG_MASK_WORDS);
-        launch_conflicts(h, s);
-        hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, h->d_acc_list,
+        launch_conflicts(h, s, cs);
+        hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, cs, h->d_acc_list,
                            h->d_nacc);
         const int limit = h->max_n > 0 ? (int)std::min<int64_t>(h->max_n - h->n, INT32_MAX) : INT32_MAX;
         const int ib = grid_for((int64_t)s * h->T, 256);
@@ -1222,10 +1235,20 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
             rc = hip_check(hipMemcpyAsync(pin + 64, h->d_acc_idx + h->n, (size_t)s * 8, hipMemcpyDeviceToHost,
                                           h->stream),
                            "accepted copy");
+        if (!rc && dev)
+            rc = hip_check(hipMemcpyAsync(pin + 32, h->d_count, 16, hipMemcpyDeviceToHost, h->stream), "count copy");
         if (!rc)
             rc = hip_check(hipStreamSynchronize(h->stream), "resolve sync");
         if (rc)
             return rc;
+        if (dev) {
+            // the filter's counts: slot 1 = survivors, slot 0 = FP64 rechecks (split form)
+            S = ((const int*)(pin + 32))[dS - h->d_count];
+            if (h->cfg.precision == UG_PREC_SPLIT_BF16 && h->full)
+                h->rechecked += ((const int*)(pin + 32))[0];
+            h->last_S = S;
+            dev = false;
+        }
         const int na = std::min(*(int*)(pin + 8), limit);
         h->filled = *(unsigned long long*)(pin + 16);
         const int64_t* ai = (const int64_t*)(pin + 64);
@@ -1234,7 +1257,7 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         *acc_total += na;
         if (h->max_n > 0 && h->n >= h->max_n)
             return UG_OK;
-        if (S > s) {
+        if (S > UG_S_MAX) {
             // retest the rest against the updated table
             int64_t S2 = 0;
             rc = filter_candidates(h, 0, S - s, list + s, h->d_list2, &S2);
