@@ -174,6 +174,15 @@ enum {
 int32_t ug_pack_profile(ug_pack* h, int32_t enable);
 int32_t ug_pack_kernel_stats(const ug_pack* h, int32_t which, ug_kernel_stats* out);
 
+/* Full-size check of the serial-RSA characterization of the unique phase (no reference
+ * counterpart; the oracle's brute force, done on the device at any size):
+ *   out[0] = stored uniques that touch another stored unique (must be 0),
+ *   out[1] = rejected candidates in [first, first + count) that touch no unique accepted
+ *            before them (must be 0),
+ *   out[2] = uniques checked, out[3] = rejected candidates checked.
+ * Candidates must lie within the consumed range.  Subpaths are not included. */
+int32_t ug_pack_check(ug_pack* h, uint64_t first, int64_t count, int64_t* out);
+
 /* Releases the handle and its device memory. */
 void ug_pack_destroy(ug_pack* h);
 

@@ -313,3 +313,54 @@ def test_euclid_collision_equals_oracle(gpu, precision, d, T, seed, cands):
         idx, gid = p.accepted()
     assert st.attempts == att
     assert list(idx) == list(want_idx) and list(gid) == list(range(len(idx)))
+
+
+def test_characterization_check_on_config1(gpu):
+    """ug_pack_check restates the oracle's brute-force characterization on the device."""
+    import ugpack
+
+    g = _gold("cfg1_d2_T16_s1")
+    with ugpack.Packer(2, 16, seed=1, max_attempts=20000) as p:
+        p.run()
+        c = p.check(0, 20000)
+    n = len(g["idx"])
+    assert c == {"colliding_uniques": 0, "unexplained_rejects": 0, "uniques": n, "rejects": 20000 - n}
+
+
+@pytest.mark.parametrize("precision", ["split", "fp64"])
+def test_config3_to_jam_satisfies_serial_rsa_at_full_size(gpu, precision):
+    """3+1 T=128 packed to jamming (~5e7 candidates): no two of the ~6.4e4 accepted paths
+    collide, and every rejected candidate of the first 2^20 and of the last 2^22 touches a
+    path accepted before it."""
+    import ugpack
+
+    prec = ugpack.PREC_SPLIT_BF16 if precision == "split" else ugpack.PREC_FP64
+    print(f"cfg3 jam check {precision}", flush=True)
+    with ugpack.Packer(3, 128, seed=1, precision=prec, stop_kind=ugpack.STOP_FIXED_WINDOW,
+                       stop_rate=1e-4) as p:
+        st = p.run()
+        a = p.check(0, 1 << 20)
+        b = p.check(st.attempts - (1 << 22), 1 << 22)
+        idx, _ = p.accepted()
+    assert st.done and st.n > 50000
+    g = _gold("cfg3_d3_T128_s1_prefix1M")
+    assert list(idx[idx < (1 << 20)]) == g["idx"]
+    for c in (a, b):
+        assert c["colliding_uniques"] == 0 and c["unexplained_rejects"] == 0
+        assert c["uniques"] == st.n
+    assert a["rejects"] == (1 << 20) - len(g["idx"])
+    assert b["rejects"] == (1 << 22) - int(np.sum(idx >= st.attempts - (1 << 22)))
+
+
+def test_config5_to_jam_satisfies_serial_rsa_at_full_size(gpu):
+    """3+1 T=256 packed to jamming (~3e8 candidates, ~3.8e5 paths): the same two properties
+    on all accepted paths and on the last 2^21 candidates."""
+    import ugpack
+
+    print("cfg5 jam check", flush=True)
+    with ugpack.Packer(3, 256, seed=1, precision=ugpack.PREC_SPLIT_BF16,
+                       stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4) as p:
+        st = p.run()
+        c = p.check(st.attempts - (1 << 21), 1 << 21)
+    assert st.done and st.n > 300000
+    assert c["colliding_uniques"] == 0 and c["unexplained_rejects"] == 0 and c["uniques"] == st.n

@@ -549,6 +549,34 @@ extern "C" int32_t ug_eval_candidates(const ug_pack* hp, const int64_t* dev_idx,
     return rc;
 }
 
+extern "C" int32_t ug_pack_check(ug_pack* hp, uint64_t first, int64_t count, int64_t* out) {
+    if (!hp || !out)
+        return fail(UG_EINVAL, "ug_pack_check: NULL argument");
+    Pack* h = (Pack*)hp;
+    if (count < 0 || first + (uint64_t)count > (uint64_t)h->attempts)
+        return fail(UG_EINVAL, "ug_pack_check: candidates [%llu, %llu) beyond the %lld consumed",
+                    (unsigned long long)first, (unsigned long long)(first + (uint64_t)count),
+                    (long long)h->attempts);
+    unsigned long long* d_out = nullptr;
+    int32_t rc = hip_check(hipMalloc((void**)&d_out, 32), "check counters");
+    if (!rc)
+        rc = hip_check(hipMemsetAsync(d_out, 0, 32, h->stream), "check clear");
+    if (!rc)
+        rc = launch_check(h, 0, 0, h->n, d_out);
+    if (!rc)
+        rc = launch_check(h, 1, first, count, d_out);
+    unsigned long long c[4] = {0, 0, 0, 0};
+    if (!rc)
+        rc = hip_check(hipMemcpyAsync(c, d_out, 32, hipMemcpyDeviceToHost, h->stream), "check copy");
+    if (!rc)
+        rc = hip_check(hipStreamSynchronize(h->stream), "check sync");
+    if (d_out)
+        (void)hipFree(d_out);
+    for (int i = 0; i < 4; i++)
+        out[i] = (int64_t)c[i];
+    return rc;
+}
+
 extern "C" int32_t ug_pack_profile(ug_pack* hp, int32_t enable) {
     if (!hp)
         return fail(UG_EINVAL, "ug_pack_profile: NULL handle");

@@ -169,6 +169,9 @@ int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, con
 // *d_bmax; enqueued on `stream` after the handle's own work.
 int32_t launch_paths_from_pack(Pack* h, hipStream_t stream, double* a2, double* A, double* B, int kmax,
                                int* d_bmax);
+// Serial-RSA characterization checks (k_check): mode 0 over the stored uniques, mode 1 over
+// candidates [first, first + count); counters into d_out[4].
+int32_t launch_check(Pack* h, int mode, uint64_t first, int64_t count, unsigned long long* d_out);
 int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out, int* gout,
                    int64_t* S, int* rs_count);
 int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_total);

@@ -463,6 +463,113 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
     }
 }
 
+// Full-size check of the serial-RSA characterization (one wave per item, every timestep
+// evaluated and looked up).  mode 0, item p < n: stored unique path p, regenerated from its
+// candidate index -- a contact with any other stored unique is a violation (no two accepted
+// paths collide).  mode 1, candidate first + i: skipped when accepted; otherwise it must touch
+// a unique accepted before it (rows < #accepted with smaller index) -- no contact is a
+// violation.  out[0..1] = violations of modes 0/1, out[2..3] = items checked.
+template <int D, int PL>
+__global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t first, int64_t count,
+                                              const int64_t* acc_idx, int64_t n,
+                                              unsigned long long* out) {
+    extern __shared__ __align__(16) int dyn_lds[];
+    const int lane = threadIdx.x;
+    const int T = m.T;
+    constexpr int NWP = 128 * PL;
+    int* sb = dyn_lds;
+    uint32_t* used = (uint32_t*)(sb + D * m.nw);
+    const int lds_head = cand64_lds_head<D>(m, PL);
+    double2* cs = (double2*)(dyn_lds + lds_head);
+    double* pos = (double*)(cs + D * NWP);
+    __shared__ int tsl[64];
+    constexpr int R = D == 2 ? 3 : 9;
+    constexpr int PER_ROUND = 64 / R;
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        uint64_t idx;
+        int64_t self = -1, limit = n;
+        if (mode == 0) {
+            idx = (uint64_t)acc_idx[i];
+            self = i;
+        } else {
+            idx = first + (uint64_t)i;
+            int64_t lo = 0, hi = n;  // first accepted index >= idx
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if ((uint64_t)acc_idx[mid] < idx)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (lo < n && (uint64_t)acc_idx[lo] == idx)
+                continue;  // accepted: mode 0 covers it (uniform across the wave)
+            limit = lo;
+        }
+        WaveCoef<D, PL> w;
+        gen_wave64<D, PL>(m, idx, lane, w, sb, used);
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int ii = 0; ii < PL; ii++)
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    const int j = 2 * (lane + 64 * ii) + k;
+                    cs[a * NWP + j] = make_double2(w.c[a][ii][k], w.s[a][ii][k]);
+                    sb[a * NWP + j] = w.b[a][ii][k];
+                }
+        __syncthreads();
+        bool hit = false;
+        for (int ps = 0; ps < T && !hit; ps += 64) {
+            const int oi = ps + lane;
+            const int pend = min(64, T - ps);
+            if (oi < T) {
+                const int t = m.order[oi];
+                double X[D];
+                eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
+                tsl[lane] = t;
+#pragma unroll
+                for (int a = 0; a < D; a++)
+                    pos[lane * D + a] = X[a];
+            }
+            __syncthreads();
+            for (int r0 = 0; r0 < pend && !hit; r0 += PER_ROUND) {
+                const int G = min(PER_ROUND, pend - r0);
+                const int sl = lane / R, rr = lane % R;
+                bool h = false;
+                if (sl < G) {
+                    const int t = tsl[r0 + sl];
+                    double x[D];
+                    int c[D];
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                        x[a] = pos[(r0 + sl) * D + a];
+                        c[a] = cell_of(x[a], m.halfT, T);
+                    }
+                    const int gx = wrap_cell(c[0], (D == 2 ? rr : rr / 3) - 1, T);
+                    const int gy = D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, T);
+                    const int64_t rid = D == 2 ? (int64_t)t * T + gx : ((int64_t)t * T + gx) * T + gy;
+                    const int* tr = m.tab + rid * m.rowlen + c[D - 1];
+                    for (int k = 0; k < 3 && !h; k++)
+                        for (int nn = tr[k]; nn >= 0 && !h;) {
+                            const size_t q = (size_t)nn * T + t;
+                            if (nn != self && nn < limit)
+                                h = contact<D>(m, x, m.pos64 + q * D);
+                            nn = __float_as_int(m.pos32[q].w);
+                        }
+                }
+                hit = __ballot(h) != 0;
+            }
+            __syncthreads();
+        }
+        if (lane == 0) {
+            if (mode == 0 ? hit : !hit)
+                atomicAdd(&out[mode], 1ull);
+            atomicAdd(&out[2 + mode], 1ull);
+        }
+        __syncthreads();
+    }
+}
+
 // Baker coefficients of stored path p (one wave per path): the canonical per-slot
 // (a cos f, a sin f) of its candidate, scattered by frequency into dense rows.
 template <int D, int PL>
@@ -1413,6 +1520,34 @@ int32_t launch_paths_from_pack(Pack* h, hipStream_t stream, double* a2, double* 
     }
 #undef UG_PFP
     return hip_check(hipGetLastError(), "k_paths_from_pack");
+}
+
+int32_t launch_check(Pack* h, int mode, uint64_t first, int64_t count, unsigned long long* d_out) {
+    if (count <= 0)
+        return UG_OK;
+    const int blocks = (int)std::min<int64_t>(count, 256 * 32);
+    const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
+    const int pl = slot_pl(h->npairs);
+#define UG_CHK(DD, PP)                                                                                       \
+    hipLaunchKernelGGL((k_check<DD, PP>), dim3(blocks), dim3(64), lds, h->stream, h->dm, mode, first, count, \
+                       h->d_acc_idx, h->n, d_out)
+    if (h->d == 2) {
+        if (pl == 1)
+            UG_CHK(2, 1);
+        else if (pl == 2)
+            UG_CHK(2, 2);
+        else
+            UG_CHK(2, 4);
+    } else {
+        if (pl == 1)
+            UG_CHK(3, 1);
+        else if (pl == 2)
+            UG_CHK(3, 2);
+        else
+            UG_CHK(3, 4);
+    }
+#undef UG_CHK
+    return hip_check(hipGetLastError(), "k_check");
 }
 
 }  // namespace ug

@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
         "ug_pack_profile": ([vp, i32], i32),
         "ug_pack_kernel_stats": ([vp, i32, C.POINTER(KernelStats)], i32),
         "ug_pack_destroy": ([vp], None),
+        "ug_pack_check": ([vp, C.c_uint64, i64, vp], i32),
         "ug_ensemble_run": ([C.POINTER(PackConfig), i32, i32, C.POINTER(vp), C.POINTER(PackStatus)], i32),
         "ug_paths_create": ([i32, i64, i32, vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
         "ug_paths_destroy": ([vp], None),
@@ -126,7 +127,7 @@ def exported_symbols() -> list[str]:
         "ug_abi_version", "ug_last_error", "ug_pack_create", "ug_pack_run", "ug_pack_run_sub",
         "ug_pack_get_status", "ug_pack_accepted", "ug_pack_params", "ug_pack_positions",
         "ug_pack_curve", "ug_eval_candidates", "ug_pack_profile", "ug_pack_kernel_stats",
-        "ug_pack_destroy", "ug_ensemble_run", "ug_paths_create", "ug_paths_destroy", "ug_paths_from_pack",
+        "ug_pack_destroy", "ug_pack_check", "ug_ensemble_run", "ug_paths_create", "ug_paths_destroy", "ug_paths_from_pack",
         "ug_frame_bake_lightcone",
         "ug_frame_bake_slices",
     ]
@@ -216,6 +217,14 @@ class Packer:
         st = PackStatus()
         check(lib().ug_pack_run_sub(self._h, C.byref(sc), max_candidates, C.byref(st)))
         return Status.of(st)
+
+    def check(self, first: int = 0, count: int = 0) -> dict:
+        """Serial-RSA characterization on the device: collisions among stored uniques and
+        rejected candidates of [first, first+count) without an earlier contact (both 0)."""
+        out = np.zeros(4, np.int64)
+        check(lib().ug_pack_check(self._h, first, count, out.ctypes.data))
+        return {"colliding_uniques": int(out[0]), "unexplained_rejects": int(out[1]),
+                "uniques": int(out[2]), "rejects": int(out[3])}
 
     def status(self) -> Status:
         st = PackStatus()
