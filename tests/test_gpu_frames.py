@@ -88,21 +88,24 @@ def test_bake_file_writes_brf1(gpu, tmp_path):
     assert len(raw) == size
 
 
-def test_slices_and_rasters(gpu):
+@pytest.mark.parametrize("d,T,seed,nz", [(3, 24, 2, 16), (2, 32, 1, 70), (3, 16, 3, 5)])
+def test_slices_and_rasters(gpu, d, T, seed, nz):
+    """FP64-MFMA slices against the oracle's per-term evaluation (frames beyond one 64-frame
+    tile and fewer than one tile included)."""
     import torch
     import ugpack
     from ugpack import frames
 
-    with ugpack.Packer(3, 24, seed=2, max_attempts=20000) as p:
+    with ugpack.Packer(d, T, seed=seed, max_attempts=20000) as p:
         p.run()
         paths = frames.Paths.from_packer(p)
         par = p.params()
     n = par.shape[0]
-    z = (np.arange(16) + 0.5) * math.pi / 16
+    z = (np.arange(nz) + 0.5) * math.pi / nz
     pos, ras = frames.slices(paths, z, raster_px=256)
     pos, ras = pos.cpu().numpy(), ras.cpu().numpy()
     axes = [{"a": par[:, a, 1::3], "b": par[:, a, 2::3], "f": par[:, a, 3::3], "a2": par[:, a, 0]}
-            for a in range(3)]
+            for a in range(d)]
     for j, zj in enumerate(z):
         want = np.stack([np.asarray(bake_oracle._comov(axes, i, zj)) for i in range(n)])
         want = want - 2.0 * np.floor((want + 1.0) * 0.5)

@@ -231,27 +231,88 @@ __global__ __launch_bounds__(64) void k_frame_hits(DevPaths P, FrontDev F, doubl
     }
 }
 
-// Equal-time slices: positions (wrapped onto [-1, 1)) of every path at the given z values,
-// optionally binned into raster_px^2 density rasters of the (x, y) projection.
-__global__ __launch_bounds__(256) void k_frame_slices(DevPaths P, const double* zs, int nz,
-                                                      float* pos, float* raster, int rpx) {
-    const int64_t tid = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (tid >= (int64_t)nz * P.n)
-        return;
-    const int j = (int)(tid / P.n), p = (int)(tid % P.n);
-    double x[3];
-    comov(P, p, zs[j], x);
-    for (int a = 0; a < P.dim; a++)
-        x[a] = x[a] - 2.0 * floor((x[a] + 1.0) * 0.5);
-    if (pos)
-        for (int a = 0; a < P.dim; a++)
-            pos[((size_t)j * P.n + p) * P.dim + a] = (float)x[a];
-    if (raster) {
-        int cx = (int)floor((x[0] + 1.0) * 0.5 * rpx), cy = (int)floor((x[1] + 1.0) * 0.5 * rpx);
-        cx = cx < 0 ? 0 : (cx >= rpx ? rpx - 1 : cx);
-        cy = cy < 0 ? 0 : (cy >= rpx ? rpx - 1 : cy);
-        atomicAdd(&raster[((size_t)j * rpx + cy) * rpx + cx], 1.0f);
+// Equal-time slices as one dense contraction on FP64 MFMA: the unwrapped positions of
+// coefficient row r = p*dim + a at frame f are
+//   X[r][f] = a2[r] + sum_k A[r][k] sin(k z_f)/sin z_f + B[r][k] (cos(k z_f) - 1)/sin z_f,
+// i.e. Coef (rows x 2kq) times Basis (2kq x frames).  A block computes a 48-row x 64-frame
+// tile with v_mfma_f64_16x16x4f64 (each wave 16 frames x 3 row tiles), stages it in LDS, and
+// the epilogue wraps, writes float positions [frame][path][dim] and bins (x, y) into the
+// frame's raster.  48 rows hold whole paths in 2+1 (24) and 3+1 (16).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+constexpr int FR_ROWS = 48, FR_COLS = 64;
+
+__global__ __launch_bounds__(256) void k_frames_mfma(DevPaths P, int kq, const double* basis, int nzp,
+                                                     int nz, int fbase, int nz_total, float* pos,
+                                                     float* raster, int rpx) {
+    __shared__ double tile[FR_ROWS][FR_COLS + 1];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t nrows = (int64_t)P.n * P.dim;
+    const int64_t r0 = (int64_t)blockIdx.x * FR_ROWS;
+    const int f0 = blockIdx.y * FR_COLS;
+    const size_t W = (size_t)P.kmax + 1;
+    // operand layouts (v_mfma_f64_16x16x4f64): A lane l = A[l % 16][l / 16],
+    // B lane l = B[l / 16][l % 16], D[v] = D[l / 16 + 4 v][l % 16]
+    const int ii = lane & 15, kk = lane >> 4;
+    int64_t row[3];
+    bool rok[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+        row[t] = r0 + 16 * t + ii;
+        rok[t] = row[t] < nrows;
     }
+    const int fcol = f0 + 16 * wv + ii;
+    dbl4 acc[3];
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+        acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+    const int ksteps = kq / 2;  // 2 kq contraction rows, 4 per step
+    for (int ks = 0; ks < ksteps; ks++) {
+        const int r = 4 * ks + kk;
+        const bool sinrow = r < kq;  // uniform within a step (kq is a multiple of 4)
+        const int k = sinrow ? r + 1 : r - kq + 1;
+        const double* src = sinrow ? P.A : P.B;
+        const double b = basis[(size_t)r * nzp + fcol];
+        double a[3];
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+            a[t] = (rok[t] && k <= P.kmax) ? src[(size_t)row[t] * W + k] : 0.0;
+#pragma unroll
+        for (int t = 0; t < 3; t++)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b, acc[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 3; t++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int rr = 16 * t + kk + 4 * v;
+            const int64_t rg = r0 + rr;
+            tile[rr][16 * wv + ii] = (rg < nrows ? P.a2[rg] : 0.0) + acc[t][v];
+        }
+    __syncthreads();
+    const int npb = FR_ROWS / P.dim;
+    const int64_t p0 = r0 / P.dim;
+    for (int q = threadIdx.x; q < npb * FR_COLS; q += 256) {
+        const int pp = q / FR_COLS, f = q % FR_COLS;
+        const int64_t p = p0 + pp;
+        if (p >= P.n || f0 + f >= nz)
+            continue;
+        const int fg = fbase + f0 + f;
+        double x[3];
+        for (int a = 0; a < P.dim; a++) {
+            const double v = tile[pp * P.dim + a][f];
+            x[a] = v - 2.0 * floor((v + 1.0) * 0.5);
+        }
+        if (pos)
+            for (int a = 0; a < P.dim; a++)
+                pos[((size_t)fg * P.n + p) * P.dim + a] = (float)x[a];
+        if (raster) {
+            int cx = (int)floor((x[0] + 1.0) * 0.5 * rpx), cy = (int)floor((x[1] + 1.0) * 0.5 * rpx);
+            cx = cx < 0 ? 0 : (cx >= rpx ? rpx - 1 : cx);
+            cy = cy < 0 ? 0 : (cy >= rpx ? rpx - 1 : cy);
+            atomicAdd(&raster[((size_t)fg * rpx + cy) * rpx + cx], 1.0f);
+        }
+    }
+    (void)nz_total;
 }
 
 __global__ void k_scan_u32(const uint32_t* counts, int n, uint64_t* offs, uint64_t* total) {
@@ -606,21 +667,42 @@ extern "C" int32_t ug_frame_bake_slices(ug_paths* P, const double* z, int32_t nz
         if (!(z[j] > 0.0 && z[j] < 3.141592653589793))
             return fail(UG_EINVAL, "ug_frame_bake_slices: z[%d] = %g outside (0, pi)", j, z[j]);
     hipStream_t st = P->stream;
-    double* dz = nullptr;
+    // the basis of up to FB frames per launch: rows r < kq hold sin(k z)/sin z (k = r + 1),
+    // rows kq + r hold (cos(k z) - 1)/sin z; libm FP64 on the host
+    const int kq = (P->kmax + 3) / 4 * 4;
+    const int FB = 512;
+    const int nzp = (std::min(nz, FB) + FR_COLS - 1) / FR_COLS * FR_COLS;
+    double* dbasis = nullptr;
     int32_t rc = UG_OK;
-    if (hipMalloc((void**)&dz, (size_t)nz * 8) != hipSuccess)
+    if (hipMalloc((void**)&dbasis, (size_t)2 * kq * nzp * 8) != hipSuccess)
         return fail(UG_ENOMEM, "ug_frame_bake_slices: allocation failed");
-    rc = hip_check(hipMemcpyAsync(dz, z, (size_t)nz * 8, hipMemcpyHostToDevice, st), "z upload");
-    if (!rc && dev_raster)
+    if (dev_raster)
         rc = hip_check(hipMemsetAsync(dev_raster, 0, (size_t)nz * raster_px * raster_px * 4, st), "raster clear");
-    if (!rc) {
-        const int64_t tot = (int64_t)nz * P->n;
-        hipLaunchKernelGGL(k_frame_slices, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, dev_of(P), dz, nz,
-                           dev_pos, dev_raster, raster_px);
-        rc = hip_check(hipGetLastError(), "k_frame_slices");
+    std::vector<double> hb((size_t)2 * kq * nzp);
+    const int64_t nrows = P->n * P->dim;
+    for (int fb = 0; fb < nz && !rc; fb += FB) {
+        const int nb = std::min(FB, nz - fb);
+        std::fill(hb.begin(), hb.end(), 0.0);
+        for (int f = 0; f < nb; f++) {
+            const double zf = z[fb + f], sz = std::sin(zf);
+            for (int k = 1; k <= kq; k++) {
+                hb[(size_t)(k - 1) * nzp + f] = std::sin(k * zf) / sz;
+                hb[(size_t)(kq + k - 1) * nzp + f] = (std::cos(k * zf) - 1.0) / sz;
+            }
+        }
+        // the previous batch's kernel must be done with the basis before it is replaced
+        rc = hip_check(hipStreamSynchronize(st), "slices sync");
+        if (!rc)
+            rc = hip_check(hipMemcpyAsync(dbasis, hb.data(), hb.size() * 8, hipMemcpyHostToDevice, st), "basis upload");
+        if (!rc) {
+            const dim3 grid((unsigned)((nrows + FR_ROWS - 1) / FR_ROWS), (unsigned)((nb + FR_COLS - 1) / FR_COLS));
+            hipLaunchKernelGGL(k_frames_mfma, grid, dim3(256), 0, st, dev_of(P), kq, dbasis, nzp, nb, fb, nz, dev_pos,
+                               dev_raster, raster_px);
+            rc = hip_check(hipGetLastError(), "k_frames_mfma");
+        }
     }
     if (!rc)
         rc = hip_check(hipStreamSynchronize(st), "slices sync");
-    (void)hipFree(dz);
+    (void)hipFree(dbasis);
     return rc;
 }
