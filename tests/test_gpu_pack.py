@@ -219,16 +219,18 @@ def _sub_reference(d, T, seed, unique_cands, sub_budget, terms=0, accept_rate=0.
     return list(idx) + list(sidx), list(gid) + list(sgid), att, satt
 
 
-@pytest.mark.parametrize("d,T,seed,unique,sub", [
-    (2, 16, 1, 20000, 200000), (2, 16, 4, 3000, 400000), (2, 32, 2, 30000, 300000),
-    (3, 12, 1, 5000, 100000), (2, 64, 3, 200000, 200000),
+@pytest.mark.parametrize("d,T,seed,unique,sub,precision", [
+    (2, 16, 1, 20000, 200000, "fp64"), (2, 16, 4, 3000, 400000, "fp64"), (2, 32, 2, 30000, 300000, "fp64"),
+    (3, 12, 1, 5000, 100000, "fp64"), (2, 64, 3, 200000, 200000, "fp64"),
+    (2, 16, 4, 3000, 400000, "split"), (2, 64, 3, 200000, 200000, "split"), (3, 16, 2, 8000, 150000, "split"),
 ])
-def test_subpath_phase_equals_oracle(gpu, d, T, seed, unique, sub):
+def test_subpath_phase_equals_oracle(gpu, d, T, seed, unique, sub, precision):
     import ugpack
 
-    print(f"subpaths d={d} T={T} seed={seed} unique={unique} sub={sub}", flush=True)
+    print(f"subpaths d={d} T={T} seed={seed} unique={unique} sub={sub} {precision}", flush=True)
     want_idx, want_gid, att, satt = _sub_reference(d, T, seed, unique, sub)
-    with ugpack.Packer(d, T, seed=seed, max_attempts=unique) as p:
+    prec = ugpack.PREC_FP64 if precision == "fp64" else ugpack.PREC_SPLIT_BF16
+    with ugpack.Packer(d, T, seed=seed, max_attempts=unique, precision=prec) as p:
         st = p.run()
         assert st.attempts == att
         st = p.run_sub(max_attempts=sub)
@@ -276,15 +278,18 @@ def test_subpath_phase_rules(gpu):
             p.run()  # the unique phase ended where the subpath phase started
 
 
+@pytest.mark.parametrize("precision", ["fp64", "split"])
 @pytest.mark.parametrize("name", ["sub_d2_T16_s1_accept2e-3", "sub_d2_T16_s2_fill3e-5"])
-def test_subpath_windowed_stop_equals_golden(gpu, name):
+def test_subpath_windowed_stop_equals_golden(gpu, name, precision):
     import hashlib
 
     import ugpack
 
-    print(f"subpath window {name}", flush=True)
+    print(f"subpath window {name} {precision}", flush=True)
     g = _gold(name)
-    with ugpack.Packer(g["dim"], g["T"], seed=g["seed"], max_attempts=g["unique_candidates"]) as p:
+    prec = ugpack.PREC_FP64 if precision == "fp64" else ugpack.PREC_SPLIT_BF16
+    with ugpack.Packer(g["dim"], g["T"], seed=g["seed"], max_attempts=g["unique_candidates"],
+                       precision=prec) as p:
         p.run()
         st = p.run_sub(max_attempts=1e15, accept_rate=g["accept_rate"], fill_rate=g["fill_rate"])
         idx, gid = p.accepted()

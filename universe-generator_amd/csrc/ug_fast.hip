@@ -198,19 +198,25 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 }
 
 // Prefilter: flags[i] = 0 when candidate i certainly collides, 1 when the FP64 test must decide.
-template <int D>
+// SUB (subpath phase): flags[i] = 0 only when candidate i certainly touches two different
+// groups among the paths stored before row nlimit (the round-start packing) -- such a
+// candidate fails both the round-start test and the adoption test.
+template <int D, bool SUB>
 __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
                                                         const int64_t* list, uint8_t* flags,
-                                                        unsigned long long* ctr) {
+                                                        unsigned long long* ctr, const int* acc_gid,
+                                                        int64_t nlimit) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevModel& m = f.m;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // basis fragments of N block 0 are read from global memory (a few tens of KB, cache
     // resident); LDS holds only each wave's position tile, which keeps occupancy up
     const bf16x8* bl = f.bfrag;
-    float* pos = (float*)lds + (size_t)wv * (D * TILE * NCOL + 3 * TILE);
+    float* pos = (float*)lds + (size_t)wv * (D * TILE * NCOL + 5 * TILE);
     float* sa1 = pos + D * TILE * NCOL;
     float* sinvw = sa1 + TILE;
+    int* smin = (int*)(sinvw + 2 * TILE);  // after ulist: per-candidate group range of a round
+    int* smax = smin + TILE;
     const int mrow = lane & 31, h = lane >> 5;
     const int64_t ntiles = (count + TILE - 1) / TILE;
     unsigned long long nvis = 0, npts = 0;
@@ -229,12 +235,17 @@ __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t 
         uint32_t unres = (uint32_t)__ballot(lane < 32 && valid && !wbad);
         uint32_t rejm = 0;
         int ncol = 0;  // lanes < 32: next column of candidate `lane`
+        int g1 = -1;   // SUB, lanes < 32: the one group candidate `lane` certainly touches so far
         int* ulist = (int*)(sinvw + TILE);
         while (unres) {
             const int U = __popc(unres);
             const int per = 64 / U;
             if (lane < 32 && ((unres >> lane) & 1u))
                 ulist[__popc(unres & ((1u << lane) - 1u))] = lane;
+            if (SUB && lane < 32) {
+                smin[lane] = INT32_MAX;
+                smax[lane] = -1;
+            }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -242,6 +253,7 @@ __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t 
             const int cm = rk < U ? ulist[rk] : 0;
             const int col = __shfl(ncol, cm, 64) + off;
             bool cert = false;
+            int gmn = INT32_MAX, gmx = -1;  // SUB: groups of this lane's certain contacts
             if (rk < U && col < f.ncols) {
                 const int t = f.colt[col];
                 const float mt = m.marg[col];
@@ -306,19 +318,48 @@ __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t 
                                 q = fmaf(d, d, q);
                             }
                             npts++;
-                            cert = cert || (m.euclid ? q <= rb2 : dmax <= r - mt);
+                            const bool sure = m.euclid ? q <= rb2 : dmax <= r - mt;
+                            if (!SUB) {
+                                cert = cert || sure;
+                            } else if (sure && node[g] < nlimit) {
+                                const int gid = acc_gid[node[g]];
+                                gmn = min(gmn, gid);
+                                gmx = max(gmx, gid);
+                                cert = gmn != gmx;  // two groups: nothing more to learn here
+                            }
                             node[g] = __float_as_int(P[g].w);
                         }
                     }
                 }
             }
-            const uint64_t hits = __ballot(cert);
-            if (lane < 32 && ((unres >> lane) & 1u)) {
-                const int myrk = __popc(unres & ((1u << lane) - 1u));
-                const uint64_t segmask = per >= 64 ? ~0ull : ((1ull << per) - 1ull);
-                if ((hits >> (myrk * per)) & segmask)
-                    rejm |= 1u << lane;
-                ncol += per;
+            if (!SUB) {
+                const uint64_t hits = __ballot(cert);
+                if (lane < 32 && ((unres >> lane) & 1u)) {
+                    const int myrk = __popc(unres & ((1u << lane) - 1u));
+                    const uint64_t segmask = per >= 64 ? ~0ull : ((1ull << per) - 1ull);
+                    if ((hits >> (myrk * per)) & segmask)
+                        rejm |= 1u << lane;
+                    ncol += per;
+                }
+            } else {
+                // merge the segment's certain groups with the candidate's earlier one
+                if (rk < U && gmx >= 0) {
+                    atomicMin(&smin[cm], gmn);
+                    atomicMax(&smax[cm], gmx);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane < 32 && ((unres >> lane) & 1u)) {
+                    const int mn = smin[lane], mx = smax[lane];
+                    if (mx >= 0) {
+                        if (mn != mx || (g1 >= 0 && g1 != mn))
+                            rejm |= 1u << lane;
+                        else
+                            g1 = mn;
+                    }
+                    ncol += per;
+                }
             }
             const uint32_t rej_all = (uint32_t)__ballot(lane < 32 && ((rejm >> lane) & 1u));
             const uint32_t done = (uint32_t)__ballot(lane < 32 && ncol >= f.ncols);
@@ -486,20 +527,15 @@ int32_t fast_setup(Pack* h) {
         rc = hip_check(hipMemcpy(h->d_marg, h->marg_host.data(), h->marg_host.size() * 4, hipMemcpyHostToDevice),
                        "margin upload");
     h->dm.marg = h->d_marg;
-    fs->lds_prefilter = (size_t)FAST_WAVES * (h->d * TILE * NCOL + 3 * TILE) * 4;
+    fs->lds_prefilter = (size_t)FAST_WAVES * (h->d * TILE * NCOL + 5 * TILE) * 4;
     h->d_basis16 = fs;
     if (!rc && fs->lds_prefilter > 160 * 1024)
         rc = fail(UG_EINVAL, "split-BF16 tile needs %zu bytes of LDS", fs->lds_prefilter);
     if (!rc) {
-        if (h->d == 2)
-            rc = hip_check(hipFuncSetAttribute((const void*)k_fast_prefilter<2>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)fs->lds_prefilter),
-                           "lds attribute");
-        else
-            rc = hip_check(hipFuncSetAttribute((const void*)k_fast_prefilter<3>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)fs->lds_prefilter),
+        const void* kf[2] = {h->d == 2 ? (const void*)k_fast_prefilter<2, false> : (const void*)k_fast_prefilter<3, false>,
+                             h->d == 2 ? (const void*)k_fast_prefilter<2, true> : (const void*)k_fast_prefilter<3, true>};
+        for (int v = 0; v < 2 && !rc; v++)
+            rc = hip_check(hipFuncSetAttribute(kf[v], hipFuncAttributeMaxDynamicSharedMemorySize, (int)fs->lds_prefilter),
                            "lds attribute");
     }
     return rc;
@@ -528,12 +564,30 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
     const FastArgs f = fast_args(h);
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
     if (h->d == 2)
-        hipLaunchKernelGGL(k_fast_prefilter<2>, dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base, count,
-                           list, flags, ctr);
+        hipLaunchKernelGGL((k_fast_prefilter<2, false>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
+                           count, list, flags, ctr, nullptr, 0);
     else
-        hipLaunchKernelGGL(k_fast_prefilter<3>, dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base, count,
-                           list, flags, ctr);
+        hipLaunchKernelGGL((k_fast_prefilter<3, false>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
+                           count, list, flags, ctr, nullptr, 0);
     return hip_check(hipGetLastError(), "k_fast_prefilter");
+}
+
+int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, int64_t nlimit) {
+    if (count <= 0)
+        return UG_OK;
+    if (!h->full || !h->d_basis16)
+        return fail(UG_ESTATE, "split-BF16 prefilter not set up for this model");
+    FastState* fs = (FastState*)h->d_basis16;
+    const int64_t ntiles = (count + TILE - 1) / TILE;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, 2048));
+    const FastArgs f = fast_args(h);
+    if (h->d == 2)
+        hipLaunchKernelGGL((k_fast_prefilter<2, true>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
+                           count, nullptr, flags, nullptr, h->d_acc_gid, nlimit);
+    else
+        hipLaunchKernelGGL((k_fast_prefilter<3, true>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
+                           count, nullptr, flags, nullptr, h->d_acc_gid, nlimit);
+    return hip_check(hipGetLastError(), "k_fast_prefilter (subpaths)");
 }
 
 int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* out) {

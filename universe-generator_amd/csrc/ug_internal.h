@@ -182,6 +182,9 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
 // split-BF16 prefilter (ug_fast.hip)
 int32_t fast_setup(Pack* h);
 int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags);
+// Subpath-phase prefilter: flags[i] = 0 when candidate base + i certainly touches two groups
+// among the rows < nlimit.
+int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, int64_t nlimit);
 int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* out);
 void fast_release(Pack* h);
 

@@ -621,9 +621,11 @@ __global__ __launch_bounds__(64) void k_paths_from_pack(DevModel m, const int64_
 // survivor count, which drives its batch size).
 template <int D, int PL>
 __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int64_t count,
-                                                 const int64_t* list, const int32_t* acc_gid,
-                                                 int64_t nlimit, int* rs_count, uint8_t* flags,
-                                                 int* gid_out) {
+                                                 const int64_t* list, const int* dcount,
+                                                 const int32_t* acc_gid, int64_t nlimit, int* rs_count,
+                                                 uint8_t* flags, int* gid_out) {
+    if (dcount)
+        count = min(count, (int64_t)*dcount);
     extern __shared__ __align__(16) int dyn_lds[];
     const int lane = threadIdx.x;
     const int T = m.T, nw = m.nw;
@@ -1382,19 +1384,32 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
     return UG_OK;
 }
 
-// Candidates -> survivors of the subpath test, with their table groups (gout).
+// Candidates -> survivors of the subpath test, with their table groups (gout).  In the
+// split-BF16 form a fresh range first goes through the prefilter's subpath mode (certain
+// two-group contacts among the round-start rows), and only the rest gets the FP64 scan.
 int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out, int* gout,
                    int64_t* S, int* rs_count) {
     if (count <= 0) {
         *S = 0;
         return UG_OK;
     }
+    int32_t rc = UG_OK;
+    const int* dcount = nullptr;
+    if (!list && h->cfg.precision == UG_PREC_SPLIT_BF16 && h->full) {
+        rc = launch_fast_sub(h, base, count, h->d_flags, h->sub_nlimit);
+        if (!rc)
+            rc = compact(h, h->d_flags, count, base, nullptr, h->d_list3, nullptr, nullptr, 0, false);
+        list = h->d_list3;
+        dcount = h->d_count;
+    }
+    if (rc)
+        return rc;
     const int blocks = (int)std::min<int64_t>(count, 256 * 32);
     const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
     const int pl = slot_pl(h->npairs);
 #define UG_SUBT(DD, PP)                                                                                      \
     hipLaunchKernelGGL((k_sub_test<DD, PP>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, list, \
-                       h->d_acc_gid, h->sub_nlimit, rs_count, h->d_flags, h->d_gflag)
+                       dcount, h->d_acc_gid, h->sub_nlimit, rs_count, h->d_flags, h->d_gflag)
     if (h->d == 2) {
         if (pl == 1)
             UG_SUBT(2, 1);
@@ -1411,9 +1426,9 @@ int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, i
             UG_SUBT(3, 4);
     }
 #undef UG_SUBT
-    int32_t rc = hip_check(hipGetLastError(), "k_sub_test");
+    rc = hip_check(hipGetLastError(), "k_sub_test");
     if (!rc)
-        rc = compact(h, h->d_flags, count, base, list, out, S, nullptr, 1, true, h->d_gflag, gout);
+        rc = compact(h, h->d_flags, count, base, list, out, S, dcount, 1, true, h->d_gflag, gout);
     return rc;
 }
 
