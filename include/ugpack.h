@@ -155,12 +155,13 @@ int32_t ug_eval_candidates(const ug_pack* h, const int64_t* dev_idx, int64_t cou
                            int32_t precision, double* dev_out);
 
 /* Per-kernel measurement of the candidate-test kernels.  When enabled, every launch is
- * bracketed by HIP events on the handle's stream and the kernels count the timesteps they
- * visit and the grid points they compare (exactly), so algorithmic bytes per launch can be
- * derived.  No counterpart in the reference (which prints only attempts/s, src/main.rs:150-159). */
+ * bracketed by a pair of timing-only HIP events on the handle's stream (read back by
+ * ug_pack_kernel_stats, so the pack waits on nothing extra) and the kernels count the timesteps they visit and the grid points
+ * they compare (exactly, in device counters), so algorithmic bytes per launch can be derived.
+ * No counterpart in the reference (which prints only attempts/s, src/main.rs:150-159). */
 typedef struct ug_kernel_stats {
     int64_t launches; /* kernel launches measured */
-    int64_t items;    /* candidates tested */
+    int64_t items;    /* candidate tests (a survivor retested after an insert counts again) */
     double ms;        /* summed event time */
     int64_t visits;   /* (candidate, timestep) pairs visited */
     int64_t points;   /* grid points compared */
@@ -172,7 +173,8 @@ enum {
 };
 
 int32_t ug_pack_profile(ug_pack* h, int32_t enable);
-int32_t ug_pack_kernel_stats(const ug_pack* h, int32_t which, ug_kernel_stats* out);
+/* waits for the handle's profiled launches still in flight */
+int32_t ug_pack_kernel_stats(ug_pack* h, int32_t which, ug_kernel_stats* out);
 
 /* Full-size check of the serial-RSA characterization of the unique phase (no reference
  * counterpart; the oracle's brute force, done on the device at any size):

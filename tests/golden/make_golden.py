@@ -31,6 +31,10 @@ CASES = {
     "d3_T16_terms4_s5": (3, 16, 4, 5, False, 20000, 0),
     **{f"cfg2_d2_T64_s{s}_jam": (2, 64, 64, s, False, 1 << 40, 1) for s in (1, 2, 3, 4)},
     "cfg3_d3_T128_s1_prefix1M": (3, 128, 128, 1, False, 1 << 20, 0),
+    # long scans: 3+1 T=256 (37 head rounds of 7 timesteps in the FP64 test, two slot pairs
+    # per lane) and 2+1 T=512 (four slot pairs per lane)
+    "d3_T256_s1": (3, 256, 256, 1, False, 6000, 0),
+    "d2_T512_s2": (2, 512, 512, 2, False, 8000, 0),
 }
 
 # Subpath phase with a windowed stop (these need >= 2e7 subpath candidates, minutes of

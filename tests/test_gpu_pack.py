@@ -6,6 +6,7 @@ index lists and group labels must equal serial RSA (oracle or committed golden f
 
 from __future__ import annotations
 
+import hashlib
 import json
 import os
 
@@ -25,6 +26,7 @@ def _gold(name):
 @pytest.mark.parametrize("d,T,terms,seed,pin", [
     (2, 16, 16, 1, False), (2, 64, 64, 2, False), (3, 12, 12, 1, False), (2, 24, 5, 3, False),
     (3, 16, 4, 5, False), (2, 16, 16, 2, True), (3, 128, 128, 1, False), (2, 200, 200, 9, False),
+    (3, 256, 256, 1, False), (2, 512, 512, 2, False),
 ])
 def test_fp64_positions_bit_exact(gpu, d, T, terms, seed, pin):
     import torch
@@ -65,6 +67,25 @@ def test_variant_accepted_lists_equal_golden(gpu, name):
                        pin_sin1=g["pin_sin1"], max_attempts=g["candidates"]) as p:
         p.run()
         idx, gid = p.accepted()
+    assert list(idx) == g["idx"] and list(gid) == g["gid"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+@pytest.mark.parametrize("name", ["d3_T256_s1", "d2_T512_s2"])
+def test_long_scan_accepted_lists_equal_golden(gpu, name, precision):
+    """3+1 T=256 and 2+1 T=512: the FP64 test's first pass walks all T timesteps in head
+    rounds (37 of them at 3+1 T=256, where a doubling round size once overflowed and the
+    last timesteps went unchecked), with two and four slot pairs per lane."""
+    import ugpack
+
+    print(f"long scan {name} {precision}", flush=True)
+    g = _gold(name)
+    prec = ugpack.PREC_FP64 if precision == "fp64" else ugpack.PREC_SPLIT_BF16
+    with ugpack.Packer(g["dim"], g["T"], terms=g["terms"], seed=g["seed"],
+                       max_attempts=g["candidates"], precision=prec) as p:
+        st = p.run()
+        idx, gid = p.accepted()
+    assert st.attempts == g["candidates"]
     assert list(idx) == g["idx"] and list(gid) == g["gid"]
 
 
@@ -281,8 +302,6 @@ def test_subpath_phase_rules(gpu):
 @pytest.mark.parametrize("precision", ["fp64", "split"])
 @pytest.mark.parametrize("name", ["sub_d2_T16_s1_accept2e-3", "sub_d2_T16_s2_fill3e-5"])
 def test_subpath_windowed_stop_equals_golden(gpu, name, precision):
-    import hashlib
-
     import ugpack
 
     print(f"subpath window {name} {precision}", flush=True)
@@ -357,15 +376,24 @@ def test_config3_to_jam_satisfies_serial_rsa_at_full_size(gpu, precision):
     assert b["rejects"] == (1 << 22) - int(np.sum(idx >= st.attempts - (1 << 22)))
 
 
-def test_config5_to_jam_satisfies_serial_rsa_at_full_size(gpu):
+@pytest.mark.parametrize("precision", ["split", "fp64"])
+def test_config5_to_jam_satisfies_serial_rsa_at_full_size(gpu, precision):
     """3+1 T=256 packed to jamming (~3e8 candidates, ~3.8e5 paths): the same two properties
-    on all accepted paths and on the last 2^21 candidates."""
+    on all accepted paths and on the last 2^21 candidates; both forms give the same list."""
     import ugpack
 
-    print("cfg5 jam check", flush=True)
-    with ugpack.Packer(3, 256, seed=1, precision=ugpack.PREC_SPLIT_BF16,
+    print(f"cfg5 jam check {precision}", flush=True)
+    prec = ugpack.PREC_SPLIT_BF16 if precision == "split" else ugpack.PREC_FP64
+    with ugpack.Packer(3, 256, seed=1, precision=prec,
                        stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4) as p:
         st = p.run()
         c = p.check(st.attempts - (1 << 21), 1 << 21)
+        idx, _ = p.accepted()
     assert st.done and st.n > 300000
     assert c["colliding_uniques"] == 0 and c["unexplained_rejects"] == 0 and c["uniques"] == st.n
+    h = hashlib.sha256(np.asarray(idx, "<i8").tobytes()).hexdigest()
+    _CFG5_DIGEST.setdefault("digest", h)
+    assert h == _CFG5_DIGEST["digest"], "split-BF16 and FP64 forms accepted different lists"
+
+
+_CFG5_DIGEST: dict = {}

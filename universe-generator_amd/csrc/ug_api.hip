@@ -131,11 +131,7 @@ void release(Pack* h) {
     free_dev(h->d_glist2);
     free_dev(h->d_gsurv);
     free_dev(h->d_rscount);
-    if (h->ev0)
-        (void)hipEventDestroy(h->ev0);
-    if (h->ev1)
-        (void)hipEventDestroy(h->ev1);
-    h->ev0 = h->ev1 = nullptr;
+    release_events(h);
     if (h->h_pinned)
         (void)hipHostFree(h->h_pinned);
     h->h_pinned = nullptr;
@@ -581,22 +577,24 @@ extern "C" int32_t ug_pack_profile(ug_pack* hp, int32_t enable) {
     if (!hp)
         return fail(UG_EINVAL, "ug_pack_profile: NULL handle");
     Pack* h = (Pack*)hp;
-    if (enable && !h->ev0) {
-        if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
-            return fail(UG_EHIP, "ug_pack_profile: event creation failed");
-        if (hipMalloc((void**)&h->d_ctr, 16) != hipSuccess)
+    if (enable && !h->d_ctr) {
+        if (hipMalloc((void**)&h->d_ctr, UG_CTR_WORDS * sizeof(unsigned long long)) != hipSuccess)
             return fail(UG_ENOMEM, "ug_pack_profile: counter allocation failed");
-        if (hipMemset(h->d_ctr, 0, 16) != hipSuccess)
+        if (hipMemset(h->d_ctr, 0, UG_CTR_WORDS * sizeof(unsigned long long)) != hipSuccess)
             return fail(UG_EHIP, "ug_pack_profile: counter clear failed");
     }
     h->profile = enable != 0;
     return UG_OK;
 }
 
-extern "C" int32_t ug_pack_kernel_stats(const ug_pack* hp, int32_t which, ug_kernel_stats* out) {
+extern "C" int32_t ug_pack_kernel_stats(ug_pack* hp, int32_t which, ug_kernel_stats* out) {
     if (!hp || !out || which < 0 || which > 1)
         return fail(UG_EINVAL, "ug_pack_kernel_stats: invalid argument");
-    *out = ((const Pack*)hp)->kstats[which];
+    Pack* h = (Pack*)hp;
+    const int32_t rc = timed_collect(h);
+    if (rc)
+        return rc;
+    *out = h->kstats[which];
     return UG_OK;
 }
 

@@ -375,9 +375,10 @@ __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t 
             a += __shfl_xor(a, s, 64);
             b += __shfl_xor(b, s, 64);
         }
+        unsigned long long* c = ctr + (blockIdx.x % UG_CTR_SLOTS) * UG_CTR_STRIDE;
         if (lane == 0 && a) {
-            atomicAdd(&ctr[0], a);
-            atomicAdd(&ctr[1], b);
+            atomicAdd(&c[0], a);
+            atomicAdd(&c[1], b);
         }
     }
 }
@@ -564,11 +565,11 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
     const FastArgs f = fast_args(h);
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
     if (h->d == 2)
-        hipLaunchKernelGGL((k_fast_prefilter<2, false>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
-                           count, list, flags, ctr, nullptr, 0);
+        UG_LAUNCH(h, (k_fast_prefilter<2, false>), dim3(blocks), dim3(256), fs->lds_prefilter, f, base, count,
+                  list, flags, ctr, nullptr, 0);
     else
-        hipLaunchKernelGGL((k_fast_prefilter<3, false>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
-                           count, list, flags, ctr, nullptr, 0);
+        UG_LAUNCH(h, (k_fast_prefilter<3, false>), dim3(blocks), dim3(256), fs->lds_prefilter, f, base, count,
+                  list, flags, ctr, nullptr, 0);
     return hip_check(hipGetLastError(), "k_fast_prefilter");
 }
 

@@ -16,6 +16,11 @@
 #define UG_S_MAX 1024     // survivors resolved per in-batch chunk
 #define UG_MASK_WORDS (UG_S_MAX / 64)
 #define UG_SUB_MAX (1 << 20)  // largest candidate sub-batch
+// Measurement counters: a wave adds into slot blockIdx % UG_CTR_SLOTS (UG_CTR_STRIDE words
+// apart), so thousands of waves ending together do not serialise on one address.
+#define UG_CTR_SLOTS 64
+#define UG_CTR_STRIDE 16
+#define UG_CTR_WORDS (2 * UG_CTR_SLOTS * UG_CTR_STRIDE)
 
 namespace ug {
 
@@ -144,9 +149,22 @@ struct Pack {
     int64_t survivors = 0, rechecked = 0;
     double seconds = 0.0;
     // measurement (ug_pack_profile)
+    // Profiled launches carry start/stop events of their own (one pair per launch, read back
+    // only when the stats are asked for, so profiling adds no host wait inside a pack).
     bool profile = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    unsigned long long* d_ctr = nullptr;  // [0] visits, [1] points
+    std::vector<std::array<hipEvent_t, 2>> ev_pool;  // event pairs, reused after a collect
+    struct TimedLaunch {
+        int which;
+        int pair;
+        int64_t items;
+    };
+    std::vector<TimedLaunch> ev_pending;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // the armed pair
+    bool timing_armed = false;                 // the next UG_LAUNCH carries ev0/ev1
+    bool timing_fired = false;                 // that launch happened
+    // cumulative device counters per kernel kind: UG_CTR_SLOTS slots of (visits, points,
+    // items of the recheck), one 128-byte line each
+    unsigned long long* d_ctr = nullptr;
     ug_kernel_stats kstats[2] = {};
     std::deque<std::pair<int64_t, int64_t>> win;
     int64_t win_att = 0, win_adm = 0;
@@ -158,8 +176,26 @@ int grid_for(int64_t items, int per_block);
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
                       double* out, bool full, unsigned long long* ctr = nullptr,
                       const int* dcount = nullptr, int head = -1);
+// Profiled timing of one launch: timed_begin arms the next UG_LAUNCH on h's stream, which
+// then records an event pair directly before and after the dispatch (timing-only events
+// without the system-scope fence, never waited on inside a pack); timed_end books the pair
+// for timed_collect.
 int32_t timed_begin(Pack* h);
 int32_t timed_end(Pack* h, int which, int64_t items);
+int32_t timed_collect(Pack* h);  // waits for pending timed launches and folds them into kstats
+void release_events(Pack* h);    // hands h's event pairs to the process-wide free list
+#define UG_LAUNCH(h, K, grid, block, lds, ...)                                                 \
+    do {                                                                                     \
+        if ((h)->timing_armed) {                                                             \
+            (h)->timing_armed = false;                                                       \
+            (h)->timing_fired = true;                                                        \
+            (void)hipEventRecord((h)->ev0, (h)->stream);                                     \
+            hipLaunchKernelGGL(K, grid, block, lds, (h)->stream, __VA_ARGS__);               \
+            (void)hipEventRecord((h)->ev1, (h)->stream);                                     \
+        } else {                                                                             \
+            hipLaunchKernelGGL(K, grid, block, lds, (h)->stream, __VA_ARGS__);               \
+        }                                                                                    \
+    } while (0)
 int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, const int64_t* list,
                 int64_t* out, int64_t* S, const int* dcount = nullptr, int slot = 1, bool sync = true,
                 const int* vin = nullptr, int* vout = nullptr);

@@ -22,6 +22,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -384,8 +386,10 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
         bool hit = false;
         int o0 = 0;
         if (!FULL) {
-            // head: chunks of 2, 4, 8 timesteps evaluated across the wave, one lookup round each
-            for (int chunk = 2; !hit && o0 < min(head, T); chunk *= 2) {
+            // head: chunks of 2, 4, 8 timesteps evaluated across the wave, one lookup round
+            // each (the chunk stops growing at PER_ROUND: a head of T = 256 in 3+1 takes 37
+            // rounds, and an ever-doubling int would overflow)
+            for (int chunk = 2; !hit && o0 < min(head, T); chunk = min(2 * chunk, PER_ROUND)) {
                 const int G = min(min(chunk, PER_ROUND), T - o0);
                 for (int g = 0; g < G; g++) {
                     double X[D];
@@ -452,13 +456,16 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
         __syncthreads();
     }
     if (!FULL && ctr) {
-        // exact counts for the algorithmic-bytes figure (one atomic per wave)
+        // exact counts for the algorithmic-bytes figure (one atomic per wave and counter)
+        unsigned long long* c = ctr + (blockIdx.x % UG_CTR_SLOTS) * UG_CTR_STRIDE;
+        if (dcount && blockIdx.x == 0 && lane == 0)
+            atomicAdd(&c[2], (unsigned long long)count);
         unsigned long long pw = npts;
         for (int h = 32; h >= 1; h >>= 1)
             pw += __shfl_xor(pw, h, 64);
         if (lane == 0 && nvis) {
-            atomicAdd(&ctr[0], nvis);
-            atomicAdd(&ctr[1], pw);
+            atomicAdd(&c[0], nvis);
+            atomicAdd(&c[1], pw);
         }
     }
 }
@@ -1096,8 +1103,8 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
     // slot pairs per lane: 1 up to 64 pairs (T <= 129), 2 up to 128, else 4
     const int pl = slot_pl(h->npairs);
 #define UG_CAND64(DD, FF, PP)                                                                         \
-    hipLaunchKernelGGL((k_cand64<DD, FF, PP>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, \
-                       list, dcount, flags, out, head, ctr)
+    UG_LAUNCH(h, (k_cand64<DD, FF, PP>), dim3(blocks), dim3(64), lds, h->dm, base, count, list, dcount, \
+              flags, out, head, ctr)
 #define UG_CAND64_PL(DD, FF)       \
     do {                           \
         if (pl == 1)               \
@@ -1123,35 +1130,111 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
     return hip_check(hipGetLastError(), "k_cand64");
 }
 
-int32_t timed_begin(Pack* h) {
-    if (!h->profile)
-        return UG_OK;
-    return hip_check(hipEventRecord(h->ev0, h->stream), "event record");
+// Event pairs of profiled launches outlive the handles that used them: a process-wide free
+// list per device, so a handle created inside a timed region creates no events once the
+// list is warm.
+static std::mutex g_ev_mu;
+static std::map<int, std::vector<std::array<hipEvent_t, 2>>> g_ev_free;
+
+void release_events(Pack* h) {
+    for (const Pack::TimedLaunch& t : h->ev_pending)  // no launch in flight still signals them
+        (void)hipEventSynchronize(h->ev_pool[(size_t)t.pair][1]);
+    std::lock_guard<std::mutex> g(g_ev_mu);
+    auto& f = g_ev_free[h->device];
+    f.insert(f.end(), h->ev_pool.begin(), h->ev_pool.end());
+    h->ev_pool.clear();
+    h->ev_pending.clear();
+    h->ev0 = h->ev1 = nullptr;
 }
 
-// Records the end event, waits for it and accumulates the launch into kstats[which].
-int32_t timed_end(Pack* h, int which, int64_t items) {
+static int32_t take_events(Pack* h) {
+    {
+        std::lock_guard<std::mutex> g(g_ev_mu);
+        auto& f = g_ev_free[h->device];
+        if (!f.empty()) {
+            h->ev_pool.push_back(f.back());
+            f.pop_back();
+            return UG_OK;
+        }
+    }
+    // timing only: no system-scope fence (cache writeback) when the events are reached
+    std::array<hipEvent_t, 2> e{nullptr, nullptr};
+    const unsigned fl = hipEventDisableSystemFence;
+    if (hipEventCreateWithFlags(&e[0], fl) != hipSuccess || hipEventCreateWithFlags(&e[1], fl) != hipSuccess) {
+        if (e[0])
+            (void)hipEventDestroy(e[0]);
+        return fail(UG_EHIP, "timed launch: event creation failed");
+    }
+    h->ev_pool.push_back(e);
+    return UG_OK;
+}
+
+int32_t timed_begin(Pack* h) {
+    h->timing_armed = h->timing_fired = false;
     if (!h->profile)
         return UG_OK;
-    int32_t rc = hip_check(hipEventRecord(h->ev1, h->stream), "event record");
-    if (!rc)
-        rc = hip_check(hipEventSynchronize(h->ev1), "event sync");
-    float ms = 0.f;
-    if (!rc)
-        rc = hip_check(hipEventElapsedTime(&ms, h->ev0, h->ev1), "event time");
-    unsigned long long c[2] = {0, 0};
-    if (!rc)
-        rc = hip_check(hipMemcpy(c, h->d_ctr, 16, hipMemcpyDeviceToHost), "counter copy");
-    if (!rc)
-        rc = hip_check(hipMemset(h->d_ctr, 0, 16), "counter clear");
-    if (rc)
-        return rc;
-    ug_kernel_stats& k = h->kstats[which];
-    k.launches++;
-    k.items += items;
-    k.ms += ms;
-    k.visits += (int64_t)c[0];
-    k.points += (int64_t)c[1];
+    if (h->ev_pending.size() >= 512) {  // bound the pool: fold what has finished
+        const int32_t rc = timed_collect(h);
+        if (rc)
+            return rc;
+    }
+    const size_t k = h->ev_pending.size();
+    if (k == h->ev_pool.size()) {
+        const int32_t rc = take_events(h);
+        if (rc)
+            return rc;
+    }
+    h->ev0 = h->ev_pool[k][0];
+    h->ev1 = h->ev_pool[k][1];
+    h->timing_armed = true;
+    return UG_OK;
+}
+
+// Books the armed launch (when it was made) under kstats[which].
+int32_t timed_end(Pack* h, int which, int64_t items) {
+    if (h->profile && h->timing_fired)
+        h->ev_pending.push_back({which, (int)h->ev_pending.size(), items});
+    h->timing_armed = h->timing_fired = false;
+    return UG_OK;
+}
+
+int32_t timed_collect(Pack* h) {
+    if (!h->profile && h->ev_pending.empty())
+        return UG_OK;
+    for (const Pack::TimedLaunch& t : h->ev_pending) {
+        const auto& e = h->ev_pool[(size_t)t.pair];
+        float ms = 0.f;
+        int32_t rc = hip_check(hipEventSynchronize(e[1]), "event sync");
+        if (!rc)
+            rc = hip_check(hipEventElapsedTime(&ms, e[0], e[1]), "event time");
+        if (rc)
+            return rc;
+        ug_kernel_stats& k = h->kstats[t.which];
+        k.launches++;
+        k.items += t.items;
+        k.ms += ms;
+    }
+    h->ev_pending.clear();
+    if (h->d_ctr) {
+        std::vector<unsigned long long> c(UG_CTR_WORDS);
+        int32_t rc = hip_check(hipStreamSynchronize(h->stream), "counter sync");
+        if (!rc)
+            rc = hip_check(hipMemcpy(c.data(), h->d_ctr, UG_CTR_WORDS * sizeof(unsigned long long),
+                                     hipMemcpyDeviceToHost),
+                           "counter copy");
+        if (rc)
+            return rc;
+        for (int w = 0; w < 2; w++) {
+            int64_t sum[3] = {0, 0, 0};
+            for (int k = 0; k < UG_CTR_SLOTS; k++)
+                for (int j = 0; j < 3; j++)
+                    sum[j] += (int64_t)c[(size_t)(w * UG_CTR_SLOTS + k) * UG_CTR_STRIDE + j];
+            h->kstats[w].visits = sum[0];
+            h->kstats[w].points = sum[1];
+            if (w == UG_KERNEL_RECHECK)
+                h->kstats[w].items = sum[2];
+        }
+    }
     return UG_OK;
 }
 
@@ -1199,12 +1282,14 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
                           int64_t* S, bool sync) {
     int32_t rc;
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
+    unsigned long long* rctr = h->profile ? h->d_ctr + UG_CTR_WORDS / 2 : nullptr;
     const bool split = h->cfg.precision == UG_PREC_SPLIT_BF16 && h->full;
+    // every pass is measured, retests of a survivor list included
     if (!split) {
         rc = timed_begin(h);
         if (!rc)
             rc = launch_cand64(h, base, count, list, h->d_flags, nullptr, false, ctr, nullptr, -1);
-        if (!rc && !list)
+        if (!rc)
             rc = timed_end(h, UG_KERNEL_PREFILTER, count);
         if (!rc)
             rc = compact(h, h->d_flags, count, base, list, out, S, nullptr, 1, sync);
@@ -1213,20 +1298,16 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
     rc = timed_begin(h);
     if (!rc)
         rc = launch_fast(h, base, count, list, h->d_flags);
-    if (!rc && !list)
+    if (!rc)
         rc = timed_end(h, UG_KERNEL_PREFILTER, count);
     if (!rc)
         rc = compact(h, h->d_flags, count, base, list, h->d_list3, nullptr, nullptr, 0, false);
     if (!rc)
         rc = timed_begin(h);
     if (!rc)
-        rc = launch_cand64(h, 0, count, h->d_list3, h->d_flags, nullptr, false, ctr, h->d_count, -1);
-    if (!rc && h->profile) {
-        int32_t n1 = 0;
-        rc = hip_check(hipMemcpy(&n1, h->d_count, 4, hipMemcpyDeviceToHost), "recheck count");
-        if (!rc)
-            rc = timed_end(h, UG_KERNEL_RECHECK, n1);
-    }
+        rc = launch_cand64(h, 0, count, h->d_list3, h->d_flags, nullptr, false, rctr, h->d_count, -1);
+    if (!rc)
+        rc = timed_end(h, UG_KERNEL_RECHECK, 0);  // items: counted on the device
     if (!rc)
         rc = compact(h, h->d_flags, count, 0, h->d_list3, out, S, h->d_count, 1, sync);
     if (!rc && sync)
