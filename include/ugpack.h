@@ -185,8 +185,16 @@ int32_t ug_pack_kernel_stats(ug_pack* h, int32_t which, ug_kernel_stats* out);
  * Candidates must lie within the consumed range.  Subpaths are not included. */
 int32_t ug_pack_check(ug_pack* h, uint64_t first, int64_t count, int64_t* out);
 
-/* Releases the handle and its device memory. */
+/* Releases the handle.  Its device buffers and pinned staging block are kept in a
+ * process-wide cache (at most 16 GiB per device) for the next handle of the same model,
+ * and the read-only basis tables of each (device, dim, T, terms, precision, margin) are
+ * built once and shared; a run takes nothing from either that it does not reinitialise. */
 void ug_pack_destroy(ug_pack* h);
+
+/* Frees the cached buffers of destroyed handles (the shared model tables stay).  No
+ * counterpart in the reference, whose engine is one process per universe; for callers that
+ * need the memory back between batches of packs. */
+int32_t ug_cache_trim(void);
 
 /* Packs n independent universes (cfgs[i], each run as ug_pack_run with max_candidates 0)
  * on the current device, `threads` at a time: one host worker thread and one HIP stream

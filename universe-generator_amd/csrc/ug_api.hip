@@ -92,23 +92,22 @@ void gen_host(const Pack* h, uint64_t idx, double* out_row) {
     }
 }
 
+// Buffers go back to the process-wide cache for the next handle (ug_cache.hip).
 template <typename P>
 void free_dev(P*& p) {
-    if (p)
-        (void)hipFree((void*)p);
+    cache_free((void*)p);
     p = nullptr;
 }
 
 void release(Pack* h) {
-    fast_release(h);
-    free_dev(h->d_S);
-    free_dev(h->d_C);
-    free_dev(h->d_ST);
-    free_dev(h->d_CT);
-    free_dev(h->d_order);
+    // model tables are shared: drop the views and this handle's reference
+    h->d_basis16 = nullptr;
+    h->d_S = h->d_C = h->d_ST = h->d_CT = nullptr;
+    h->d_order = nullptr;
+    h->d_marg = nullptr;
+    h->tabs.reset();
     free_dev(h->d_tab);
     free_dev(h->d_occ);
-    free_dev(h->d_marg);
     free_dev(h->d_pos64);
     free_dev(h->d_pos32);
     free_dev(h->d_acc_idx);
@@ -132,8 +131,7 @@ void release(Pack* h) {
     free_dev(h->d_gsurv);
     free_dev(h->d_rscount);
     release_events(h);
-    if (h->h_pinned)
-        (void)hipHostFree(h->h_pinned);
+    cache_host_free(h->h_pinned);
     h->h_pinned = nullptr;
 }
 
@@ -195,33 +193,19 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     h->modmax = modmax;
     h->full = h->nw == modmax;
     h->max_n = cfg.max_n;
-    const int T = h->T, W = T + 1;
+    const int T = h->T;
 
-    // canonical basis tables on the host (ug_arith.h), uploaded once
-    std::vector<double> S((size_t)T * W), C((size_t)T * W);
-    const uint64_t Dn = (uint64_t)T + 1;
-    for (int t = 0; t < T; t++) {
-        const double s1 = sincospi_rat((uint64_t)(t + 1), Dn).s;
-        for (int k = 0; k <= T; k++) {
-            const uint64_t nk = (uint64_t)k * (uint64_t)(t + 1);
-            const double sk = sincospi_rat(nk, Dn).s;
-            const double hk = sincospi_rat(nk, 2 * Dn).s;
-            S[(size_t)t * W + k] = sk / s1;
-            C[(size_t)t * W + k] = (-2.0 * (hk * hk)) / s1;
-        }
+    // canonical basis tables (and the split-BF16 ones), shared by every handle of the model
+    int32_t rc = model_tables(h->d, T, h->nw, cfg.precision == UG_PREC_SPLIT_BF16 && h->full, cfg.margin, &h->tabs);
+    if (rc != UG_OK) {
+        delete h;
+        return rc;
     }
-    std::vector<double> ST((size_t)W * T), CT((size_t)W * T);
-    for (int t = 0; t < T; t++)
-        for (int k = 0; k <= T; k++) {
-            ST[(size_t)k * T + t] = S[(size_t)t * W + k];
-            CT[(size_t)k * T + t] = C[(size_t)t * W + k];
-        }
-    std::vector<int> order(T);
-    for (int i = 0, o = 0; o < T; i++) {
-        order[o++] = i;
-        if (o < T)
-            order[o++] = T - 1 - i;
-    }
+    h->d_S = h->tabs->d_S;
+    h->d_C = h->tabs->d_C;
+    h->d_ST = h->tabs->d_ST;
+    h->d_CT = h->tabs->d_CT;
+    h->d_order = h->tabs->d_order;
     int64_t tab_len = (int64_t)T * T * (T + 2);
     if (h->d == 3)
         tab_len *= T;
@@ -230,17 +214,11 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     h->occ_len = tab_len / (T + 2) * occw;
     int64_t cap0 = cfg.max_paths > 0 ? cfg.max_paths : 4096;
 
-    int32_t rc = UG_OK;
-#define UG_ALLOC(ptr, bytes)                                                             \
-    do {                                                                                 \
-        if (rc == UG_OK && hipMalloc((void**)&(ptr), (bytes)) != hipSuccess)            \
-            rc = fail(UG_ENOMEM, "device allocation of %zu bytes failed", (size_t)(bytes)); \
+#define UG_ALLOC(ptr, bytes)                                                                 \
+    do {                                                                                     \
+        if (rc == UG_OK && !((ptr) = (decltype(ptr))cache_alloc(bytes)))                     \
+            rc = fail(UG_ENOMEM, "device allocation of %zu bytes failed", (size_t)(bytes));     \
     } while (0)
-    UG_ALLOC(h->d_S, S.size() * 8);
-    UG_ALLOC(h->d_C, C.size() * 8);
-    UG_ALLOC(h->d_ST, ST.size() * 8);
-    UG_ALLOC(h->d_CT, CT.size() * 8);
-    UG_ALLOC(h->d_order, (size_t)T * 4);
     UG_ALLOC(h->d_tab, (size_t)tab_len * 4);
     UG_ALLOC(h->d_occ, (size_t)h->occ_len * 4);
     UG_ALLOC(h->d_filled, 8);
@@ -257,25 +235,13 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     UG_ALLOC(h->d_acc_list, (size_t)UG_S_MAX * 4);
     UG_ALLOC(h->d_nacc, 4);
 #undef UG_ALLOC
-    if (rc == UG_OK && hipHostMalloc(&h->h_pinned, 64 + 8 * UG_S_MAX) != hipSuccess)
+    if (rc == UG_OK && !(h->h_pinned = cache_host_alloc(64 + 8 * UG_S_MAX)))
         rc = fail(UG_ENOMEM, "pinned host allocation failed");
     if (rc == UG_OK) {
-        rc = hip_check(hipMemcpyAsync(h->d_S, S.data(), S.size() * 8, hipMemcpyHostToDevice, h->stream), "upload S");
+        hipLaunchKernelGGL(k_fill_tab, dim3(grid_for(tab_len, 256 * 8)), dim3(256), 0, h->stream, h->d_tab, tab_len);
+        rc = hip_check(hipMemsetAsync(h->d_filled, 0, 8, h->stream), "clear filled");
         if (!rc)
-            rc = hip_check(hipMemcpyAsync(h->d_C, C.data(), C.size() * 8, hipMemcpyHostToDevice, h->stream), "upload C");
-        if (!rc)
-            rc = hip_check(hipMemcpyAsync(h->d_ST, ST.data(), ST.size() * 8, hipMemcpyHostToDevice, h->stream), "upload ST");
-        if (!rc)
-            rc = hip_check(hipMemcpyAsync(h->d_CT, CT.data(), CT.size() * 8, hipMemcpyHostToDevice, h->stream), "upload CT");
-        if (!rc)
-            rc = hip_check(hipMemcpyAsync(h->d_order, order.data(), (size_t)T * 4, hipMemcpyHostToDevice, h->stream),
-                           "upload order");
-        if (!rc) {
-            hipLaunchKernelGGL(k_fill_tab, dim3(grid_for(tab_len, 256 * 8)), dim3(256), 0, h->stream, h->d_tab, tab_len);
-            rc = hip_check(hipMemsetAsync(h->d_filled, 0, 8, h->stream), "clear filled");
-            if (!rc)
-                rc = hip_check(hipMemsetAsync(h->d_occ, 0, (size_t)h->occ_len * 4, h->stream), "clear occupancy");
-        }
+            rc = hip_check(hipMemsetAsync(h->d_occ, 0, (size_t)h->occ_len * 4, h->stream), "clear occupancy");
     }
     DevModel& m = h->dm;
     m.T = T;
@@ -290,7 +256,7 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     m.euclid = cfg.euclid ? 1 : 0;
     m.r2 = m.r * m.r;
     m.halfT = 0.5 * (double)T;
-    m.ldb = W;
+    m.ldb = T + 1;
     m.rowlen = T + 2;
     m.S = h->d_S;
     m.C = h->d_C;
@@ -578,7 +544,7 @@ extern "C" int32_t ug_pack_profile(ug_pack* hp, int32_t enable) {
         return fail(UG_EINVAL, "ug_pack_profile: NULL handle");
     Pack* h = (Pack*)hp;
     if (enable && !h->d_ctr) {
-        if (hipMalloc((void**)&h->d_ctr, UG_CTR_WORDS * sizeof(unsigned long long)) != hipSuccess)
+        if (!(h->d_ctr = (unsigned long long*)cache_alloc(UG_CTR_WORDS * sizeof(unsigned long long))))
             return fail(UG_ENOMEM, "ug_pack_profile: counter allocation failed");
         if (hipMemset(h->d_ctr, 0, UG_CTR_WORDS * sizeof(unsigned long long)) != hipSuccess)
             return fail(UG_EHIP, "ug_pack_profile: counter clear failed");
@@ -607,11 +573,11 @@ extern "C" int32_t ug_pack_run_sub(ug_pack* hp, const ug_sub_config* sc, int64_t
         return fail(UG_EINVAL, "error: --sub-terms %d differs from --terms %d (not supported)", sc->sub_terms,
                     h->cfg.terms);
     if (h->d_gflag == nullptr) {
-        if (hipMalloc((void**)&h->d_gflag, (size_t)UG_SUB_MAX * 4) != hipSuccess ||
-            hipMalloc((void**)&h->d_glist, (size_t)UG_SUB_MAX * 4) != hipSuccess ||
-            hipMalloc((void**)&h->d_glist2, (size_t)UG_SUB_MAX * 4) != hipSuccess ||
-            hipMalloc((void**)&h->d_gsurv, (size_t)UG_S_MAX * 4) != hipSuccess ||
-            hipMalloc((void**)&h->d_rscount, 16) != hipSuccess)
+        if (!(h->d_gflag = (int*)cache_alloc((size_t)UG_SUB_MAX * 4)) ||
+            !(h->d_glist = (int*)cache_alloc((size_t)UG_SUB_MAX * 4)) ||
+            !(h->d_glist2 = (int*)cache_alloc((size_t)UG_SUB_MAX * 4)) ||
+            !(h->d_gsurv = (int*)cache_alloc((size_t)UG_S_MAX * 4)) ||
+            !(h->d_rscount = (int*)cache_alloc(16)))
             return fail(UG_ENOMEM, "ug_pack_run_sub: allocation failed");
         h->done = true;  // the unique phase ends where the subpath phase starts
     }

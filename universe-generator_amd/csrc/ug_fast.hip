@@ -437,26 +437,15 @@ struct FastState {
 
 namespace ug {
 
-int32_t fast_setup(Pack* h) {
-    if (!h->full)
-        return UG_OK;  // partial spectra run the FP64 candidate test (see launch_fast)
-    const int T = h->T, W = T + 1, npairs = h->npairs;
+// Split-BF16 tables of one full-spectrum model, from the host copies of its canonical basis
+// (built once per model, see ug_cache.hip).
+int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vector<double>& C,
+                   const std::vector<int>& order, double margin_req) {
+    const int T = tb.T, W = T + 1, npairs = (tb.nw + 1) / 2;
     FastState* fs = new FastState();
     fs->ks = (npairs + 3) / 4;
     fs->nb = (T + NCOL - 1) / NCOL;
     fs->ncols = std::min(T, NCOL);
-    // host copies of the canonical tables (same arithmetic as ug_pack_create)
-    std::vector<double> S((size_t)T * W), C((size_t)T * W);
-    int32_t rc = hip_check(hipMemcpy(S.data(), h->d_S, S.size() * 8, hipMemcpyDeviceToHost), "fast S");
-    if (!rc)
-        rc = hip_check(hipMemcpy(C.data(), h->d_C, C.size() * 8, hipMemcpyDeviceToHost), "fast C");
-    std::vector<int> order(T);
-    if (!rc)
-        rc = hip_check(hipMemcpy(order.data(), h->d_order, (size_t)T * 4, hipMemcpyDeviceToHost), "fast order");
-    if (rc) {
-        delete fs;
-        return rc;
-    }
     std::vector<int> colt((size_t)fs->nb * NCOL, -1);
     for (int oi = 0; oi < T; oi++)
         colt[oi] = order[oi];
@@ -495,7 +484,7 @@ int32_t fast_setup(Pack* h) {
     // certainty margin per examined column: B_t bounds sum |coef * basis| for unit budget
     const double n_add = 48.0 * fs->ks;
     const double eps = 3.1 * 0x1p-16 + n_add * 0x1p-24 + 0x1p-20;
-    h->marg_host.assign(fs->ncols, 0.f);
+    std::vector<float> marg(fs->ncols, 0.f);
     double mmax = 0.0;
     for (int col = 0; col < fs->ncols; col++) {
         const int t = colt[col];
@@ -507,39 +496,49 @@ int32_t fast_setup(Pack* h) {
             Bt = std::max(Bt, v / k);
         }
         double mt = Bt * eps + 2e-6;
-        mt = std::max(mt, h->cfg.margin);
-        h->marg_host[col] = (float)mt;
+        mt = std::max(mt, margin_req);
+        marg[col] = (float)mt;
         mmax = std::max(mmax, mt);
     }
-    h->margin = mmax;
+    tb.margin = mmax;
+    tb.fast = fs;  // owned by the tables from here on (kept for the process)
     if (hipMalloc((void**)&fs->d_bfrag, frag.size() * 2) != hipSuccess ||
         hipMalloc((void**)&fs->d_invk, invk.size() * 4) != hipSuccess ||
         hipMalloc((void**)&fs->d_colt, colt.size() * 4) != hipSuccess ||
-        hipMalloc((void**)&h->d_marg, h->marg_host.size() * 4) != hipSuccess) {
-        h->d_basis16 = fs;
+        hipMalloc((void**)&tb.d_marg, marg.size() * 4) != hipSuccess)
         return fail(UG_ENOMEM, "split-BF16 table allocation failed");
-    }
-    rc = hip_check(hipMemcpy(fs->d_bfrag, frag.data(), frag.size() * 2, hipMemcpyHostToDevice), "frag upload");
+    int32_t rc = hip_check(hipMemcpy(fs->d_bfrag, frag.data(), frag.size() * 2, hipMemcpyHostToDevice), "frag upload");
     if (!rc)
         rc = hip_check(hipMemcpy(fs->d_invk, invk.data(), invk.size() * 4, hipMemcpyHostToDevice), "invk upload");
     if (!rc)
         rc = hip_check(hipMemcpy(fs->d_colt, colt.data(), colt.size() * 4, hipMemcpyHostToDevice), "colt upload");
     if (!rc)
-        rc = hip_check(hipMemcpy(h->d_marg, h->marg_host.data(), h->marg_host.size() * 4, hipMemcpyHostToDevice),
-                       "margin upload");
-    h->dm.marg = h->d_marg;
-    fs->lds_prefilter = (size_t)FAST_WAVES * (h->d * TILE * NCOL + 5 * TILE) * 4;
-    h->d_basis16 = fs;
+        rc = hip_check(hipMemcpy(tb.d_marg, marg.data(), marg.size() * 4, hipMemcpyHostToDevice), "margin upload");
+    fs->lds_prefilter = (size_t)FAST_WAVES * (tb.d * TILE * NCOL + 5 * TILE) * 4;
     if (!rc && fs->lds_prefilter > 160 * 1024)
         rc = fail(UG_EINVAL, "split-BF16 tile needs %zu bytes of LDS", fs->lds_prefilter);
     if (!rc) {
-        const void* kf[2] = {h->d == 2 ? (const void*)k_fast_prefilter<2, false> : (const void*)k_fast_prefilter<3, false>,
-                             h->d == 2 ? (const void*)k_fast_prefilter<2, true> : (const void*)k_fast_prefilter<3, true>};
+        const void* kf[2] = {tb.d == 2 ? (const void*)k_fast_prefilter<2, false> : (const void*)k_fast_prefilter<3, false>,
+                             tb.d == 2 ? (const void*)k_fast_prefilter<2, true> : (const void*)k_fast_prefilter<3, true>};
         for (int v = 0; v < 2 && !rc; v++)
             rc = hip_check(hipFuncSetAttribute(kf[v], hipFuncAttributeMaxDynamicSharedMemorySize, (int)fs->lds_prefilter),
                            "lds attribute");
     }
     return rc;
+}
+
+// A handle's view of its model's split-BF16 tables.
+int32_t fast_setup(Pack* h) {
+    if (!h->full)
+        return UG_OK;  // partial spectra run the FP64 candidate test (see launch_fast)
+    const ModelTables& tb = *h->tabs;
+    if (!tb.fast)
+        return fail(UG_ESTATE, "split-BF16 tables missing for this model");
+    h->d_basis16 = tb.fast;
+    h->d_marg = tb.d_marg;
+    h->dm.marg = tb.d_marg;
+    h->margin = tb.margin;
+    return UG_OK;
 }
 
 static FastArgs fast_args(Pack* h) {
@@ -611,20 +610,6 @@ int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* ou
     else
         hipLaunchKernelGGL(k_fast_eval<3>, dim3(blocks), dim3(256), lds, h->stream, f, fs->nb, list, count, out);
     return hip_check(hipGetLastError(), "k_fast_eval");
-}
-
-void fast_release(Pack* h) {
-    FastState* fs = (FastState*)h->d_basis16;
-    if (!fs)
-        return;
-    if (fs->d_bfrag)
-        (void)hipFree(fs->d_bfrag);
-    if (fs->d_invk)
-        (void)hipFree(fs->d_invk);
-    if (fs->d_colt)
-        (void)hipFree(fs->d_colt);
-    delete fs;
-    h->d_basis16 = nullptr;
 }
 
 }  // namespace ug

@@ -5,6 +5,7 @@
 
 #include <array>
 #include <deque>
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -86,14 +87,28 @@ __device__ __forceinline__ uint32_t occ_bits3(const DevModel& m, int64_t row, in
     return (uint32_t)(v >> (c & 31)) & 7u;
 }
 
+// Read-only tables of one model, built once per (device, dim, T, terms, form, margin) and
+// shared by its handles (ug_cache.hip).
+struct ModelTables {
+    int device = 0, d = 0, T = 0, nw = 0;
+    bool split = false;
+    double *d_S = nullptr, *d_C = nullptr, *d_ST = nullptr, *d_CT = nullptr;
+    int* d_order = nullptr;
+    // split-BF16 form of full spectra (ug_fast.hip)
+    void* fast = nullptr;     // basis fragments and launch shape
+    float* d_marg = nullptr;  // certainty margin per examined column
+    double margin = 0.0;      // its maximum
+};
+
 struct Pack {
     ug_pack_config cfg;
+    std::shared_ptr<const ModelTables> tabs;
     hipStream_t stream;
     int device;
     int d, T, nw, npairs, modmax;
     bool full;
     DevModel dm;
-    // tables
+    // tables (d_S .. d_order, d_marg: views of tabs)
     double *d_S = nullptr, *d_C = nullptr, *d_ST = nullptr, *d_CT = nullptr;
     int* d_order = nullptr;
     int* d_tab = nullptr;
@@ -101,7 +116,6 @@ struct Pack {
     uint32_t* d_occ = nullptr;
     int64_t occ_len = 0;
     float* d_marg = nullptr;
-    std::vector<float> marg_host;
     double margin = 0.0;
     // accepted paths
     double* d_pos64 = nullptr;
@@ -171,6 +185,13 @@ struct Pack {
 };
 
 int32_t fail(int32_t code, const char* fmt, ...);
+// recycled memory and shared model tables (ug_cache.hip)
+void* cache_alloc(size_t bytes);  // device memory of the current device; nullptr on failure
+void cache_free(void* p);
+void* cache_host_alloc(size_t bytes);  // pinned host memory
+void cache_host_free(void* p);
+int32_t model_tables(int d, int T, int nw, bool split_full, double margin_req,
+                     std::shared_ptr<const ModelTables>* out);
 int32_t hip_check(hipError_t e, const char* what);
 int grid_for(int64_t items, int per_block);
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
@@ -216,12 +237,13 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
 int32_t ensure_paths(Pack* h, int64_t need);
 int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total, const int* dS = nullptr);
 // split-BF16 prefilter (ug_fast.hip)
+int32_t fast_build(ModelTables& t, const std::vector<double>& S, const std::vector<double>& C,
+                   const std::vector<int>& order, double margin_req);
 int32_t fast_setup(Pack* h);
 int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags);
 // Subpath-phase prefilter: flags[i] = 0 when candidate base + i certainly touches two groups
 // among the rows < nlimit.
 int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, int64_t nlimit);
 int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* out);
-void fast_release(Pack* h);
 
 }  // namespace ug

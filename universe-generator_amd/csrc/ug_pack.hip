@@ -1324,13 +1324,12 @@ int32_t ensure_paths(Pack* h, int64_t need) {
     int64_t* ai = nullptr;
     int32_t* ag = nullptr;
     const size_t T = (size_t)h->T, D = (size_t)h->d;
-    if (hipMalloc(&p64, (size_t)nc * T * D * 8) != hipSuccess ||
-        hipMalloc(&p32, (size_t)nc * T * 16) != hipSuccess ||
-        hipMalloc(&ai, (size_t)nc * 8) != hipSuccess || hipMalloc(&ag, (size_t)nc * 4) != hipSuccess) {
-        (void)hipFree(p64);
-        (void)hipFree(p32);
-        (void)hipFree(ai);
-        (void)hipFree(ag);
+    if (!(p64 = (double*)cache_alloc((size_t)nc * T * D * 8)) || !(p32 = (float4*)cache_alloc((size_t)nc * T * 16)) ||
+        !(ai = (int64_t*)cache_alloc((size_t)nc * 8)) || !(ag = (int32_t*)cache_alloc((size_t)nc * 4))) {
+        cache_free(p64);
+        cache_free(p32);
+        cache_free(ai);
+        cache_free(ag);
         return fail(UG_ENOMEM, "cannot grow path storage to %lld paths", (long long)nc);
     }
     const size_t rows = (size_t)(h->n + h->nsub);  // uniques, then subpaths
@@ -1340,11 +1339,12 @@ int32_t ensure_paths(Pack* h, int64_t need) {
         (void)hipMemcpyAsync(ai, h->d_acc_idx, rows * 8, hipMemcpyDeviceToDevice, h->stream);
         (void)hipMemcpyAsync(ag, h->d_acc_gid, rows * 4, hipMemcpyDeviceToDevice, h->stream);
     }
+    // the copies have finished before the old buffers can serve another handle
     int32_t rc = hip_check(hipStreamSynchronize(h->stream), "grow copy");
-    (void)hipFree(h->d_pos64);
-    (void)hipFree(h->d_pos32);
-    (void)hipFree(h->d_acc_idx);
-    (void)hipFree(h->d_acc_gid);
+    cache_free(h->d_pos64);
+    cache_free(h->d_pos32);
+    cache_free(h->d_acc_idx);
+    cache_free(h->d_acc_gid);
     h->d_pos64 = p64;
     h->d_pos32 = p32;
     h->d_acc_idx = ai;

@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
         "ug_pack_profile": ([vp, i32], i32),
         "ug_pack_kernel_stats": ([vp, i32, C.POINTER(KernelStats)], i32),
         "ug_pack_destroy": ([vp], None),
+        "ug_cache_trim": ([], i32),
         "ug_pack_check": ([vp, C.c_uint64, i64, vp], i32),
         "ug_ensemble_run": ([C.POINTER(PackConfig), i32, i32, C.POINTER(vp), C.POINTER(PackStatus)], i32),
         "ug_paths_create": ([i32, i64, i32, vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
@@ -122,12 +123,18 @@ def lib() -> C.CDLL:
     return L
 
 
+def cache_trim() -> None:
+    """Free the cached device buffers of destroyed handles (model tables stay)."""
+    check(lib().ug_cache_trim())
+
+
 def exported_symbols() -> list[str]:
     return [
         "ug_abi_version", "ug_last_error", "ug_pack_create", "ug_pack_run", "ug_pack_run_sub",
         "ug_pack_get_status", "ug_pack_accepted", "ug_pack_params", "ug_pack_positions",
         "ug_pack_curve", "ug_eval_candidates", "ug_pack_profile", "ug_pack_kernel_stats",
-        "ug_pack_destroy", "ug_pack_check", "ug_ensemble_run", "ug_paths_create", "ug_paths_destroy", "ug_paths_from_pack",
+        "ug_pack_destroy", "ug_cache_trim", "ug_pack_check", "ug_ensemble_run", "ug_paths_create",
+        "ug_paths_destroy", "ug_paths_from_pack",
         "ug_frame_bake_lightcone",
         "ug_frame_bake_slices",
     ]
