@@ -30,14 +30,24 @@ struct u32x4 {
     uint32_t x, y, z, w;
 };
 
+// a ^ b ^ c: one three-input bitwise instruction on gfx950 (truth table 0x96), where the
+// round function would otherwise spend two XORs per output word.
+UG_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 UG_HD u32x4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
                    uint32_t k1) {
 #pragma unroll
     for (int rnd = 0; rnd < 10; rnd++) {
         const uint64_t m0 = (uint64_t)0xD2511F53u * c0;
         const uint64_t m1 = (uint64_t)0xCD9E8D57u * c2;
-        const uint32_t n0 = (uint32_t)(m1 >> 32) ^ c1 ^ k0;
-        const uint32_t n2 = (uint32_t)(m0 >> 32) ^ c3 ^ k1;
+        const uint32_t n0 = xor3((uint32_t)(m1 >> 32), c1, k0);
+        const uint32_t n2 = xor3((uint32_t)(m0 >> 32), c3, k1);
         c1 = (uint32_t)m1;
         c3 = (uint32_t)m0;
         c0 = n0;

@@ -55,22 +55,18 @@ __device__ __forceinline__ void sincospi_f32_small(float r, float& s, float& c) 
 
 // cos(pi u), sin(pi u) for u = w 2^-32 in [0, 1).
 __device__ __forceinline__ void phase_cs(uint32_t w, float& cf, float& sf) {
-    // q = nearest integer to 2u, r = u - q/2 computed exactly in integers
-    const uint32_t q = (uint32_t)(((uint64_t)w + 0x40000000ull) >> 31);  // 0, 1 or 2
-    const int64_t num = (int64_t)w - (int64_t)q * 0x80000000ll;          // |num| <= 2^30
+    // q = nearest integer to 2u (0, 1 or 2) and r = u - q/2, exactly in 32-bit integers:
+    // w - q 2^31 wraps modulo 2^32 onto the int32 value num, |num| <= 2^30
+    const uint32_t q = ((w >> 30) + 1u) >> 1;
+    const int32_t num = (int32_t)(w - (q << 31));
     const float r = (float)num * 0x1p-32f;
     float s, c;
     sincospi_f32_small(r, s, c);
-    if (q == 0) {
-        cf = c;
-        sf = s;
-    } else if (q == 1) {
-        cf = -s;
-        sf = c;
-    } else {
-        cf = -c;
-        sf = -s;
-    }
+    // q = 0: (c, s); q = 1: (-s, c); q = 2: (-c, -s) -- selects, no branches
+    const bool odd = q == 1u;
+    const float c0 = odd ? s : c, s0 = odd ? c : s;
+    cf = q != 0u ? -c0 : c0;
+    sf = q == 2u ? -s0 : s0;
 }
 
 __device__ __forceinline__ float expo_f32(uint32_t w) {
@@ -96,9 +92,8 @@ __device__ __forceinline__ void gen_kstep(const DevModel& m, uint64_t idx, bool 
             const u32x4 r = philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, m.key0, m.key1);
             const int j0 = 2 * p;
             const float e0 = expo_f32(r.x);
-            float ae = e0 * invk[j0 + 2];
-            if (r.x & 1u)
-                ae = -ae;
+            // bit 0 of the Dirichlet word is the slot's sign
+            const float ae = __uint_as_float(__float_as_uint(e0 * invk[j0 + 2]) ^ (r.x << 31));
             float cf, sf;
             phase_cs(r.z, cf, sf);
             v0 = ae * cf;
@@ -106,10 +101,7 @@ __device__ __forceinline__ void gen_kstep(const DevModel& m, uint64_t idx, bool 
             Wpart += e0;
             if (j0 + 1 < m.nw) {
                 const float e1 = expo_f32(r.y);
-                float ao = e1 * invk[j0 + 3];
-                if (r.y & 1u)
-                    ao = -ao;
-                v2 = ao;
+                v2 = __uint_as_float(__float_as_uint(e1 * invk[j0 + 3]) ^ (r.y << 31));
                 Wpart += e1;
             }
         }
