@@ -117,6 +117,7 @@ void release(Pack* h) {
     free_dev(h->d_list);
     free_dev(h->d_list2);
     free_dev(h->d_list3);
+    free_dev(h->d_list4);
     free_dev(h->d_bcount);
     free_dev(h->d_count);
     free_dev(h->d_spos);
@@ -226,6 +227,7 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     UG_ALLOC(h->d_list, (size_t)UG_SUB_MAX * 8);
     UG_ALLOC(h->d_list2, (size_t)UG_SUB_MAX * 8);
     UG_ALLOC(h->d_list3, (size_t)UG_SUB_MAX * 8);
+    UG_ALLOC(h->d_list4, (size_t)UG_SUB_MAX * 8);
     h->cmp_blocks_cap = UG_SUB_MAX / 4096 + 1;
     UG_ALLOC(h->d_bcount, (size_t)h->cmp_blocks_cap * 4);
     UG_ALLOC(h->d_count, 16);

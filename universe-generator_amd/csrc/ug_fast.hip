@@ -123,7 +123,7 @@ __device__ __forceinline__ void gen_kstep(const DevModel& m, uint64_t idx, bool 
 struct FastArgs {
     DevModel m;
     int ks;                 // K steps (4 pairs each)
-    int ncols;              // ordered timesteps examined (min(T, 32))
+    int ncols;              // ordered timesteps with a certainty margin (T, in N blocks of 32)
     const bf16x8* bfrag;    // [nb][ks][2][64] basis fragments, columns in endpoint-first order
     const float* invk;      // [T+2] 1/k as float
     const int* colt;        // [nb*32] timestep of each column (-1 = padding)
@@ -195,7 +195,8 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 // candidate fails both the round-start test and the adoption test.
 template <int D, bool SUB>
 __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
-                                                        const int64_t* list, uint8_t* flags,
+                                                        const int64_t* list, const int* dcount,
+                                                        int cb0, int cb1, int par, uint8_t* flags,
                                                         unsigned long long* ctr, const int* acc_gid,
                                                         int64_t nlimit) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -210,24 +211,37 @@ __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t 
     int* smin = (int*)(sinvw + 2 * TILE);  // after ulist: per-candidate group range of a round
     int* smax = smin + TILE;
     const int mrow = lane & 31, h = lane >> 5;
+    if (dcount)
+        count = min(count, (int64_t)*dcount);
     const int64_t ntiles = (count + TILE - 1) / TILE;
+    // par: one work item per (tile, N block), flags preset to 1 and cleared by whichever block
+    // rejects; otherwise one item per tile walking its N blocks in order
+    const int nbk = par ? cb1 - cb0 : 1;
     unsigned long long nvis = 0, npts = 0;
     const float r = (float)m.r;
-    for (int64_t tile = (int64_t)blockIdx.x * FAST_WAVES + wv; tile < ntiles;
-         tile += (int64_t)gridDim.x * FAST_WAVES) {
+    for (int64_t item = (int64_t)blockIdx.x * FAST_WAVES + wv; item < ntiles * nbk;
+         item += (int64_t)gridDim.x * FAST_WAVES) {
+        const int64_t tile = item / nbk;
+        const int nbs = par ? cb0 + (int)(item % nbk) : cb0, nbe = par ? nbs + 1 : cb1;
         const int64_t i = tile * TILE + mrow;
         const bool valid = i < count;
         const uint64_t idx = list ? (valid ? (uint64_t)list[i] : 0ull) : base + (uint64_t)(valid ? i : 0);
-        const bool wbad = tile_positions<D>(f, idx, valid, lane, 0, bl, pos, sa1, sinvw);
+        uint32_t rejm = 0;
+        int g1 = -1;  // SUB, lanes < 32: the one group candidate `lane` certainly touches so far
+        // N blocks of 32 ordered timesteps each; the coefficients are generated again per
+        // block (a later block only sees the candidates no earlier one rejected)
+        for (int nb = nbs; nb < nbe; nb++) {
+        const bool wbad = tile_positions<D>(f, idx, valid, lane, nb, bl, pos, sa1, sinvw);
         // Adaptive cooperative lookup: every round hands all 64 lanes to the still
         // unresolved candidates of the tile (64/U consecutive columns each), so the tile
         // finishes in a few dependent memory round trips.  A candidate whose exponential
         // spacings sum below nw/4 (wbad) is outside the error bound's premise and goes
         // straight to the FP64 test.
-        uint32_t unres = (uint32_t)__ballot(lane < 32 && valid && !wbad);
-        uint32_t rejm = 0;
-        int ncol = 0;  // lanes < 32: next column of candidate `lane`
-        int g1 = -1;   // SUB, lanes < 32: the one group candidate `lane` certainly touches so far
+        uint32_t unres = (uint32_t)__ballot(lane < 32 && valid && !wbad && !((rejm >> lane) & 1u));
+        if (!unres)
+            break;
+        const int col0 = NCOL * nb, bcols = min(NCOL, f.ncols - col0);  // this block's columns
+        int ncol = 0;  // lanes < 32: next column (within the block) of candidate `lane`
         int* ulist = (int*)(sinvw + TILE);
         while (unres) {
             const int U = __popc(unres);
@@ -246,9 +260,9 @@ __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t 
             const int col = __shfl(ncol, cm, 64) + off;
             bool cert = false;
             int gmn = INT32_MAX, gmx = -1;  // SUB: groups of this lane's certain contacts
-            if (rk < U && col < f.ncols) {
-                const int t = f.colt[col];
-                const float mt = m.marg[col];
+            if (rk < U && col < bcols) {
+                const int t = f.colt[col0 + col];
+                const float mt = m.marg[col0 + col];
                 // L2 ball: the FP64 distance exceeds the FP32 one by at most sqrt(D)*mt, and
                 // the FP32 sum of squares is within (D+1)*2^-24 of its exact value
                 const float rb = r - sqrtf((float)D) * mt;
@@ -354,12 +368,17 @@ __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t 
                 }
             }
             const uint32_t rej_all = (uint32_t)__ballot(lane < 32 && ((rejm >> lane) & 1u));
-            const uint32_t done = (uint32_t)__ballot(lane < 32 && ncol >= f.ncols);
+            const uint32_t done = (uint32_t)__ballot(lane < 32 && ncol >= bcols);
             unres &= ~(rej_all | done);
             rejm = rej_all;
         }
-        if (valid && h == 0)
-            flags[i] = ((rejm >> lane) & 1u) ? 0 : 1;
+        }
+        if (valid && h == 0) {
+            if (!par)
+                flags[i] = ((rejm >> lane) & 1u) ? 0 : 1;
+            else if ((rejm >> lane) & 1u)
+                flags[i] = 0;
+        }
     }
     if (ctr) {
         unsigned long long a = nvis, b = npts;
@@ -373,6 +392,14 @@ __global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t 
             atomicAdd(&c[1], b);
         }
     }
+}
+
+// flags[0 .. min(count, *dcount)) = 1 (the parallel-block mode's starting value).
+__global__ void k_flags_one(uint8_t* flags, int64_t count, const int* dcount) {
+    if (dcount)
+        count = min(count, (int64_t)*dcount);
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256)
+        flags[i] = 1;
 }
 
 // Split-BF16 positions of listed candidates at every timestep (error reporting, tests).
@@ -437,7 +464,7 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
     FastState* fs = new FastState();
     fs->ks = (npairs + 3) / 4;
     fs->nb = (T + NCOL - 1) / NCOL;
-    fs->ncols = std::min(T, NCOL);
+    fs->ncols = T;
     std::vector<int> colt((size_t)fs->nb * NCOL, -1);
     for (int oi = 0; oi < T; oi++)
         colt[oi] = order[oi];
@@ -476,7 +503,7 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
     // certainty margin per examined column: B_t bounds sum |coef * basis| for unit budget
     const double n_add = 48.0 * fs->ks;
     const double eps = 3.1 * 0x1p-16 + n_add * 0x1p-24 + 0x1p-20;
-    std::vector<float> marg(fs->ncols, 0.f);
+    std::vector<float> marg((size_t)fs->nb * NCOL, 0.f);  // padding columns are never examined
     double mmax = 0.0;
     for (int col = 0; col < fs->ncols; col++) {
         const int t = colt[col];
@@ -545,22 +572,37 @@ static FastArgs fast_args(Pack* h) {
     return f;
 }
 
-int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags) {
+int fast_blocks(const Pack* h) {
+    return h->d_basis16 ? ((const FastState*)h->d_basis16)->nb : 0;
+}
+
+int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
+                    const int* dcount, int cb0, int cb1) {
     if (count <= 0)
         return UG_OK;
     if (!h->full || !h->d_basis16)
         return fail(UG_ESTATE, "split-BF16 prefilter not set up for this model");
     FastState* fs = (FastState*)h->d_basis16;
     const int64_t ntiles = (count + TILE - 1) / TILE;
-    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, 2048));
+    // a device-counted list is short (what an earlier stage left): fewer persistent blocks
+    const int cap = dcount ? 512 : 2048;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, cap));
     const FastArgs f = fast_args(h);
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
+    cb1 = std::min(cb1, fs->nb);
+    if (cb0 >= cb1)
+        return fail(UG_EINVAL, "prefilter: empty column-block range [%d, %d)", cb0, cb1);
+    // several N blocks of a short list run in parallel (one work item per tile and block)
+    const int par = (dcount && cb1 - cb0 > 1) ? 1 : 0;
+    const int nblk = par ? std::min(2048, blocks * (cb1 - cb0)) : blocks;
+    if (par)
+        hipLaunchKernelGGL(k_flags_one, dim3(grid_for(count, 256 * 16)), dim3(256), 0, h->stream, flags, count, dcount);
     if (h->d == 2)
-        UG_LAUNCH(h, (k_fast_prefilter<2, false>), dim3(blocks), dim3(256), fs->lds_prefilter, f, base, count,
-                  list, flags, ctr, nullptr, 0);
+        UG_LAUNCH(h, (k_fast_prefilter<2, false>), dim3(nblk), dim3(256), fs->lds_prefilter, f, base, count,
+                  list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0);
     else
-        UG_LAUNCH(h, (k_fast_prefilter<3, false>), dim3(blocks), dim3(256), fs->lds_prefilter, f, base, count,
-                  list, flags, ctr, nullptr, 0);
+        UG_LAUNCH(h, (k_fast_prefilter<3, false>), dim3(nblk), dim3(256), fs->lds_prefilter, f, base, count,
+                  list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0);
     return hip_check(hipGetLastError(), "k_fast_prefilter");
 }
 
@@ -575,10 +617,10 @@ int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, i
     const FastArgs f = fast_args(h);
     if (h->d == 2)
         hipLaunchKernelGGL((k_fast_prefilter<2, true>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
-                           count, nullptr, flags, nullptr, h->d_acc_gid, nlimit);
+                           count, nullptr, nullptr, 0, 1, 0, flags, nullptr, h->d_acc_gid, nlimit);
     else
         hipLaunchKernelGGL((k_fast_prefilter<3, true>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
-                           count, nullptr, flags, nullptr, h->d_acc_gid, nlimit);
+                           count, nullptr, nullptr, 0, 1, 0, flags, nullptr, h->d_acc_gid, nlimit);
     return hip_check(hipGetLastError(), "k_fast_prefilter (subpaths)");
 }
 

@@ -130,6 +130,8 @@ struct Pack {
     int64_t* d_list = nullptr;
     int64_t* d_list2 = nullptr;
     int64_t* d_list3 = nullptr;
+    int64_t* d_list4 = nullptr;
+    int recheck_slot = 0;  // d_count slot holding the last filter's FP64 recheck count
     int* d_bcount = nullptr;
     int cmp_blocks_cap = 0;
     int* d_count = nullptr;
@@ -240,7 +242,12 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
 int32_t fast_build(ModelTables& t, const std::vector<double>& S, const std::vector<double>& C,
                    const std::vector<int>& order, double margin_req);
 int32_t fast_setup(Pack* h);
-int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags);
+// Split-BF16 certain-contact test of fresh candidates base + i (list == nullptr) or of
+// list[i], i < count (bounded by *dcount when given), over the N blocks [cb0, cb1) of 32
+// ordered timesteps: flags[i] = 0 when candidate i certainly collides.
+int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
+                    const int* dcount = nullptr, int cb0 = 0, int cb1 = 1);
+int fast_blocks(const Pack* h);  // N blocks of the model's ordered timesteps (0: no split form)
 // Subpath-phase prefilter: flags[i] = 0 when candidate base + i certainly touches two groups
 // among the rows < nlimit.
 int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, int64_t nlimit);

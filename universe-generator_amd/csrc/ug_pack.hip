@@ -960,27 +960,46 @@ __global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos
 }
 
 // Greedy in index order over the conflict bits (one wave): survivor i is accepted iff it
-// conflicts with no earlier accepted survivor -- serial RSA restricted to the chunk.
+// conflicts with no earlier accepted survivor -- serial RSA restricted to the chunk.  Lane
+// i of block blk needs words 0..blk of its mask row; they are loaded together, and the next
+// block's while this one is decided, so the pass pays about one load latency in all.
 __global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, const int* dS, int* acc_list,
                                                int* nacc) {
     if (dS)
         S = min(S, *dS);
     __shared__ uint64_t accw[UG_MASK_WORDS];
     const int lane = threadIdx.x;
+    uint64_t cur[UG_MASK_WORDS], nxt[UG_MASK_WORDS];
+    auto load = [&](uint64_t* w, int blk) {
+        const int i = blk * 64 + lane;
+#pragma unroll
+        for (int wd = 0; wd < UG_MASK_WORDS; wd++)
+            w[wd] = (i < S && wd <= blk) ? mask[(size_t)i * UG_MASK_WORDS + wd] : 0ull;
+    };
+    load(cur, 0);
     int na = 0;
     for (int blk = 0; blk * 64 < S; blk++) {
+        if ((blk + 1) * 64 < S)
+            load(nxt, blk + 1);
         const int i = blk * 64 + lane;
         const bool valid = i < S;
-        bool dead = !valid;
-        for (int wd = 0; wd < blk && valid; wd++)
-            if (mask[(size_t)i * UG_MASK_WORDS + wd] & accw[wd])
-                dead = true;
-        const uint64_t inrow = valid ? mask[(size_t)i * UG_MASK_WORDS + blk] : 0ull;
-        const uint64_t deadm = __ballot(dead);
+        uint64_t killed = 0ull, inrow = 0ull;
+#pragma unroll
+        for (int wd = 0; wd < UG_MASK_WORDS; wd++) {
+            if (wd < blk)
+                killed |= cur[wd] & accw[wd];
+            if (wd == blk)
+                inrow = cur[wd];
+        }
+        const uint64_t deadm = __ballot(!valid || killed != 0ull);
+        // the serial decision runs on the scalar unit: lane k's row is read into scalar
+        // registers (v_readlane), so no step waits on a cross-lane shuffle through LDS
         uint64_t accb = 0;
         const int nin = min(64, S - blk * 64);
+        const uint32_t in_lo = (uint32_t)inrow, in_hi = (uint32_t)(inrow >> 32);
         for (int k = 0; k < nin; k++) {
-            const uint64_t rowk = __shfl(inrow, k, 64);
+            const uint64_t rowk = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)in_lo, k) |
+                                  ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)in_hi, k) << 32);
             if (!((deadm >> k) & 1ull) && (rowk & accb) == 0)
                 accb |= 1ull << k;
         }
@@ -992,6 +1011,9 @@ __global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, cons
             acc_list[pos] = i;
         }
         na += __popcll(accb);
+#pragma unroll
+        for (int wd = 0; wd < UG_MASK_WORDS; wd++)
+            cur[wd] = nxt[wd];
     }
     if (lane == 0)
         *nacc = na;
@@ -1264,7 +1286,7 @@ int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, con
     int32_t rc = hip_check(hipGetLastError(), "compact");
     if (rc || !sync)
         return rc;
-    rc = hip_check(hipMemcpyAsync(h->h_pinned, h->d_count, 8, hipMemcpyDeviceToHost, h->stream),
+    rc = hip_check(hipMemcpyAsync(h->h_pinned, h->d_count, 16, hipMemcpyDeviceToHost, h->stream),
                    "compact count copy");
     if (!rc)
         rc = hip_check(hipStreamSynchronize(h->stream), "compact sync");
@@ -1295,6 +1317,8 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
             rc = compact(h, h->d_flags, count, base, list, out, S, nullptr, 1, sync);
         return rc;
     }
+    // stage 1: the first 32 ordered timesteps of every candidate -> survivors in d_list3
+    // (count in d_count[0])
     rc = timed_begin(h);
     if (!rc)
         rc = launch_fast(h, base, count, list, h->d_flags);
@@ -1302,16 +1326,35 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
         rc = timed_end(h, UG_KERNEL_PREFILTER, count);
     if (!rc)
         rc = compact(h, h->d_flags, count, base, list, h->d_list3, nullptr, nullptr, 0, false);
+    const int64_t* rlist = h->d_list3;
+    int rslot = 0;
+    if (!rc && fast_blocks(h) >= 6) {
+        // stage 2 (T > 160): the remaining ordered timesteps, split-BF16 as well, for the few
+        // that stage 1 could not reject -> d_list4 (count in d_count[2]); the FP64 recheck
+        // then sees only candidates no timestep certainly rejects.  Measured on one MI355X:
+        // 3+1 T=256 packs 1.23x faster with it, 3+1 T=128 about 4% slower (its FP64 recheck
+        // is cheap enough), so shorter models skip it.
+        rc = timed_begin(h);
+        if (!rc)
+            rc = launch_fast(h, 0, count, h->d_list3, h->d_flags, h->d_count, 1, fast_blocks(h));
+        if (!rc)
+            rc = timed_end(h, UG_KERNEL_PREFILTER, 0);  // items: stage-1 survivors, on the device
+        if (!rc)
+            rc = compact(h, h->d_flags, count, 0, h->d_list3, h->d_list4, nullptr, h->d_count, 2, false);
+        rlist = h->d_list4;
+        rslot = 2;
+    }
+    h->recheck_slot = rslot;
     if (!rc)
         rc = timed_begin(h);
     if (!rc)
-        rc = launch_cand64(h, 0, count, h->d_list3, h->d_flags, nullptr, false, rctr, h->d_count, -1);
+        rc = launch_cand64(h, 0, count, rlist, h->d_flags, nullptr, false, rctr, h->d_count + rslot, -1);
     if (!rc)
         rc = timed_end(h, UG_KERNEL_RECHECK, 0);  // items: counted on the device
     if (!rc)
-        rc = compact(h, h->d_flags, count, 0, h->d_list3, out, S, h->d_count, 1, sync);
+        rc = compact(h, h->d_flags, count, 0, rlist, out, S, h->d_count + rslot, 1, sync);
     if (!rc && sync)
-        h->rechecked += ((int*)h->h_pinned)[0];
+        h->rechecked += ((int*)h->h_pinned)[rslot];
     return rc;
 }
 
@@ -1432,10 +1475,11 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         if (rc)
             return rc;
         if (dev) {
-            // the filter's counts: slot 1 = survivors, slot 0 = FP64 rechecks (split form)
+            // the filter's counts: slot 1 = survivors; FP64 rechecks (split form) in slot 0,
+            // or in slot 2 after the second split-BF16 stage
             S = ((const int*)(pin + 32))[dS - h->d_count];
             if (h->cfg.precision == UG_PREC_SPLIT_BF16 && h->full)
-                h->rechecked += ((const int*)(pin + 32))[0];
+                h->rechecked += ((const int*)(pin + 32))[h->recheck_slot];
             h->last_S = S;
             dev = false;
         }
