@@ -147,7 +147,8 @@ struct Pack {
     int* d_gsurv = nullptr;     // adopted groups of a resolve chunk
     int* d_rscount = nullptr;   // survivors of a round against its round-start packing
     int sub_done = 0;  // UG_SUB_* once the subpath phase stopped
-    int64_t last_S = 0;  // survivors of the last filter whose count stayed on the device
+    int64_t last_S = UG_S_MAX;  // survivors of the last filter whose count stayed on the device
+                                // (a fresh table lets a whole first sub-batch survive)
     // sub_batch: the round size, adapted as cuda/braid_cuda.cu:952,1000-1003;
     // sub_nlimit: rows stored before the current round started
     int64_t sub_attempts = 0, sub_batch = 1 << 16, sub_nlimit = 0;
