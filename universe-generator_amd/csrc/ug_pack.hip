@@ -1046,6 +1046,24 @@ __global__ __launch_bounds__(256) void k_conflicts_tiled(DevModel m, const doubl
         atomicOr((unsigned long long*)&mask[(size_t)(64 * bi + tid) * UG_MASK_WORDS + bj], word[tid]);
 }
 
+// A resolve chunk's results, stored straight into the pinned staging block (one dispatch in
+// place of four copies): [8] accepted count, [16] occupied (t, cell) buckets, [32] the
+// filter's four counts when they were still on the device, [64 ..] the accepted candidate
+// indices (at most s).  The host reads the block after synchronising the stream.
+__global__ __launch_bounds__(256) void k_report(const int* nacc, const unsigned long long* filled,
+                                                const int64_t* acc_idx, int s, const int* counts, char* pin) {
+    const int na = min(*nacc, s);
+    if (threadIdx.x == 0) {
+        *(int*)(pin + 8) = *nacc;
+        *(unsigned long long*)(pin + 16) = *filled;
+    }
+    if (counts && threadIdx.x < 4)
+        ((int*)(pin + 32))[threadIdx.x] = counts[threadIdx.x];
+    int64_t* out = (int64_t*)(pin + 64);
+    for (int k = threadIdx.x; k < na; k += 256)
+        out[k] = acc_idx[k];
+}
+
 // Greedy in index order over the conflict bits (one wave): survivor i is accepted iff it
 // conflicts with no earlier accepted survivor -- serial RSA restricted to the chunk.  Lane
 // i of block blk needs words 0..blk of its mask row; they are loaded together, and the next
@@ -1561,17 +1579,12 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         rc = hip_check(hipGetLastError(), "resolve kernels");
         if (rc)
             return rc;
-        // one round trip: accepted count, their candidate indices and the filled-cell count
+        // one round trip: accepted count, their candidate indices and the filled-cell count,
+        // written into the pinned block by one small kernel
         char* pin = (char*)h->h_pinned;
-        rc = hip_check(hipMemcpyAsync(pin + 8, h->d_nacc, 4, hipMemcpyDeviceToHost, h->stream), "nacc copy");
-        if (!rc)
-            rc = hip_check(hipMemcpyAsync(pin + 16, h->d_filled, 8, hipMemcpyDeviceToHost, h->stream), "filled copy");
-        if (!rc)
-            rc = hip_check(hipMemcpyAsync(pin + 64, h->d_acc_idx + h->n, (size_t)s * 8, hipMemcpyDeviceToHost,
-                                          h->stream),
-                           "accepted copy");
-        if (!rc && dev)
-            rc = hip_check(hipMemcpyAsync(pin + 32, h->d_count, 16, hipMemcpyDeviceToHost, h->stream), "count copy");
+        hipLaunchKernelGGL(k_report, dim3(1), dim3(256), 0, h->stream, h->d_nacc, h->d_filled, h->d_acc_idx + h->n, s,
+                           dev ? h->d_count : nullptr, pin);
+        rc = hip_check(hipGetLastError(), "k_report");
         if (!rc)
             rc = hip_check(hipStreamSynchronize(h->stream), "resolve sync");
         if (rc)
