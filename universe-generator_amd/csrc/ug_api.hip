@@ -118,6 +118,11 @@ void release(Pack* h) {
     free_dev(h->d_list2);
     free_dev(h->d_list3);
     free_dev(h->d_list4);
+    free_dev(h->d_listb);
+    free_dev(h->d_spec_count);
+    if (h->ev_report)
+        (void)hipEventDestroy(h->ev_report);
+    h->ev_report = nullptr;
     free_dev(h->d_bcount);
     free_dev(h->d_count);
     free_dev(h->d_spos);
@@ -228,6 +233,8 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     UG_ALLOC(h->d_list2, (size_t)UG_SUB_MAX * 8);
     UG_ALLOC(h->d_list3, (size_t)UG_SUB_MAX * 8);
     UG_ALLOC(h->d_list4, (size_t)UG_SUB_MAX * 8);
+    UG_ALLOC(h->d_listb, (size_t)UG_SUB_MAX * 8);
+    UG_ALLOC(h->d_spec_count, 4);
     h->cmp_blocks_cap = UG_SUB_MAX / 4096 + 1;
     UG_ALLOC(h->d_bcount, (size_t)h->cmp_blocks_cap * 4);
     UG_ALLOC(h->d_count, 16);
@@ -239,6 +246,8 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
 #undef UG_ALLOC
     if (rc == UG_OK && !(h->h_pinned = cache_host_alloc(64 + 8 * UG_S_MAX)))
         rc = fail(UG_ENOMEM, "pinned host allocation failed");
+    if (rc == UG_OK && hipEventCreateWithFlags(&h->ev_report, hipEventDisableTiming) != hipSuccess)
+        rc = fail(UG_EHIP, "event creation failed");
     if (rc == UG_OK) {
         hipLaunchKernelGGL(k_fill_tab, dim3(grid_for(tab_len, 256 * 8)), dim3(256), 0, h->stream, h->d_tab, tab_len);
         rc = hip_check(hipMemsetAsync(h->d_filled, 0, 8, h->stream), "clear filled");
@@ -331,6 +340,28 @@ static bool fixed_window_stop(const Pack* h) {
     return (double)cnt < c.stop_rate * (double)c.stop_window;
 }
 
+// The smallest accepted count of the next sub-batch (sub candidates) for which the run goes
+// on: the loop stops after it iff its uniques number fewer (0: it cannot stop there).  The
+// fixed window counts accepts with index >= end - window, all of the sub-batch's among them.
+static int64_t stop_limit(const Pack* h, int64_t sub) {
+    const ug_pack_config& c = h->cfg;
+    const int64_t at = h->attempts + sub;
+    if (c.stop_kind == UG_STOP_FIXED_WINDOW) {
+        if (at % c.stop_batch != 0 || at < c.stop_window)
+            return 0;
+        const int64_t lo = at - c.stop_window;
+        const int64_t prev = (int64_t)(h->acc_host.end() - std::lower_bound(h->acc_host.begin(), h->acc_host.end(), lo));
+        const double thr = c.stop_rate * (double)c.stop_window;
+        int64_t x = std::max<int64_t>(0, (int64_t)std::ceil(thr) - prev);
+        while (x > 0 && !((double)(prev + x - 1) < thr))
+            x--;
+        while ((double)(prev + x) < thr)
+            x++;
+        return x;
+    }
+    return 0;
+}
+
 extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_status* st) {
     if (!hp)
         return fail(UG_EINVAL, "ug_pack_run: NULL handle");
@@ -346,6 +377,28 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
                                    ? (int64_t)std::min(5e9, std::max(2e7, 30.0 / cfg.stop_rate))
                                    : 0;
     int32_t rc = UG_OK;
+    // The size of the sub-batch starting at `from`: h->sub, cut at the budget, the call's end
+    // and (fixed window) the next stop-rule boundary; 0 when none may start there.
+    auto size_at = [&](int64_t from) -> int64_t {
+        if (from >= budget || from >= call_end)
+            return 0;
+        int64_t sub = std::min(h->sub, sub_cap);
+        sub = std::min(sub, budget - from);
+        sub = std::min(sub, call_end - from);
+        if (cfg.stop_kind == UG_STOP_FIXED_WINDOW) {
+            const int64_t nb = (from / cfg.stop_batch + 1) * cfg.stop_batch;
+            sub = std::min(sub, nb - from);
+        }
+        return sub;
+    };
+    // Sub-batches are pipelined one deep: while the host reads sub-batch k's results, the
+    // device already filters sub-batch k+1 against the table as it stands after k's first
+    // resolve chunk (all of k's inserts when that chunk was the only one; otherwise the
+    // filter is redone).  A stop decided after k drops the filter run ahead -- it changed
+    // nothing but scratch buffers -- and the device already gave it no candidates when the
+    // stop rule or max_n fires after k.
+    SpecFilter ahead;
+    int64_t* cur_list = h->d_list;
     while (!h->done && rc == UG_OK) {
         if (cfg.stop_kind == UG_STOP_FIXED_WINDOW && fixed_window_stop(h)) {
             h->done = true;
@@ -357,24 +410,39 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         }
         if (h->attempts >= call_end)
             break;
-        int64_t sub = std::min(h->sub, sub_cap);
-        sub = std::min(sub, budget - h->attempts);
-        sub = std::min(sub, call_end - h->attempts);
-        if (cfg.stop_kind == UG_STOP_FIXED_WINDOW) {
-            const int64_t nb = (h->attempts / cfg.stop_batch + 1) * cfg.stop_batch;
-            sub = std::min(sub, nb - h->attempts);
-        }
         const uint64_t base = (uint64_t)h->attempts;
+        int64_t sub = 0;
+        if (ahead.launched && ahead.base == base) {
+            // survivors already filtered, their count still on the device
+            sub = ahead.count;
+            cur_list = ahead.out;
+        } else {
+            sub = size_at(h->attempts);
+            cur_list = h->d_list;
+            int64_t S0 = 0;
+            // survivors stay on the device: the resolve reads their count there, and it comes
+            // back with the resolve's results (one host round trip per sub-batch)
+            rc = filter_candidates(h, base, sub, nullptr, cur_list, &S0, false);
+            if (rc)
+                break;
+        }
+        ahead = SpecFilter();
+        ahead.base = base + (uint64_t)sub;
+        // Only in the steady state (full-size sub-batches, few survivors): earlier, sizes still
+        // grow from the measured rates and a sub-batch often needs several resolve chunks,
+        // which would void the run ahead.  The reference window's verdict depends on the
+        // sub-batch sizes, so that rule keeps the plain loop.
+        const bool steady = h->sub >= sub_cap && h->last_S <= UG_S_MAX / 4;
+        ahead.count = (cfg.stop_kind == UG_STOP_REF_WINDOW || !steady) ? 0 : size_at(h->attempts + sub);
+        ahead.out = cur_list == h->d_list ? h->d_listb : h->d_list;
+        ahead.dcount = h->d_spec_count;
+        ahead.stop_lim = stop_limit(h, sub);
+        ahead.n_room = h->max_n > 0 ? h->max_n - h->n : INT64_MAX;
         int64_t S = 0;
         const int64_t rechecked_before = h->rechecked;
-        // survivors stay on the device: the resolve reads their count there, and it comes
-        // back with the resolve's results (one host round trip per sub-batch)
-        rc = filter_candidates(h, base, sub, nullptr, h->d_list, &S, false);
-        if (rc)
-            break;
         int64_t acc = 0;
         const int64_t n_before = h->n;
-        rc = resolve_and_insert(h, h->d_list, 0, &acc, h->d_count + 1);
+        rc = resolve_and_insert(h, cur_list, 0, &acc, h->d_count + 1, &ahead);
         if (rc)
             break;
         S = h->last_S;

@@ -585,7 +585,7 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
     FastState* fs = (FastState*)h->d_basis16;
     const int64_t ntiles = (count + TILE - 1) / TILE;
     // a device-counted list is short (what an earlier stage left): fewer persistent blocks
-    const int cap = dcount ? 512 : 2048;
+    const int cap = (dcount && list) ? 512 : 2048;
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, cap));
     const FastArgs f = fast_args(h);
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
@@ -593,7 +593,7 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
     if (cb0 >= cb1)
         return fail(UG_EINVAL, "prefilter: empty column-block range [%d, %d)", cb0, cb1);
     // several N blocks of a short list run in parallel (one work item per tile and block)
-    const int par = (dcount && cb1 - cb0 > 1) ? 1 : 0;
+    const int par = (dcount && list && cb1 - cb0 > 1) ? 1 : 0;
     const int nblk = par ? std::min(2048, blocks * (cb1 - cb0)) : blocks;
     if (par)
         hipLaunchKernelGGL(k_flags_one, dim3(grid_for(count, 256 * 16)), dim3(256), 0, h->stream, flags, count, dcount);

@@ -131,6 +131,9 @@ struct Pack {
     int64_t* d_list2 = nullptr;
     int64_t* d_list3 = nullptr;
     int64_t* d_list4 = nullptr;
+    int64_t* d_listb = nullptr;     // survivors of a filter run ahead (alternates with d_list)
+    int* d_spec_count = nullptr;    // that filter's candidate count, set on the device
+    hipEvent_t ev_report = nullptr;  // after a resolve chunk's report (timing disabled)
     int recheck_slot = 0;  // d_count slot holding the last filter's FP64 recheck count
     int* d_bcount = nullptr;
     int cmp_blocks_cap = 0;
@@ -236,9 +239,26 @@ int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, i
                    int64_t* S, int* rs_count);
 int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_total);
 int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out,
-                          int64_t* S, bool sync = true);
+                          int64_t* S, bool sync = true, const int* dfresh = nullptr);
 int32_t ensure_paths(Pack* h, int64_t need);
-int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total, const int* dS = nullptr);
+// A filter run ahead: fresh candidates [base, base + count) into out, enqueued behind the
+// current sub-batch's first resolve chunk; launched stays true only if no later chunk of the
+// current sub-batch changed the table first (then its survivors in out and the counts in
+// d_count are the ones a filter run now would give).
+// The device sets the filter's count (*dcount) from the chunk's accepted count: 0 when the
+// stop rule fires after the current sub-batch (nacc < stop_lim) or max_n is reached
+// (nacc >= n_room), so a filter nobody will use costs a few empty launches.
+struct SpecFilter {
+    uint64_t base = 0;
+    int64_t count = 0;
+    int64_t* out = nullptr;
+    int* dcount = nullptr;
+    int64_t stop_lim = 0;
+    int64_t n_room = INT64_MAX;
+    bool launched = false;
+};
+int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total, const int* dS = nullptr,
+                           SpecFilter* spec = nullptr);
 // split-BF16 prefilter (ug_fast.hip)
 int32_t fast_build(ModelTables& t, const std::vector<double>& S, const std::vector<double>& C,
                    const std::vector<int>& order, double margin_req);

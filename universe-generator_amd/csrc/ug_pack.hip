@@ -1051,11 +1051,19 @@ __global__ __launch_bounds__(256) void k_conflicts_tiled(DevModel m, const doubl
 // filter's four counts when they were still on the device, [64 ..] the accepted candidate
 // indices (at most s).  The host reads the block after synchronising the stream.
 __global__ __launch_bounds__(256) void k_report(const int* nacc, const unsigned long long* filled,
-                                                const int64_t* acc_idx, int s, const int* counts, char* pin) {
+                                                const int64_t* acc_idx, int s, const int* counts, char* pin,
+                                                int* spec_count, int spec_n, int64_t stop_lim, int64_t n_room) {
     const int na = min(*nacc, s);
     if (threadIdx.x == 0) {
         *(int*)(pin + 8) = *nacc;
         *(unsigned long long*)(pin + 16) = *filled;
+        if (spec_count) {
+            // the count of the filter run ahead: none when the run stops after this sub-batch
+            const int64_t a = *nacc;
+            const int v = (a < stop_lim || a >= n_room) ? 0 : spec_n;
+            *spec_count = v;
+            *(int*)(pin + 48) = v;
+        }
     }
     if (counts && threadIdx.x < 4)
         ((int*)(pin + 32))[threadIdx.x] = counts[threadIdx.x];
@@ -1406,7 +1414,7 @@ int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, con
 // prefilters with k_fast_prefilter and rechecks only the candidates it could not reject,
 // chained through a device-side count.  Profiled runs time both stages.
 int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out,
-                          int64_t* S, bool sync) {
+                          int64_t* S, bool sync, const int* dfresh) {
     int32_t rc;
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
     unsigned long long* rctr = h->profile ? h->d_ctr + UG_CTR_WORDS / 2 : nullptr;
@@ -1415,22 +1423,22 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
     if (!split) {
         rc = timed_begin(h);
         if (!rc)
-            rc = launch_cand64(h, base, count, list, h->d_flags, nullptr, false, ctr, nullptr, -1);
+            rc = launch_cand64(h, base, count, list, h->d_flags, nullptr, false, ctr, dfresh, -1);
         if (!rc)
             rc = timed_end(h, UG_KERNEL_PREFILTER, count);
         if (!rc)
-            rc = compact(h, h->d_flags, count, base, list, out, S, nullptr, 1, sync);
+            rc = compact(h, h->d_flags, count, base, list, out, S, dfresh, 1, sync);
         return rc;
     }
     // stage 1: the first 32 ordered timesteps of every candidate -> survivors in d_list3
     // (count in d_count[0])
     rc = timed_begin(h);
     if (!rc)
-        rc = launch_fast(h, base, count, list, h->d_flags);
+        rc = launch_fast(h, base, count, list, h->d_flags, dfresh);
     if (!rc)
         rc = timed_end(h, UG_KERNEL_PREFILTER, count);
     if (!rc)
-        rc = compact(h, h->d_flags, count, base, list, h->d_list3, nullptr, nullptr, 0, false);
+        rc = compact(h, h->d_flags, count, base, list, h->d_list3, nullptr, dfresh, 0, false);
     const int64_t* rlist = h->d_list3;
     int rslot = 0;
     if (!rc && fast_blocks(h) >= 6) {
@@ -1540,7 +1548,8 @@ void launch_conflicts(Pack* h, int s, const int* dS = nullptr, int64_t expect = 
 // Resolve the survivors list[0..S) (already clear of the cell table) in index order and
 // insert the accepted ones.  Survivors beyond one chunk are retested against the updated
 // table before their turn.  Returns the number accepted via *acc_total.
-int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total, const int* dS) {
+int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total, const int* dS,
+                           SpecFilter* spec) {
     int32_t rc;
     *acc_total = 0;
     // dS: the survivor count is still on the device (filter without a host round trip); the
@@ -1582,11 +1591,25 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         // one round trip: accepted count, their candidate indices and the filled-cell count,
         // written into the pinned block by one small kernel
         char* pin = (char*)h->h_pinned;
+        const bool ahead = dev && spec && spec->count > 0;
         hipLaunchKernelGGL(k_report, dim3(1), dim3(256), 0, h->stream, h->d_nacc, h->d_filled, h->d_acc_idx + h->n, s,
-                           dev ? h->d_count : nullptr, pin);
+                           dev ? h->d_count : nullptr, pin, ahead ? spec->dcount : nullptr,
+                           ahead ? (int)spec->count : 0, ahead ? spec->stop_lim : 0, ahead ? spec->n_room : 0);
         rc = hip_check(hipGetLastError(), "k_report");
-        if (!rc)
+        if (!rc && ahead) {
+            // the next sub-batch's filter goes in behind the report, so the device works on
+            // it while the host reads this chunk's results (it only reads the table)
+            rc = hip_check(hipEventRecord(h->ev_report, h->stream), "report event");
+            if (!rc)
+                rc = filter_candidates(h, spec->base, spec->count, nullptr, spec->out, nullptr, false, spec->dcount);
+            if (!rc) {
+                rc = hip_check(hipEventSynchronize(h->ev_report), "report wait");
+                // usable only with the full count (a stop decided on the device leaves 0)
+                spec->launched = !rc && *(const int*)(pin + 48) == (int)spec->count;
+            }
+        } else if (!rc) {
             rc = hip_check(hipStreamSynchronize(h->stream), "resolve sync");
+        }
         if (rc)
             return rc;
         if (dev) {
@@ -1607,7 +1630,10 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         if (h->max_n > 0 && h->n >= h->max_n)
             return UG_OK;
         if (S > UG_S_MAX) {
-            // retest the rest against the updated table
+            // retest the rest against the updated table (a filter run ahead saw the table
+            // before these inserts: it is dropped)
+            if (spec)
+                spec->launched = false;
             int64_t S2 = 0;
             rc = filter_candidates(h, 0, S - s, list + s, h->d_list2, &S2);
             if (rc)
