@@ -422,7 +422,9 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
             int64_t S0 = 0;
             // survivors stay on the device: the resolve reads their count there, and it comes
             // back with the resolve's results (one host round trip per sub-batch)
-            rc = filter_candidates(h, base, sub, nullptr, cur_list, &S0, false);
+            // an empty table rejects nothing: the first sub-batch goes straight to the resolve
+            rc = h->n + h->nsub == 0 ? all_survive(h, base, sub, cur_list)
+                                     : filter_candidates(h, base, sub, nullptr, cur_list, &S0, false);
             if (rc)
                 break;
         }

@@ -241,6 +241,7 @@ int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_tot
 int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out,
                           int64_t* S, bool sync = true, const int* dfresh = nullptr);
 int32_t ensure_paths(Pack* h, int64_t need);
+int32_t all_survive(Pack* h, uint64_t base, int64_t count, int64_t* out);
 // A filter run ahead: fresh candidates [base, base + count) into out, enqueued behind the
 // current sub-batch's first resolve chunk; launched stays true only if no later chunk of the
 // current sub-batch changed the table first (then its survivors in out and the counts in

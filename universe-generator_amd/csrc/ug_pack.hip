@@ -1408,6 +1408,26 @@ int32_t compact(Pack* h, const uint8_t* flags, int64_t count, uint64_t base, con
     return rc;
 }
 
+// out[i] = base + i for i < count; counts[1] = count, counts[0] = counts[2] = 0 (no rechecks).
+__global__ void k_all_survive(uint64_t base, int64_t count, int64_t* out, int* counts) {
+    const int64_t i0 = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (int64_t i = i0; i < count; i += (int64_t)gridDim.x * 256)
+        out[i] = (int64_t)(base + (uint64_t)i);
+    if (i0 == 0) {
+        counts[0] = 0;
+        counts[1] = (int)count;
+        counts[2] = 0;
+    }
+}
+
+// The filter of a sub-batch against an empty table: nothing can touch a path, so every
+// candidate survives (survivor count left in d_count[1] as filter_candidates leaves it).
+int32_t all_survive(Pack* h, uint64_t base, int64_t count, int64_t* out) {
+    hipLaunchKernelGGL(k_all_survive, dim3(grid_for(count, 256 * 4)), dim3(256), 0, h->stream, base, count, out,
+                       h->d_count);
+    return hip_check(hipGetLastError(), "k_all_survive");
+}
+
 // Candidates [base, base+count) or list[0..count) -> survivors of the exact FP64 test
 // against the cell table, in index order, into out (*S of them; without sync the count
 // stays in d_count[1] and *S is not set).  The split-BF16 form
