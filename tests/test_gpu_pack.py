@@ -144,6 +144,41 @@ def test_max_n(gpu):
     assert st.n == 10 and list(idx) == list(ref_idx) and st.attempts == att
 
 
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_max_n_in_the_pipelined_steady_state(gpu, precision):
+    """max_n reached in a full-size sub-batch, while the next sub-batch's filter runs ahead:
+    the run stops at the max_n-th accept of the golden jam list, as serial RSA does."""
+    import ugpack
+
+    g = _gold("cfg2_d2_T64_s1_jam")
+    k = len(g["idx"]) - 5
+    prec = ugpack.PREC_FP64 if precision == "fp64" else ugpack.PREC_SPLIT_BF16
+    with ugpack.Packer(2, 64, seed=1, stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4, max_n=k,
+                       precision=prec) as p:
+        st = p.run()
+        idx, _ = p.accepted()
+    assert g["idx"][k - 1] >= 1 << 20  # the cut falls after the first full-size sub-batch
+    assert st.done and st.n == k and list(idx) == g["idx"][:k]
+    assert st.attempts == g["idx"][k - 1] + 1
+
+
+def test_recycled_buffers_and_cache_trim(gpu):
+    """Handles reuse the buffers of destroyed ones (other models in between, the cache freed
+    midway): every pack still equals its golden list."""
+    import ugpack
+
+    g1, g2 = _gold("cfg1_d2_T16_s1"), _gold("d3_T12_s1")
+    for rep in range(3):
+        with ugpack.Packer(2, 16, seed=1, max_attempts=20000) as p:
+            p.run()
+            assert list(p.accepted()[0]) == g1["idx"]
+        with ugpack.Packer(3, 12, seed=1, max_attempts=g2["candidates"]) as p:
+            p.run()
+            assert list(p.accepted()[0]) == g2["idx"]
+        if rep == 1:
+            ugpack.abi.cache_trim()
+
+
 def test_invalid_config_is_an_error_not_an_exit(gpu):
     import ugpack
 

@@ -1105,16 +1105,19 @@ __global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, cons
                 inrow = cur[wd];
         }
         const uint64_t deadm = __ballot(!valid || killed != 0ull);
-        // the serial decision runs on the scalar unit: lane k's row is read into scalar
-        // registers (v_readlane), so no step waits on a cross-lane shuffle through LDS
-        uint64_t accb = 0;
-        const int nin = min(64, S - blk * 64);
-        const uint32_t in_lo = (uint32_t)inrow, in_hi = (uint32_t)(inrow >> 32);
-        for (int k = 0; k < nin; k++) {
-            const uint64_t rowk = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)in_lo, k) |
-                                  ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)in_hi, k) << 32);
-            if (!((deadm >> k) & 1ull) && (rowk & accb) == 0)
-                accb |= 1ull << k;
+        // Index-order greedy inside the block, decided in waves of lanes: a lane is accepted
+        // once every earlier lane it conflicts with is decided and none of them accepted, and
+        // dropped once one of them is accepted.  The lowest undecided lane is always decidable,
+        // so the loop ends; it takes as many steps as the longest chain of conflicts (usually
+        // a few) instead of 64.
+        uint64_t accb = 0, undec = ~deadm;
+        const uint64_t earlier = inrow & ((1ull << lane) - 1ull);  // (bits >= lane are zero)
+        while (undec) {
+            const bool mine = (undec >> lane) & 1ull;
+            const bool drop = mine && (earlier & accb) != 0ull;
+            const bool take = mine && !drop && (earlier & undec) == 0ull;
+            accb |= __ballot(take);
+            undec &= ~(__ballot(take) | __ballot(drop));
         }
         if (lane == 0)
             accw[blk] = accb;
