@@ -200,9 +200,13 @@ int32_t model_tables(int d, int T, int nw, bool split_full, double margin_req,
                      std::shared_ptr<const ModelTables>* out);
 int32_t hip_check(hipError_t e, const char* what);
 int grid_for(int64_t items, int per_block);
+// FP64 candidate kernel: the exact test (full = false: flags) or the trajectories of a
+// resolve chunk (full = true: out, and with cells / mask also their packed cells and cleared
+// conflict rows).
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
                       double* out, bool full, unsigned long long* ctr = nullptr,
-                      const int* dcount = nullptr, int head = -1);
+                      const int* dcount = nullptr, int head = -1, int* cells = nullptr,
+                      uint64_t* mask = nullptr);
 // Profiled timing of one launch: timed_begin arms the next UG_LAUNCH on h's stream, which
 // then records an event pair directly before and after the dispatch (timing-only events
 // without the system-scope fence, never waited on inside a pack); timed_end books the pair

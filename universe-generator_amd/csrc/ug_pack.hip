@@ -363,7 +363,8 @@ template <int D, bool FULL, int PL>
 __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_t count,
                                                const int64_t* list, const int* dcount,
                                                uint8_t* flags, double* out, int head,
-                                               unsigned long long* ctr) {
+                                               unsigned long long* ctr, int* cells,
+                                               uint64_t* mask) {
     if (dcount)
         count = min(count, (int64_t)*dcount);
     extern __shared__ __align__(16) int dyn_lds[];
@@ -380,6 +381,8 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
     constexpr int PER_ROUND = 64 / R;
     unsigned long long nvis = 0, npts = 0;
     for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        if (FULL && mask && lane < UG_MASK_WORDS)
+            mask[(size_t)i * UG_MASK_WORDS + lane] = 0ull;  // the conflict row, cleared
         const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
         WaveCoef<D, PL> w;
         gen_wave64<D, PL>(m, idx, lane, w, sb, used);
@@ -430,9 +433,16 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
                     double X[D];
                     eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
                     if (FULL) {
+                        // trajectories of a resolve chunk; with cells, also their packed
+                        // cells (the conflict test's first filter)
+                        int v = 0;
 #pragma unroll
-                        for (int a = 0; a < D; a++)
+                        for (int a = 0; a < D; a++) {
                             out[((size_t)i * T + t) * D + a] = X[a];
+                            v |= cell_of(X[a], m.halfT, T) << (10 * a);
+                        }
+                        if (cells)
+                            cells[(size_t)i * T + t] = v;
                     } else {
                         tsl[lane] = t;
 #pragma unroll
@@ -884,22 +894,8 @@ __device__ __forceinline__ bool pair_hit(const DevModel& m, const double* x, con
     return h && contact<D>(m, x, y);
 }
 
-// Packed cells (10 bits per axis) of the first S survivors at every timestep.
-template <int D>
-__global__ __launch_bounds__(256) void k_cells(DevModel m, const double* pos, int S, const int* dS, int* cells) {
-    if (dS)
-        S = min(S, *dS);
-    const int64_t n = (int64_t)S * m.T;
-    for (int64_t it = (int64_t)blockIdx.x * 256 + threadIdx.x; it < n; it += (int64_t)gridDim.x * 256) {
-        int v = 0;
-#pragma unroll
-        for (int a = 0; a < D; a++)
-            v |= cell_of(pos[(size_t)it * D + a], m.halfT, m.T) << (10 * a);
-        cells[it] = v;
-    }
-}
-
-// Neighbouring cells on every axis (torus), from packed cells.
+// Neighbouring cells on every axis (torus), from packed cells (10 bits per axis, written
+// with a resolve chunk's trajectories).
 template <int D>
 __device__ __forceinline__ bool cells_adjacent(int u, int v, int T) {
     bool adj = true;
@@ -1216,11 +1212,6 @@ __global__ void k_fill_i32(int* p, int64_t n, int v) {
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
         p[i] = v;
 }
-__global__ void k_fill_u64(uint64_t* p, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        p[i] = 0ull;
-}
-
 // =====================================================================================
 // host driver
 namespace ug {
@@ -1231,7 +1222,8 @@ int grid_for(int64_t items, int per_block) {
 }
 
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
-                      double* out, bool full, unsigned long long* ctr, const int* dcount, int head) {
+                      double* out, bool full, unsigned long long* ctr, const int* dcount, int head,
+                      int* cells, uint64_t* mask) {
     if (count <= 0)
         return UG_OK;
     if (head < 0)
@@ -1242,7 +1234,7 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
     const int pl = slot_pl(h->npairs);
 #define UG_CAND64(DD, FF, PP)                                                                         \
     UG_LAUNCH(h, (k_cand64<DD, FF, PP>), dim3(blocks), dim3(64), lds, h->dm, base, count, list, dcount, \
-              flags, out, head, ctr)
+              flags, out, head, ctr, cells, mask)
 #define UG_CAND64_PL(DD, FF)       \
     do {                           \
         if (pl == 1)               \
@@ -1536,6 +1528,7 @@ int32_t ensure_paths(Pack* h, int64_t need) {
 
 // Conflict bits among the chunk's s survivors (trajectories in d_spos, mask cleared); with
 // dS the count is min(s, *dS), known only on the device (grid-stride launches).
+// The chunk's packed cells (d_scell) and cleared mask rows come from its trajectory launch.
 // expect: the survivors the host expects (s itself when known); large chunks take the
 // tiled kernel, small ones the pair-per-thread kernel -- the bits are the same either way.
 void launch_conflicts(Pack* h, int s, const int* dS = nullptr, int64_t expect = -1) {
@@ -1546,11 +1539,9 @@ void launch_conflicts(Pack* h, int s, const int* dS = nullptr, int64_t expect = 
         expect = s;
     const bool tiled = expect >= 512;
     const int64_t items = npairs * ((h->T + CONF_T - 1) / CONF_T);
-    const int cb = grid_for((int64_t)s * h->T, 256);
     const int kb = tiled ? conf_tiles(s) * ((h->T + CONF_TB - 1) / CONF_TB)
                          : (dS ? std::min(grid_for(items, 256), 2048) : grid_for(items, 256));
     if (h->d == 2) {
-        hipLaunchKernelGGL(k_cells<2>, dim3(cb), dim3(256), 0, h->stream, h->dm, h->d_spos, s, dS, h->d_scell);
         if (tiled)
             hipLaunchKernelGGL(k_conflicts_tiled<2>, dim3(kb), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell,
                                s, dS, h->d_mask);
@@ -1558,7 +1549,6 @@ void launch_conflicts(Pack* h, int s, const int* dS = nullptr, int64_t expect = 
             hipLaunchKernelGGL(k_conflicts<2>, dim3(kb), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell, s,
                                dS, h->d_mask);
     } else {
-        hipLaunchKernelGGL(k_cells<3>, dim3(cb), dim3(256), 0, h->stream, h->dm, h->d_spos, s, dS, h->d_scell);
         if (tiled)
             hipLaunchKernelGGL(k_conflicts_tiled<3>, dim3(kb), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell,
                                s, dS, h->d_mask);
@@ -1584,11 +1574,10 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         rc = ensure_paths(h, h->n + s);
         if (rc)
             return rc;
-        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, cs, 0);
+        // trajectories, packed cells and cleared conflict rows of the chunk in one launch
+        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, cs, 0, h->d_scell, h->d_mask);
         if (rc)
             return rc;
-        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((int64_t)s * UG_MASK_WORDS, 256)), dim3(256), 0,
-                           h->stream, h->d_mask, (int64_t)s * UG_MASK_WORDS);
         // with the count still on the device, the last filter's survivors are the estimate
         launch_conflicts(h, s, cs, dev ? h->last_S : s);
         hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, cs, h->d_acc_list,
@@ -1732,11 +1721,9 @@ int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_tot
         rc = ensure_paths(h, n0 + s);
         if (rc)
             return rc;
-        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, nullptr, 0);
+        rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, nullptr, 0, h->d_scell, h->d_mask);
         if (rc)
             return rc;
-        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for((int64_t)s * UG_MASK_WORDS, 256)), dim3(256), 0, h->stream,
-                           h->d_mask, (int64_t)s * UG_MASK_WORDS);
         launch_conflicts(h, s);
         hipLaunchKernelGGL(k_greedy_sub, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, gl, h->d_gsurv,
                            h->d_acc_list, h->d_nacc);
