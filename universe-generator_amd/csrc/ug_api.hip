@@ -287,6 +287,7 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     if (rc == UG_OK)
         rc = hip_check(hipStreamSynchronize(h->stream), "create sync");
     if (rc != UG_OK) {
+        (void)hipStreamSynchronize(h->stream);  // nothing may still write a recycled buffer
         release(h);
         delete h;
         return rc;
@@ -420,9 +421,9 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
             sub = size_at(h->attempts);
             cur_list = h->d_list;
             int64_t S0 = 0;
-            // survivors stay on the device: the resolve reads their count there, and it comes
-            // back with the resolve's results (one host round trip per sub-batch)
-            // an empty table rejects nothing: the first sub-batch goes straight to the resolve
+            // Survivors stay on the device: the resolve reads their count there, and it comes
+            // back with the resolve's results (one host round trip per sub-batch).  An empty
+            // table rejects nothing, so the first sub-batch goes straight to the resolve.
             rc = h->n + h->nsub == 0 ? all_survive(h, base, sub, cur_list)
                                      : filter_candidates(h, base, sub, nullptr, cur_list, &S0, false);
             if (rc)
