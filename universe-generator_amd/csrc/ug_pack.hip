@@ -279,6 +279,76 @@ __device__ __forceinline__ void eval_lane(const DevModel& m, const double2* cs, 
     }
 }
 
+// eval_lane for full spectra: slot j carries frequency j + 2 on every axis (2 on padding
+// slots), so the 32 basis values of a slot group are loaded once for all axes; each axis's
+// fma chains and tree see exactly the operations, in the same order, as in eval_lane.
+template <int D, int PL>
+__device__ __forceinline__ void eval_lane_full(const DevModel& m, const double2* cs, const int* bb,
+                                               const double* a1, int t, double* X) {
+    double L0[D], L1[D], L2[D], total[D];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        L0[a] = L1[a] = L2[a] = total[a] = 0.0;
+#pragma unroll 1
+    for (int g = 0; g < 8; g++) {
+        const int rg = bitrev3(g);
+        double e[D][8];
+#pragma unroll
+        for (int a = 0; a < D; a++)
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                e[a][k] = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < PL; ii++) {
+            double sv[8][2], cv[8][2];
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+#pragma unroll
+                for (int kk = 0; kk < 2; kk++) {
+                    const int j = 2 * (8 * bitrev3(k) + rg + 64 * ii) + kk;
+                    const int b = bb[j];
+                    sv[k][kk] = m.ST[(size_t)b * m.T + t];
+                    cv[k][kk] = m.CT[(size_t)b * m.T + t];
+                }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                const double2* ca = cs + a * (128 * PL);
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+#pragma unroll
+                    for (int kk = 0; kk < 2; kk++) {
+                        const double2 c = ca[2 * (8 * bitrev3(k) + rg + 64 * ii) + kk];
+                        e[a][k] = fma(c.x, sv[k][kk], e[a][k]);
+                        e[a][k] = fma(c.y, cv[k][kk], e[a][k]);
+                    }
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            double G = ((e[a][0] + e[a][1]) + (e[a][2] + e[a][3])) + ((e[a][4] + e[a][5]) + (e[a][6] + e[a][7]));
+            if ((g & 1) == 0) {
+                L0[a] = G;
+                continue;
+            }
+            G = L0[a] + G;
+            if ((g & 2) == 0) {
+                L1[a] = G;
+                continue;
+            }
+            G = L1[a] + G;
+            if ((g & 4) == 0) {
+                L2[a] = G;
+                continue;
+            }
+            total[a] = L2[a] + G;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        X[a] = torus_wrap(a1[a] + total[a]);
+}
+
 // The slot arrays of the lane-per-timestep evaluation use a padded stride of 128 PL slots per
 // axis (PL = slot pairs per lane: 1 up to 64 pairs, 2 up to 128, else 4).
 __host__ __device__ inline int slot_pl(int npairs) { return npairs <= 64 ? 1 : (npairs <= 128 ? 2 : 4); }
@@ -359,7 +429,10 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
 // head = T; lists of likely survivors use a short head.
 // FULL = false: flags[i] = 1 iff candidate i survives the cell table.
 // FULL = true: out[i][T][D] = canonical trajectory (no lookup).
-template <int D, bool FULL, int PL>
+// SB: the lane-per-timestep evaluation shares each slot group's basis loads across axes
+// (full spectra, one slot pair per lane).  A separate instantiation, because the shared
+// evaluation's extra registers would lower the occupancy of launches that do not use it.
+template <int D, bool FULL, int PL, bool SB = false>
 __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_t count,
                                                const int64_t* list, const int* dcount,
                                                uint8_t* flags, double* out, int head,
@@ -431,7 +504,10 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
                 if (oi < T) {
                     const int t = FULL ? oi : m.order[oi];
                     double X[D];
-                    eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
+                    if constexpr (SB)
+                        eval_lane_full<D, PL>(m, cs, sb, w.a1, t, X);
+                    else
+                        eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
                     if (FULL) {
                         // trajectories of a resolve chunk; with cells, also their packed
                         // cells (the conflict test's first filter)
@@ -542,7 +618,10 @@ __global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t fir
             if (oi < T) {
                 const int t = m.order[oi];
                 double X[D];
-                eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
+                if (PL == 1 && m.full)  // (more slot pairs per lane would spill registers)
+                    eval_lane_full<D, PL>(m, cs, sb, w.a1, t, X);
+                else
+                    eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
                 tsl[lane] = t;
 #pragma unroll
                 for (int a = 0; a < D; a++)
@@ -679,7 +758,10 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
             if (oi < T) {
                 const int t = m.order[oi];
                 double X[D];
-                eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
+                if (PL == 1 && m.full)  // (more slot pairs per lane would spill registers)
+                    eval_lane_full<D, PL>(m, cs, sb, w.a1, t, X);
+                else
+                    eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
                 tsl[lane] = t;
 #pragma unroll
                 for (int a = 0; a < D; a++)
@@ -1232,17 +1314,22 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
     const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
     // slot pairs per lane: 1 up to 64 pairs (T <= 129), 2 up to 128, else 4
     const int pl = slot_pl(h->npairs);
-#define UG_CAND64(DD, FF, PP)                                                                         \
-    UG_LAUNCH(h, (k_cand64<DD, FF, PP>), dim3(blocks), dim3(64), lds, h->dm, base, count, list, dcount, \
+#define UG_CAND64(DD, FF, PP, SS)                                                                         \
+    UG_LAUNCH(h, (k_cand64<DD, FF, PP, SS>), dim3(blocks), dim3(64), lds, h->dm, base, count, list, dcount, \
               flags, out, head, ctr, cells, mask)
-#define UG_CAND64_PL(DD, FF)       \
-    do {                           \
-        if (pl == 1)               \
-            UG_CAND64(DD, FF, 1);  \
-        else if (pl == 2)          \
-            UG_CAND64(DD, FF, 2);  \
-        else                       \
-            UG_CAND64(DD, FF, 4);  \
+    // shared basis loads where the lane-per-timestep evaluation does the work: rechecks and
+    // retests of a list, and trajectories (a fresh first pass scans its head across the wave)
+    const bool sb = pl == 1 && h->full && (full || list);
+#define UG_CAND64_PL(DD, FF)              \
+    do {                                  \
+        if (pl == 1 && sb)                \
+            UG_CAND64(DD, FF, 1, true);   \
+        else if (pl == 1)                 \
+            UG_CAND64(DD, FF, 1, false);  \
+        else if (pl == 2)                 \
+            UG_CAND64(DD, FF, 2, false);  \
+        else                              \
+            UG_CAND64(DD, FF, 4, false);  \
     } while (0)
     if (h->d == 2) {
         if (full)
