@@ -1319,13 +1319,15 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
               flags, out, head, ctr, cells, mask)
     // shared basis loads where the lane-per-timestep evaluation does the work: rechecks and
     // retests of a list, and trajectories (a fresh first pass scans its head across the wave)
-    const bool sb = pl == 1 && h->full && (full || list);
+    const bool sb = pl <= 2 && h->full && (full || list);
 #define UG_CAND64_PL(DD, FF)              \
     do {                                  \
         if (pl == 1 && sb)                \
             UG_CAND64(DD, FF, 1, true);   \
         else if (pl == 1)                 \
             UG_CAND64(DD, FF, 1, false);  \
+        else if (pl == 2 && sb)           \
+            UG_CAND64(DD, FF, 2, true);   \
         else if (pl == 2)                 \
             UG_CAND64(DD, FF, 2, false);  \
         else                              \
