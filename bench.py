@@ -251,8 +251,8 @@ def main() -> int:
                     help="first copy 256 MiB device-to-device (counter calibration for rocprof --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the packer runs one universe per "
-              "GPU; see DESIGN.md for how it shards)", file=sys.stderr)
+        print("bench.py: only one GPU is measured here (a universe packs on one GPU; "
+              "ensembles shard by universe)", file=sys.stderr)
         return 2
 
     import torch

@@ -6,9 +6,10 @@
  * C99, the serial random-sequential-adsorption admission of the reference
  * (cuda/braid_cuda.cu:703-722 host_collides, :831-879 evalPath/insertPath,
  * cuda/braid_knot.cu:433-462 index-ordered admission) on the Philox candidate
- * stream whose layout is fixed in DESIGN.md ("Candidate stream and canonical
- * arithmetic").  Every function below is written independently of the HIP
- * implementation; the two must agree bit for bit.
+ * stream: Philox4x32-10 keyed by the seed, one counter per (candidate, stage,
+ * axis, slot pair), evaluated with explicit fma in a fixed summation order.
+ * Every function below is written independently of the HIP implementation; the
+ * two must agree bit for bit.
  */
 #ifndef UG_ORACLE_H
 #define UG_ORACLE_H

@@ -65,7 +65,7 @@ def command(dim: int, T: int, *, attempts: float, seed: int, accept_rate: float,
             terms: int = 2, subpaths: bool = False, sub_attempts: float = 0.0,
             euclid: bool = False, curve_path: str = "curve.csv",
             binary: str | None = None) -> list[str]:
-    """The argv braidlab.engine.build_command would produce, aimed at this build's binary."""
+    """The argv braidlab.engine.build_command would produce, aimed at this package's binary."""
     exe = binary or os.path.join(BIN_DIR, "braid_ug" if dim == 2 else "braid_ug3d")
     cmd = [exe, "-t", str(T), "--attempts", repr(attempts), "--uniform", "--torus", "--phase",
            "--seed", str(seed), "--until-accept-rate", repr(accept_rate)]
