@@ -70,6 +70,32 @@ def measured_traffic(workload: str, split: bool):
     return None
 
 
+def measured_valu_busy(workload: str, split: bool):
+    """VALU-busy percentage of the prefilter from the committed rocprofv3 PMC summary of the
+    same workload (None when no summary exists)."""
+    f = os.path.join(ROOT, "profiles", f"{workload}_prefilter_valu.md")
+    if not split or not os.path.exists(f):
+        return None
+    for line in open(f):
+        if line.startswith("valu_busy_pct:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def mfma_roofline(stats: dict, dim: int, T: int) -> dict:
+    """Algorithmic flops of the prefilter's evaluation per launch over its average duration:
+    2*d*K*32 per candidate tested (K = 2(F-1) coefficient rows, the 32 timesteps of the first
+    column block).  The split's three products, the padding column of each frequency pair and
+    the later column blocks (T > 160) are not counted."""
+    launches = max(stats["launches"], 1)
+    flops = stats["items"] * 2 * dim * 2 * (T - 1) * min(T, 32)
+    avg_s = stats["ms"] / launches * 1e-3
+    achieved = (flops / launches) / avg_s / 1e12 if avg_s > 0 else 0.0
+    return {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_BF16_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 6),
+            "flops_per_launch": round(flops / launches, 1)}
+
+
 def sum_stats(stats) -> dict:
     out = {"launches": 0, "items": 0, "ms": 0.0, "visits": 0, "points": 0}
     for st in stats:
@@ -165,8 +191,15 @@ def ensemble_configs(ugpack, n: int, cands: float, precision: int):
 
 def run_ensemble(args, ugpack, torch, forms, prec_main) -> int:
     """One step = the whole ensemble; value = all universes' candidates / wall time."""
+    from ugpack import tune
+
     w = WORKLOADS["cfg4"]
     n, cands, threads = w["universes"], w["max_attempts"], args.threads
+    sweep = None
+    if threads <= 0:
+        # the in-flight universe count is tuned on this GPU: a 128-universe sample of the same
+        # mix (seeds 1..128) at 2*10^6 candidates each, outside the timed region
+        threads, sweep = tune.ensemble_threads(ensemble_configs(ugpack, 128, 2e6, prec_main))
     res = {}
     for prec in forms:
         cfgs = ensemble_configs(ugpack, n, cands, prec)
@@ -197,20 +230,85 @@ def run_ensemble(args, ugpack, torch, forms, prec_main) -> int:
         "data": "synthetic (Philox candidate streams, seeds 1..512)",
         "config": {"workload": w["text"], "universes": n, "candidates_per_universe": cands,
                    "threads": threads,
+                   "threads_sweep_s": {str(k): round(v, 4) for k, v in sweep.items()} if sweep else None,
                    "T_counts": {str(T): sum(1 for c in cfgs if c.T == T) for T in (16, 32, 64, 128)},
                    "dim3_count": sum(1 for c in cfgs if c.dim == 3),
                    "precision": "split-bf16" if split else "fp64"},
         "accepted_total": sum(main_r["n"]),
         "roofline": dict(roofline_of(stats, 3, 128, split), class_profiled="T=128 3+1 universe, 10^7 candidates"),
     }
+    if split:
+        line["roofline"]["mfma"] = mfma_roofline(stats, 3, 128)
     if ugpack.PREC_FP64 in res and split:
         f = res[ugpack.PREC_FP64]
         line["fp64_form"] = {"value": round(f["value"], 1), "ms_per_step": round(1e3 * f["wall"], 3),
                              "same_accepts_as_split": f["n"] == main_r["n"]}
         line["speedup_split_vs_fp64"] = round(main_r["value"] / f["value"], 3)
-    line["cpu_baseline"] = None
+    line["cpu_baseline"] = None if args.no_cpu else cpu_sample(ugpack, torch, "cfg4")
     print(json.dumps(line))
     return 0
+
+
+# CPU samples of the larger workloads (about 10-30 s of oracle work on 16 host cores): the
+# first candidates of seed 1, the early, unjammed part of a pack, where every candidate is
+# tested at many timesteps and most are admitted -- the CPU's slowest regime.
+CPU_PREFIX = {"cfg3": 1 << 20, "cfg5": 1 << 15}
+CPU_ENSEMBLE = (32, 1 << 16)  # the ensemble's first 32 universes, 2^16 candidates each
+
+
+def cpu_sample(ugpack, torch, workload: str, threads: int = 0) -> dict:
+    """The CPU oracle on a bounded sample of configs 3-5, with the GPU timed on the same
+    sample and the accepted lists compared."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+
+    nproc = os.cpu_count() or 1
+    threads = threads or min(nproc, 16)
+    split = ugpack.PREC_SPLIT_BF16
+    if workload == "cfg4":
+        nu, M = CPU_ENSEMBLE
+        cfgs = ensemble_configs(ugpack, nu, float(M), split)
+
+        def one(c):
+            idx, _, att = pyoracle.Model(c.dim, c.T, seed=c.seed).pack(M)
+            return idx, att
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:  # one universe per core (ctypes drops the GIL)
+            ref = list(ex.map(one, cfgs))
+        dt = time.perf_counter() - t0
+        att = sum(a for _, a in ref)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, packers = ugpack.ensemble(cfgs, threads=16, keep=True)
+        torch.cuda.synchronize()
+        gdt = time.perf_counter() - t0
+        same = all(list(p.accepted()[0]) == list(r[0]) for p, r in zip(packers, ref))
+        for p in packers:
+            p.close()
+        sample = f"the ensemble's first {nu} universes, {M} candidates each, one universe per core"
+    else:
+        w = WORKLOADS[workload]
+        M = CPU_PREFIX[workload]
+        t0 = time.perf_counter()
+        idx, att = pyoracle.Model(w["dim"], w["T"], seed=1).pack_mt(M, threads)
+        dt = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        with ugpack.Packer(w["dim"], w["T"], seed=1, precision=split, max_attempts=M) as p:
+            p.run()
+            torch.cuda.synchronize()
+            gdt = time.perf_counter() - t0
+            same = list(p.accepted()[0]) == list(idx)
+        sample = (f"the first {M} candidates of seed 1 (N={len(idx)}), parallel prefilter per "
+                  "block and serial in-order admission")
+    return {"value": round(att / dt, 1), "unit": "candidates/s", "cores": threads, "kind": "port",
+            "sample": sample, "seconds": round(dt, 3),
+            "gpu_same_sample": {"value": round(att / gdt, 1), "seconds": round(gdt, 4),
+                                "same_accepts": same},
+            "nproc": nproc, "cpu_model": _cpu_model()}
 
 
 def cpu_baseline(threads: int = 0) -> dict:
@@ -247,7 +345,8 @@ def main() -> int:
     ap.add_argument("--precision", default="both", choices=["split", "fp64", "both"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
-    ap.add_argument("--threads", type=int, default=16, help="ensemble workers (cfg4)")
+    ap.add_argument("--threads", type=int, default=0,
+                    help="ensemble workers (cfg4); 0 = tuned on this GPU")
     ap.add_argument("--calib", action="store_true",
                     help="first copy 256 MiB device-to-device (counter calibration for rocprof --pmc)")
     args = ap.parse_args()
@@ -356,11 +455,19 @@ def main() -> int:
                              "same_accepts_as_split": f["accepted"] == main_r["accepted"]}
         line["speedup_split_vs_fp64"] = round(main_r["value"] / f["value"], 3)
     if split:
+        line["roofline"]["mfma"] = mfma_roofline(main_r["stats"], WORKLOADS[args.workload]["dim"],
+                                                 WORKLOADS[args.workload]["T"])
+        line["roofline"]["valu_busy_pct"] = measured_valu_busy(args.workload, split)
         line["recheck"] = main_r["rstats"]
         line["split_error"] = split_error(ugpack, torch, args.workload)
     if frame_stats:
         line["frames"] = frame_stats
-    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline()
+    if args.no_cpu:
+        line["cpu_baseline"] = None
+    elif args.workload == "cfg2":
+        line["cpu_baseline"] = cpu_baseline()
+    else:
+        line["cpu_baseline"] = cpu_sample(ugpack, torch, args.workload)
     print(json.dumps(line))
     return 0
 

@@ -57,3 +57,17 @@ def test_ensemble_reports_a_failing_universe(gpu):
     cfgs.append(ugpack.pack_config(2, 16, terms=40, seed=1, max_attempts=100))  # terms > pool
     with pytest.raises(ugpack.UgError, match="universe 3"):
         ugpack.ensemble(cfgs, threads=2)
+
+
+def test_tuned_worker_count_is_one_of_the_choices(gpu):
+    import ugpack
+    from ugpack import tune
+
+    cfgs = _configs(ugpack, ugpack.PREC_SPLIT_BF16)
+    best, times = tune.ensemble_threads(cfgs, choices=(1, 4, 12))
+    assert best in (1, 4, 12) and sorted(times) == [1, 4, 12]
+    assert times[best] == min(times.values()) and all(t > 0 for t in times.values())
+    # the tuned count packs the same universes as any other
+    a = ugpack.ensemble(cfgs, threads=best)
+    b = ugpack.ensemble(cfgs, threads=1)
+    assert [(s.n, s.attempts) for s in a] == [(s.n, s.attempts) for s in b]
