@@ -175,12 +175,16 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // D layout of 32x32 MFMA: lane holds column n = lane & 31, rows 8(v/4) + 4h + (v%4)
+        // per-lane offsets kept opaque to the compiler here, so that the 16 stores use one
+        // base register with immediate offsets instead of 16 hoisted addresses
+        int poff = (a * TILE + 4 * h) * NCOL + mrow, roff = 4 * h;
+        asm volatile("" : "+v"(poff), "+v"(roff));
 #pragma unroll
         for (int v = 0; v < 16; v++) {
-            const int row = 8 * (v >> 2) + 4 * h + (v & 3);
-            float X = fmaf(acc[a][v], sinvw[row], sa1[row]);
+            const int ro = 8 * (v >> 2) + (v & 3);
+            float X = fmaf(acc[a][v], sinvw[roff + ro], sa1[roff + ro]);
             X = X - 2.0f * floorf((X + 1.0f) * 0.5f);
-            pos[(a * TILE + row) * NCOL + mrow] = X;
+            pos[poff + ro * NCOL] = X;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -193,8 +197,11 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 // SUB (subpath phase): flags[i] = 0 only when candidate i certainly touches two different
 // groups among the paths stored before row nlimit (the round-start packing) -- such a
 // candidate fails both the round-start test and the adoption test.
+// Occupancy: 2+1 runs 4 waves per SIMD (at most 128 VGPRs, no spills; 4 blocks of 35 KB LDS
+// fit a CU), 2-4% faster per launch than 3 on one MI355X; 3+1 stays at 3, which is also what
+// its 52 KB of LDS per block allows.
 template <int D, bool SUB>
-__global__ __launch_bounds__(256, 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
+__global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
                                                         const int64_t* list, const int* dcount,
                                                         int cb0, int cb1, int par, uint8_t* flags,
                                                         unsigned long long* ctr, const int* acc_gid,
@@ -584,8 +591,12 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
         return fail(UG_ESTATE, "split-BF16 prefilter not set up for this model");
     FastState* fs = (FastState*)h->d_basis16;
     const int64_t ntiles = (count + TILE - 1) / TILE;
-    // a device-counted list is short (what an earlier stage left): fewer persistent blocks
-    const int cap = (dcount && list) ? 512 : 2048;
+    // a device-counted list is short (what an earlier stage left): fewer persistent blocks.
+    // Otherwise one tile per wave up to 2^20 candidates: tiles finish at uneven times (the
+    // lookup stops at the first certain contact), and finer work balances the tail better
+    // (one MI355X, 2^20-candidate launches: 2+1 T=64 142 -> 138 us, 3+1 T=128 436 -> 413 us
+    // against a 2048-block cap; a 1024-block cap was slower still)
+    const int cap = (dcount && list) ? 512 : 8192;
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, cap));
     const FastArgs f = fast_args(h);
     unsigned long long* ctr = h->profile ? h->d_ctr : nullptr;
