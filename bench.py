@@ -47,13 +47,15 @@ WORKLOADS = {
 }
 
 
-def pack_once(ugpack, seed: int, precision: int, profile: bool = False, workload: str = "cfg2"):
+def pack_once(ugpack, seed: int, precision: int, profile: int = 0, workload: str = "cfg2"):
+    """One pack of the workload; `profile` = the kernels timed with HIP events (PROFILE_*
+    bits; counters run whenever it is nonzero)."""
     w = WORKLOADS[workload]
     p = ugpack.Packer(w["dim"], w["T"], seed=seed, precision=precision,
                       stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4,
                       stop_batch=1 << 20, stop_window=1 << 22, max_attempts=w["max_attempts"])
     if profile:
-        p.profile(True)
+        p.profile(True, kernels=profile)
     st = p.run()
     return p, st
 
@@ -383,22 +385,25 @@ def main() -> int:
     results = {}
     frame_stats = None
     for prec in forms:
+        timed_kernels = ugpack.abi.PROFILE_PREFILTER
         for w in range(args.warmup):  # profiled (warms the event pool)
-            p, _ = pack_once(ugpack, 1001 + w, prec, profile=True, workload=args.workload)
+            p, _ = pack_once(ugpack, 1001 + w, prec, profile=timed_kernels, workload=args.workload)
             p.close()
         torch.cuda.synchronize()
         cands, accepted, per_pack, parity = 0, [], [], []
-        # Every timed pack is profiled: each candidate-test launch is bracketed by its own
-        # timing-only events and the kernels count visits and points in device counters
-        # (about 2% of a 3 ms pack).  A step is one pack -- handle creation, the run to jam and
-        # a device synchronisation -- bracketed by synchronisations; between steps the stats
-        # are read, the list is checked against the golden fixture and the handle is released
-        # (its buffers serve the next pack), outside the step's time.
+        # Every timed pack is profiled: each prefilter launch is bracketed by its own
+        # timing-only events and the kernels count visits and points in device counters (the
+        # event pairs cost about 4% of a config-2 pack; timing the FP64 recheck as well would
+        # double that, so its time comes from one separate pack below).  A step is one pack --
+        # handle creation, the run to jam and a device synchronisation -- bracketed by
+        # synchronisations; between steps the stats are read, the list is checked against the
+        # golden fixture and the handle is released (its buffers serve the next pack),
+        # outside the step's time.
         pstats, rstats_l = [], []
         for k in range(args.steps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            p, st = pack_once(ugpack, k + 1, prec, profile=True, workload=args.workload)
+            p, st = pack_once(ugpack, k + 1, prec, profile=timed_kernels, workload=args.workload)
             torch.cuda.synchronize()
             per_pack.append(time.perf_counter() - t0)
             cands += st.attempts
@@ -411,6 +416,12 @@ def main() -> int:
             p.close()
         wall = sum(per_pack)
         stats, rstats = sum_stats(pstats), sum_stats(rstats_l)
+        if prec == ugpack.PREC_SPLIT_BF16:  # the FP64 recheck timed in one untimed pack (seed 1)
+            p, _ = pack_once(ugpack, 1, prec, profile=ugpack.abi.PROFILE_ALL, workload=args.workload)
+            one = p.kernel_stats(ugpack.abi.KERNEL_RECHECK)
+            p.close()
+            rstats = {"items": rstats["items"], "visits": rstats["visits"], "points": rstats["points"],
+                      "timed_pack_seed1": one}
         wl = WORKLOADS[args.workload]
         if wl.get("frames") and prec == prec_main:
             p, _ = pack_once(ugpack, 1, prec, workload=args.workload)

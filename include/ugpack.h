@@ -154,10 +154,12 @@ int32_t ug_pack_curve(const ug_pack* h, int64_t* rows4, int64_t cap, int64_t* n)
 int32_t ug_eval_candidates(const ug_pack* h, const int64_t* dev_idx, int64_t count,
                            int32_t precision, double* dev_out);
 
-/* Per-kernel measurement of the candidate-test kernels.  When enabled, every launch is
- * bracketed by a pair of timing-only HIP events on the handle's stream (read back by
- * ug_pack_kernel_stats, so the pack waits on nothing extra) and the kernels count the timesteps they visit and the grid points
- * they compare (exactly, in device counters), so algorithmic bytes per launch can be derived.
+/* Per-kernel measurement of the candidate-test kernels.  When enabled, every launch of the
+ * selected kernels is bracketed by a pair of timing-only HIP events on the handle's stream
+ * (read back by ug_pack_kernel_stats, so the pack waits on nothing extra; each event still
+ * holds the next launch back by a few microseconds) and the kernels count the timesteps they
+ * visit and the grid points they compare (exactly, in device counters), so algorithmic
+ * bytes per launch can be derived.
  * No counterpart in the reference (which prints only attempts/s, src/main.rs:150-159). */
 typedef struct ug_kernel_stats {
     int64_t launches; /* kernel launches measured */
@@ -172,7 +174,16 @@ enum {
     UG_KERNEL_RECHECK = 1    /* FP64 recheck of the split-BF16 form's uncertain candidates */
 };
 
-int32_t ug_pack_profile(ug_pack* h, int32_t enable);
+/* ug_pack_profile's `kernels`: which launches are timed (bit UG_KERNEL_* set); any nonzero
+ * value turns the counters on, 0 turns measurement off.  Values above UG_PROFILE_ALL are
+ * rejected. */
+enum {
+    UG_PROFILE_PREFILTER = 1 << UG_KERNEL_PREFILTER,
+    UG_PROFILE_RECHECK = 1 << UG_KERNEL_RECHECK,
+    UG_PROFILE_ALL = UG_PROFILE_PREFILTER | UG_PROFILE_RECHECK
+};
+
+int32_t ug_pack_profile(ug_pack* h, int32_t kernels);
 /* waits for the handle's profiled launches still in flight */
 int32_t ug_pack_kernel_stats(ug_pack* h, int32_t which, ug_kernel_stats* out);
 

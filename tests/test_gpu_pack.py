@@ -179,6 +179,33 @@ def test_recycled_buffers_and_cache_trim(gpu):
             ugpack.abi.cache_trim()
 
 
+def test_profiling_selects_the_timed_kernels(gpu):
+    """ug_pack_profile times only the selected kernels, counts in all of them, never changes
+    the accepted list, and rejects values outside the UG_PROFILE_* bits."""
+    import ugpack
+    from ugpack import abi
+
+    g = _gold("cfg1_d2_T16_s1")
+    seen = {}
+    for kernels in (abi.PROFILE_PREFILTER, abi.PROFILE_RECHECK, abi.PROFILE_ALL):
+        with ugpack.Packer(2, 16, seed=1, max_attempts=20000, precision=ugpack.PREC_SPLIT_BF16) as p:
+            p.profile(True, kernels=kernels)
+            p.run()
+            assert list(p.accepted()[0]) == g["idx"]
+            pre, re = p.kernel_stats(abi.KERNEL_PREFILTER), p.kernel_stats(abi.KERNEL_RECHECK)
+            seen[kernels] = (pre, re)
+            assert (pre["launches"] > 0) == bool(kernels & abi.PROFILE_PREFILTER)
+            assert (re["launches"] > 0) == bool(kernels & abi.PROFILE_RECHECK)
+            assert pre["visits"] > 0 and re["items"] > 0  # the counters run either way
+    # the counters do not depend on which launches are timed
+    counts = {k: (v[0]["visits"], v[0]["points"], v[1]["items"], v[1]["visits"]) for k, v in seen.items()}
+    assert len(set(counts.values())) == 1
+    with ugpack.Packer(2, 16, seed=1, max_attempts=100) as p:
+        for bad in (-1, abi.PROFILE_ALL + 1):
+            with pytest.raises(ugpack.UgError, match="UG_PROFILE"):
+                abi.check(abi.lib().ug_pack_profile(p._h, bad))
+
+
 def test_invalid_config_is_an_error_not_an_exit(gpu):
     import ugpack
 

@@ -612,17 +612,20 @@ extern "C" int32_t ug_pack_check(ug_pack* hp, uint64_t first, int64_t count, int
     return rc;
 }
 
-extern "C" int32_t ug_pack_profile(ug_pack* hp, int32_t enable) {
+extern "C" int32_t ug_pack_profile(ug_pack* hp, int32_t kernels) {
     if (!hp)
         return fail(UG_EINVAL, "ug_pack_profile: NULL handle");
+    if (kernels < 0 || kernels > UG_PROFILE_ALL)
+        return fail(UG_EINVAL, "ug_pack_profile: kernels=%d is not a set of UG_PROFILE_* bits", kernels);
     Pack* h = (Pack*)hp;
-    if (enable && !h->d_ctr) {
+    if (kernels && !h->d_ctr) {
         if (!(h->d_ctr = (unsigned long long*)cache_alloc(UG_CTR_WORDS * sizeof(unsigned long long))))
             return fail(UG_ENOMEM, "ug_pack_profile: counter allocation failed");
         if (hipMemset(h->d_ctr, 0, UG_CTR_WORDS * sizeof(unsigned long long)) != hipSuccess)
             return fail(UG_EHIP, "ug_pack_profile: counter clear failed");
     }
-    h->profile = enable != 0;
+    h->profile = kernels != 0;
+    h->profile_mask = kernels;
     return UG_OK;
 }
 

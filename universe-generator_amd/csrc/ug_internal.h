@@ -172,6 +172,7 @@ struct Pack {
     // Profiled launches carry start/stop events of their own (one pair per launch, read back
     // only when the stats are asked for, so profiling adds no host wait inside a pack).
     bool profile = false;
+    int profile_mask = 0;  // kernels whose launches are timed (bit UG_KERNEL_*)
     std::vector<std::array<hipEvent_t, 2>> ev_pool;  // event pairs, reused after a collect
     struct TimedLaunch {
         int which;
@@ -211,7 +212,7 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
 // then records an event pair directly before and after the dispatch (timing-only events
 // without the system-scope fence, never waited on inside a pack); timed_end books the pair
 // for timed_collect.
-int32_t timed_begin(Pack* h);
+int32_t timed_begin(Pack* h, int which);
 int32_t timed_end(Pack* h, int which, int64_t items);
 int32_t timed_collect(Pack* h);  // waits for pending timed launches and folds them into kstats
 void release_events(Pack* h);    // hands h's event pairs to the process-wide free list

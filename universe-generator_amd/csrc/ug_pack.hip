@@ -1388,9 +1388,9 @@ static int32_t take_events(Pack* h) {
     return UG_OK;
 }
 
-int32_t timed_begin(Pack* h) {
+int32_t timed_begin(Pack* h, int which) {
     h->timing_armed = h->timing_fired = false;
-    if (!h->profile)
+    if (!h->profile || !((h->profile_mask >> which) & 1))
         return UG_OK;
     if (h->ev_pending.size() >= 512) {  // bound the pool: fold what has finished
         const int32_t rc = timed_collect(h);
@@ -1525,7 +1525,7 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
     const bool split = h->cfg.precision == UG_PREC_SPLIT_BF16 && h->full;
     // every pass is measured, retests of a survivor list included
     if (!split) {
-        rc = timed_begin(h);
+        rc = timed_begin(h, UG_KERNEL_PREFILTER);
         if (!rc)
             rc = launch_cand64(h, base, count, list, h->d_flags, nullptr, false, ctr, dfresh, -1);
         if (!rc)
@@ -1536,7 +1536,7 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
     }
     // stage 1: the first 32 ordered timesteps of every candidate -> survivors in d_list3
     // (count in d_count[0])
-    rc = timed_begin(h);
+    rc = timed_begin(h, UG_KERNEL_PREFILTER);
     if (!rc)
         rc = launch_fast(h, base, count, list, h->d_flags, dfresh);
     if (!rc)
@@ -1551,7 +1551,7 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
         // then sees only candidates no timestep certainly rejects.  Measured on one MI355X:
         // 3+1 T=256 packs 1.23x faster with it, 3+1 T=128 about 4% slower (its FP64 recheck
         // is cheap enough), so shorter models skip it.
-        rc = timed_begin(h);
+        rc = timed_begin(h, UG_KERNEL_PREFILTER);
         if (!rc)
             rc = launch_fast(h, 0, count, h->d_list3, h->d_flags, h->d_count, 1, fast_blocks(h));
         if (!rc)
@@ -1563,7 +1563,7 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
     }
     h->recheck_slot = rslot;
     if (!rc)
-        rc = timed_begin(h);
+        rc = timed_begin(h, UG_KERNEL_RECHECK);
     if (!rc)
         rc = launch_cand64(h, 0, count, rlist, h->d_flags, nullptr, false, rctr, h->d_count + rslot, -1);
     if (!rc)

@@ -59,6 +59,8 @@ class KernelStats(C.Structure):
 
 
 KERNEL_PREFILTER, KERNEL_RECHECK = 0, 1
+PROFILE_PREFILTER, PROFILE_RECHECK = 1 << KERNEL_PREFILTER, 1 << KERNEL_RECHECK
+PROFILE_ALL = PROFILE_PREFILTER | PROFILE_RECHECK
 SUB_BUDGET, SUB_ACCEPT_RATE, SUB_FILL_RATE = 1, 2, 3
 
 
@@ -266,8 +268,10 @@ class Packer:
         check(lib().ug_pack_positions(self._h, out.data_ptr(), out.shape[0]))
         return out[:n]
 
-    def profile(self, enable: bool = True) -> None:
-        check(lib().ug_pack_profile(self._h, int(enable)))
+    def profile(self, enable: bool = True, kernels: int = PROFILE_ALL) -> None:
+        """Count visits and points in every candidate-test kernel and time the launches of
+        `kernels` (PROFILE_* bits) with HIP events; enable=False turns both off."""
+        check(lib().ug_pack_profile(self._h, kernels if enable else 0))
 
     def kernel_stats(self, which: int = KERNEL_PREFILTER) -> dict:
         ks = KernelStats()
