@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define UG_ABI_VERSION 2
+#define UG_ABI_VERSION 3
 
 enum {
     UG_OK = 0,

@@ -18,7 +18,7 @@ from ._build import LIB_PATH
 UG_OK, UG_EINVAL, UG_ENOMEM, UG_EHIP, UG_EOVERFLOW, UG_ESTATE = 0, -1, -2, -3, -4, -5
 PREC_FP64, PREC_SPLIT_BF16 = 0, 1
 STOP_BUDGET, STOP_FIXED_WINDOW, STOP_REF_WINDOW = 0, 1, 2
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 
 class UgError(RuntimeError):
