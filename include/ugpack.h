@@ -176,7 +176,9 @@ enum {
 
 /* ug_pack_profile's `kernels`: which launches are timed (bit UG_KERNEL_* set); any nonzero
  * value turns the counters on, 0 turns measurement off.  Values above UG_PROFILE_ALL are
- * rejected. */
+ * rejected.  In ug_kernel_stats, launches, ms and the prefilter's items cover the timed
+ * launches only; visits, points and the recheck's items are counted in every launch while
+ * measurement is on. */
 enum {
     UG_PROFILE_PREFILTER = 1 << UG_KERNEL_PREFILTER,
     UG_PROFILE_RECHECK = 1 << UG_KERNEL_RECHECK,
