@@ -7,6 +7,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import ugpack
 from ugpack import abi
 
@@ -38,3 +40,19 @@ def test_config_struct_layout_matches_header():
     # ug_pack_config: 4 i32, u64, 2 i32, 2 f64, 3 i64, f64, i64, 2 i32
     assert ctypes.sizeof(abi.PackConfig) == 4 * 4 + 8 + 2 * 4 + 2 * 8 + 3 * 8 + 8 + 8 + 2 * 4
     assert ctypes.sizeof(abi.PackStatus) == 3 * 8 + 2 * 4 + 2 * 8 + 2 * 8 + 2 * 8
+
+
+def test_profile_and_kernel_constants_match_header():
+    text = open(os.path.join(ROOT, "include", "ugpack.h")).read()
+    assert re.search(r"UG_KERNEL_PREFILTER\s*=\s*0", text) and re.search(r"UG_KERNEL_RECHECK\s*=\s*1", text)
+    assert (abi.KERNEL_PREFILTER, abi.KERNEL_RECHECK) == (0, 1)
+    assert "UG_PROFILE_PREFILTER = 1 << UG_KERNEL_PREFILTER" in text
+    assert "UG_PROFILE_RECHECK = 1 << UG_KERNEL_RECHECK" in text
+    assert (abi.PROFILE_PREFILTER, abi.PROFILE_RECHECK, abi.PROFILE_ALL) == (1, 2, 3)
+
+
+def test_tuner_rejects_an_empty_sample():
+    from ugpack import tune
+
+    with pytest.raises(ValueError):
+        tune.ensemble_threads([])
