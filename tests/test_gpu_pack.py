@@ -401,6 +401,38 @@ def test_euclid_collision_equals_oracle(gpu, precision, d, T, seed, cands):
     assert list(idx) == list(want_idx) and list(gid) == list(range(len(idx)))
 
 
+_UNEVEN_REF = {}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+@pytest.mark.parametrize("d,T,seed,cands,euclid", [
+    (3, 161, 7, 12000, False),  # split form: stage 2 on (6 column blocks, the last one column wide)
+    (2, 161, 8, 20000, False),
+    (3, 170, 3, 6000, True),    # stage 2 with the L2 ball
+    (2, 33, 4, 20000, False),   # two column blocks, the second one column wide
+    (3, 33, 5, 8000, False),
+    (2, 100, 6, 15000, False),
+    (3, 65, 2, 6000, False),
+])
+def test_uneven_column_blocks_equal_oracle(gpu, precision, d, T, seed, cands, euclid):
+    """T not a multiple of 32: the prefilter's last block of 32 timesteps is partial, and past
+    T = 160 the split form's second stage walks it.  The accepted lists equal the oracle's
+    serial RSA (about 10 s of oracle work for all cases, shared by both forms)."""
+    import ugpack
+
+    print(f"uneven {precision} d={d} T={T} seed={seed} euclid={euclid}", flush=True)
+    key = (d, T, seed, cands, euclid)
+    if key not in _UNEVEN_REF:
+        _UNEVEN_REF[key] = po.Model(d, T, seed=seed, euclid=euclid).pack(cands)
+    want_idx, _, att = _UNEVEN_REF[key]
+    prec = ugpack.PREC_FP64 if precision == "fp64" else ugpack.PREC_SPLIT_BF16
+    with ugpack.Packer(d, T, seed=seed, max_attempts=cands, precision=prec, euclid=euclid) as p:
+        st = p.run()
+        idx, gid = p.accepted()
+    assert st.attempts == att
+    assert list(idx) == list(want_idx) and list(gid) == list(range(len(idx)))
+
+
 def test_characterization_check_on_config1(gpu):
     """ug_pack_check restates the oracle's brute-force characterization on the device."""
     import ugpack
