@@ -1128,9 +1128,9 @@ __global__ __launch_bounds__(256) void k_conflicts_tiled(DevModel m, const doubl
 // place of four copies): [8] accepted count, [16] occupied (t, cell) buckets, [32] the
 // filter's four counts when they were still on the device, [64 ..] the accepted candidate
 // indices (at most s).  The host reads the block after synchronising the stream.
-__global__ __launch_bounds__(256) void k_report(const int* nacc, const unsigned long long* filled,
-                                                const int64_t* acc_idx, int s, const int* counts, char* pin,
-                                                int* spec_count, int spec_n, int64_t stop_lim, int64_t n_room) {
+__device__ __forceinline__ void report_block(const int* nacc, const unsigned long long* filled,
+                                             const int64_t* acc_idx, int s, const int* counts, char* pin,
+                                             int* spec_count, int spec_n, int64_t stop_lim, int64_t n_room) {
     const int na = min(*nacc, s);
     if (threadIdx.x == 0) {
         *(int*)(pin + 8) = *nacc;
@@ -1267,9 +1267,8 @@ __global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, c
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_halo(DevModel m, const double* pos, const int* acc_list,
-                                              const int* nacc, int limit) {
-    const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void halo_item(const DevModel& m, const double* pos, const int* acc_list,
+                                          const int* nacc, int limit, int64_t item) {
     const int na = min(*nacc, limit);
     const int k = (int)(item / m.T), t = (int)(item % m.T);
     if (k >= na)
@@ -1288,6 +1287,27 @@ __global__ __launch_bounds__(256) void k_halo(DevModel m, const double* pos, con
         m.tab[row + m.T + 1] = m.tab[row + 1];
     if (cl == m.T - 1)
         m.tab[row] = m.tab[row + m.T];
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_halo(DevModel m, const double* pos, const int* acc_list,
+                                              const int* nacc, int limit) {
+    halo_item<D>(m, pos, acc_list, nacc, limit, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+// The halo copies in blocks [0, gridDim.x - 1) and the chunk's report in the last block: the
+// report reads only what k_insert and the filter wrote, never the halo cells, so it needs no
+// order against the copies and the chunk ends with one dispatch instead of two.
+template <int D>
+__global__ __launch_bounds__(256) void k_halo_report(DevModel m, const double* pos, const int* acc_list,
+                                                     const int* nacc, int limit, const unsigned long long* filled,
+                                                     const int64_t* acc_idx, int s, const int* counts, char* pin,
+                                                     int* spec_count, int spec_n, int64_t stop_lim,
+                                                     int64_t n_room) {
+    if (blockIdx.x == gridDim.x - 1)
+        report_block(nacc, filled, acc_idx, s, counts, pin, spec_count, spec_n, stop_lim, n_room);
+    else
+        halo_item<D>(m, pos, acc_list, nacc, limit, (int64_t)blockIdx.x * 256 + threadIdx.x);
 }
 
 __global__ void k_fill_i32(int* p, int64_t n, int v) {
@@ -1673,30 +1693,30 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
                            h->d_nacc);
         const int limit = h->max_n > 0 ? (int)std::min<int64_t>(h->max_n - h->n, INT32_MAX) : INT32_MAX;
         const int ib = grid_for((int64_t)s * h->T, 256);
+        // one round trip: accepted count, their candidate indices and the filled-cell count,
+        // written into the pinned block by the last block of the halo launch
+        char* pin = (char*)h->h_pinned;
+        const bool ahead = dev && spec && spec->count > 0;
+        const int* rcounts = dev ? h->d_count : nullptr;
+        int* spec_count = ahead ? spec->dcount : nullptr;
+        const int spec_n = ahead ? (int)spec->count : 0;
+        const int64_t stop_lim = ahead ? spec->stop_lim : 0, n_room = ahead ? spec->n_room : 0;
         if (h->d == 2) {
             hipLaunchKernelGGL(k_insert<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
                                h->d_nacc, limit, list, nullptr, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
                                h->d_pos32, h->d_filled);
-            hipLaunchKernelGGL(k_halo<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
-                               h->d_nacc, limit);
+            hipLaunchKernelGGL(k_halo_report<2>, dim3(ib + 1), dim3(256), 0, h->stream, h->dm, h->d_spos,
+                               h->d_acc_list, h->d_nacc, limit, h->d_filled, h->d_acc_idx + h->n, s, rcounts, pin,
+                               spec_count, spec_n, stop_lim, n_room);
         } else {
             hipLaunchKernelGGL(k_insert<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
                                h->d_nacc, limit, list, nullptr, h->n, h->d_acc_idx, h->d_acc_gid, h->d_pos64,
                                h->d_pos32, h->d_filled);
-            hipLaunchKernelGGL(k_halo<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_acc_list,
-                               h->d_nacc, limit);
+            hipLaunchKernelGGL(k_halo_report<3>, dim3(ib + 1), dim3(256), 0, h->stream, h->dm, h->d_spos,
+                               h->d_acc_list, h->d_nacc, limit, h->d_filled, h->d_acc_idx + h->n, s, rcounts, pin,
+                               spec_count, spec_n, stop_lim, n_room);
         }
         rc = hip_check(hipGetLastError(), "resolve kernels");
-        if (rc)
-            return rc;
-        // one round trip: accepted count, their candidate indices and the filled-cell count,
-        // written into the pinned block by one small kernel
-        char* pin = (char*)h->h_pinned;
-        const bool ahead = dev && spec && spec->count > 0;
-        hipLaunchKernelGGL(k_report, dim3(1), dim3(256), 0, h->stream, h->d_nacc, h->d_filled, h->d_acc_idx + h->n, s,
-                           dev ? h->d_count : nullptr, pin, ahead ? spec->dcount : nullptr,
-                           ahead ? (int)spec->count : 0, ahead ? spec->stop_lim : 0, ahead ? spec->n_room : 0);
-        rc = hip_check(hipGetLastError(), "k_report");
         if (!rc && ahead) {
             // the next sub-batch's filter goes in behind the report, so the device works on
             // it while the host reads this chunk's results (it only reads the table)
