@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define UG_ABI_VERSION 3
+#define UG_ABI_VERSION 4
 
 enum {
     UG_OK = 0,
@@ -48,8 +48,11 @@ enum {
     UG_STOP_BUDGET = 0,       /* candidates [0, max_attempts) */
     UG_STOP_FIXED_WINDOW = 1, /* at each multiple c of stop_batch with c >= stop_window:
                                  stop when accepts in [c-stop_window, c) < stop_rate*stop_window */
-    UG_STOP_REF_WINDOW = 2    /* the reference's --until-accept-rate window
-                                 (cuda/braid_cuda.cu:585-590,930-943) over stop_batch batches */
+    UG_STOP_REF_WINDOW = 2    /* the reference engines' round schedule and --until-accept-rate
+                                 window (cuda/braid_cuda.cu:585-590,885-944 for dim 2,
+                                 cuda/braid_cuda3d.cu:1446-1566 for dim 3): rounds of 2^16
+                                 candidates adapted on their survivors (4096..2^26), the
+                                 last round may pass max_attempts; stop_rate 0 = no window */
 };
 
 /* One packing run: the flags of cuda/braid_cuda.cu:534-584 / cuda/braid_cuda3d.cu:862-913. */
@@ -147,6 +150,12 @@ int32_t ug_pack_positions(const ug_pack* h, double* dev_out, int64_t cap);
 /* Log-spaced growth curve (attempts, n, nsub, filled) as cuda/braid_cuda.cu:925-929. */
 int32_t ug_pack_curve(const ug_pack* h, int64_t* rows4, int64_t cap, int64_t* n);
 
+/* Rounds run under UG_STOP_REF_WINDOW, 5 values each: first candidate, candidates, survivors
+ * against the table at the round's start (the engines' `got`, cuda/braid_cuda.cu:894-897),
+ * uniques admitted, (timestep, cell) buckets first occupied.  The reference keeps these in
+ * locals of its loop; exposed so a caller can audit the schedule.  *n receives the total. */
+int32_t ug_pack_rounds(const ug_pack* h, int64_t* rows5, int64_t cap, int64_t* n);
+
 /* Positions of arbitrary candidates of the handle's model: dev_idx[count] -> dev_out
  * [count][T][dim] (FP64).  precision UG_PREC_FP64 gives the canonical values
  * (collides_dev's evaluation, cuda/braid_cuda.cu:418-441); UG_PREC_SPLIT_BF16 gives the
@@ -195,7 +204,9 @@ int32_t ug_pack_kernel_stats(ug_pack* h, int32_t which, ug_kernel_stats* out);
  *   out[1] = rejected candidates in [first, first + count) that touch no unique accepted
  *            before them (must be 0),
  *   out[2] = uniques checked, out[3] = rejected candidates checked.
- * Candidates must lie within the consumed range.  Subpaths are not included. */
+ * Candidates must lie within the decided range (the consumed one, except that max_n
+ * reached inside a UG_STOP_REF_WINDOW round leaves the rest of the round untested).
+ * Subpaths are not included. */
 int32_t ug_pack_check(ug_pack* h, uint64_t first, int64_t count, int64_t* out);
 
 /* Releases the handle.  Its device buffers and pinned staging block are kept in a
@@ -213,12 +224,14 @@ int32_t ug_cache_trim(void);
  * on the current device, `threads` at a time: one host worker thread and one HIP stream
  * each, universes taken in index order from a shared counter (no reference counterpart:
  * the reference runs one universe per engine process, braidlab/orchestrator.py).
- * status[i] receives universe i's final status.  keep == NULL: every handle is destroyed
- * after its run; else keep[i] receives universe i's handle (on the null stream) for the
- * caller to query and destroy.  On error the remaining universes are not started and the
- * first failure's message is in ug_last_error(). */
+ * status[i] receives universe i's final status; digests (may be NULL) receives the 64-bit
+ * FNV-1a hash of universe i's accepted candidate indices (int64 little-endian bytes in
+ * admission order).  keep == NULL: every handle is destroyed after its run; else keep[i]
+ * receives universe i's handle (on a stream of its own, released with it) for the caller
+ * to query and destroy.  On error the remaining universes are not started, no handle is
+ * kept, and the first failure's message is in ug_last_error(). */
 int32_t ug_ensemble_run(const ug_pack_config* cfgs, int32_t n, int32_t threads, ug_pack** keep,
-                        ug_pack_status* status);
+                        ug_pack_status* status, uint64_t* digests);
 
 /* ------------------------------------------------------------------ frame baker
  * Replaces the frame_bake executable (cuda/frame_bake.cu:398-635), whose BRF1 output

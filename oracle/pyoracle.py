@@ -60,8 +60,18 @@ def lib() -> C.CDLL:
         L.or_pack_mt.restype = C.c_int64
         L.or_pack_sub.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_uint64,
                                   C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
-                                  C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+                                  C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]
         L.or_pack_sub.restype = C.c_int64
+        L.or_pack_ref.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int64,
+                                  C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64), C.c_void_p, C.c_int64,
+                                  C.POINTER(C.c_int64)]
+        L.or_pack_ref.restype = C.c_int64
+        L.or_positions.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.or_positions.restype = None
+        L.or_count_clear.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64]
+        L.or_count_clear.restype = C.c_int64
         L.or_check_rsa.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
         L.or_check_rsa.restype = C.c_int64
         _lib = L
@@ -205,22 +215,66 @@ class Model:
         return idx[:min(n, cap)].copy(), att.value
 
     def pack_sub(self, pos0: np.ndarray, gid0: np.ndarray, first: int, budget: float,
-                 accept_rate: float = 0.0, fill_rate: float = 0.0, cap: int = 1 << 20):
+                 accept_rate: float = 0.0, fill_rate: float = 0.0, cap: int = 1 << 20,
+                 rounds: bool = False):
         """Subpath phase after the uniques pos0 (n0,T,d) with groups gid0: returns
-        (idx int64[Nsub], gid int32[Nsub], subpath candidates drawn, newly filled cells)."""
+        (idx int64[Nsub], gid int32[Nsub], subpath candidates drawn, newly filled cells)
+        and, with rounds=True, the round records (start, size, survivors, admitted, fresh)."""
         pos0 = np.ascontiguousarray(pos0, np.float64)
         gid0 = np.ascontiguousarray(gid0, np.int32)
         idx = np.zeros(cap, np.int64)
         gid = np.zeros(cap, np.int32)
         att = C.c_int64(0)
         filled = C.c_int64(0)
+        rcap = 1 << 16
+        rec = np.zeros((rcap, 5), np.int64)
+        nr = C.c_int64(0)
         n = lib().or_pack_sub(self.ref, _p(pos0), _p(gid0), len(gid0), first, float(budget),
                               float(accept_rate), float(fill_rate), _p(idx), _p(gid), cap,
-                              C.byref(att), C.byref(filled))
+                              C.byref(att), C.byref(filled), _p(rec), rcap, C.byref(nr))
         if n < 0:
             raise RuntimeError(f"or_pack_sub failed ({n})")
         n = min(n, cap)
-        return idx[:n].copy(), gid[:n].copy(), att.value, filled.value
+        out = (idx[:n].copy(), gid[:n].copy(), att.value, filled.value)
+        if rounds:
+            out = out + (rec[:min(nr.value, rcap)].copy(),)
+        return out
+
+    def pack_ref(self, budget: float, rule: int | None = None, stop_rate: float = 0.0,
+                 max_n: int = 0, cap: int = 1 << 22):
+        """Serial RSA on the reference engine's round schedule (rule 2 = braid_cuda,
+        3 = braid_cuda3d; default: the model's dimension).  Returns a dict with idx, gid,
+        attempts, decided and rounds (start, size, survivors, admitted, fresh)."""
+        rule = self.d if rule is None else rule
+        idx = np.zeros(cap, np.int64)
+        gid = np.zeros(cap, np.int32)
+        att, dec, nr = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        rcap = 1 << 16
+        rec = np.zeros((rcap, 5), np.int64)
+        n = lib().or_pack_ref(self.ref, float(budget), rule, float(stop_rate), max_n, _p(idx),
+                              _p(gid), cap, C.byref(att), C.byref(dec), _p(rec), rcap,
+                              C.byref(nr))
+        if n < 0:
+            raise RuntimeError(f"or_pack_ref failed ({n})")
+        n = min(n, cap)
+        return {"idx": idx[:n].copy(), "gid": gid[:n].copy(), "attempts": att.value,
+                "decided": dec.value, "rounds": rec[:min(nr.value, rcap)].copy()}
+
+    def count_clear(self, first: int, count: int, acc) -> int:
+        """Candidates in [first, first + count) touching none of acc (pairwise, no grid)."""
+        acc = np.ascontiguousarray(acc, np.int64)
+        n = lib().or_count_clear(self.ref, first, count, _p(acc), len(acc))
+        if n < 0:
+            raise RuntimeError(f"or_count_clear failed ({n})")
+        return n
+
+    def positions_of(self, idx) -> np.ndarray:
+        """Canonical wrapped positions (n, T, d) of candidates idx (evaluated in C)."""
+        idx = np.ascontiguousarray(idx, np.int64)
+        out = np.zeros((len(idx), self.T, self.d))
+        if len(idx):
+            lib().or_positions(self.ref, _p(idx), len(idx), _p(out))
+        return out
 
     def touches(self, x, y) -> bool:
         x = np.ascontiguousarray(x, np.float64)

@@ -768,10 +768,221 @@ static int64_t window_target(double rate) {
     return (int64_t)t;
 }
 
+/* Appends one round record {start, size, survivors, admitted, fresh} when there is room. */
+static void round_log(int64_t* rounds, int64_t rounds_cap, int64_t* nrounds, int64_t start,
+                      int64_t size, int64_t surv, int64_t adm, int64_t fresh) {
+    if (!nrounds)
+        return;
+    if (rounds && *nrounds < rounds_cap) {
+        int64_t* r = rounds + 5 * *nrounds;
+        r[0] = start;
+        r[1] = size;
+        r[2] = surv;
+        r[3] = adm;
+        r[4] = fresh;
+    }
+    (*nrounds)++;
+}
+
+/* Candidates [start, start + size) that touch no path of P (the packing as the round found
+ * it; read-only, so tested in parallel) -> *surv (ascending), their number returned. */
+static int64_t round_survivors(const or_model* m, const packing* P, int64_t start, int64_t size,
+                               char** flag, int64_t* flag_alloc, int64_t** surv,
+                               int64_t* surv_alloc) {
+    const int T = m->T, d = m->d;
+    if (size > *flag_alloc) {
+        free(*flag);
+        *flag = (char*)malloc((size_t)size);
+        *flag_alloc = *flag ? size : 0;
+        if (!*flag)
+            return -2;
+    }
+    char* fl = *flag;
+    int par_err = 0;
+#pragma omp parallel reduction(| : par_err)
+    {
+        cand_buf tb;
+        if (cb_init(&tb, m)) {
+            par_err = 1;
+        } else {
+#pragma omp for schedule(dynamic, 256)
+            for (int64_t k = 0; k < size; k++) {
+                or_candidate(m, (uint64_t)(start + k), tb.a1, tb.b, tb.c, tb.s, NULL, NULL);
+                int hit = 0;
+                for (int oi = 0; oi < T && !hit; oi++) {
+                    int t = m->order[oi];
+                    or_eval_t(m, t, tb.a1, tb.b, tb.c, tb.s, tb.X + (size_t)t * d);
+                    hit = pk_scan(P, t, tb.X + (size_t)t * d, 0, NULL);
+                }
+                fl[k] = (char)!hit;
+            }
+            cb_free(&tb);
+        }
+    }
+    if (par_err)
+        return -2;
+    int64_t n = 0;
+    for (int64_t k = 0; k < size; k++) {
+        if (!fl[k])
+            continue;
+        if (n == *surv_alloc) {
+            int64_t na = *surv_alloc ? 2 * *surv_alloc : 1024;
+            int64_t* ns = (int64_t*)realloc(*surv, (size_t)na * 8);
+            if (!ns)
+                return -2;
+            *surv = ns;
+            *surv_alloc = na;
+        }
+        (*surv)[n++] = start + k;
+    }
+    return n;
+}
+
+/* The reference engines' batch adaptation toward ~256 survivors a round
+ * (cuda/braid_cuda.cu:920-924, cuda/braid_cuda3d.cu:1540-1543); got is capped at the
+ * survivor buffer (2^16 at KMAX_WIGGLE > 16, :760), which never changes the outcome. */
+static int64_t adapt_batch(int64_t batch, int64_t surv) {
+    const int64_t got = surv < ((int64_t)1 << 16) ? surv : ((int64_t)1 << 16);
+    if (got > 512 && batch > 4096)
+        return batch / 2;
+    if (got < 64 && batch < ((int64_t)1 << 26))
+        return batch * 2;
+    return batch;
+}
+
+int64_t or_pack_ref(const or_model* m, double budget, int rule, double stop_rate, int64_t max_n,
+                    int64_t* idx, int32_t* gid, int64_t cap, int64_t* attempts,
+                    int64_t* decided, int64_t* rounds, int64_t rounds_cap, int64_t* nrounds) {
+    packing P;
+    cand_buf cb;
+    if ((rule != 2 && rule != 3) || pk_init(&P, m) || cb_init(&cb, m)) {
+        pk_free(&P);
+        cb_free(&cb);
+        return rule != 2 && rule != 3 ? -1 : -2;
+    }
+    const int T = m->T, d = m->d;
+    char* flag = NULL;
+    int64_t* surv = NULL;
+    int64_t flag_alloc = 0, surv_alloc = 0, nacc = 0, nacc_cap = 1024, ret = 0;
+    int64_t* acc = (int64_t*)malloc((size_t)nacc_cap * 8);
+    rate_window w = {0};
+    w.target = window_target(stop_rate);
+    int64_t att = 0, dec = 0, batch = (int64_t)1 << 16;
+    /* rule 3: the round after the current one is sized (enqueued) before the current one is
+     * drained, and a stop still drains it (cuda/braid_cuda3d.cu:1446-1566) */
+    int64_t next = batch, enq = batch;
+    int stop = 0;
+    if (nrounds)
+        *nrounds = 0;
+    for (;;) {
+        int64_t size;
+        if (rule == 2) {
+            if ((double)att >= budget || (max_n > 0 && nacc >= max_n))
+                break;
+            size = batch;
+        } else {
+            if (next == 0 || (max_n > 0 && nacc >= max_n))
+                break;
+            size = next;
+            next = 0;
+            if ((double)enq < budget && !stop) {
+                next = batch;
+                enq += next;
+            }
+        }
+        const int64_t ns = round_survivors(m, &P, att, size, &flag, &flag_alloc, &surv,
+                                           &surv_alloc);
+        if (ns < 0) {
+            ret = -2;
+            break;
+        }
+        /* admission in index order against the up-to-date packing (host_collides, :703-722) */
+        int64_t adm = 0, fresh = 0;
+        dec = att + size;
+        for (int64_t s = 0; s < ns && ret == 0; s++) {
+            or_candidate(m, (uint64_t)surv[s], cb.a1, cb.b, cb.c, cb.s, NULL, NULL);
+            int hit = 0;
+            for (int oi = 0; oi < T && !hit; oi++) {
+                int t = m->order[oi];
+                or_eval_t(m, t, cb.a1, cb.b, cb.c, cb.s, cb.X + (size_t)t * d);
+                hit = pk_scan(&P, t, cb.X + (size_t)t * d, 0, NULL);
+            }
+            if (hit)
+                continue;
+            int64_t f = pk_insert(&P, cb.X, (int32_t)P.n);
+            if (f < 0) {
+                ret = -2;
+                break;
+            }
+            fresh += f;
+            if (nacc == nacc_cap) {
+                nacc_cap *= 2;
+                acc = (int64_t*)realloc(acc, (size_t)nacc_cap * 8);
+            }
+            acc[nacc++] = surv[s];
+            adm++;
+            if (max_n > 0 && nacc >= max_n) {
+                dec = surv[s] + 1;
+                break;
+            }
+        }
+        if (ret)
+            break;
+        round_log(rounds, rounds_cap, nrounds, att, size, ns, adm, fresh);
+        att += size;
+        batch = adapt_batch(batch, ns);
+        /* the 2+1 engine's window records the adapted batch (cuda/braid_cuda.cu:930-936), the
+         * 3+1 engine's the round just drained (cuda/braid_cuda3d.cu:1549-1551) */
+        if (window_push(&w, rule == 2 ? batch : size, adm, att, stop_rate)) {
+            if (rule == 2)
+                break;
+            stop = 1;
+        }
+    }
+    if (ret == 0) {
+        for (int64_t i = 0; i < nacc && i < cap; i++) {
+            if (idx)
+                idx[i] = acc[i];
+            if (gid)
+                gid[i] = P.gid[i];
+        }
+        ret = nacc;
+        if (attempts)
+            *attempts = att;
+        if (decided)
+            *decided = dec;
+    }
+    free(acc);
+    free(flag);
+    free(surv);
+    free(w.att);
+    free(w.val);
+    pk_free(&P);
+    cb_free(&cb);
+    return ret;
+}
+
+void or_positions(const or_model* m, const int64_t* idx, int64_t n, double* out) {
+    const int T = m->T, d = m->d;
+#pragma omp parallel
+    {
+        cand_buf tb;
+        if (!cb_init(&tb, m)) {
+#pragma omp for schedule(static)
+            for (int64_t i = 0; i < n; i++) {
+                or_candidate(m, (uint64_t)idx[i], tb.a1, tb.b, tb.c, tb.s, NULL, NULL);
+                for (int t = 0; t < T; t++)
+                    or_eval_t(m, t, tb.a1, tb.b, tb.c, tb.s, out + ((size_t)i * T + t) * d);
+            }
+            cb_free(&tb);
+        }
+    }
+}
+
 int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, int64_t n0,
                     uint64_t first, double budget, double accept_rate, double fill_rate,
                     int64_t* idx, int32_t* gid, int64_t cap, int64_t* attempts,
-                    int64_t* filled_new) {
+                    int64_t* filled_new, int64_t* rounds, int64_t rounds_cap, int64_t* nrounds) {
     packing P;
     cand_buf cb;
     if (pk_init(&P, m) || cb_init(&cb, m)) {
@@ -779,7 +990,6 @@ int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, 
         cb_free(&cb);
         return -2;
     }
-    const int64_t surv_cap = 1 << 16; /* the reference's survivor buffer at KMAX_WIGGLE > 16 */
     int64_t ret = 0, nsub = 0, fresh = 0, att = 0, batch = 1 << 16;
     uint64_t* surv = NULL;
     int64_t surv_n = 0, surv_alloc = 0, flag_alloc = 0;
@@ -787,6 +997,8 @@ int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, 
     rate_window aw = {0}, fw = {0};
     aw.target = window_target(accept_rate);
     fw.target = window_target(fill_rate);
+    if (nrounds)
+        *nrounds = 0;
     for (int64_t i = 0; i < n0; i++)
         if (pk_insert(&P, pos0 + (size_t)i * m->T * m->d, gid0[i]) < 0)
             ret = -2;
@@ -852,11 +1064,8 @@ int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, 
             admitted++;
         }
         fresh += round_new;
-        int64_t got = surv_n < surv_cap ? surv_n : surv_cap;
-        if (got > 512 && batch > 4096)
-            batch /= 2;
-        else if (got < 64 && batch < ((int64_t)1 << 26))
-            batch *= 2;
+        round_log(rounds, rounds_cap, nrounds, att - batch, batch, surv_n, admitted, round_new);
+        batch = adapt_batch(batch, surv_n);
         /* the windows record the adapted batch, as the reference does */
         if (window_push(&aw, batch, admitted, att, accept_rate))
             break;
@@ -944,4 +1153,39 @@ int64_t or_check_rsa(const or_model* m, int64_t M, const int64_t* acc, int64_t n
     free(member);
     cb_free(&cb);
     return bad;
+}
+
+int64_t or_count_clear(const or_model* m, int64_t first, int64_t count, const int64_t* acc,
+                       int64_t na) {
+    const int T = m->T, d = m->d;
+    double* A = (double*)malloc((size_t)(na > 0 ? na : 1) * T * d * 8);
+    if (!A)
+        return -2;
+    or_positions(m, acc, na, A);
+    int64_t clear = 0;
+    int par_err = 0;
+#pragma omp parallel reduction(+ : clear) reduction(| : par_err)
+    {
+        double* X = (double*)malloc((size_t)T * d * 8);
+        cand_buf tb;
+        if (!X || cb_init(&tb, m)) {
+            par_err = 1;
+        } else {
+#pragma omp for schedule(dynamic, 1024)
+            for (int64_t k = 0; k < count; k++) {
+                or_candidate(m, (uint64_t)(first + k), tb.a1, tb.b, tb.c, tb.s, NULL, NULL);
+                for (int t = 0; t < T; t++)
+                    or_eval_t(m, t, tb.a1, tb.b, tb.c, tb.s, X + (size_t)t * d);
+                int hit = 0;
+                for (int64_t j = 0; j < na && !hit; j++)
+                    for (int t = 0; t < T && !hit; t++)
+                        hit = pair_hit(m, X + (size_t)t * d, A + ((size_t)j * T + t) * d);
+                clear += !hit;
+            }
+            cb_free(&tb);
+        }
+        free(X);
+    }
+    free(A);
+    return par_err ? -2 : clear;
 }

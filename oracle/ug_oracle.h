@@ -93,11 +93,41 @@ int64_t or_pack_mt(const or_model* m, int64_t max_candidates, int stop_kind, int
  * starts at 2^16 and adapts on min(survivors, 2^16) (:1000-1003); rounds run while fewer
  * than `budget` subpath candidates were drawn; accept_rate / fill_rate > 0 arm the
  * windowed stops of :1008-1037.  Uniques are pos0[n0][T][d] with groups gid0[n0].
+ * rounds (may be NULL) receives up to rounds_cap records {start, size, survivors,
+ * admitted, fresh cells} (start counted from `first`), *nrounds the number of rounds.
  * Returns the number of admitted subpaths (idx/gid hold the first cap), or < 0. */
 int64_t or_pack_sub(const or_model* m, const double* pos0, const int32_t* gid0, int64_t n0,
                     uint64_t first, double budget, double accept_rate, double fill_rate,
                     int64_t* idx, int32_t* gid, int64_t cap, int64_t* attempts,
-                    int64_t* filled_new);
+                    int64_t* filled_new, int64_t* rounds, int64_t rounds_cap, int64_t* nrounds);
+
+/* Serial RSA consumed on the reference engines' round schedule (the unique phase of
+ * cuda/braid_cuda.cu:885-944 for rule 2, cuda/braid_cuda3d.cu:1446-1566 for rule 3).
+ * Round k holds candidates [start_k, start_k + size_k); its survivors are the candidates
+ * that touch no path accepted before start_k (the grid the round's kernel sees), admitted
+ * in index order against the up-to-date packing.  Sizes start at 2^16 and adapt on the
+ * survivor count (halve above 512 down to 4096, double below 64 up to 2^26).  Rule 2:
+ * rounds run while attempts < budget; the --until-accept-rate window (stop_rate > 0,
+ * clamp(30/rate, 2e7, 5e9) attempts, attempts > 1e6) records the adapted size and stops
+ * after the round.  Rule 3: the next round is sized before the current one is drained
+ * (while enqueued attempts < budget and no stop), the window records the drained round,
+ * and a stop still drains the round already sized.  max_n > 0 stops at the max_n-th
+ * accept; *decided = the first candidate whose fate was not decided (= *attempts unless
+ * max_n stopped mid-round).  rounds/nrounds as in or_pack_sub (fresh = newly occupied
+ * cells).  Deviation (documented): the reference's 3+1 kernel sees a grid that lags by at
+ * least one round; here every round sees the packing at its start.  Returns N or < 0. */
+int64_t or_pack_ref(const or_model* m, double budget, int rule, double stop_rate, int64_t max_n,
+                    int64_t* idx, int32_t* gid, int64_t cap, int64_t* attempts,
+                    int64_t* decided, int64_t* rounds, int64_t rounds_cap, int64_t* nrounds);
+
+/* Candidates in [first, first + count) that touch none of the candidates acc[0..na), by
+ * direct pairwise tests at every timestep (no grid): an independent count of a round's
+ * survivors against the paths accepted before it.  Returns the count or < 0. */
+int64_t or_count_clear(const or_model* m, int64_t first, int64_t count, const int64_t* acc,
+                       int64_t na);
+
+/* Canonical wrapped positions out[n][T][d] of candidates idx[0..n). */
+void or_positions(const or_model* m, const int64_t* idx, int64_t n, double* out);
 
 /* Brute-force check that acc[0..na) is exactly the serial RSA output over candidate
  * indices [0, M): (i) no two members collide, (ii) every non-member collides with an

@@ -40,9 +40,23 @@ CASES = {
 # Subpath phase with a windowed stop (these need >= 2e7 subpath candidates, minutes of
 # oracle time): name -> (d, T, seed, unique-phase candidates, accept_rate, fill_rate).
 # Stored as counts, a digest of the lists and their first/last entries.
+# The unique phase is serial RSA over a budget ("budget", the API's UG_STOP_BUDGET) or the
+# reference engine's round schedule over it ("ref", what the braid_ug front end runs).
+# The full subpath lists and the round records {start, size, survivors, admitted, fresh
+# cells} go to a .npz beside the .json.
 SUB_CASES = {
-    "sub_d2_T16_s1_accept2e-3": (2, 16, 1, 20000, 2e-3, 0.0),
-    "sub_d2_T16_s2_fill3e-5": (2, 16, 2, 20000, 0.0, 3e-5),
+    "sub_d2_T16_s1_accept2e-3": (2, 16, 1, 20000, 2e-3, 0.0, "budget"),
+    "sub_d2_T16_s2_fill3e-5": (2, 16, 2, 20000, 0.0, 3e-5, "budget"),
+    "subref_d2_T16_s1_accept2e-3": (2, 16, 1, 20000, 2e-3, 0.0, "ref"),
+}
+
+# Unique phase on the reference engine's round schedule with its --until-accept-rate window
+# (rule 2 = braid_cuda, 3 = braid_cuda3d): name -> (d, T, seed, budget, rate).
+REF_CASES = {
+    "ref_d2_T16_s1_rate2e-6": (2, 16, 1, 1e12, 2e-6),
+    "ref_d3_T8_s1_rate1.5e-6": (3, 8, 1, 1e12, 1.5e-6),
+    "ref_d2_T64_s1_rate1e-4": (2, 64, 1, 1e12, 1e-4),
+    "ref_d3_T32_s2_rate1e-4": (3, 32, 2, 1e12, 1e-4),
 }
 
 
@@ -60,18 +74,24 @@ def list_digest(idx, gid) -> str:
 def make_sub(name: str) -> dict:
     import numpy as np
 
-    d, T, seed, uniq, ar, fr = SUB_CASES[name]
+    d, T, seed, uniq, ar, fr, rule = SUB_CASES[name]
     m = pyoracle.Model(d, T, seed=seed)
-    idx, gid, att = m.pack(uniq)
-    pos = np.stack([m.positions(int(i)) for i in idx])
+    if rule == "ref":
+        r = m.pack_ref(uniq)
+        idx, gid, att = r["idx"], r["gid"], r["attempts"]
+    else:
+        idx, gid, att = m.pack(uniq)
+    pos = m.positions_of(idx)
     t0 = time.time()
-    sidx, sgid, satt, fresh = m.pack_sub(pos, gid, first=att, budget=1e15, accept_rate=ar, fill_rate=fr)
+    sidx, sgid, satt, fresh, rounds = m.pack_sub(pos, gid, first=att, budget=1e15, accept_rate=ar,
+                                                 fill_rate=fr, rounds=True)
     dt = time.time() - t0
     all_idx = list(idx) + list(sidx)
     all_gid = list(gid) + list(sgid)
     rec = {
-        "name": name, "dim": d, "T": T, "seed": seed, "unique_candidates": uniq,
+        "name": name, "dim": d, "T": T, "seed": seed, "unique_candidates": uniq, "unique_rule": rule,
         "accept_rate": ar, "fill_rate": fr, "n": int(len(idx)), "nsub": int(len(sidx)),
+        "unique_attempts": int(att),
         "attempts": int(att + satt), "sub_attempts": int(satt), "filled_new": int(fresh),
         "digest": list_digest(all_idx, all_gid),
         "head_idx": [int(x) for x in sidx[:64]], "head_gid": [int(x) for x in sgid[:64]],
@@ -80,7 +100,29 @@ def make_sub(name: str) -> dict:
     }
     with open(os.path.join(HERE, name + ".json"), "w") as f:
         json.dump(rec, f)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), uidx=np.asarray(idx, np.int64),
+                        ugid=np.asarray(gid, np.int32), sidx=np.asarray(sidx, np.int64),
+                        sgid=np.asarray(sgid, np.int32), rounds=np.asarray(rounds, np.int64))
     print(f"{name}: N={len(idx)} Nsub={len(sidx)} sub candidates={satt} ({dt:.1f} s)")
+    return rec
+
+
+def make_ref(name: str) -> dict:
+    d, T, seed, budget, rate = REF_CASES[name]
+    m = pyoracle.Model(d, T, seed=seed)
+    t0 = time.time()
+    r = m.pack_ref(budget, stop_rate=rate)
+    dt = time.time() - t0
+    rec = {
+        "name": name, "dim": d, "T": T, "seed": seed, "budget": budget, "rate": rate,
+        "rule": d, "attempts": int(r["attempts"]), "decided": int(r["decided"]),
+        "n": int(len(r["idx"])), "idx": [int(x) for x in r["idx"]],
+        "filled": int(r["rounds"][:, 4].sum()),
+        "rounds": [[int(v) for v in row] for row in r["rounds"]], "oracle_seconds": round(dt, 2),
+    }
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(rec, f)
+    print(f"{name}: N={rec['n']} attempts={rec['attempts']} rounds={len(rec['rounds'])} ({dt:.1f} s)")
     return rec
 
 
@@ -103,5 +145,5 @@ def make(name: str) -> dict:
 
 
 if __name__ == "__main__":
-    for n in sys.argv[1:] or list(CASES) + list(SUB_CASES):
-        make_sub(n) if n in SUB_CASES else make(n)
+    for n in sys.argv[1:] or list(CASES) + list(SUB_CASES) + list(REF_CASES):
+        make_sub(n) if n in SUB_CASES else make_ref(n) if n in REF_CASES else make(n)

@@ -86,35 +86,38 @@ def _check_curve(path, attempts, n, four=False):
 
 
 @pytest.mark.parametrize("seed", [1, 3])
-def test_braid_ug_engine_argv_equals_golden(gpu, tmp_path, seed):
+def test_braid_ug_engine_argv_equals_oracle(gpu, tmp_path, seed):
+    # --attempts 20000 without a window: one engine round of 2^16 candidates, as the
+    # reference's loop runs it (cuda/braid_cuda.cu:885,900)
     import ugpack
 
-    g = _gold(f"cfg1_d2_T16_s{seed}")
+    m = po.Model(2, 16, seed=seed)
+    r = m.pack_ref(20000)
     curve, dump = str(tmp_path / "c.csv"), str(tmp_path / "d.csv")
     cmd = ugpack.command(2, 16, attempts=20000.0, seed=seed, accept_rate=0.0, terms=16,
                          curve_path=curve) + ["--dump-params", dump]
     err = _run(cmd)
     n, att = map(int, DONE_RE.search(err).groups())
-    assert n == len(g["idx"]) and att == 20000
-    assert re.search(r"filled \d+/4096 cells", err)
+    assert n == len(r["idx"]) and att == r["attempts"] == 1 << 16
+    assert re.search(rf"filled {r['rounds'][:, 4].sum()}/4096 cells", err)
     _check_curve(curve, att, n)
-    _check_dump(dump, 2, 16, po.Model(2, 16, seed=seed), g["idx"])
+    _check_dump(dump, 2, 16, m, r["idx"])
 
 
 def test_braid_ug_terms2_legacy_dump(gpu, tmp_path):
     import ugpack
 
-    g = _gold("d2_T32_terms2_s1")
+    m = po.Model(2, 32, terms=2, seed=1)
+    r = m.pack_ref(50000)
     dump = str(tmp_path / "d.csv")
-    err = _run(ugpack.command(2, 32, attempts=float(g["candidates"]), seed=1, accept_rate=0.0,
+    err = _run(ugpack.command(2, 32, attempts=50000.0, seed=1, accept_rate=0.0,
                               curve_path=str(tmp_path / "c.csv")) + ["--dump-params", dump])
-    n, _ = map(int, DONE_RE.search(err).groups())
-    assert n == len(g["idx"])
+    n, att = map(int, DONE_RE.search(err).groups())
+    assert (n, att) == (len(r["idx"]), r["attempts"])
     rows = list(csv.DictReader(open(dump)))
     assert list(rows[0].keys()) == ["ax", "ay", "bx", "by", "ax2", "ay2", "fx", "fy"]
-    m = po.Model(2, 32, terms=2, seed=1)
     for i in (0, 1, len(rows) - 1):
-        cd = m.candidate(g["idx"][i])
+        cd = m.candidate(int(r["idx"][i]))
         assert float(rows[i]["ax2"]) == pytest.approx(cd.a1[0], abs=1e-9)
         assert float(rows[i]["bx"]) == cd.b[0][0] and float(rows[i]["by"]) == cd.b[1][0]
 
@@ -122,13 +125,13 @@ def test_braid_ug_terms2_legacy_dump(gpu, tmp_path):
 def test_braid_ug_subpaths_equals_golden(gpu, tmp_path):
     import ugpack
 
-    g = _gold("sub_d2_T16_s1_accept2e-3")
+    g = _gold("subref_d2_T16_s1_accept2e-3")
     curve, dump = str(tmp_path / "c.csv"), str(tmp_path / "d.csv")
     cmd = ugpack.command(2, 16, attempts=20000.0, seed=1, accept_rate=g["accept_rate"], terms=16,
                          subpaths=True, sub_attempts=1e15, curve_path=curve)
     err = _run(cmd + ["--dump-params", dump])
     n, att = map(int, DONE_RE.search(err).groups())
-    assert (n, att) == (g["n"], 20000)
+    assert (n, att) == (g["n"], g["unique_attempts"])
     m = re.search(r"sub done \((.+?)\): Nsub=(\d+) in (\d+) attempts", err)
     assert m.group(1) == "accept-rate"
     assert int(m.group(2)) == g["nsub"] and int(m.group(3)) == g["sub_attempts"]
@@ -145,7 +148,8 @@ def test_braid_ug3d_engine_argv(gpu, tmp_path, euclid):
     import ugpack
 
     m = po.Model(3, 12, seed=1, euclid=euclid)
-    want, _, att = m.pack(20000)
+    r = m.pack_ref(20000)
+    want, att = r["idx"], r["attempts"]
     curve, dump = str(tmp_path / "c.csv"), str(tmp_path / "d.csv")
     cmd = ugpack.command(3, 12, attempts=20000.0, seed=1, accept_rate=0.0, terms=12,
                          euclid=euclid, curve_path=curve) + ["--dump-params", dump]
@@ -162,8 +166,8 @@ def test_pack_api_equals_cli(gpu, tmp_path):
     import ugpack
 
     r = ugpack.pack(2, 24, terms=5, seed=3, attempts=30000)
-    g = _gold("d2_T24_terms5_s3")
-    assert list(r.idx) == g["idx"] and r.n == len(g["idx"]) and r.attempts == 30000
+    want = po.Model(2, 24, terms=5, seed=3).pack_ref(30000)
+    assert list(r.idx) == list(want["idx"]) and r.attempts == want["attempts"]
     assert r.params.shape == (r.n, 2, 1 + 3 * 4)
 
 

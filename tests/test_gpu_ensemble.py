@@ -41,12 +41,60 @@ def test_ensemble_equals_single_universe_packs(gpu, precision):
 
 
 def test_ensemble_is_independent_of_worker_count(gpu):
+    # destroyed handles hand their buffers to the next universes of other workers' streams;
+    # the accepted lists (digests) equal the oracle's for 1 and 12 workers
     import ugpack
 
     cfgs = _configs(ugpack, ugpack.PREC_SPLIT_BF16)
-    a = ugpack.ensemble(cfgs, threads=1)
-    b = ugpack.ensemble(cfgs, threads=12)
+    a, da = ugpack.ensemble(cfgs, threads=1, digests=True)
+    b, db = ugpack.ensemble(cfgs, threads=12, digests=True)
     assert [(s.n, s.attempts, s.filled) for s in a] == [(s.n, s.attempts, s.filled) for s in b]
+    want = [ugpack.abi.list_digest(po.Model(d, T, seed=s).pack(m)[0]) for d, T, s, m in MIX]
+    assert da == want and db == want
+
+
+def _mix_slice(ugpack, first: int, count: int, cands: float, precision: int):
+    """Universes first..first+count-1 of the 512-universe ensemble workload (bench.py
+    ensemble_configs): T from {16, 32, 64, 128} and dim from {2, 3} by a fixed-seed draw,
+    seed = index + 1."""
+    rng = np.random.default_rng(20260709)
+    Ts = rng.choice([16, 32, 64, 128], size=512)
+    ds = rng.choice([2, 3], size=512)
+    return [ugpack.pack_config(int(ds[i]), int(Ts[i]), seed=i + 1, max_attempts=cands, precision=precision)
+            for i in range(first, first + count)]
+
+
+def test_ensemble_workload_slice_at_full_size(gpu):
+    # the first 20 universes of the ensemble workload at its 10^7 candidates each (three of
+    # them 3+1 T=128), plus a 3+1 T=128 seed-1 universe whose first 2^20 candidates have an
+    # oracle golden list: every universe satisfies the serial-RSA characterization over its
+    # whole range, kept and discarded runs report the same digests
+    import json
+    import os
+
+    import ugpack
+
+    cfgs = _mix_slice(ugpack, 0, 20, 1e7, ugpack.PREC_SPLIT_BF16)
+    cfgs.append(ugpack.pack_config(3, 128, seed=1, max_attempts=1e7, precision=ugpack.PREC_SPLIT_BF16))
+    assert sum(1 for c in cfgs if (c.dim, c.T) == (3, 128)) == 4
+    print("ensemble slice: 21 universes x 1e7", flush=True)
+    status, packers, dig = ugpack.ensemble(cfgs, threads=8, keep=True, digests=True)
+    _, dig2 = ugpack.ensemble(cfgs, threads=8, digests=True)
+    assert dig == dig2
+    try:
+        for c, st, p, dg in zip(cfgs, status, packers, dig):
+            assert st.done and st.attempts == 10 ** 7
+            chk = p.check(0, st.attempts)
+            assert chk == {"colliding_uniques": 0, "unexplained_rejects": 0, "uniques": st.n,
+                           "rejects": st.attempts - st.n}, (c.dim, c.T, c.seed)
+            idx, _ = p.accepted()
+            assert ugpack.abi.list_digest(idx) == dg
+        g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cfg3_d3_T128_s1_prefix1M.json")))
+        idx, _ = packers[-1].accepted()
+        assert list(idx[idx < (1 << 20)]) == g["idx"]
+    finally:
+        for p in packers:
+            p.close()
 
 
 def test_ensemble_reports_a_failing_universe(gpu):
@@ -56,6 +104,9 @@ def test_ensemble_reports_a_failing_universe(gpu):
     cfgs.append(ugpack.pack_config(2, 16, terms=40, seed=1, max_attempts=100))  # terms > pool
     with pytest.raises(ugpack.UgError, match="universe 3"):
         ugpack.ensemble(cfgs, threads=2)
+    # with keep, the universes finished before the failure are released by the library
+    with pytest.raises(ugpack.UgError, match="universe 3"):
+        ugpack.ensemble(cfgs, threads=2, keep=True)
 
 
 def test_tuned_worker_count_is_one_of_the_choices(gpu):

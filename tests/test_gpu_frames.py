@@ -143,3 +143,45 @@ def test_paths_from_pack_equals_dump_parameters(gpu, d, T, terms, seed):
         got = frames.bake(dev, 0.5 * math.pi, 0.1 * math.pi, observer_path=1, front="fit")
         want = frames.bake(host, 0.5 * math.pi, 0.1 * math.pi, observer_path=1, front="fit")
         match_hits(got, want, 1e-5)
+
+
+def test_config5_frames_equal_numpy_histograms(gpu):
+    """3+1 F=256 packed to jamming (~3.8e5 worldlines), then three of the workload's 512
+    instants z_j = (j + 1/2) pi / 512 at 2048^2: each raster is numpy.histogram2d of the
+    worldlines' positions evaluated from their dump parameters with the reference's direct
+    expression (braidlab/corrdim.py:116-137), and holds exactly N."""
+    import torch
+
+    import ugpack
+    from ugpack import frames
+
+    print("cfg5 pack + frames", flush=True)
+    js = np.array([0, 255, 511])
+    z = (js + 0.5) * math.pi / 512
+    px = 2048
+    with ugpack.Packer(3, 256, seed=1, precision=ugpack.PREC_SPLIT_BF16,
+                       stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4) as p:
+        st = p.run()
+        n = st.n
+        paths = frames.Paths.from_packer(p)
+        pos, ras = frames.slices(paths, z, raster_px=px)
+        paths.close()
+        ras = ras.cpu().numpy()
+        pos = pos.cpu().numpy()
+        want = np.zeros((len(z), n, 3))
+        sz = np.sin(z)
+        for first in range(0, n, 16384):
+            par = p.params(first, min(16384, n - first))          # (k, 3, 1 + 3 nw)
+            a2, amp, frq, phs = par[:, :, 0], par[:, :, 1::3], par[:, :, 2::3], par[:, :, 3::3]
+            for j, zj in enumerate(z):
+                s = (amp * (np.sin(frq * zj + phs) - np.sin(phs))).sum(-1)
+                want[j, first:first + len(par)] = (a2 * sz[j] + s) / sz[j]
+    assert n > 300000
+    want = want - 2.0 * np.floor((want + 1.0) * 0.5)
+    for j in range(len(z)):
+        d = np.abs(pos[j] - want[j])
+        assert np.minimum(d, 2.0 - d).max() < 2e-6
+        h, _, _ = np.histogram2d(want[j][:, 1], want[j][:, 0], bins=px, range=[[-1, 1], [-1, 1]])
+        assert ras[j].sum() == n
+        # positions within float32 rounding of a bin edge may land in the neighbour bin
+        assert np.abs(h - ras[j]).sum() <= 2 * int(0.001 * n)

@@ -445,11 +445,23 @@ def test_characterization_check_on_config1(gpu):
     assert c == {"colliding_uniques": 0, "unexplained_rejects": 0, "uniques": n, "rejects": 20000 - n}
 
 
+def _check_whole_range(p, attempts, chunk=1 << 26):
+    """ug_pack_check over every consumed candidate, in chunks; the summed counts."""
+    tot = {"colliding_uniques": 0, "unexplained_rejects": 0, "uniques": 0, "rejects": 0}
+    for first in range(0, attempts, chunk):
+        c = p.check(first, min(chunk, attempts - first))
+        for k in ("unexplained_rejects", "rejects"):
+            tot[k] += c[k]
+        tot["colliding_uniques"] = c["colliding_uniques"]
+        tot["uniques"] = c["uniques"]
+    return tot
+
+
 @pytest.mark.parametrize("precision", ["split", "fp64"])
 def test_config3_to_jam_satisfies_serial_rsa_at_full_size(gpu, precision):
-    """3+1 T=128 packed to jamming (~5e7 candidates): no two of the ~6.4e4 accepted paths
-    collide, and every rejected candidate of the first 2^20 and of the last 2^22 touches a
-    path accepted before it."""
+    """3+1 T=128 packed to jamming (~5.6e7 candidates): no two of the ~6.4e4 accepted paths
+    collide, and every rejected candidate of the whole consumed range touches a path
+    accepted before it."""
     import ugpack
 
     prec = ugpack.PREC_SPLIT_BF16 if precision == "split" else ugpack.PREC_FP64
@@ -457,37 +469,30 @@ def test_config3_to_jam_satisfies_serial_rsa_at_full_size(gpu, precision):
     with ugpack.Packer(3, 128, seed=1, precision=prec, stop_kind=ugpack.STOP_FIXED_WINDOW,
                        stop_rate=1e-4) as p:
         st = p.run()
-        a = p.check(0, 1 << 20)
-        b = p.check(st.attempts - (1 << 22), 1 << 22)
+        c = _check_whole_range(p, st.attempts)
         idx, _ = p.accepted()
     assert st.done and st.n > 50000
     g = _gold("cfg3_d3_T128_s1_prefix1M")
     assert list(idx[idx < (1 << 20)]) == g["idx"]
-    for c in (a, b):
-        assert c["colliding_uniques"] == 0 and c["unexplained_rejects"] == 0
-        assert c["uniques"] == st.n
-    assert a["rejects"] == (1 << 20) - len(g["idx"])
-    assert b["rejects"] == (1 << 22) - int(np.sum(idx >= st.attempts - (1 << 22)))
+    assert c == {"colliding_uniques": 0, "unexplained_rejects": 0, "uniques": st.n,
+                 "rejects": st.attempts - st.n}
 
 
-@pytest.mark.parametrize("precision", ["split", "fp64"])
-def test_config5_to_jam_satisfies_serial_rsa_at_full_size(gpu, precision):
-    """3+1 T=256 packed to jamming (~3e8 candidates, ~3.8e5 paths): the same two properties
-    on all accepted paths and on the last 2^21 candidates; both forms give the same list."""
+def test_config5_to_jam_satisfies_serial_rsa_at_full_size(gpu):
+    """3+1 T=256 packed to jamming (~3.3e8 candidates, ~3.8e5 paths) in both forms: the same
+    accepted list, and both properties over every accepted path and every consumed
+    candidate."""
     import ugpack
 
-    print(f"cfg5 jam check {precision}", flush=True)
-    prec = ugpack.PREC_SPLIT_BF16 if precision == "split" else ugpack.PREC_FP64
-    with ugpack.Packer(3, 256, seed=1, precision=prec,
-                       stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4) as p:
-        st = p.run()
-        c = p.check(st.attempts - (1 << 21), 1 << 21)
-        idx, _ = p.accepted()
-    assert st.done and st.n > 300000
-    assert c["colliding_uniques"] == 0 and c["unexplained_rejects"] == 0 and c["uniques"] == st.n
-    h = hashlib.sha256(np.asarray(idx, "<i8").tobytes()).hexdigest()
-    _CFG5_DIGEST.setdefault("digest", h)
-    assert h == _CFG5_DIGEST["digest"], "split-BF16 and FP64 forms accepted different lists"
-
-
-_CFG5_DIGEST: dict = {}
+    lists = {}
+    for precision, prec in (("split", ugpack.PREC_SPLIT_BF16), ("fp64", ugpack.PREC_FP64)):
+        print(f"cfg5 jam check {precision}", flush=True)
+        with ugpack.Packer(3, 256, seed=1, precision=prec,
+                           stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4) as p:
+            st = p.run()
+            c = _check_whole_range(p, st.attempts)
+            lists[precision], _ = p.accepted()
+        assert st.done and st.n > 300000
+        assert c == {"colliding_uniques": 0, "unexplained_rejects": 0, "uniques": st.n,
+                     "rejects": st.attempts - st.n}
+    assert np.array_equal(lists["split"], lists["fp64"]), "the two forms accepted different lists"

@@ -463,3 +463,176 @@ def test_subpath_round_schedule_and_budget():
     assert a3 == a2 and filled > 0 and np.all(np.diff(sidx) > 0)
     assert np.all(sidx >= att) and np.all(sidx < att + a3)
 
+
+
+# ---------------------------------------------------------------- reference round schedule
+def _halve_double(batch, surv):
+    """Next round size from a round's survivors (cuda/braid_cuda.cu:920-924)."""
+    got = min(surv, 1 << 16)
+    if got > 512 and batch > 4096:
+        return batch // 2
+    if got < 64 and batch < (1 << 26):
+        return batch * 2
+    return batch
+
+
+def _window_verdicts(entries, attempts_after, rate):
+    """Per round: does the --until-accept-rate window fire (cuda/braid_cuda.cu:585-590,
+    930-943)?  The window after entry k is the shortest suffix of entries[..k] whose
+    attempts reach the target (all of them while the total is below it); it fires when it
+    reaches the target, more than 10^6 candidates were consumed, and its acceptance rate
+    is below `rate`."""
+    target = int(min(5e9, max(2e7, 30.0 / rate)))
+    out = []
+    for k in range(len(entries)):
+        att = adm = 0
+        for a, n in reversed(entries[:k + 1]):
+            att += a
+            adm += n
+            if att >= target:
+                break
+        out.append(att >= target and attempts_after[k] > 1_000_000 and adm / att < rate)
+    return out
+
+
+def _ref_cells(X, T):
+    """(t, cell) keys of positions X (T, d) with the reference's host cell index
+    floor((v + 1)/cell), cell = 2/T (cuda/braid_cuda.cu:699,868-870)."""
+    c = np.floor((X + 1.0) / (2.0 / T)).astype(np.int64)
+    key = np.arange(T, dtype=np.int64)
+    for a in range(X.shape[1]):
+        key = key * (T + 1) + c[:, a]
+    return key
+
+
+@pytest.mark.parametrize("d,T,rate", [(2, 16, 2e-6), (3, 8, 1.5e-6)])
+def test_reference_round_schedule_by_hand(d, T, rate):
+    # The unique phase consumed in the reference engine's rounds with --until-accept-rate:
+    # every field of every round record is recomputed here from outside the schedule code.
+    m = po.Model(d, T, seed=1)
+    r = m.pack_ref(1e12, stop_rate=rate)
+    rec, acc = r["rounds"], r["idx"]
+    start, size, surv, adm, fresh = rec.T
+    # contiguous rounds from 0; attempts = their sum; nothing left undecided
+    assert start[0] == 0 and np.array_equal(start[1:], np.cumsum(size)[:-1])
+    assert r["attempts"] == size.sum() == r["decided"]
+    # the accepted list is serial RSA over the consumed range, whatever the round sizes
+    plain, _ = m.pack_mt(r["attempts"], 8)
+    assert np.array_equal(plain, acc) and np.array_equal(r["gid"], np.arange(len(acc)))
+    # admitted per round from the accepted list
+    assert np.array_equal(adm, [np.sum((acc >= s) & (acc < s + z)) for s, z in zip(start, size)])
+    # survivors: candidates touching none of the paths accepted before the round, by pairwise
+    # tests without a grid (rounds up to 2^21 candidates; later ones have doubled from them)
+    for k in range(len(rec)):
+        if size[k] <= 1 << 21:
+            assert surv[k] == m.count_clear(int(start[k]), int(size[k]), acc[acc < start[k]]), k
+        assert surv[k] >= adm[k]
+    # sizes: braid_cuda adapts the batch after each round; braid_cuda3d sizes round k+1
+    # before round k is drained, so its sizes lag one round (cuda/braid_cuda3d.cu:1446-1452)
+    b, want = 1 << 16, []
+    if d == 2:
+        for k in range(len(rec)):
+            want.append(b)
+            b = _halve_double(b, int(surv[k]))
+    else:
+        want.append(b)
+        for k in range(len(rec) - 1):
+            want.append(b)
+            b = _halve_double(b, int(surv[k]))
+    assert list(size) == want
+    # stop: braid_cuda's window records the adapted batch and ends after the firing round;
+    # braid_cuda3d's records the drained round and drains the one already sized after it
+    if d == 2:
+        entries = [(_halve_double(int(want[k]), int(surv[k])), int(adm[k])) for k in range(len(rec))]
+    else:
+        entries = [(int(size[k]), int(adm[k])) for k in range(len(rec))]
+    fires = _window_verdicts(entries, np.cumsum(size), rate)
+    last = len(rec) - 1 if d == 2 else len(rec) - 2
+    assert fires[last] and not any(fires[:last])
+    # fresh cells per round: (t, cell) buckets first occupied by the round's accepts, in
+    # admission order (insertPath, cuda/braid_cuda.cu:865-879)
+    seen, per_round = set(), []
+    X = m.positions_of(acc)
+    for k in range(len(rec)):
+        new = 0
+        for i in np.nonzero((acc >= start[k]) & (acc < start[k] + size[k]))[0]:
+            keys = set(_ref_cells(X[i], T).tolist())
+            new += len(keys - seen)
+            seen |= keys
+        per_round.append(new)
+    assert list(fresh) == per_round and fresh.sum() == len(seen)
+
+
+def test_reference_schedule_budget_and_max_n():
+    # without a window the last round overshoots the budget (the loop tests attempts <
+    # budget before each round); max_n ends the admissions mid-round
+    m = po.Model(2, 16, seed=2)
+    r = m.pack_ref(90000)
+    assert list(r["rounds"][:, 1]) == [1 << 16, 1 << 15] and r["attempts"] == 3 << 15
+    assert np.array_equal(r["idx"], m.pack(r["attempts"])[0])
+    r3 = m.pack_ref(90000, rule=3)
+    assert list(r3["rounds"][:, 1]) == [1 << 16, 1 << 16]
+    assert r3["attempts"] == 1 << 17 and np.array_equal(r3["idx"], m.pack(1 << 17)[0])
+    r = m.pack_ref(10**7, max_n=40)
+    assert len(r["idx"]) == 40 and r["decided"] == r["idx"][-1] + 1
+    assert r["attempts"] == r["rounds"][:, 1].sum() > r["decided"]
+
+
+@pytest.mark.parametrize("name", ["sub_d2_T16_s1_accept2e-3", "sub_d2_T16_s2_fill3e-5",
+                                  "subref_d2_T16_s1_accept2e-3"])
+def test_subpath_windows_and_fresh_cells_by_hand(name):
+    # The subpath fixtures (the oracle's windowed stops, minutes of oracle time to rerun) carry
+    # their full lists and round records; every record is recomputed here: admissions from the
+    # list, fresh cells from the positions, the sizes from the survivor counts and both
+    # windows (cuda/braid_cuda.cu:1000-1037) from the records.
+    import hashlib
+    import json
+    import os
+
+    gdir = os.path.join(os.path.dirname(__file__), "golden")
+    g = json.load(open(os.path.join(gdir, name + ".json")))
+    z = np.load(os.path.join(gdir, name + ".npz"))
+    uidx, ugid, sidx, sgid, rec = z["uidx"], z["ugid"], z["sidx"], z["sgid"], z["rounds"]
+    m = po.Model(g["dim"], g["T"], seed=g["seed"])
+    # the unique phase is the oracle's (budget or engine schedule over unique_candidates)
+    if g["unique_rule"] == "ref":
+        r = m.pack_ref(g["unique_candidates"])
+        want_u, att = r["idx"], r["attempts"]
+    else:
+        want_u, _, att = m.pack(g["unique_candidates"])
+    assert np.array_equal(uidx, want_u) and att == g["unique_attempts"]
+    h = hashlib.sha256()
+    h.update(np.concatenate([uidx, sidx]).astype("<i8").tobytes())
+    h.update(np.concatenate([ugid, sgid]).astype("<i4").tobytes())
+    assert h.hexdigest() == g["digest"] and len(sidx) == g["nsub"]
+    start, size, surv, adm, fresh = rec.T
+    assert start[0] == 0 and np.array_equal(start[1:], np.cumsum(size)[:-1])
+    assert size.sum() == g["sub_attempts"]
+    first = g["unique_attempts"]
+    assert np.array_equal(adm, [np.sum((sidx >= first + s) & (sidx < first + s + k))
+                                for s, k in zip(start, size)])
+    b, want = 1 << 16, []
+    for k in range(len(rec)):
+        want.append(b)
+        b = _halve_double(b, int(surv[k]))
+    assert list(size) == want
+    adapted = [_halve_double(int(size[k]), int(surv[k])) for k in range(len(rec))]
+    att_after = np.cumsum(size)
+    acc_f = _window_verdicts(list(zip(adapted, adm.tolist())), att_after, g["accept_rate"]) \
+        if g["accept_rate"] > 0 else [False] * len(rec)
+    fill_f = _window_verdicts(list(zip(adapted, fresh.tolist())), att_after, g["fill_rate"]) \
+        if g["fill_rate"] > 0 else [False] * len(rec)
+    fired = [a or f for a, f in zip(acc_f, fill_f)]
+    assert fired[-1] and not any(fired[:-1])
+    # fresh cells: buckets first occupied, uniques first, then the subpaths in admission order
+    X = m.positions_of(np.concatenate([uidx, sidx]))
+    seen = set()
+    for i in range(len(uidx)):
+        seen |= set(_ref_cells(X[i], g["T"]).tolist())
+    per_round = np.zeros(len(rec), np.int64)
+    rnd = np.searchsorted(first + start, sidx, side="right") - 1
+    for j in range(len(sidx)):
+        keys = set(_ref_cells(X[len(uidx) + j], g["T"]).tolist())
+        per_round[rnd[j]] += len(keys - seen)
+        seen |= keys
+    assert np.array_equal(per_round, fresh) and fresh.sum() == g["filled_new"]

@@ -136,6 +136,8 @@ void release(Pack* h) {
     free_dev(h->d_glist2);
     free_dev(h->d_gsurv);
     free_dev(h->d_rscount);
+    free_dev(h->d_rlist);
+    h->rlist_cap = 0;
     release_events(h);
     cache_host_free(h->h_pinned);
     h->h_pinned = nullptr;
@@ -177,8 +179,10 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
         return fail(UG_EINVAL, "error: unknown precision %d", cfg.precision);
     if (cfg.stop_kind < UG_STOP_BUDGET || cfg.stop_kind > UG_STOP_REF_WINDOW)
         return fail(UG_EINVAL, "error: unknown stop rule %d", cfg.stop_kind);
-    if (cfg.stop_kind != UG_STOP_BUDGET && !(cfg.stop_rate > 0))
+    if (cfg.stop_kind == UG_STOP_FIXED_WINDOW && !(cfg.stop_rate > 0))
         return fail(UG_EINVAL, "error: the stop rule needs stop_rate > 0");
+    if (cfg.stop_kind == UG_STOP_REF_WINDOW && cfg.stop_rate < 0)
+        return fail(UG_EINVAL, "error: --until-accept-rate must not be negative");
     if (cfg.stop_batch <= 0)
         cfg.stop_batch = 1 << 20;
     if (cfg.stop_window <= 0)
@@ -302,6 +306,8 @@ extern "C" void ug_pack_destroy(ug_pack* hp) {
     Pack* h = (Pack*)hp;
     (void)hipStreamSynchronize(h->stream);
     release(h);
+    if (h->own_stream)
+        (void)hipStreamDestroy(h->stream);
     delete h;
 }
 
@@ -363,6 +369,140 @@ static int64_t stop_limit(const Pack* h, int64_t sub) {
     return 0;
 }
 
+// One round of the reference schedule: candidates [base, base + size) are tested against
+// the table as the round found it (pieces of at most UG_SUB_MAX, no inserts in between), so
+// *S is the round's survivor count the engines adapt on (the `got` of
+// cuda/braid_cuda.cu:894-897); then the survivors are admitted in index order.
+static int32_t ref_round(Pack* h, uint64_t base, int64_t size, int64_t* S_out) {
+    const bool empty = h->n + h->nsub == 0;
+    int64_t S = 0;
+    int32_t rc = UG_OK;
+    for (int64_t off = 0; off < size && rc == UG_OK; off += UG_SUB_MAX) {
+        const int64_t piece = std::min<int64_t>(UG_SUB_MAX, size - off);
+        if (S > UG_SUB_MAX)
+            return fail(UG_EOVERFLOW, "a round of %lld candidates left more than %d survivors",
+                        (long long)size, UG_SUB_MAX);
+        if (S + piece > h->rlist_cap) {
+            const int64_t nc = std::max<int64_t>(2 * UG_SUB_MAX, S + piece);
+            int64_t* nl = (int64_t*)cache_alloc((size_t)nc * 8);
+            if (!nl)
+                return fail(UG_ENOMEM, "round survivor list of %lld entries", (long long)nc);
+            if (S > 0)
+                rc = hip_check(hipMemcpyAsync(nl, h->d_rlist, (size_t)S * 8, hipMemcpyDeviceToDevice, h->stream),
+                               "round list copy");
+            if (!rc)
+                rc = hip_check(hipStreamSynchronize(h->stream), "round list sync");
+            cache_free(h->d_rlist);
+            h->d_rlist = nl;
+            h->rlist_cap = nc;
+            if (rc)
+                return rc;
+        }
+        int64_t Sp = piece;
+        if (empty)
+            rc = all_survive(h, base + (uint64_t)off, piece, h->d_rlist + S);
+        else
+            rc = filter_candidates(h, base + (uint64_t)off, piece, nullptr, h->d_rlist + S, &Sp, true);
+        S += Sp;
+    }
+    if (rc)
+        return rc;
+    if (S > UG_SUB_MAX)
+        return fail(UG_EOVERFLOW, "a round of %lld candidates left more than %d survivors", (long long)size,
+                    UG_SUB_MAX);
+    *S_out = S;
+    int64_t acc = 0;
+    return S > 0 ? resolve_and_insert(h, h->d_rlist, S, &acc) : UG_OK;
+}
+
+// The unique phase on the reference engines' round schedule (cuda/braid_cuda.cu:885-944 for
+// 2+1, cuda/braid_cuda3d.cu:1446-1566 for 3+1): rounds start at 2^16 candidates and adapt
+// toward a few hundred survivors (halve above 512 down to 4096, double below 64 up to 2^26);
+// the --until-accept-rate window (clamp(30/rate, 2e7, 5e9) attempts, attempts > 1e6) stops
+// after a round.  The 2+1 engine's window records the adapted batch and its loop runs while
+// attempts < budget; the 3+1 engine sizes round k+1 before round k is drained (while the
+// sized candidates stay below the budget and the window has not fired), records the drained
+// round, and drains the sized round after a stop.  Every round here sees the table as it
+// stood at its start (the 3+1 engine's kernel sees a grid that lags by at least a round,
+// :1247-1249, which can only raise its survivor count).  Survivors are admitted in index
+// order, so the accepted list is serial RSA over the consumed candidates and the schedule
+// depends only on it: both precision forms run the same rounds.
+static int32_t run_ref(Pack* h, int64_t call_end) {
+    const ug_pack_config& cfg = h->cfg;
+    const double budget = cfg.max_attempts;
+    const int64_t target =
+        cfg.stop_rate > 0 ? (int64_t)std::min(5e9, std::max(2e7, 30.0 / cfg.stop_rate)) : 0;
+    const bool lag = h->d == 3;
+    if (h->ref_next < 0) {
+        h->ref_batch = 1 << 16;
+        h->ref_next = h->ref_batch;
+        h->ref_enq = h->ref_batch;
+    }
+    int32_t rc = UG_OK;
+    while (!h->done && rc == UG_OK) {
+        if (h->max_n > 0 && h->n >= h->max_n) {
+            h->done = true;
+            break;
+        }
+        if (!lag && (double)h->attempts >= budget) {
+            h->done = true;
+            break;
+        }
+        if (lag && h->ref_next == 0) {
+            h->done = true;
+            break;
+        }
+        if (h->attempts >= call_end)
+            break;  // rounds are never split between calls
+        int64_t size = h->ref_batch;
+        if (lag) {
+            size = h->ref_next;
+            h->ref_next = 0;
+            if ((double)h->ref_enq < budget && !h->ref_stop) {
+                h->ref_next = h->ref_batch;
+                h->ref_enq += h->ref_next;
+            }
+        }
+        const int64_t start = h->attempts, n0 = h->n;
+        const unsigned long long f0 = h->filled;
+        int64_t S = 0;
+        rc = ref_round(h, (uint64_t)start, size, &S);
+        if (rc)
+            break;
+        const int64_t adm = h->n - n0;
+        h->rounds.push_back({start, size, S, adm, (int64_t)(h->filled - f0)});
+        h->survivors += S;
+        if (h->max_n > 0 && h->n >= h->max_n)
+            h->undecided_from = h->acc_host[(size_t)h->max_n - 1] + 1;
+        h->attempts += size;
+        const int64_t got = std::min<int64_t>(S, 1 << 16);  // survivor buffer, cuda/braid_cuda.cu:760
+        if (got > 512 && h->ref_batch > 4096)
+            h->ref_batch /= 2;
+        else if (got < 64 && h->ref_batch < (int64_t(1) << 26))
+            h->ref_batch *= 2;
+        rc = record_curve(h);
+        if (target) {
+            const int64_t e = lag ? size : h->ref_batch;
+            h->win.push_back({e, adm});
+            h->win_att += e;
+            h->win_adm += adm;
+            while (h->win.size() > 1 && h->win_att - h->win.front().first >= target) {
+                h->win_att -= h->win.front().first;
+                h->win_adm -= h->win.front().second;
+                h->win.pop_front();
+            }
+            if (h->win_att >= target && h->attempts > 1000000 &&
+                (double)h->win_adm / (double)h->win_att < cfg.stop_rate) {
+                if (lag)
+                    h->ref_stop = true;
+                else
+                    h->done = true;
+            }
+        }
+    }
+    return rc;
+}
+
 extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_status* st) {
     if (!hp)
         return fail(UG_EINVAL, "ug_pack_run: NULL handle");
@@ -373,10 +513,15 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
     const ug_pack_config& cfg = h->cfg;
     const int64_t budget = (int64_t)std::min(cfg.max_attempts, 9.2e18);
     const int64_t call_end = max_candidates > 0 ? h->attempts + max_candidates : INT64_MAX;
+    if (cfg.stop_kind == UG_STOP_REF_WINDOW) {
+        int32_t rc = run_ref(h, call_end);
+        if (rc == UG_OK)
+            rc = hip_check(hipStreamSynchronize(h->stream), "run sync");
+        h->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        fill_status(h, st);
+        return rc;
+    }
     const int64_t sub_cap = std::min<int64_t>(UG_SUB_MAX, cfg.stop_kind == UG_STOP_FIXED_WINDOW ? cfg.stop_batch : UG_SUB_MAX);
-    const int64_t win_target = cfg.stop_kind == UG_STOP_REF_WINDOW
-                                   ? (int64_t)std::min(5e9, std::max(2e7, 30.0 / cfg.stop_rate))
-                                   : 0;
     int32_t rc = UG_OK;
     // The size of the sub-batch starting at `from`: h->sub, cut at the budget, the call's end
     // and (fixed window) the next stop-rule boundary; 0 when none may start there.
@@ -433,10 +578,9 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         ahead.base = base + (uint64_t)sub;
         // Only in the steady state (full-size sub-batches, few survivors): earlier, sizes still
         // grow from the measured rates and a sub-batch often needs several resolve chunks,
-        // which would void the run ahead.  The reference window's verdict depends on the
-        // sub-batch sizes, so that rule keeps the plain loop.
+        // which would void the run ahead.
         const bool steady = h->sub >= sub_cap && h->last_S <= UG_S_MAX / 4;
-        ahead.count = (cfg.stop_kind == UG_STOP_REF_WINDOW || !steady) ? 0 : size_at(h->attempts + sub);
+        ahead.count = steady ? size_at(h->attempts + sub) : 0;
         ahead.out = cur_list == h->d_list ? h->d_listb : h->d_list;
         ahead.dcount = h->d_spec_count;
         ahead.stop_lim = stop_limit(h, sub);
@@ -444,7 +588,6 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         int64_t S = 0;
         const int64_t rechecked_before = h->rechecked;
         int64_t acc = 0;
-        const int64_t n_before = h->n;
         rc = resolve_and_insert(h, cur_list, 0, &acc, h->d_count + 1, &ahead);
         if (rc)
             break;
@@ -469,20 +612,6 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
             while (next * 2 <= (int64_t)want && next * 2 <= sub_cap)
                 next *= 2;
             h->sub = next;
-        }
-        if (win_target) {
-            const int64_t adm = h->n - n_before;
-            h->win.push_back({sub, adm});
-            h->win_att += sub;
-            h->win_adm += adm;
-            while (h->win.size() > 1 && h->win_att - h->win.front().first >= win_target) {
-                h->win_att -= h->win.front().first;
-                h->win_adm -= h->win.front().second;
-                h->win.pop_front();
-            }
-            if (h->win_att >= win_target && h->attempts > 1000000 &&
-                (double)h->win_adm / (double)h->win_att < cfg.stop_rate)
-                h->done = true;
         }
         rc = record_curve(h);
     }
@@ -567,6 +696,19 @@ extern "C" int32_t ug_pack_curve(const ug_pack* hp, int64_t* rows4, int64_t cap,
     return UG_OK;
 }
 
+extern "C" int32_t ug_pack_rounds(const ug_pack* hp, int64_t* rows5, int64_t cap, int64_t* n) {
+    if (!hp)
+        return fail(UG_EINVAL, "ug_pack_rounds: NULL handle");
+    const Pack* h = (const Pack*)hp;
+    if (n)
+        *n = (int64_t)h->rounds.size();
+    const int64_t k = std::min<int64_t>(cap, (int64_t)h->rounds.size());
+    for (int64_t i = 0; i < k && rows5; i++)
+        for (int j = 0; j < 5; j++)
+            rows5[i * 5 + j] = h->rounds[(size_t)i][(size_t)j];
+    return UG_OK;
+}
+
 extern "C" int32_t ug_eval_candidates(const ug_pack* hp, const int64_t* dev_idx, int64_t count, int32_t precision,
                                       double* dev_out) {
     if (!hp || (count > 0 && (!dev_idx || !dev_out)))
@@ -588,10 +730,11 @@ extern "C" int32_t ug_pack_check(ug_pack* hp, uint64_t first, int64_t count, int
     if (!hp || !out)
         return fail(UG_EINVAL, "ug_pack_check: NULL argument");
     Pack* h = (Pack*)hp;
-    if (count < 0 || first + (uint64_t)count > (uint64_t)h->attempts)
-        return fail(UG_EINVAL, "ug_pack_check: candidates [%llu, %llu) beyond the %lld consumed",
+    const int64_t decided = h->undecided_from >= 0 ? h->undecided_from : h->attempts;
+    if (count < 0 || first + (uint64_t)count > (uint64_t)decided)
+        return fail(UG_EINVAL, "ug_pack_check: candidates [%llu, %llu) beyond the %lld decided",
                     (unsigned long long)first, (unsigned long long)(first + (uint64_t)count),
-                    (long long)h->attempts);
+                    (long long)decided);
     unsigned long long* d_out = nullptr;
     int32_t rc = hip_check(hipMalloc((void**)&d_out, 32), "check counters");
     if (!rc)
@@ -649,12 +792,23 @@ extern "C" int32_t ug_pack_run_sub(ug_pack* hp, const ug_sub_config* sc, int64_t
         return fail(UG_EINVAL, "error: --sub-terms %d differs from --terms %d (not supported)", sc->sub_terms,
                     h->cfg.terms);
     if (h->d_gflag == nullptr) {
-        if (!(h->d_gflag = (int*)cache_alloc((size_t)UG_SUB_MAX * 4)) ||
-            !(h->d_glist = (int*)cache_alloc((size_t)UG_SUB_MAX * 4)) ||
-            !(h->d_glist2 = (int*)cache_alloc((size_t)UG_SUB_MAX * 4)) ||
-            !(h->d_gsurv = (int*)cache_alloc((size_t)UG_S_MAX * 4)) ||
-            !(h->d_rscount = (int*)cache_alloc(16)))
+        // all or nothing: d_gflag marks the phase as started, so it is set last
+        int *gflag = (int*)cache_alloc((size_t)UG_SUB_MAX * 4), *glist = (int*)cache_alloc((size_t)UG_SUB_MAX * 4);
+        int *glist2 = (int*)cache_alloc((size_t)UG_SUB_MAX * 4), *gsurv = (int*)cache_alloc((size_t)UG_S_MAX * 4);
+        int* rscount = (int*)cache_alloc(16);
+        if (!gflag || !glist || !glist2 || !gsurv || !rscount) {
+            cache_free(gflag);
+            cache_free(glist);
+            cache_free(glist2);
+            cache_free(gsurv);
+            cache_free(rscount);
             return fail(UG_ENOMEM, "ug_pack_run_sub: allocation failed");
+        }
+        h->d_glist = glist;
+        h->d_glist2 = glist2;
+        h->d_gsurv = gsurv;
+        h->d_rscount = rscount;
+        h->d_gflag = gflag;
         h->done = true;  // the unique phase ends where the subpath phase starts
     }
     const auto t0 = std::chrono::steady_clock::now();

@@ -11,10 +11,14 @@
 // Everything runs through the C ABI of libugpack.so (include/ugpack.h); a failing call
 // prints ug_last_error() and exits 1, like the reference's CK().
 //
+// Candidates are consumed on the engines' round schedule (rounds of 2^16 adapted on their
+// survivors; the last round may pass --attempts) with their --until-accept-rate window, so
+// the done: line's attempts count the same rounds (UG_STOP_REF_WINDOW).
+//
 // Flags beyond the reference's: --precision fp64|split (default split: split-BF16 MFMA
-// prefilter + FP64 recheck; both give the same accepts), --fixed-window (the fixed rule:
-// stop at the first 2^20 boundary where the last 2^22 candidates hold fewer than
-// rate * 2^22 accepts, rate from --until-accept-rate).
+// prefilter + FP64 recheck; both give the same accepts and rounds), --fixed-window (the
+// fixed rule: stop at the first 2^20 boundary where the last 2^22 candidates hold fewer
+// than rate * 2^22 accepts, rate from --until-accept-rate).
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -260,7 +264,9 @@ int main(int argc, char** argv) {
     cfg.pin_sin1 = a.pin ? 1 : 0;
     cfg.max_attempts = a.budget;
     cfg.stop_rate = a.accept;
-    cfg.stop_kind = a.accept > 0 ? (a.fixed_window ? UG_STOP_FIXED_WINDOW : UG_STOP_REF_WINDOW) : UG_STOP_BUDGET;
+    // the engines' round schedule (a last round past --attempts included), and with
+    // --until-accept-rate their window; --fixed-window swaps in the fixed 2^20/2^22 rule
+    cfg.stop_kind = a.fixed_window && a.accept > 0 ? UG_STOP_FIXED_WINDOW : UG_STOP_REF_WINDOW;
     cfg.stop_batch = 1 << 20;
     cfg.stop_window = 1 << 22;
     cfg.max_n = a.max_n;

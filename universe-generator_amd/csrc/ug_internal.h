@@ -104,6 +104,7 @@ struct Pack {
     ug_pack_config cfg;
     std::shared_ptr<const ModelTables> tabs;
     hipStream_t stream;
+    bool own_stream = false;  // the handle destroys its stream (ensemble handles kept for the caller)
     int device;
     int d, T, nw, npairs, modmax;
     bool full;
@@ -189,6 +190,15 @@ struct Pack {
     ug_kernel_stats kstats[2] = {};
     std::deque<std::pair<int64_t, int64_t>> win;
     int64_t win_att = 0, win_adm = 0;
+    // reference round schedule (UG_STOP_REF_WINDOW): the engines' adaptive batch, the 3+1
+    // engine's already-sized next round (-1 before the first round, 0 = none) and the
+    // candidates it has sized, and whether its window fired (the sized round still runs)
+    int64_t ref_batch = 1 << 16, ref_next = -1, ref_enq = 0;
+    bool ref_stop = false;
+    int64_t* d_rlist = nullptr;  // a round's survivors (all pieces of the round)
+    int64_t rlist_cap = 0;
+    std::vector<std::array<int64_t, 5>> rounds;  // {start, size, survivors, admitted, fresh}
+    int64_t undecided_from = -1;  // max_n reached mid-round: candidates from here are untested
 };
 
 int32_t fail(int32_t code, const char* fmt, ...);

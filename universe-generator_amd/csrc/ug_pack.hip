@@ -578,6 +578,7 @@ __global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t fir
     __shared__ int tsl[64];
     constexpr int R = D == 2 ? 3 : 9;
     constexpr int PER_ROUND = 64 / R;
+    unsigned long long nviol = 0, nitems = 0;  // this block's counts (one atomic each at the end)
     for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
         uint64_t idx;
         int64_t self = -1, limit = n;
@@ -657,12 +658,16 @@ __global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t fir
             }
             __syncthreads();
         }
-        if (lane == 0) {
-            if (mode == 0 ? hit : !hit)
-                atomicAdd(&out[mode], 1ull);
-            atomicAdd(&out[2 + mode], 1ull);
-        }
+        if (mode == 0 ? hit : !hit)
+            nviol++;
+        nitems++;
         __syncthreads();
+    }
+    if (lane == 0) {
+        if (nviol)
+            atomicAdd(&out[mode], nviol);
+        if (nitems)
+            atomicAdd(&out[2 + mode], nitems);
     }
 }
 

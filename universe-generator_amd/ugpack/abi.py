@@ -18,7 +18,7 @@ from ._build import LIB_PATH
 UG_OK, UG_EINVAL, UG_ENOMEM, UG_EHIP, UG_EOVERFLOW, UG_ESTATE = 0, -1, -2, -3, -4, -5
 PREC_FP64, PREC_SPLIT_BF16 = 0, 1
 STOP_BUDGET, STOP_FIXED_WINDOW, STOP_REF_WINDOW = 0, 1, 2
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 
 class UgError(RuntimeError):
@@ -101,13 +101,15 @@ def lib() -> C.CDLL:
         "ug_pack_params": ([vp, i64, i64, vp], i32),
         "ug_pack_positions": ([vp, vp, i64], i32),
         "ug_pack_curve": ([vp, vp, i64, C.POINTER(i64)], i32),
+        "ug_pack_rounds": ([vp, vp, i64, C.POINTER(i64)], i32),
         "ug_eval_candidates": ([vp, vp, i64, i32, vp], i32),
         "ug_pack_profile": ([vp, i32], i32),
         "ug_pack_kernel_stats": ([vp, i32, C.POINTER(KernelStats)], i32),
         "ug_pack_destroy": ([vp], None),
         "ug_cache_trim": ([], i32),
         "ug_pack_check": ([vp, C.c_uint64, i64, vp], i32),
-        "ug_ensemble_run": ([C.POINTER(PackConfig), i32, i32, C.POINTER(vp), C.POINTER(PackStatus)], i32),
+        "ug_ensemble_run": ([C.POINTER(PackConfig), i32, i32, C.POINTER(vp), C.POINTER(PackStatus), vp],
+                            i32),
         "ug_paths_create": ([i32, i64, i32, vp, vp, vp, vp, vp, C.POINTER(vp)], i32),
         "ug_paths_destroy": ([vp], None),
         "ug_paths_from_pack": ([vp, vp, C.POINTER(vp)], i32),
@@ -134,7 +136,7 @@ def exported_symbols() -> list[str]:
     return [
         "ug_abi_version", "ug_last_error", "ug_pack_create", "ug_pack_run", "ug_pack_run_sub",
         "ug_pack_get_status", "ug_pack_accepted", "ug_pack_params", "ug_pack_positions",
-        "ug_pack_curve", "ug_eval_candidates", "ug_pack_profile", "ug_pack_kernel_stats",
+        "ug_pack_curve", "ug_pack_rounds", "ug_eval_candidates", "ug_pack_profile", "ug_pack_kernel_stats",
         "ug_pack_destroy", "ug_cache_trim", "ug_pack_check", "ug_ensemble_run", "ug_paths_create",
         "ug_paths_destroy", "ug_paths_from_pack",
         "ug_frame_bake_lightcone",
@@ -287,6 +289,16 @@ class Packer:
             check(lib().ug_pack_curve(self._h, rows.ctypes.data, n.value, C.byref(n)))
         return rows
 
+    def rounds(self) -> np.ndarray:
+        """Rounds of the reference schedule: (rows, 5) = first candidate, candidates,
+        survivors at the round's start, admitted, newly occupied cells."""
+        n = C.c_int64()
+        check(lib().ug_pack_rounds(self._h, None, 0, C.byref(n)))
+        rows = np.zeros((n.value, 5), np.int64)
+        if n.value:
+            check(lib().ug_pack_rounds(self._h, rows.ctypes.data, n.value, C.byref(n)))
+        return rows
+
     def eval_candidates(self, idx, precision: int = PREC_FP64):
         """Positions (count, T, dim) float64 of candidate indices (torch cuda int64 tensor)."""
         import torch
@@ -308,16 +320,28 @@ def pack_config(dim: int, T: int, terms: int = 0, seed: int = 1, precision: int 
                       int(euclid), 0)
 
 
-def ensemble(configs: list[PackConfig], threads: int = 16, keep: bool = False):
+def list_digest(idx) -> int:
+    """FNV-1a (64-bit) of an accepted index list's int64 little-endian bytes (the digest
+    ug_ensemble_run reports per universe)."""
+    h = 1469598103934665603
+    for byte in np.asarray(idx, "<i8").tobytes():
+        h = ((h ^ byte) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def ensemble(configs: list[PackConfig], threads: int = 16, keep: bool = False, digests: bool = False):
     """Pack independent universes `threads` at a time (ug_ensemble_run).  Returns the list
-    of final Status, and with keep=True also the list of Packer handles."""
+    of final Status; with keep=True also the list of Packer handles, with digests=True also
+    each universe's accepted-list digest (list_digest)."""
     n = len(configs)
     arr = (PackConfig * n)(*configs)
     st = (PackStatus * n)()
     handles = (C.c_void_p * n)() if keep else None
-    check(lib().ug_ensemble_run(arr, n, threads, handles, st))
-    status = [Status.of(st[i]) for i in range(n)]
-    if not keep:
-        return status
-    packers = [Packer._adopt(C.c_void_p(handles[i]), c.dim, c.T, c.terms) for i, c in enumerate(configs)]
-    return status, packers
+    dig = np.zeros(n, np.uint64)
+    check(lib().ug_ensemble_run(arr, n, threads, handles, st, dig.ctypes.data if digests else None))
+    out = [[Status.of(st[i]) for i in range(n)]]
+    if keep:
+        out.append([Packer._adopt(C.c_void_p(handles[i]), c.dim, c.T, c.terms) for i, c in enumerate(configs)])
+    if digests:
+        out.append([int(v) for v in dig])
+    return out[0] if len(out) == 1 else tuple(out)

@@ -40,13 +40,11 @@ def pack(dim: int, T: int, *, terms: int = 2, seed: int = 12345, attempts: float
          sub_attempts: float = 0.0, sub_accept_rate: float = 0.0, sub_fill_rate: float = 0.0,
          precision: str = "split", want_params: bool = True) -> PackResult:
     """One run with the engine's flag semantics (defaults are the engine's: T=120 aside,
-    --terms 2, --seed 12345, --attempts 1e8); accept_rate > 0 arms the windowed stop of
+    --terms 2, --seed 12345, --attempts 1e8): the engine's round schedule, whose last round
+    may pass `attempts`, and with accept_rate > 0 the windowed stop of
     cuda/braid_cuda.cu:930-943 (fixed_window: the fixed 2^20/2^22 rule instead)."""
     prec = abi.PREC_SPLIT_BF16 if precision == "split" else abi.PREC_FP64
-    if accept_rate > 0:
-        kind = abi.STOP_FIXED_WINDOW if fixed_window else abi.STOP_REF_WINDOW
-    else:
-        kind = abi.STOP_BUDGET
+    kind = abi.STOP_FIXED_WINDOW if fixed_window and accept_rate > 0 else abi.STOP_REF_WINDOW
     with abi.Packer(dim, T, terms=max(2, terms), seed=seed, precision=prec, pin_sin1=pin_sin1,
                     stop_kind=kind, max_attempts=attempts, stop_rate=accept_rate, max_n=max_n,
                     euclid=euclid) as p:
