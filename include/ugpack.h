@@ -297,6 +297,18 @@ int32_t ug_frame_bake_lightcone(ug_paths* p, const ug_front* front, double z_obs
 int32_t ug_frame_bake_slices(ug_paths* p, const double* z, int32_t nz, float* dev_pos,
                              float* dev_raster, int32_t raster_px);
 
+/* ug_frame_bake_slices without any allocation: no reference counterpart.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc): the same slices and rasters, bit for bit, with the frame basis
+ * kept in a fixed 4 MiB device array (one per GPU, held until the process ends) and a host
+ * staging array of the same size instead of per-call buffers.  Limit: 2 * kq * 64 <= 2^19
+ * doubles, kq = the paths' largest frequency rounded up to a multiple of 4 (frequencies up
+ * to 4096); larger returns UG_EOVERFLOW.  Runs on the current device, which must be the
+ * paths' device.  Calls share the arrays: a lock serialises them within a process, so
+ * calls that overlap (several threads, several devices) wait for each other. */
+int32_t ug_frame_bake_slices_noalloc(ug_paths* p, const double* z, int32_t nz, float* dev_pos,
+                                     float* dev_raster, int32_t raster_px);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ug_internal.h"
@@ -657,32 +658,38 @@ extern "C" int32_t ug_frame_bake_lightcone(ug_paths* P, const ug_front* fr, doub
     return rc;
 }
 
-extern "C" int32_t ug_frame_bake_slices(ug_paths* P, const double* z, int32_t nz, float* dev_pos, float* dev_raster,
-                                        int32_t raster_px) {
-    if (!P || !z || nz < 0 || (dev_raster && raster_px <= 0))
-        return fail(UG_EINVAL, "ug_frame_bake_slices: invalid argument");
-    if (nz == 0 || P->n == 0)
-        return UG_OK;
-    for (int j = 0; j < nz; j++)
-        if (!(z[j] > 0.0 && z[j] < 3.141592653589793))
-            return fail(UG_EINVAL, "ug_frame_bake_slices: z[%d] = %g outside (0, pi)", j, z[j]);
+namespace {
+
+__global__ void k_fill_f32(float* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        p[i] = 0.0f;
+}
+
+// Slices of frames z[0..nz) in batches of at most FB frames.  The basis of a batch (rows
+// r < kq: sin(k z)/sin z with k = r + 1; rows kq + r: (cos(k z) - 1)/sin z; FP64 libm on the
+// host) is built in hb and uploaded to dbasis, both holding 2 kq nzp doubles.  The raster is
+// cleared by a fill kernel.
+int32_t slices_run(ug_paths* P, const double* z, int nz, float* dev_pos, float* dev_raster, int raster_px,
+                   double* dbasis, double* hb, int FB) {
     hipStream_t st = P->stream;
-    // the basis of up to FB frames per launch: rows r < kq hold sin(k z)/sin z (k = r + 1),
-    // rows kq + r hold (cos(k z) - 1)/sin z; libm FP64 on the host
     const int kq = (P->kmax + 3) / 4 * 4;
-    const int FB = 512;
     const int nzp = (std::min(nz, FB) + FR_COLS - 1) / FR_COLS * FR_COLS;
-    double* dbasis = nullptr;
+    const size_t nb_words = (size_t)2 * kq * nzp;
     int32_t rc = UG_OK;
-    if (hipMalloc((void**)&dbasis, (size_t)2 * kq * nzp * 8) != hipSuccess)
-        return fail(UG_ENOMEM, "ug_frame_bake_slices: allocation failed");
-    if (dev_raster)
-        rc = hip_check(hipMemsetAsync(dev_raster, 0, (size_t)nz * raster_px * raster_px * 4, st), "raster clear");
-    std::vector<double> hb((size_t)2 * kq * nzp);
+    if (dev_raster) {
+        const size_t nr = (size_t)nz * raster_px * raster_px;
+        hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)std::min<size_t>((nr + 255) / 256, 8192)), dim3(256), 0, st,
+                           dev_raster, nr);
+        rc = hip_check(hipGetLastError(), "raster clear");
+    }
     const int64_t nrows = P->n * P->dim;
     for (int fb = 0; fb < nz && !rc; fb += FB) {
         const int nb = std::min(FB, nz - fb);
-        std::fill(hb.begin(), hb.end(), 0.0);
+        // the previous batch's upload and kernel must be done with both buffers first
+        rc = hip_check(hipStreamSynchronize(st), "slices sync");
+        if (rc)
+            break;
+        std::fill(hb, hb + nb_words, 0.0);
         for (int f = 0; f < nb; f++) {
             const double zf = z[fb + f], sz = std::sin(zf);
             for (int k = 1; k <= kq; k++) {
@@ -690,10 +697,7 @@ extern "C" int32_t ug_frame_bake_slices(ug_paths* P, const double* z, int32_t nz
                 hb[(size_t)(kq + k - 1) * nzp + f] = (std::cos(k * zf) - 1.0) / sz;
             }
         }
-        // the previous batch's kernel must be done with the basis before it is replaced
-        rc = hip_check(hipStreamSynchronize(st), "slices sync");
-        if (!rc)
-            rc = hip_check(hipMemcpyAsync(dbasis, hb.data(), hb.size() * 8, hipMemcpyHostToDevice, st), "basis upload");
+        rc = hip_check(hipMemcpyAsync(dbasis, hb, nb_words * 8, hipMemcpyHostToDevice, st), "basis upload");
         if (!rc) {
             const dim3 grid((unsigned)((nrows + FR_ROWS - 1) / FR_ROWS), (unsigned)((nb + FR_COLS - 1) / FR_COLS));
             hipLaunchKernelGGL(k_frames_mfma, grid, dim3(256), 0, st, dev_of(P), kq, dbasis, nzp, nb, fb, nz, dev_pos,
@@ -703,6 +707,74 @@ extern "C" int32_t ug_frame_bake_slices(ug_paths* P, const double* z, int32_t nz
     }
     if (!rc)
         rc = hip_check(hipStreamSynchronize(st), "slices sync");
+    return rc;
+}
+
+int32_t slices_check(const ug_paths* P, const double* z, int32_t nz, const float* dev_raster, int32_t raster_px,
+                     const char* who) {
+    if (!P || (!z && nz > 0) || nz < 0 || (dev_raster && raster_px <= 0))
+        return fail(UG_EINVAL, "%s: invalid argument", who);
+    for (int j = 0; j < nz; j++)
+        if (!(z[j] > 0.0 && z[j] < 3.141592653589793))
+            return fail(UG_EINVAL, "%s: z[%d] = %g outside (0, pi)", who, j, z[j]);
+    return UG_OK;
+}
+
+// The frame basis of ug_frame_bake_slices_noalloc: a file-scope device array (each GPU holds
+// its own copy until the process ends) and a host staging array of the same size.  Host code
+// never uses the device array's name as an address: its per-device address comes from
+// hipGetSymbolAddress on the first call on that device.  One lock serialises the calls of a
+// process, since they share both arrays.
+constexpr size_t NOALLOC_BASIS_DOUBLES = (size_t)2 * 512 * 512;  // 4 MiB
+double g_noalloc_host[NOALLOC_BASIS_DOUBLES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_noalloc_mu;
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_BASIS_DOUBLES];
+
+extern "C" int32_t ug_frame_bake_slices(ug_paths* P, const double* z, int32_t nz, float* dev_pos, float* dev_raster,
+                                        int32_t raster_px) {
+    int32_t rc = slices_check(P, z, nz, dev_raster, raster_px, "ug_frame_bake_slices");
+    if (rc || nz == 0 || P->n == 0)
+        return rc;
+    const int kq = (P->kmax + 3) / 4 * 4;
+    const int FB = 512;
+    const int nzp = (std::min(nz, FB) + FR_COLS - 1) / FR_COLS * FR_COLS;
+    double* dbasis = nullptr;
+    if (hipMalloc((void**)&dbasis, (size_t)2 * kq * nzp * 8) != hipSuccess)
+        return fail(UG_ENOMEM, "ug_frame_bake_slices: allocation failed");
+    std::vector<double> hb((size_t)2 * kq * nzp);
+    rc = slices_run(P, z, nz, dev_pos, dev_raster, raster_px, dbasis, hb.data(), FB);
     (void)hipFree(dbasis);
     return rc;
+}
+
+extern "C" int32_t ug_frame_bake_slices_noalloc(ug_paths* P, const double* z, int32_t nz, float* dev_pos,
+                                                float* dev_raster, int32_t raster_px) {
+    int32_t rc = slices_check(P, z, nz, dev_raster, raster_px, "ug_frame_bake_slices_noalloc");
+    if (rc || nz == 0 || P->n == 0)
+        return rc;
+    const int kq = (P->kmax + 3) / 4 * 4;
+    // frames per batch: as many 64-frame columns as the scratch holds, at most 512
+    const int FB = (int)std::min<size_t>(512, NOALLOC_BASIS_DOUBLES / ((size_t)2 * kq) / FR_COLS * FR_COLS);
+    if (FB < FR_COLS)
+        return fail(UG_EOVERFLOW,
+                    "ug_frame_bake_slices_noalloc: largest frequency %d needs %zu basis doubles per 64 frames, "
+                    "above the %zu of the scratch",
+                    P->kmax, (size_t)2 * kq * FR_COLS, NOALLOC_BASIS_DOUBLES);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(UG_EHIP, "ug_frame_bake_slices_noalloc: no usable current device");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        rc = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "scratch address");
+        if (rc)
+            return rc;
+        g_scratch_ptr[dev] = (double*)ptr;
+    }
+    return slices_run(P, z, nz, dev_pos, dev_raster, raster_px, g_scratch_ptr[dev], g_noalloc_host, FB);
 }

@@ -116,6 +116,7 @@ def lib() -> C.CDLL:
         "ug_frame_bake_lightcone": ([vp, C.POINTER(Front), C.c_double, C.POINTER(Observer), vp,
                                      C.c_uint64, C.POINTER(FrameInfo)], i32),
         "ug_frame_bake_slices": ([vp, vp, i32, vp, vp, i32], i32),
+        "ug_frame_bake_slices_noalloc": ([vp, vp, i32, vp, vp, i32], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -140,7 +141,7 @@ def exported_symbols() -> list[str]:
         "ug_pack_destroy", "ug_cache_trim", "ug_pack_check", "ug_ensemble_run", "ug_paths_create",
         "ug_paths_destroy", "ug_paths_from_pack",
         "ug_frame_bake_lightcone",
-        "ug_frame_bake_slices",
+        "ug_frame_bake_slices", "ug_frame_bake_slices_noalloc",
     ]
 
 
