@@ -233,6 +233,58 @@ int32_t ug_cache_trim(void);
 int32_t ug_ensemble_run(const ug_pack_config* cfgs, int32_t n, int32_t threads, ug_pack** keep,
                         ug_pack_status* status, uint64_t* digests);
 
+/* ------------------------------------------------------------------ knot regime
+ * Replaces the braid_knot executable (cuda/braid_knot.cu): few strands, very many
+ * candidates, no spatial grid.  Candidates are the packer's counter-indexed Philox
+ * candidates, admitted in index order against every resident strand (the canonical FP64
+ * evaluation and pair predicate), so a run is serial RSA per species and independent of
+ * batch sizes (the reference's own determinism model, :14-20,433-462). */
+
+/* The flags of cuda/braid_knot.cu:255-289. */
+typedef struct ug_knot_config {
+    int32_t dim;         /* --dim: 2 or 3 */
+    int32_t T;           /* -t */
+    int32_t terms;       /* --terms (0 = T, the full band) */
+    int32_t pin_sin1;    /* --pin-sin1 */
+    uint64_t seed;       /* --seed */
+    double max_attempts; /* --attempts: primary-species candidates [0, max_attempts) */
+    double nu_attempts;  /* --nu-attempts: nu-species candidates after them (0 = off) */
+    double nu_div;       /* --nu-div k >= 1: the nu exclusion is (2/T)/k */
+    int32_t max_strands; /* strand capacity, 0 = 16384 (kMaxStrands, :94-97) */
+    int32_t reserved;
+} ug_knot_config;
+
+typedef struct ug_knot_status {
+    int64_t attempts;         /* candidates consumed, both phases */
+    int64_t n;                /* strands admitted, both species */
+    int64_t n_primary;        /* primary-species strands */
+    int64_t primary_attempts; /* attempts when the primary phase ended */
+    int32_t phase;            /* 0 primary, 1 nu, 2 finished */
+    int32_t done;
+    double seconds;
+} ug_knot_status;
+
+typedef struct ug_knot ug_knot;
+
+/* Validates cfg like the tool's argv checks (:290-306) and binds the handle to hip_stream on
+ * the current device. */
+int32_t ug_knot_create(const ug_knot_config* cfg, void* hip_stream, ug_knot** out);
+/* Runs the phases (:404-476) until both budgets are spent or the strand capacity is reached,
+ * or until max_candidates more candidates were consumed (0 = no limit; batches are never
+ * split between calls).  st may be NULL. */
+int32_t ug_knot_run(ug_knot* h, int64_t max_candidates, ug_knot_status* st);
+int32_t ug_knot_get_status(const ug_knot* h, ug_knot_status* st);
+/* Admitted strands in admission order: candidate index (the reference logs index + 1) and
+ * species (0 primary, 1 nu); *n receives the total. */
+int32_t ug_knot_strands(const ug_knot* h, int64_t* idx, int32_t* species, int64_t cap, int64_t* n);
+/* Dump parameters of strands [first, first + count): per strand and axis a1 then nw
+ * (amplitude, frequency, phase), as ug_pack_params (the tool's --dump rows, :442-452). */
+int32_t ug_knot_params(const ug_knot* h, int64_t first, int64_t count, double* out);
+/* Batches run, 5 values each: first candidate, candidates, survivors against the strands of
+ * the batch's start (the ramp's `sc`), admitted, species. */
+int32_t ug_knot_batches(const ug_knot* h, int64_t* rows5, int64_t cap, int64_t* n);
+void ug_knot_destroy(ug_knot* h);
+
 /* ------------------------------------------------------------------ frame baker
  * Replaces the frame_bake executable (cuda/frame_bake.cu:398-635), whose BRF1 output
  * braidlab/frames.py:88-144 reads. */

@@ -72,6 +72,11 @@ def lib() -> C.CDLL:
         L.or_positions.restype = None
         L.or_count_clear.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p, C.c_int64]
         L.or_count_clear.restype = C.c_int64
+        L.or_pack_knot.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                   C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_void_p,
+                                   C.c_int64, C.POINTER(C.c_int64)]
+        L.or_pack_knot.restype = C.c_int64
         L.or_check_rsa.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]
         L.or_check_rsa.restype = C.c_int64
         _lib = L
@@ -267,6 +272,24 @@ class Model:
         if n < 0:
             raise RuntimeError(f"or_count_clear failed ({n})")
         return n
+
+    def pack_knot(self, budget: float, nu_attempts: float = 0.0, nu_div: float = 1.0,
+                  max_strands: int = 16384):
+        """The knot-regime tool's two phases: dict with idx, species, attempts,
+        primary_attempts, n_primary and batches (start, size, survivors, admitted, species)."""
+        idx = np.zeros(max_strands, np.int64)
+        sp = np.zeros(max_strands, np.int32)
+        att, patt, npri, nr = C.c_int64(0), C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        rcap = 1 << 16
+        rec = np.zeros((rcap, 5), np.int64)
+        n = lib().or_pack_knot(self.ref, float(budget), float(nu_attempts), float(nu_div), max_strands,
+                               _p(idx), _p(sp), max_strands, C.byref(att), C.byref(patt),
+                               C.byref(npri), _p(rec), rcap, C.byref(nr))
+        if n < 0:
+            raise RuntimeError(f"or_pack_knot failed ({n})")
+        return {"idx": idx[:n].copy(), "species": sp[:n].copy(), "attempts": att.value,
+                "primary_attempts": patt.value, "n_primary": npri.value,
+                "batches": rec[:min(nr.value, rcap)].copy()}
 
     def positions_of(self, idx) -> np.ndarray:
         """Canonical wrapped positions (n, T, d) of candidates idx (evaluated in C)."""

@@ -1189,3 +1189,109 @@ int64_t or_count_clear(const or_model* m, int64_t first, int64_t count, const in
     free(A);
     return par_err ? -2 : clear;
 }
+
+int64_t or_pack_knot(const or_model* m, double budget, double nu_attempts, double nu_div,
+                     int64_t max_strands, int64_t* idx, int32_t* species, int64_t cap,
+                     int64_t* attempts, int64_t* primary_attempts, int64_t* n_primary,
+                     int64_t* rounds, int64_t rounds_cap, int64_t* nrounds) {
+    packing P;
+    cand_buf cb;
+    if (nu_div < 1.0 || pk_init(&P, m) || cb_init(&cb, m)) {
+        pk_free(&P);
+        cb_free(&cb);
+        return nu_div < 1.0 ? -1 : -2;
+    }
+    const int T = m->T, d = m->d;
+    or_model mnu = *m; /* the nu species: same model, exclusion (2/T)/nu_div */
+    mnu.r = m->r / nu_div;
+    char* flag = NULL;
+    int64_t *surv = NULL, *acc = NULL;
+    int32_t* sp = NULL;
+    int64_t flag_alloc = 0, surv_alloc = 0, nacc = 0, nacc_cap = 0, ret = 0, att = 0;
+    if (nrounds)
+        *nrounds = 0;
+    if (primary_attempts)
+        *primary_attempts = 0;
+    if (n_primary)
+        *n_primary = 0;
+    for (int ph = 0; ph < 2 && ret == 0; ph++) {
+        if (ph == 1 && !(nu_attempts > 0))
+            break;
+        const or_model* mp = ph == 0 ? m : &mnu;
+        P.m = mp;
+        const double end = ph == 0 ? budget : budget + nu_attempts;
+        int64_t batch = 4096;
+        while ((double)att < end && nacc < max_strands) {
+            int64_t b = batch;
+            if ((double)(att + b) > end)
+                b = (int64_t)(end - (double)att);
+            if (b == 0)
+                break;
+            /* survivors against the strands of the batch's start, then index-order admission
+             * against the up-to-date strand set (cuda/braid_knot.cu:421-462) */
+            const int64_t ns = round_survivors(mp, &P, att, b, &flag, &flag_alloc, &surv, &surv_alloc);
+            if (ns < 0) {
+                ret = -2;
+                break;
+            }
+            int64_t adm = 0;
+            for (int64_t k = 0; k < ns && nacc < max_strands; k++) {
+                or_candidate(mp, (uint64_t)surv[k], cb.a1, cb.b, cb.c, cb.s, NULL, NULL);
+                int hit = 0;
+                for (int oi = 0; oi < T && !hit; oi++) {
+                    int t = mp->order[oi];
+                    or_eval_t(mp, t, cb.a1, cb.b, cb.c, cb.s, cb.X + (size_t)t * d);
+                    hit = pk_scan(&P, t, cb.X + (size_t)t * d, 0, NULL);
+                }
+                if (hit)
+                    continue;
+                if (pk_insert(&P, cb.X, (int32_t)P.n) < 0) {
+                    ret = -2;
+                    break;
+                }
+                if (nacc == nacc_cap) {
+                    nacc_cap = nacc_cap ? 2 * nacc_cap : 1024;
+                    acc = (int64_t*)realloc(acc, (size_t)nacc_cap * 8);
+                    sp = (int32_t*)realloc(sp, (size_t)nacc_cap * 4);
+                }
+                acc[nacc] = surv[k];
+                sp[nacc] = ph;
+                nacc++;
+                adm++;
+            }
+            if (ret)
+                break;
+            round_log(rounds, rounds_cap, nrounds, att, b, ns, adm, ph);
+            att += b;
+            if (ns > b / 8 && batch > 4096)
+                batch = batch / 4 > 4096 ? batch / 4 : 4096;
+            else if (ns < b / 64 && batch < ((int64_t)1 << 24))
+                batch *= 4;
+        }
+        if (ph == 0) {
+            if (primary_attempts)
+                *primary_attempts = att;
+            if (n_primary)
+                *n_primary = nacc;
+        }
+    }
+    if (ret == 0) {
+        for (int64_t i = 0; i < nacc && i < cap; i++) {
+            if (idx)
+                idx[i] = acc[i];
+            if (species)
+                species[i] = sp[i];
+        }
+        ret = nacc;
+        if (attempts)
+            *attempts = att;
+    }
+    P.m = m;
+    free(flag);
+    free(surv);
+    free(acc);
+    free(sp);
+    pk_free(&P);
+    cb_free(&cb);
+    return ret;
+}

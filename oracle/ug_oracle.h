@@ -126,6 +126,21 @@ int64_t or_pack_ref(const or_model* m, double budget, int rule, double stop_rate
 int64_t or_count_clear(const or_model* m, int64_t first, int64_t count, const int64_t* acc,
                        int64_t na);
 
+/* The knot-regime tool (cuda/braid_knot.cu:387-476) on the Philox stream: the primary
+ * species over candidates [0, budget) at the exclusion r = 2/T, then, when nu_attempts > 0,
+ * the nu species over the next nu_attempts candidates at r/nu_div, against every strand of
+ * both species.  Each batch's survivors are the candidates touching no strand of the batch's
+ * start; they are admitted in index order against the up-to-date strands, at most
+ * max_strands in all.  Batches start at 4096 per phase, shrink /4 (not below 4096) when
+ * survivors exceed size/8 and grow x4 (up to 2^24) below size/64, and are cut at the phase's
+ * budget.  Outputs: idx/species of the strands (admission order), *attempts,
+ * *primary_attempts and *n_primary (at the end of the primary phase), batch records
+ * {start, size, survivors, admitted, species}.  Returns the strand count or < 0. */
+int64_t or_pack_knot(const or_model* m, double budget, double nu_attempts, double nu_div,
+                     int64_t max_strands, int64_t* idx, int32_t* species, int64_t cap,
+                     int64_t* attempts, int64_t* primary_attempts, int64_t* n_primary,
+                     int64_t* rounds, int64_t rounds_cap, int64_t* nrounds);
+
 /* Canonical wrapped positions out[n][T][d] of candidates idx[0..n). */
 void or_positions(const or_model* m, const int64_t* idx, int64_t n, double* out);
 

@@ -19,7 +19,7 @@ def run(*argv):
     return r.returncode, r.stderr
 
 
-@pytest.mark.parametrize("name", ["braid_ug", "braid_ug3d", "frame_bake_ug"])
+@pytest.mark.parametrize("name", ["braid_ug", "braid_ug3d", "frame_bake_ug", "braid_knot_ug"])
 def test_binaries_are_built(name):
     exe = os.path.join(BIN, name)
     assert os.access(exe, os.X_OK), f"{exe} missing: run __graft_entry__.build()"
@@ -72,3 +72,16 @@ def test_engine_argv_matches_build_command_layout():
                        "--subpaths", "--sub-until-accept-rate", "1e-06", "--sub-attempts",
                        "1000000000.0", "--maxfreq", "64", "--curve", "c.csv"]
     assert ugpack.command(3, 12, attempts=1e6, seed=1, accept_rate=0.0)[0].endswith("braid_ug3d")
+
+
+def test_knot_argv_errors():
+    # cuda/braid_knot.cu:284-306: unknown flags, --nu-div < 1 and a bad --dim end the run
+    exe = os.path.join(BIN, "braid_knot_ug")
+    rc, err = run(exe, "--bogus")
+    assert rc == 1 and "unknown flag --bogus" in err
+    rc, err = run(exe, "--nu-div", "0.5")
+    assert rc == 1 and "--nu-div must be >= 1" in err
+    rc, err = run(exe, "--dim", "4")
+    assert rc == 1 and "--dim must be 2 or 3" in err
+    rc, err = run(exe, "-t", "16", "--terms", "40")
+    assert rc == 1 and "terms 40" in err

@@ -636,3 +636,69 @@ def test_subpath_windows_and_fresh_cells_by_hand(name):
         per_round[rnd[j]] += len(keys - seen)
         seen |= keys
     assert np.array_equal(per_round, fresh) and fresh.sum() == g["filled_new"]
+
+
+# ---------------------------------------------------------------- knot regime
+def _touch_any(X, others, r, T):
+    """Does X (T, d) touch any of others (n, T, d) at some timestep?  The canonical pair
+    predicate: neighbouring cells on every axis and |torus_delta| <= r on every axis."""
+    if len(others) == 0:
+        return False
+    cx = np.floor((X + 1.0) * (T / 2.0)).astype(np.int64) % T
+    co = np.floor((others + 1.0) * (T / 2.0)).astype(np.int64) % T
+    dc = np.abs(cx[None] - co)
+    adj = np.all((dc <= 1) | (dc == T - 1), axis=2)
+    d = X[None] - others
+    d = np.where(d > 1.0, d - 2.0, np.where(d < -1.0, d + 2.0, d))
+    return bool(np.any(adj & np.all(np.abs(d) <= r, axis=2)))
+
+
+@pytest.mark.parametrize("d,T,seed,budget,nu,div", [(2, 16, 1, 2e6, 1e5, 2.0), (3, 8, 2, 4e5, 1e5, 4.0)])
+def test_knot_oracle_by_hand(d, T, seed, budget, nu, div):
+    # cuda/braid_knot.cu's two phases: the primary species is serial RSA over [0, budget)
+    # (the packer's own list); the nu species, checked here by dense numpy tests, admits a
+    # candidate iff it touches no earlier strand of either species at the exclusion r/div;
+    # the batches follow the ramp of :464-468 on survivor counts recounted pairwise
+    m = po.Model(d, T, seed=seed, pin=True)
+    r = m.pack_knot(budget, nu_attempts=nu, nu_div=div)
+    idx, sp, rec = r["idx"], r["species"], r["batches"]
+    assert r["primary_attempts"] == budget and r["attempts"] == budget + nu
+    prim, _, _ = m.pack(int(budget))
+    assert np.array_equal(idx[sp == 0], prim) and r["n_primary"] == len(prim)
+    assert np.all(sp[:len(prim)] == 0) and np.all(sp[len(prim):] == 1) and len(idx) > len(prim)
+    rnu = m.r / div
+    cand = np.arange(int(budget), int(budget + nu))
+    Xc = m.positions_of(cand)
+    Xs = m.positions_of(idx)
+    member = set(idx[sp == 1].tolist())
+    for c, X in zip(cand, Xc):
+        earlier = Xs[(sp == 0) | (idx < c)]
+        assert _touch_any(X, earlier, rnu, T) == (int(c) not in member), c
+    # batches: contiguous per phase, cut at the phase budget, sizes from the ramp
+    start, size, surv, adm, spc = rec.T
+    assert start[0] == 0 and np.array_equal(start[1:], np.cumsum(size)[:-1])
+    for ph, end in ((0, budget), (1, budget + nu)):
+        b = 4096
+        for k in np.nonzero(spc == ph)[0]:
+            assert size[k] == min(b, int(end - start[k]))
+            if surv[k] > size[k] // 8 and b > 4096:
+                b = max(4096, b // 4)
+            elif surv[k] < size[k] // 64 and b < 1 << 24:
+                b *= 4
+        assert adm[spc == ph].sum() == np.sum(sp == ph)
+    # survivors of the nu batches: pairwise count against the strands of each batch's start
+    mn = po.Model(d, T, seed=seed, pin=True)
+    mn._m.r = rnu
+    for k in np.nonzero(spc == 1)[0]:
+        before = idx[(sp == 0) | (idx < start[k])]
+        assert surv[k] == mn.count_clear(int(start[k]), int(size[k]), before)
+
+
+def test_knot_strand_capacity_ends_at_the_batch():
+    # the capacity stops admissions inside a batch; the batch is still counted
+    m = po.Model(2, 16, seed=3)
+    r = m.pack_knot(1e6, max_strands=5)
+    full = m.pack_knot(1e6)
+    assert len(r["idx"]) == 5 and np.array_equal(r["idx"], full["idx"][:5])
+    last = r["batches"][-1]
+    assert r["attempts"] == last[0] + last[1] and r["idx"][-1] < r["attempts"] < 1e6

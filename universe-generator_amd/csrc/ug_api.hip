@@ -18,9 +18,11 @@ using namespace ug;
 
 namespace {
 
+}  // namespace
+
 // Host copy of the canonical generator (same arithmetic as the device's gen_wave64):
 // dump parameters of accepted paths, per axis a1 then (amplitude, frequency, phase).
-void gen_host(const Pack* h, uint64_t idx, double* out_row) {
+void ug::gen_params_host(const Pack* h, uint64_t idx, double* out_row) {
     const int D = h->d, nw = h->nw, npairs = h->npairs;
     const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
     const uint32_t k0 = h->dm.key0, k1 = h->dm.key1;
@@ -92,6 +94,8 @@ void gen_host(const Pack* h, uint64_t idx, double* out_row) {
     }
 }
 
+namespace {
+
 // Buffers go back to the process-wide cache for the next handle (ug_cache.hip).
 template <typename P>
 void free_dev(P*& p) {
@@ -162,6 +166,15 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     if (!cfg_in || !out)
         return fail(UG_EINVAL, "ug_pack_create: NULL argument");
     *out = nullptr;
+    Pack* h = nullptr;
+    const int32_t rc = create_handle(cfg_in, hip_stream, true, &h);
+    if (rc == UG_OK)
+        *out = (ug_pack*)h;
+    return rc;
+}
+
+int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool with_table, Pack** out) {
+    *out = nullptr;
     ug_pack_config cfg = *cfg_in;
     if (cfg.dim != 2 && cfg.dim != 3)
         return fail(UG_EINVAL, "error: dim must be 2 or 3, got %d", cfg.dim);
@@ -219,8 +232,10 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     int64_t tab_len = (int64_t)T * T * (T + 2);
     if (h->d == 3)
         tab_len *= T;
-    h->tab_len = tab_len;
     const int occw = (T + 2 + 31) / 32 + 1;  // one spare word: a 3-bit window may cross a word
+    if (!with_table)
+        tab_len = 0;
+    h->tab_len = tab_len;
     h->occ_len = tab_len / (T + 2) * occw;
     int64_t cap0 = cfg.max_paths > 0 ? cfg.max_paths : 4096;
 
@@ -229,8 +244,10 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
         if (rc == UG_OK && !((ptr) = (decltype(ptr))cache_alloc(bytes)))                     \
             rc = fail(UG_ENOMEM, "device allocation of %zu bytes failed", (size_t)(bytes));     \
     } while (0)
-    UG_ALLOC(h->d_tab, (size_t)tab_len * 4);
-    UG_ALLOC(h->d_occ, (size_t)h->occ_len * 4);
+    if (with_table) {
+        UG_ALLOC(h->d_tab, (size_t)tab_len * 4);
+        UG_ALLOC(h->d_occ, (size_t)h->occ_len * 4);
+    }
     UG_ALLOC(h->d_filled, 8);
     UG_ALLOC(h->d_flags, (size_t)UG_SUB_MAX);
     UG_ALLOC(h->d_list, (size_t)UG_SUB_MAX * 8);
@@ -253,9 +270,11 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     if (rc == UG_OK && hipEventCreateWithFlags(&h->ev_report, hipEventDisableTiming) != hipSuccess)
         rc = fail(UG_EHIP, "event creation failed");
     if (rc == UG_OK) {
-        hipLaunchKernelGGL(k_fill_tab, dim3(grid_for(tab_len, 256 * 8)), dim3(256), 0, h->stream, h->d_tab, tab_len);
+        if (with_table)
+            hipLaunchKernelGGL(k_fill_tab, dim3(grid_for(tab_len, 256 * 8)), dim3(256), 0, h->stream, h->d_tab,
+                               tab_len);
         rc = hip_check(hipMemsetAsync(h->d_filled, 0, 8, h->stream), "clear filled");
-        if (!rc)
+        if (!rc && with_table)
             rc = hip_check(hipMemsetAsync(h->d_occ, 0, (size_t)h->occ_len * 4, h->stream), "clear occupancy");
     }
     DevModel& m = h->dm;
@@ -284,7 +303,7 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
     m.pos64 = nullptr;
     m.pos32 = nullptr;
     m.marg = nullptr;
-    if (rc == UG_OK)
+    if (rc == UG_OK && with_table)
         rc = ensure_paths(h, cap0);
     if (rc == UG_OK && cfg.precision == UG_PREC_SPLIT_BF16)
         rc = fast_setup(h);
@@ -296,19 +315,21 @@ extern "C" int32_t ug_pack_create(const ug_pack_config* cfg_in, void* hip_stream
         delete h;
         return rc;
     }
-    *out = (ug_pack*)h;
+    *out = h;
     return UG_OK;
 }
 
-extern "C" void ug_pack_destroy(ug_pack* hp) {
-    if (!hp)
-        return;
-    Pack* h = (Pack*)hp;
+void ug::destroy_handle(Pack* h) {
     (void)hipStreamSynchronize(h->stream);
     release(h);
     if (h->own_stream)
         (void)hipStreamDestroy(h->stream);
     delete h;
+}
+
+extern "C" void ug_pack_destroy(ug_pack* hp) {
+    if (hp)
+        destroy_handle((Pack*)hp);
 }
 
 static void fill_status(const Pack* h, ug_pack_status* st) {
@@ -662,7 +683,7 @@ extern "C" int32_t ug_pack_params(const ug_pack* hp, int64_t first, int64_t coun
         return rc;
     const size_t row = (size_t)h->d * (1 + 3 * h->nw);
     for (int64_t i = 0; i < count; i++)
-        gen_host(h, (uint64_t)idx[i], out + (size_t)i * row);
+        gen_params_host(h, (uint64_t)idx[i], out + (size_t)i * row);
     return UG_OK;
 }
 

@@ -289,5 +289,23 @@ int fast_blocks(const Pack* h);  // N blocks of the model's ordered timesteps (0
 // among the rows < nlimit.
 int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, int64_t nlimit);
 int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* out);
+// Conflict bits among a resolve chunk's s survivors (trajectories in d_spos, packed cells in
+// d_scell, mask rows cleared); dS: count bounded on the device; expect: survivors expected.
+void launch_conflicts(Pack* h, int s, const int* dS = nullptr, int64_t expect = -1);
+// Index-order greedy over the chunk's conflict bits -> d_acc_list, d_nacc.
+int32_t launch_greedy(Pack* h, int s);
+// Knot regime: brute-force test of fresh candidates base + i (list == nullptr) or list[i]
+// against the nk strands (kcell[t][ks], kpos[t][ks][D]): flags[i] = 1 iff clear.
+int32_t launch_knot_test(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
+                         const int* dcount, const int* kcell, const double* kpos, int64_t ks, int nk);
+// Appends the chunk's accepted survivors (d_acc_list, at most limit) of list as strands n0, ...
+int32_t launch_knot_append(Pack* h, const int64_t* list, int s, int64_t n0, int* kcell, double* kpos, int64_t ks,
+                           int64_t* kidx, int limit);
+// Handle set-up shared by the packer and the knot engine: with_table = false builds no cell
+// table, occupancy bits or path records (the knot engine keeps its own strands).
+int32_t create_handle(const ug_pack_config* cfg, void* hip_stream, bool with_table, Pack** out);
+void destroy_handle(Pack* h);
+// Dump parameters of candidate idx: per axis a1 then nw (amplitude, frequency, phase).
+void gen_params_host(const Pack* h, uint64_t idx, double* out_row);
 
 }  // namespace ug

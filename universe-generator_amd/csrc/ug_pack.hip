@@ -671,6 +671,149 @@ __global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t fir
     }
 }
 
+// =====================================================================================
+// knot regime (cuda/braid_knot.cu): no grid, every candidate is compared with every resident
+// strand.  Strands are stored timestep-major -- packed cells kcell[t][ks] and FP64 positions
+// kpos[t][ks][D] -- so the strands of one timestep are contiguous; the FP64 positions are
+// read only where the packed cells neighbour (the canonical pair predicate: neighbouring
+// cells on every axis and |torus_delta| <= r, r = the phase's exclusion).
+
+template <int D>
+__device__ __forceinline__ bool cells_adjacent(int u, int v, int T);
+
+// Any contact between the positions pos[g] (timesteps tsl[g], g < G) and the nk strands:
+// lanes take (g, strand) items 64 at a time.
+template <int D>
+__device__ __forceinline__ bool knot_round(const DevModel& m, const double* pos, const int* tsl, int G, int lane,
+                                           const int* kcell, const double* kpos, int64_t ks, int nk) {
+    const int items = G * nk;
+    for (int it0 = 0; it0 < items; it0 += 64) {
+        const int it = it0 + lane;
+        bool h = false;
+        if (it < items) {
+            const int g = it / nk, sidx = it - g * nk;
+            const int t = tsl[g];
+            double x[D];
+            int cx = 0;
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                x[a] = pos[g * D + a];
+                cx |= cell_of(x[a], m.halfT, m.T) << (10 * a);
+            }
+            const size_t q = (size_t)t * ks + sidx;
+            if (cells_adjacent<D>(cx, kcell[q], m.T))
+                h = contact<D>(m, x, kpos + q * D);
+        }
+        if (__ballot(h))
+            return true;
+    }
+    return false;
+}
+
+// One wave per candidate, as k_cand64's test (head chunks of 2, 4, ... 64 timesteps across
+// the wave, then one lane per timestep) with the brute-force lookup: flags[i] = 1 iff
+// candidate i touches none of the nk strands.
+template <int D, int PL, bool SB>
+__global__ __launch_bounds__(64) void k_knot_test(DevModel m, uint64_t base, int64_t count, const int64_t* list,
+                                                  const int* dcount, uint8_t* flags, int head, const int* kcell,
+                                                  const double* kpos, int64_t ks, int nk) {
+    if (dcount)
+        count = min(count, (int64_t)*dcount);
+    extern __shared__ __align__(16) int dyn_lds[];
+    const int lane = threadIdx.x;
+    const int T = m.T;
+    int* sb = dyn_lds;
+    uint32_t* used = (uint32_t*)(sb + D * m.nw);
+    constexpr int NWP = 128 * PL;
+    const int lds_head = cand64_lds_head<D>(m, PL);
+    double2* cs = (double2*)(dyn_lds + lds_head);
+    double* pos = (double*)(cs + D * NWP);
+    __shared__ int tsl[64];
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
+        WaveCoef<D, PL> w;
+        gen_wave64<D, PL>(m, idx, lane, w, sb, used);
+        bool hit = false;
+        int o0 = 0;
+        for (int chunk = 2; !hit && o0 < min(head, T); chunk = min(2 * chunk, 64)) {
+            const int G = min(chunk, T - o0);
+            for (int g = 0; g < G; g++) {
+                double X[D];
+                const int t = m.order[o0 + g];
+                eval_wave64<D, PL>(m, w, t, X);
+                if (lane == g) {
+                    tsl[g] = t;
+#pragma unroll
+                    for (int a = 0; a < D; a++)
+                        pos[g * D + a] = X[a];
+                }
+            }
+            __syncthreads();
+            hit = knot_round<D>(m, pos, tsl, G, lane, kcell, kpos, ks, nk);
+            o0 += G;
+            __syncthreads();
+        }
+        if (!hit && o0 < T) {
+#pragma unroll
+            for (int a = 0; a < D; a++)
+#pragma unroll
+                for (int ii = 0; ii < PL; ii++)
+#pragma unroll
+                    for (int k = 0; k < 2; k++) {
+                        const int j = 2 * (lane + 64 * ii) + k;
+                        cs[a * NWP + j] = make_double2(w.c[a][ii][k], w.s[a][ii][k]);
+                        sb[a * NWP + j] = w.b[a][ii][k];
+                    }
+            __syncthreads();
+            for (int ps = o0; ps < T && !hit; ps += 64) {
+                const int oi = ps + lane;
+                const int pend = min(64, T - ps);
+                if (oi < T) {
+                    const int t = m.order[oi];
+                    double X[D];
+                    if constexpr (SB)
+                        eval_lane_full<D, PL>(m, cs, sb, w.a1, t, X);
+                    else
+                        eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
+                    tsl[lane] = t;
+#pragma unroll
+                    for (int a = 0; a < D; a++)
+                        pos[lane * D + a] = X[a];
+                }
+                __syncthreads();
+                hit = knot_round<D>(m, pos, tsl, pend, lane, kcell, kpos, ks, nk);
+                __syncthreads();
+            }
+        }
+        if (lane == 0)
+            flags[i] = hit ? 0 : 1;
+        __syncthreads();
+    }
+}
+
+// Appends a resolve chunk's accepted survivors to the strands (one thread per (strand,
+// timestep)): trajectory rows of pos (k_cand64 FULL order [s][t][D]) and packed cells go to
+// timestep-major rows n0 + k; t == 0 also records the candidate index.
+template <int D>
+__global__ __launch_bounds__(256) void k_knot_append(DevModel m, const double* pos, const int* cells,
+                                                     const int* acc_list, const int* nacc, int limit,
+                                                     const int64_t* surv_idx, int64_t n0, int* kcell, double* kpos,
+                                                     int64_t ks, int64_t* kidx) {
+    const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int na = min(*nacc, limit);
+    const int k = (int)(item / m.T), t = (int)(item % m.T);
+    if (k >= na)
+        return;
+    const int sidx = acc_list[k];
+    const size_t src = (size_t)sidx * m.T + t, dst = (size_t)t * ks + (size_t)(n0 + k);
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        kpos[dst * D + a] = pos[src * D + a];
+    kcell[dst] = cells[src];
+    if (t == 0)
+        kidx[n0 + k] = surv_idx[sidx];
+}
+
 // Baker coefficients of stored path p (one wave per path): the canonical per-slot
 // (a cos f, a sin f) of its candidate, scattered by frequency into dense rows.
 template <int D, int PL>
@@ -1645,7 +1788,7 @@ int32_t ensure_paths(Pack* h, int64_t need) {
 // The chunk's packed cells (d_scell) and cleared mask rows come from its trajectory launch.
 // expect: the survivors the host expects (s itself when known); large chunks take the
 // tiled kernel, small ones the pair-per-thread kernel -- the bits are the same either way.
-void launch_conflicts(Pack* h, int s, const int* dS = nullptr, int64_t expect = -1) {
+void launch_conflicts(Pack* h, int s, const int* dS, int64_t expect) {
     const int64_t npairs = (int64_t)s * (s - 1) / 2;
     if (npairs <= 0)
         return;
@@ -1953,6 +2096,57 @@ int32_t launch_check(Pack* h, int mode, uint64_t first, int64_t count, unsigned 
     }
 #undef UG_CHK
     return hip_check(hipGetLastError(), "k_check");
+}
+
+int32_t launch_knot_test(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
+                         const int* dcount, const int* kcell, const double* kpos, int64_t ks, int nk) {
+    if (count <= 0)
+        return UG_OK;
+    const int head = list ? (h->d == 2 ? 14 : 13) : h->T;
+    const int blocks = (int)std::min<int64_t>(count, 256 * 32);
+    const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
+    const int pl = slot_pl(h->npairs);
+    const bool sb = pl <= 2 && h->full && list;
+#define UG_KNOT(DD, PP, SS)                                                                                \
+    hipLaunchKernelGGL((k_knot_test<DD, PP, SS>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, \
+                       list, dcount, flags, head, kcell, kpos, ks, nk)
+#define UG_KNOT_PL(DD)                \
+    do {                              \
+        if (pl == 1 && sb)            \
+            UG_KNOT(DD, 1, true);     \
+        else if (pl == 1)             \
+            UG_KNOT(DD, 1, false);    \
+        else if (pl == 2 && sb)       \
+            UG_KNOT(DD, 2, true);     \
+        else if (pl == 2)             \
+            UG_KNOT(DD, 2, false);    \
+        else                          \
+            UG_KNOT(DD, 4, false);    \
+    } while (0)
+    if (h->d == 2)
+        UG_KNOT_PL(2);
+    else
+        UG_KNOT_PL(3);
+#undef UG_KNOT_PL
+#undef UG_KNOT
+    return hip_check(hipGetLastError(), "k_knot_test");
+}
+
+int32_t launch_knot_append(Pack* h, const int64_t* list, int s, int64_t n0, int* kcell, double* kpos, int64_t ks,
+                           int64_t* kidx, int limit) {
+    const int ib = grid_for((int64_t)s * h->T, 256);
+    if (h->d == 2)
+        hipLaunchKernelGGL(k_knot_append<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell,
+                           h->d_acc_list, h->d_nacc, limit, list, n0, kcell, kpos, ks, kidx);
+    else
+        hipLaunchKernelGGL(k_knot_append<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell,
+                           h->d_acc_list, h->d_nacc, limit, list, n0, kcell, kpos, ks, kidx);
+    return hip_check(hipGetLastError(), "k_knot_append");
+}
+
+int32_t launch_greedy(Pack* h, int s) {
+    hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, nullptr, h->d_acc_list, h->d_nacc);
+    return hip_check(hipGetLastError(), "k_greedy");
 }
 
 }  // namespace ug

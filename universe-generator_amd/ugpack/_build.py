@@ -9,7 +9,8 @@ import subprocess
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(PKG_DIR), "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libugpack.so")
-SOURCES = ["ug_pack.hip", "ug_fast.hip", "ug_api.hip", "ug_cache.hip", "ug_frames.hip", "ug_ensemble.hip"]
+SOURCES = ["ug_pack.hip", "ug_fast.hip", "ug_api.hip", "ug_cache.hip", "ug_frames.hip", "ug_ensemble.hip",
+           "ug_knot.hip"]
 
 # -ffp-contract=off: the canonical FP64 arithmetic uses explicit fma only, so host and
 # device produce identical bits.  Packed FP32 instructions are disabled for all HIP code.
@@ -38,6 +39,7 @@ CLIS = {
     "braid_ug3d": ("ug_cli_pack.cpp", ["-DUG_CLI_DIM=3"]),
     "frame_bake_ug": ("ug_cli_bake.cpp", ["-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
                                           f"-L{ROCM}/lib", "-lamdhip64"]),
+    "braid_knot_ug": ("ug_cli_knot.cpp", []),
 }
 
 

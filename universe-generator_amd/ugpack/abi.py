@@ -58,6 +58,18 @@ class KernelStats(C.Structure):
                 ("visits", C.c_int64), ("points", C.c_int64)]
 
 
+class KnotConfig(C.Structure):
+    _fields_ = [("dim", C.c_int32), ("T", C.c_int32), ("terms", C.c_int32), ("pin_sin1", C.c_int32),
+                ("seed", C.c_uint64), ("max_attempts", C.c_double), ("nu_attempts", C.c_double),
+                ("nu_div", C.c_double), ("max_strands", C.c_int32), ("reserved", C.c_int32)]
+
+
+class KnotStatus(C.Structure):
+    _fields_ = [("attempts", C.c_int64), ("n", C.c_int64), ("n_primary", C.c_int64),
+                ("primary_attempts", C.c_int64), ("phase", C.c_int32), ("done", C.c_int32),
+                ("seconds", C.c_double)]
+
+
 KERNEL_PREFILTER, KERNEL_RECHECK = 0, 1
 PROFILE_PREFILTER, PROFILE_RECHECK = 1 << KERNEL_PREFILTER, 1 << KERNEL_RECHECK
 PROFILE_ALL = PROFILE_PREFILTER | PROFILE_RECHECK
@@ -116,6 +128,13 @@ def lib() -> C.CDLL:
         "ug_frame_bake_lightcone": ([vp, C.POINTER(Front), C.c_double, C.POINTER(Observer), vp,
                                      C.c_uint64, C.POINTER(FrameInfo)], i32),
         "ug_frame_bake_slices": ([vp, vp, i32, vp, vp, i32], i32),
+        "ug_knot_create": ([C.POINTER(KnotConfig), vp, C.POINTER(vp)], i32),
+        "ug_knot_run": ([vp, i64, C.POINTER(KnotStatus)], i32),
+        "ug_knot_get_status": ([vp, C.POINTER(KnotStatus)], i32),
+        "ug_knot_strands": ([vp, vp, vp, i64, C.POINTER(i64)], i32),
+        "ug_knot_params": ([vp, i64, i64, vp], i32),
+        "ug_knot_batches": ([vp, vp, i64, C.POINTER(i64)], i32),
+        "ug_knot_destroy": ([vp], None),
         "ug_frame_bake_slices_noalloc": ([vp, vp, i32, vp, vp, i32], i32),
     }
     for name, (args, res) in sigs.items():
@@ -141,7 +160,8 @@ def exported_symbols() -> list[str]:
         "ug_pack_destroy", "ug_cache_trim", "ug_pack_check", "ug_ensemble_run", "ug_paths_create",
         "ug_paths_destroy", "ug_paths_from_pack",
         "ug_frame_bake_lightcone",
-        "ug_frame_bake_slices", "ug_frame_bake_slices_noalloc",
+        "ug_frame_bake_slices", "ug_frame_bake_slices_noalloc", "ug_knot_create", "ug_knot_run",
+        "ug_knot_get_status", "ug_knot_strands", "ug_knot_params", "ug_knot_batches", "ug_knot_destroy",
     ]
 
 
@@ -308,6 +328,81 @@ class Packer:
         out = torch.empty((idx.numel(), self.T, self.dim), dtype=torch.float64, device="cuda")
         check(lib().ug_eval_candidates(self._h, idx.data_ptr(), idx.numel(), precision,
                                        out.data_ptr()))
+        return out
+
+
+@dataclass
+class KnotRun:
+    attempts: int
+    n: int
+    n_primary: int
+    primary_attempts: int
+    phase: int
+    done: bool
+    seconds: float
+
+
+class Knot:
+    """The knot-regime packer (ug_knot_*): the primary species over [0, max_attempts), then
+    the nu species over the next nu_attempts candidates at the exclusion (2/T)/nu_div."""
+
+    def __init__(self, dim: int, T: int, terms: int = 0, seed: int = 1, pin_sin1: bool = False,
+                 max_attempts: float = 1e10, nu_attempts: float = 0.0, nu_div: float = 1.0,
+                 max_strands: int = 0, stream: int = 0):
+        self._h = None
+        cfg = KnotConfig(dim, T, terms, int(pin_sin1), seed, float(max_attempts), float(nu_attempts),
+                         float(nu_div), max_strands, 0)
+        h = C.c_void_p()
+        check(lib().ug_knot_create(C.byref(cfg), C.c_void_p(stream), C.byref(h)))
+        self._h = h
+        self.dim, self.T = dim, T
+        self.nw = (terms if terms else T) - 1
+
+    def close(self) -> None:
+        if self._h is not None:
+            lib().ug_knot_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, max_candidates: int = 0) -> KnotRun:
+        st = KnotStatus()
+        check(lib().ug_knot_run(self._h, max_candidates, C.byref(st)))
+        return KnotRun(st.attempts, st.n, st.n_primary, st.primary_attempts, st.phase, bool(st.done),
+                       st.seconds)
+
+    def strands(self) -> tuple[np.ndarray, np.ndarray]:
+        n = C.c_int64()
+        check(lib().ug_knot_strands(self._h, None, None, 0, C.byref(n)))
+        idx = np.zeros(n.value, np.int64)
+        sp = np.zeros(n.value, np.int32)
+        if n.value:
+            check(lib().ug_knot_strands(self._h, idx.ctypes.data, sp.ctypes.data, n.value, C.byref(n)))
+        return idx, sp
+
+    def batches(self) -> np.ndarray:
+        n = C.c_int64()
+        check(lib().ug_knot_batches(self._h, None, 0, C.byref(n)))
+        rows = np.zeros((n.value, 5), np.int64)
+        if n.value:
+            check(lib().ug_knot_batches(self._h, rows.ctypes.data, n.value, C.byref(n)))
+        return rows
+
+    def params(self, first: int = 0, count: int | None = None) -> np.ndarray:
+        if count is None:
+            count = len(self.strands()[0]) - first
+        out = np.zeros((count, self.dim, 1 + 3 * self.nw))
+        check(lib().ug_knot_params(self._h, first, count, out.ctypes.data))
         return out
 
 
