@@ -111,8 +111,8 @@ __global__ __launch_bounds__(256) void k_frame_grid(DevPaths P, int M, const dou
 }
 
 // One wave per path; lanes over torus images (nx, ny) in [-K, K]^2.  COUNT: per-path
-// number of crossings into counts[p].  Otherwise: bisect each crossing and write hits at
-// offs[p] in (image, row) order.
+// number of crossings into counts[p].  Otherwise: the crossings' descriptors (path, image,
+// row, side) at offs[p] in (image, row) order, refined into hits by k_frame_bisect.
 template <bool COUNT>
 __global__ __launch_bounds__(64) void k_frame_hits(DevPaths P, FrontDev F, double ox, double oy,
                                                    int K, int obs_idx, const double2* rows,
@@ -180,42 +180,13 @@ __global__ __launch_bounds__(64) void k_frame_hits(DevPaths P, FrontDev F, doubl
                         if (COUNT) {
                             cnt++;
                         } else {
-                            const bool inside_prev = prev < 0.0;
-                            double chi, zemit, q[3];
-                            if (F.budget) {
-                                double lo = F.zg[i - 1], hi = F.zg[i];
-                                for (int it = 0; it < 24; it++) {
-                                    const double mid = 0.5 * (lo + hi);
-                                    comov(P, p, mid, q);
-                                    const double gm = dist_of(q[0] + bx, q[1] + by, F.cheb) - reach_dev(F, mid);
-                                    if ((gm < 0.0) == inside_prev)
-                                        lo = mid;
-                                    else
-                                        hi = mid;
-                                }
-                                zemit = 0.5 * (lo + hi);
-                                chi = reach_dev(F, zemit);
-                            } else {
-                                double lo = F.chig[i - 1], hi = F.chig[i];
-                                for (int it = 0; it < 24; it++) {
-                                    const double mid = 0.5 * (lo + hi);
-                                    comov(P, p, z_at_chi(F, mid), q);
-                                    const double gm = dist_of(q[0] + bx, q[1] + by, F.cheb) - mid;
-                                    if ((gm < 0.0) == inside_prev)
-                                        lo = mid;
-                                    else
-                                        hi = mid;
-                                }
-                                chi = 0.5 * (lo + hi);
-                                zemit = z_at_chi(F, chi);
-                            }
-                            comov(P, p, zemit, q);
+                            // the crossing's descriptor; k_frame_bisect refines it in place
                             ug_hit hrec;
                             hrec.src = (uint32_t)p;
-                            hrec.dx = (float)(q[0] + bx);
-                            hrec.dy = (float)(q[1] + by);
-                            hrec.chi = (float)chi;
-                            hrec.z_emit = (float)zemit;
+                            hrec.dx = __int_as_float(nx);
+                            hrec.dy = __int_as_float(ny);
+                            hrec.chi = __int_as_float(i);
+                            hrec.z_emit = __int_as_float(prev < 0.0 ? 1 : 0);
                             hits[wpos++] = hrec;
                         }
                     }
@@ -230,6 +201,58 @@ __global__ __launch_bounds__(64) void k_frame_hits(DevPaths P, FrontDev F, doubl
         if (lane == 0)
             counts[p] = cnt;
     }
+}
+
+// One thread per crossing found by k_frame_hits<false>: 24 bisections between rewind rows
+// i - 1 and i (in z for the budget front, in chi otherwise) with fresh evaluations, then the
+// hit record (cuda/frame_bake.cu:326-369).  Separate from the walk so that every lane works
+// on a crossing: in the walk only the few images with crossings of a path would be busy.
+__global__ __launch_bounds__(256) void k_frame_bisect(DevPaths P, FrontDev F, double ox, double oy, ug_hit* hits,
+                                                      uint64_t n) {
+    const uint64_t h = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (h >= n)
+        return;
+    const ug_hit d = hits[h];
+    const int p = (int)d.src, nx = __float_as_int(d.dx), ny = __float_as_int(d.dy);
+    const int i = __float_as_int(d.chi);
+    const bool inside_prev = __float_as_int(d.z_emit) != 0;
+    const double bx = 2.0 * nx - ox, by = 2.0 * ny - oy;
+    double chi, zemit, q[3];
+    if (F.budget) {
+        double lo = F.zg[i - 1], hi = F.zg[i];
+        for (int it = 0; it < 24; it++) {
+            const double mid = 0.5 * (lo + hi);
+            comov(P, p, mid, q);
+            const double gm = dist_of(q[0] + bx, q[1] + by, F.cheb) - reach_dev(F, mid);
+            if ((gm < 0.0) == inside_prev)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        zemit = 0.5 * (lo + hi);
+        chi = reach_dev(F, zemit);
+    } else {
+        double lo = F.chig[i - 1], hi = F.chig[i];
+        for (int it = 0; it < 24; it++) {
+            const double mid = 0.5 * (lo + hi);
+            comov(P, p, z_at_chi(F, mid), q);
+            const double gm = dist_of(q[0] + bx, q[1] + by, F.cheb) - mid;
+            if ((gm < 0.0) == inside_prev)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        chi = 0.5 * (lo + hi);
+        zemit = z_at_chi(F, chi);
+    }
+    comov(P, p, zemit, q);
+    ug_hit hrec;
+    hrec.src = (uint32_t)p;
+    hrec.dx = (float)(q[0] + bx);
+    hrec.dy = (float)(q[1] + by);
+    hrec.chi = (float)chi;
+    hrec.z_emit = (float)zemit;
+    hits[h] = hrec;
 }
 
 // Equal-time slices as one dense contraction on FP64 MFMA: the unwrapped positions of
@@ -643,6 +666,8 @@ extern "C" int32_t ug_frame_bake_lightcone(ug_paths* P, const ug_front* fr, doub
         if (!rc && total > 0) {
             hipLaunchKernelGGL(k_frame_hits<false>, dim3((unsigned)P->n), dim3(64), lds, st, D, F, ox, oy, K,
                                obs_idx, drows, dcnt, doff, dev_hits);
+            hipLaunchKernelGGL(k_frame_bisect, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, D, F, ox, oy,
+                               dev_hits, total);
             rc = hip_check(hipGetLastError(), "frame hits");
             if (!rc)
                 rc = hip_check(hipStreamSynchronize(st), "frame sync");
