@@ -60,13 +60,17 @@ WORKLOADS = {
 }
 
 
+SPARSE = False  # --sparse: the packs of configs 2, 3 and 5 use the sparse occupied-cell grid
+
+
 def pack_once(ugpack, seed: int, precision: int, profile: int = 0, workload: str = "cfg2"):
     """One pack of the workload; `profile` = the kernels timed with HIP events (PROFILE_*
     bits; counters run whenever it is nonzero)."""
     w = WORKLOADS[workload]
     p = ugpack.Packer(w["dim"], w["T"], seed=seed, precision=precision,
                       stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4,
-                      stop_batch=1 << 20, stop_window=1 << 22, max_attempts=w["max_attempts"])
+                      stop_batch=1 << 20, stop_window=1 << 22, max_attempts=w["max_attempts"],
+                      sparse=SPARSE)
     if profile:
         p.profile(True, kernels=profile)
     st = p.run()
@@ -531,7 +535,11 @@ def main() -> int:
                     help="ensemble workers (cfg4); 0 = tuned on this GPU")
     ap.add_argument("--calib", action="store_true",
                     help="first copy 256 MiB device-to-device (counter calibration for rocprof --pmc)")
+    ap.add_argument("--sparse", action="store_true",
+                    help="pack on the sparse occupied-cell grid instead of the dense cell table")
     args = ap.parse_args()
+    global SPARSE
+    SPARSE = args.sparse
     if args.gpus != 1:
         print("bench.py: only one GPU is measured here (a universe packs on one GPU; "
               "ensembles shard by universe)", file=sys.stderr)
@@ -643,6 +651,7 @@ def main() -> int:
             "dim": WORKLOADS[args.workload]["dim"], "T": WORKLOADS[args.workload]["T"],
             "seeds": list(range(1, args.steps + 1)),
             "precision": "split-bf16" if split else "fp64",
+            "grid": "sparse occupied cells" if SPARSE else "dense cell table",
         },
         "s_to_jam_mean": round(main_r["wall"] / args.steps, 5),
         "cold_first_pack_s": round(cold[prec_main], 5),

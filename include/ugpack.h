@@ -73,7 +73,9 @@ typedef struct ug_pack_config {
     int64_t max_paths;   /* initial path capacity (0 = automatic; grows on demand) */
     int32_t euclid;      /* --euclid-collision: L2 ball sum(d_a^2) <= r^2 instead of the
                             Chebyshev cube (cuda/braid_cuda3d.cu:605-608) */
-    int32_t reserved;
+    int32_t sparse;      /* --sparse: the occupied cells only (O(N T) memory, a hash of
+                            (timestep, cell) buckets) instead of the dense T^(d+1) cell table
+                            (cuda/braid_cuda3d.cu:82-91,1262-1353); same accepted lists */
 } ug_pack_config;
 
 typedef struct ug_pack_status {

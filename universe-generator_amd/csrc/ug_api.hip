@@ -142,6 +142,8 @@ void release(Pack* h) {
     free_dev(h->d_rscount);
     free_dev(h->d_rlist);
     h->rlist_cap = 0;
+    free_dev(h->d_hash);
+    h->hash_cap = 0;
     release_events(h);
     cache_host_free(h->h_pinned);
     h->h_pinned = nullptr;
@@ -233,10 +235,13 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
     if (h->d == 3)
         tab_len *= T;
     const int occw = (T + 2 + 31) / 32 + 1;  // one spare word: a 3-bit window may cross a word
-    if (!with_table)
+    const bool sparse = with_table && cfg.sparse;
+    // occupancy bits with either grid (a 32nd of the dense table), the dense head table only
+    // without --sparse
+    h->occ_len = with_table ? tab_len / (T + 2) * occw : 0;
+    if (!with_table || sparse)
         tab_len = 0;
     h->tab_len = tab_len;
-    h->occ_len = tab_len / (T + 2) * occw;
     int64_t cap0 = cfg.max_paths > 0 ? cfg.max_paths : 4096;
 
 #define UG_ALLOC(ptr, bytes)                                                                 \
@@ -244,10 +249,10 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
         if (rc == UG_OK && !((ptr) = (decltype(ptr))cache_alloc(bytes)))                     \
             rc = fail(UG_ENOMEM, "device allocation of %zu bytes failed", (size_t)(bytes));     \
     } while (0)
-    if (with_table) {
+    if (tab_len > 0)
         UG_ALLOC(h->d_tab, (size_t)tab_len * 4);
+    if (h->occ_len > 0)
         UG_ALLOC(h->d_occ, (size_t)h->occ_len * 4);
-    }
     UG_ALLOC(h->d_filled, 8);
     UG_ALLOC(h->d_flags, (size_t)UG_SUB_MAX);
     UG_ALLOC(h->d_list, (size_t)UG_SUB_MAX * 8);
@@ -270,11 +275,11 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
     if (rc == UG_OK && hipEventCreateWithFlags(&h->ev_report, hipEventDisableTiming) != hipSuccess)
         rc = fail(UG_EHIP, "event creation failed");
     if (rc == UG_OK) {
-        if (with_table)
+        if (tab_len > 0)
             hipLaunchKernelGGL(k_fill_tab, dim3(grid_for(tab_len, 256 * 8)), dim3(256), 0, h->stream, h->d_tab,
                                tab_len);
         rc = hip_check(hipMemsetAsync(h->d_filled, 0, 8, h->stream), "clear filled");
-        if (!rc && with_table)
+        if (!rc && h->occ_len > 0)
             rc = hip_check(hipMemsetAsync(h->d_occ, 0, (size_t)h->occ_len * 4, h->stream), "clear occupancy");
     }
     DevModel& m = h->dm;
@@ -303,6 +308,11 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
     m.pos64 = nullptr;
     m.pos32 = nullptr;
     m.marg = nullptr;
+    m.sparse = sparse ? 1 : 0;
+    m.hslots = nullptr;
+    m.hmask = 0;
+    if (rc == UG_OK && sparse)
+        rc = ensure_hash(h, std::max<int64_t>(cap0, 1024) * T);
     if (rc == UG_OK && with_table)
         rc = ensure_paths(h, cap0);
     if (rc == UG_OK && cfg.precision == UG_PREC_SPLIT_BF16)

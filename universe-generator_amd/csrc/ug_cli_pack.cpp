@@ -271,6 +271,7 @@ int main(int argc, char** argv) {
     cfg.stop_window = 1 << 22;
     cfg.max_n = a.max_n;
     cfg.euclid = a.euclid ? 1 : 0;
+    cfg.sparse = a.sparse ? 1 : 0;
     ug_pack* h = nullptr;
     if (ug_pack_create(&cfg, nullptr, &h) != UG_OK) {
         fprintf(stderr, "%s\n", ug_last_error());
@@ -287,12 +288,15 @@ int main(int argc, char** argv) {
             fprintf(stderr, "subpaths: terms=%d  max-n=%lld\n", a.sub_terms > 0 ? a.sub_terms : a.terms,
                     a.max_n);
     } else {
-        // the dense cell table holds T^3 rows of T + 2 heads (one halo cell each side)
-        fprintf(stderr,
-                "3+1 (torus, phase)%s: T=%d  modmax=%u (maxfreq=%u)  terms=%d  grid cells=%.2e  "
-                "(~%.1f GB)\n",
-                a.sparse ? " (dense cell table)" : "", a.T, modmax, modmax + 1, a.terms, ncell,
-                T * T * T * (T + 2) * 4.0 / 1e9);
+        // the dense cell table holds T^3 rows of T + 2 heads (one halo cell each side); the
+        // sparse grid holds the occupied cells only
+        if (a.sparse)
+            fprintf(stderr, "3+1 (torus, phase) (sparse grid): T=%d  modmax=%u (maxfreq=%u)  terms=%d\n", a.T,
+                    modmax, modmax + 1, a.terms);
+        else
+            fprintf(stderr,
+                    "3+1 (torus, phase): T=%d  modmax=%u (maxfreq=%u)  terms=%d  grid cells=%.2e  (~%.1f GB)\n",
+                    a.T, modmax, modmax + 1, a.terms, ncell, T * T * T * (T + 2) * 4.0 / 1e9);
     }
     ug_pack_status st;
     if (ug_pack_run(h, 0, &st) != UG_OK)

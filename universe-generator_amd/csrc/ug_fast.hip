@@ -200,7 +200,7 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 // Occupancy: 2+1 runs 4 waves per SIMD (at most 128 VGPRs, no spills; 4 blocks of 35 KB LDS
 // fit a CU), 2-4% faster per launch than 3 on one MI355X; 3+1 stays at 3, which is also what
 // its 52 KB of LDS per block allows.
-template <int D, bool SUB>
+template <int D, bool SUB, bool SPARSE>
 __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
                                                         const int64_t* list, const int* dcount,
                                                         int cb0, int cb1, int par, uint8_t* flags,
@@ -291,10 +291,33 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
                 auto row_of = [&](int rr) -> int64_t {
                     return D == 2 ? (int64_t)t * m.T + gx_of(rr) : ((int64_t)t * m.T + gx_of(rr)) * m.T + gy_of(rr);
                 };
+                // one grid point of this lane's visit: certain contact (or, SUB, its group)
+                auto point = [&](int node, const float4& P) {
+                    const float p3[3] = {P.x, P.y, P.z};
+                    float dmax = 0.f, q = 0.f;
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                        float d = X[a] - p3[a];
+                        d = d > 1.f ? d - 2.f : (d < -1.f ? d + 2.f : d);
+                        dmax = fmaxf(dmax, fabsf(d));
+                        q = fmaf(d, d, q);
+                    }
+                    npts++;
+                    const bool sure = m.euclid ? q <= rb2 : dmax <= r - mt;
+                    if (!SUB) {
+                        cert = cert || sure;
+                    } else if (sure && node < nlimit) {
+                        const int gid = acc_gid[node];
+                        gmn = min(gmn, gid);
+                        gmx = max(gmx, gid);
+                        cert = gmn != gmx;  // two groups: nothing more to learn here
+                    }
+                };
                 // occupancy of all neighbour cells first (independent loads, issued together)
                 // as one mask; then up to 2 occupied cells at a time load their heads together
-                // and walk their chains together, so a visit costs about three dependent
-                // memory steps (bits, heads, points)
+                // (from the dense table, or by probing the sparse grid) and walk their chains
+                // together, so a visit costs about three dependent memory steps (bits, heads,
+                // points)
                 uint32_t cells = 0;
 #pragma unroll
                 for (int rr = 0; rr < ROWS; rr++)
@@ -308,7 +331,8 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
                         if (cells) {
                             const int b = __builtin_ctz(cells);
                             cells &= cells - 1u;
-                            node[g] = m.tab[row_of(b / 3) * m.rowlen + c[D - 1] + b % 3];
+                            node[g] = SPARSE ? hash_head(m, cell_hkey(row_of(b / 3), wrap_cell(c[D - 1], b % 3 - 1, m.T), m.T))
+                                             : m.tab[row_of(b / 3) * m.rowlen + c[D - 1] + b % 3];
                         }
                     }
                     while (!cert && (node[0] >= 0 || node[1] >= 0)) {
@@ -321,25 +345,7 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
                         for (int g = 0; g < NG; g++) {
                             if (node[g] < 0)
                                 continue;
-                            const float p3[3] = {P[g].x, P[g].y, P[g].z};
-                            float dmax = 0.f, q = 0.f;
-#pragma unroll
-                            for (int a = 0; a < D; a++) {
-                                float d = X[a] - p3[a];
-                                d = d > 1.f ? d - 2.f : (d < -1.f ? d + 2.f : d);
-                                dmax = fmaxf(dmax, fabsf(d));
-                                q = fmaf(d, d, q);
-                            }
-                            npts++;
-                            const bool sure = m.euclid ? q <= rb2 : dmax <= r - mt;
-                            if (!SUB) {
-                                cert = cert || sure;
-                            } else if (sure && node[g] < nlimit) {
-                                const int gid = acc_gid[node[g]];
-                                gmn = min(gmn, gid);
-                                gmx = max(gmx, gid);
-                                cert = gmn != gmx;  // two groups: nothing more to learn here
-                            }
+                            point(node[g], P[g]);
                             node[g] = __float_as_int(P[g].w);
                         }
                     }
@@ -544,9 +550,12 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
     if (!rc && fs->lds_prefilter > 160 * 1024)
         rc = fail(UG_EINVAL, "split-BF16 tile needs %zu bytes of LDS", fs->lds_prefilter);
     if (!rc) {
-        const void* kf[2] = {tb.d == 2 ? (const void*)k_fast_prefilter<2, false> : (const void*)k_fast_prefilter<3, false>,
-                             tb.d == 2 ? (const void*)k_fast_prefilter<2, true> : (const void*)k_fast_prefilter<3, true>};
-        for (int v = 0; v < 2 && !rc; v++)
+        const void* kf[4] = {
+            tb.d == 2 ? (const void*)k_fast_prefilter<2, false, false> : (const void*)k_fast_prefilter<3, false, false>,
+            tb.d == 2 ? (const void*)k_fast_prefilter<2, true, false> : (const void*)k_fast_prefilter<3, true, false>,
+            tb.d == 2 ? (const void*)k_fast_prefilter<2, false, true> : (const void*)k_fast_prefilter<3, false, true>,
+            tb.d == 2 ? (const void*)k_fast_prefilter<2, true, true> : (const void*)k_fast_prefilter<3, true, true>};
+        for (int v = 0; v < 4 && !rc; v++)
             rc = hip_check(hipFuncSetAttribute(kf[v], hipFuncAttributeMaxDynamicSharedMemorySize, (int)fs->lds_prefilter),
                            "lds attribute");
     }
@@ -608,12 +617,18 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
     const int nblk = par ? std::min(2048, blocks * (cb1 - cb0)) : blocks;
     if (par)
         hipLaunchKernelGGL(k_flags_one, dim3(grid_for(count, 256 * 16)), dim3(256), 0, h->stream, flags, count, dcount);
-    if (h->d == 2)
-        UG_LAUNCH(h, (k_fast_prefilter<2, false>), dim3(nblk), dim3(256), fs->lds_prefilter, f, base, count,
-                  list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0);
+#define UG_FAST(DD, SP)                                                                                     \
+    UG_LAUNCH(h, (k_fast_prefilter<DD, false, SP>), dim3(nblk), dim3(256), fs->lds_prefilter, f, base, count, \
+              list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0)
+    if (h->d == 2 && !f.m.sparse)
+        UG_FAST(2, false);
+    else if (h->d == 2)
+        UG_FAST(2, true);
+    else if (!f.m.sparse)
+        UG_FAST(3, false);
     else
-        UG_LAUNCH(h, (k_fast_prefilter<3, false>), dim3(nblk), dim3(256), fs->lds_prefilter, f, base, count,
-                  list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0);
+        UG_FAST(3, true);
+#undef UG_FAST
     return hip_check(hipGetLastError(), "k_fast_prefilter");
 }
 
@@ -626,12 +641,18 @@ int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, i
     const int64_t ntiles = (count + TILE - 1) / TILE;
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, 2048));
     const FastArgs f = fast_args(h);
-    if (h->d == 2)
-        hipLaunchKernelGGL((k_fast_prefilter<2, true>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
-                           count, nullptr, nullptr, 0, 1, 0, flags, nullptr, h->d_acc_gid, nlimit);
+#define UG_FASTSUB(DD, SP)                                                                                         \
+    hipLaunchKernelGGL((k_fast_prefilter<DD, true, SP>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, \
+                       base, count, nullptr, nullptr, 0, 1, 0, flags, nullptr, h->d_acc_gid, nlimit)
+    if (h->d == 2 && !f.m.sparse)
+        UG_FASTSUB(2, false);
+    else if (h->d == 2)
+        UG_FASTSUB(2, true);
+    else if (!f.m.sparse)
+        UG_FASTSUB(3, false);
     else
-        hipLaunchKernelGGL((k_fast_prefilter<3, true>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, base,
-                           count, nullptr, nullptr, 0, 1, 0, flags, nullptr, h->d_acc_gid, nlimit);
+        UG_FASTSUB(3, true);
+#undef UG_FASTSUB
     return hip_check(hipGetLastError(), "k_fast_prefilter (subpaths)");
 }
 

@@ -46,6 +46,12 @@ struct DevModel {
     const float4* pos32;  // [n][T] (x, y, w, next path in the same cell as int bits)
     // split-BF16 form
     const float* marg;    // [T] certainty margin per timestep
+    // sparse occupied-cell grid (--sparse): the occupancy bits as above, and an
+    // open-addressing hash of the occupied cells' heads in place of tab; slot = {key + 1
+    // (0 = empty), head path in the low 32 bits}
+    int sparse;
+    ulonglong2* hslots;
+    uint64_t hmask;       // slots - 1 (a power of two)
 };
 
 // Canonical FP64 contact predicate between two points at one timestep (torus min-image
@@ -85,6 +91,49 @@ __device__ __forceinline__ uint32_t occ_bits3(const DevModel& m, int64_t row, in
     const uint32_t* w = m.occ + row * m.occw + (c >> 5);
     const uint64_t v = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
     return (uint32_t)(v >> (c & 31)) & 7u;
+}
+
+// Sparse grid: the key of cell c (0 <= c < T) of table row `row`, and the slot probe sequence
+// (linear from a 64-bit finaliser of the key).
+__host__ __device__ __forceinline__ uint64_t cell_hkey(int64_t row, int cell, int T) {
+    return (uint64_t)row * (uint64_t)T + (uint64_t)cell;
+}
+__host__ __device__ __forceinline__ uint64_t hmix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+// Head path of an occupied cell, -1 when the cell is empty.
+__device__ __forceinline__ int hash_head(const DevModel& m, uint64_t key) {
+    uint64_t h = hmix64(key) & m.hmask;
+    for (;;) {
+        const ulonglong2 s = m.hslots[h];
+        if (s.x == key + 1)
+            return (int)(uint32_t)s.y;
+        if (s.x == 0)
+            return -1;
+        h = (h + 1) & m.hmask;
+    }
+}
+
+// Heads of the cells c-1, c, c+1 (wrapped) of table row `row` (t*T + x in 2+1, (t*T + x)*T + y
+// in 3+1), -1 for empty cells: the occupancy bits first, then the occupied cells' heads from
+// the dense table's padded row or from the sparse grid.
+__device__ __forceinline__ void row_heads3(const DevModel& m, int64_t row, int c, int node[3]) {
+    const uint32_t bits = occ_bits3(m, row, c);
+    if (m.sparse) {
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            node[k] = ((bits >> k) & 1u) ? hash_head(m, cell_hkey(row, wrap_cell(c, k - 1, m.T), m.T)) : -1;
+        return;
+    }
+    const int* tr = m.tab + row * m.rowlen + c;  // padded cells c-1..c+1 at c..c+2
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        node[k] = ((bits >> k) & 1u) ? tr[k] : -1;
 }
 
 // Read-only tables of one model, built once per (device, dim, T, terms, form, margin) and
@@ -199,6 +248,9 @@ struct Pack {
     int64_t rlist_cap = 0;
     std::vector<std::array<int64_t, 5>> rounds;  // {start, size, survivors, admitted, fresh}
     int64_t undecided_from = -1;  // max_n reached mid-round: candidates from here are untested
+    // sparse occupied-cell grid
+    ulonglong2* d_hash = nullptr;
+    uint64_t hash_cap = 0;
 };
 
 int32_t fail(int32_t code, const char* fmt, ...);
@@ -256,6 +308,9 @@ int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_tot
 int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* list, int64_t* out,
                           int64_t* S, bool sync = true, const int* dfresh = nullptr);
 int32_t ensure_paths(Pack* h, int64_t need);
+// Sparse grid: room for `points` (path, timestep) entries at load <= 1/2; a larger table is
+// rebuilt from the stored paths.  No-op for the dense table.
+int32_t ensure_hash(Pack* h, int64_t points);
 int32_t all_survive(Pack* h, uint64_t base, int64_t count, int64_t* out);
 // A filter run ahead: fresh candidates [base, base + count) into out, enqueued behind the
 // current sub-batch's first resolve chunk; launched stays true only if no later chunk of the

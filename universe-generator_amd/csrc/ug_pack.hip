@@ -387,13 +387,9 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
         const int gx = wrap_cell(c[0], (D == 2 ? rr : rr / 3) - 1, T);
         const int gy = D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, T);
         const int64_t rid = D == 2 ? (int64_t)t * T + gx : ((int64_t)t * T + gx) * T + gy;
-        const uint32_t bits = occ_bits3(m, rid, c[D - 1]);
-        const int* tr = m.tab + rid * m.rowlen + c[D - 1];  // padded cells c-1..c+1 at c..c+2
         // the occupied cells' heads load together and their chains are walked together
         int node[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-            node[k] = ((bits >> k) & 1u) ? tr[k] : -1;
+        row_heads3(m, rid, c[D - 1], node);
         while (!h && (node[0] >= 0 || node[1] >= 0 || node[2] >= 0)) {
             double P[3][D];
             int nxt[3];
@@ -645,7 +641,8 @@ __global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t fir
                     const int gx = wrap_cell(c[0], (D == 2 ? rr : rr / 3) - 1, T);
                     const int gy = D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, T);
                     const int64_t rid = D == 2 ? (int64_t)t * T + gx : ((int64_t)t * T + gx) * T + gy;
-                    const int* tr = m.tab + rid * m.rowlen + c[D - 1];
+                    int tr[3];
+                    row_heads3(m, rid, c[D - 1], tr);
                     for (int k = 0; k < 3 && !h; k++)
                         for (int nn = tr[k]; nn >= 0 && !h;) {
                             const size_t q = (size_t)nn * T + t;
@@ -932,10 +929,10 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
                     const int gx = wrap_cell(c[0], (D == 2 ? rr : rr / 3) - 1, T);
                     const int gy = D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, T);
                     const int64_t rid = D == 2 ? (int64_t)t * T + gx : ((int64_t)t * T + gx) * T + gy;
-                    const uint32_t bits = occ_bits3(m, rid, c[D - 1]);
-                    const int* tr = m.tab + rid * m.rowlen + c[D - 1];
+                    int tr[3];
+                    row_heads3(m, rid, c[D - 1], tr);
                     for (int k = 0; k < 3; k++)
-                        for (int n = ((bits >> k) & 1u) ? tr[k] : -1; n >= 0;) {
+                        for (int n = tr[k]; n >= 0;) {
                             const size_t q = (size_t)n * T + t;
                             const double* P = m.pos64 + q * D;
                             const bool hh = contact<D>(m, x, P);
@@ -1361,6 +1358,50 @@ __global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, cons
         *nacc = na;
 }
 
+// Sparse grid: path n becomes the head of the cell `key` (the slot claimed by compare-and-swap
+// when the cell was empty, *fresh set then); returns the previous head, -1 if none.  Slots of an
+// empty table hold {0, 0xffffffff}; a table is never more than half full.
+__device__ __forceinline__ int hash_insert(const DevModel& m, uint64_t key, int n, bool* fresh) {
+    uint64_t h = hmix64(key) & m.hmask;
+    for (;;) {
+        const unsigned long long old = atomicCAS(&m.hslots[h].x, 0ull, (unsigned long long)(key + 1));
+        if (old == 0ull || old == key + 1) {
+            *fresh = old == 0ull;
+            return atomicExch((int*)&m.hslots[h].y, n);
+        }
+        h = (h + 1) & m.hmask;
+    }
+}
+
+__global__ void k_hash_clear(ulonglong2* slots, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        slots[i] = make_ulonglong2(0ull, 0xffffffffull);
+}
+
+// Rebuild of the sparse grid into a larger (cleared) table: every stored (path, timestep)
+// is inserted again from its canonical position and its chain link rewritten.
+template <int D>
+__global__ __launch_bounds__(256) void k_hash_rebuild(DevModel m, int64_t rows, float4* pos32) {
+    const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (item >= rows * m.T)
+        return;
+    const int64_t n = item / m.T;
+    const int t = (int)(item % m.T);
+    const size_t q = (size_t)n * m.T + t;
+    int c[D];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        c[a] = cell_of(m.pos64[q * D + a], m.halfT, m.T);
+    int64_t rid = (int64_t)t * m.T + c[0];
+    if (D == 3)
+        rid = rid * m.T + c[1];
+    bool fresh = false;
+    const int prev = hash_insert(m, cell_hkey(rid, c[D - 1], m.T), (int)n, &fresh);
+    float4 r = pos32[q];
+    r.w = __int_as_float(prev);
+    pos32[q] = r;
+}
+
 // Insert the chunk's accepted survivors into the cell table, one thread per (path, timestep):
 // path n becomes the head of its (t, cell) chain and links to the previous head
 // (cuda/braid_cuda.cu:865-879).  Accepted paths of one chunk never collide, so each cell
@@ -1392,9 +1433,14 @@ __global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, c
         int64_t rid = (int64_t)t * m.T + c[0];
         if (D == 3)
             rid = rid * m.T + c[1];
-        const int64_t row = rid * m.rowlen;
         const int cl = c[D - 1];
-        const int prev = atomicExch(&m.tab[row + cl + 1], (int)n);
+        int prev;
+        if (m.sparse) {
+            prev = hash_insert(m, cell_hkey(rid, cl, m.T), (int)n, &fresh);
+        } else {
+            prev = atomicExch(&m.tab[rid * m.rowlen + cl + 1], (int)n);
+            fresh = prev < 0;
+        }
         // occupancy bit of the padded cell, and of its halo copy at the row ends
         uint32_t* ob = m.occ + rid * m.occw;
         atomicOr(&ob[(cl + 1) >> 5], 1u << ((cl + 1) & 31));
@@ -1402,7 +1448,6 @@ __global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, c
             atomicOr(&ob[(m.T + 1) >> 5], 1u << ((m.T + 1) & 31));
         if (cl == m.T - 1)
             atomicOr(&ob[0], 1u);
-        fresh = prev < 0;
         pos32[o] = make_float4(f[0], f[1], f[2], __int_as_float(prev));
         if (t == 0) {
             acc_idx[n] = surv_idx[s];
@@ -1419,7 +1464,7 @@ __device__ __forceinline__ void halo_item(const DevModel& m, const double* pos, 
                                           const int* nacc, int limit, int64_t item) {
     const int na = min(*nacc, limit);
     const int k = (int)(item / m.T), t = (int)(item % m.T);
-    if (k >= na)
+    if (k >= na || m.sparse)  // the sparse grid has no halo cells
         return;
     const double* X = pos + ((size_t)acc_list[k] * m.T + t) * D;
     int c[D];
@@ -1783,6 +1828,38 @@ int32_t ensure_paths(Pack* h, int64_t need) {
     return rc;
 }
 
+int32_t ensure_hash(Pack* h, int64_t points) {
+    if (!h->dm.sparse || (uint64_t)points * 2 <= h->hash_cap)
+        return UG_OK;
+    // load at most 1/2 before the next insert; a new table starts at most 1/3 full
+    uint64_t cap = 1u << 16;
+    while (cap < (uint64_t)points * 3)
+        cap *= 2;
+    ulonglong2* slots = (ulonglong2*)cache_alloc((size_t)cap * sizeof(ulonglong2));
+    if (!slots)
+        return fail(UG_ENOMEM, "sparse grid of %llu slots", (unsigned long long)cap);
+    hipLaunchKernelGGL(k_hash_clear, dim3(grid_for((int64_t)cap, 256 * 8)), dim3(256), 0, h->stream, slots, cap);
+    ulonglong2* old = h->d_hash;
+    h->d_hash = slots;
+    h->hash_cap = cap;
+    h->dm.hslots = slots;
+    h->dm.hmask = cap - 1;
+    const int64_t rows = h->n + h->nsub;
+    if (rows > 0) {
+        const int nb = grid_for(rows * h->T, 256);
+        if (h->d == 2)
+            hipLaunchKernelGGL(k_hash_rebuild<2>, dim3(nb), dim3(256), 0, h->stream, h->dm, rows, h->d_pos32);
+        else
+            hipLaunchKernelGGL(k_hash_rebuild<3>, dim3(nb), dim3(256), 0, h->stream, h->dm, rows, h->d_pos32);
+    }
+    int32_t rc = hip_check(hipGetLastError(), "sparse grid rebuild");
+    // the old table may serve another handle only once nothing reads it
+    if (!rc)
+        rc = hip_check(hipStreamSynchronize(h->stream), "sparse grid sync");
+    cache_free(old);
+    return rc;
+}
+
 // Conflict bits among the chunk's s survivors (trajectories in d_spos, mask cleared); with
 // dS the count is min(s, *dS), known only on the device (grid-stride launches).
 // The chunk's packed cells (d_scell) and cleared mask rows come from its trajectory launch.
@@ -1829,6 +1906,8 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         const int s = dev ? UG_S_MAX : (int)std::min<int64_t>(S, UG_S_MAX);
         const int* cs = dev ? dS : nullptr;
         rc = ensure_paths(h, h->n + s);
+        if (!rc)
+            rc = ensure_hash(h, (h->n + h->nsub + s) * h->T);
         if (rc)
             return rc;
         // trajectories, packed cells and cleared conflict rows of the chunk in one launch
@@ -1976,6 +2055,8 @@ int32_t resolve_sub(Pack* h, int64_t* list, int* gl, int64_t S, int64_t* acc_tot
         const int s = (int)std::min<int64_t>(S, UG_S_MAX);
         const int64_t n0 = h->n + h->nsub;
         rc = ensure_paths(h, n0 + s);
+        if (!rc)
+            rc = ensure_hash(h, (n0 + s) * h->T);
         if (rc)
             return rc;
         rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, nullptr, 0, h->d_scell, h->d_mask);

@@ -36,7 +36,7 @@ class PackResult:
 
 def pack(dim: int, T: int, *, terms: int = 2, seed: int = 12345, attempts: float = 1e8,
          accept_rate: float = 0.0, fixed_window: bool = False, max_n: int = 0,
-         pin_sin1: bool = False, euclid: bool = False, subpaths: bool = False,
+         pin_sin1: bool = False, euclid: bool = False, sparse: bool = False, subpaths: bool = False,
          sub_attempts: float = 0.0, sub_accept_rate: float = 0.0, sub_fill_rate: float = 0.0,
          precision: str = "split", want_params: bool = True) -> PackResult:
     """One run with the engine's flag semantics (defaults are the engine's: T=120 aside,
@@ -47,7 +47,7 @@ def pack(dim: int, T: int, *, terms: int = 2, seed: int = 12345, attempts: float
     kind = abi.STOP_FIXED_WINDOW if fixed_window and accept_rate > 0 else abi.STOP_REF_WINDOW
     with abi.Packer(dim, T, terms=max(2, terms), seed=seed, precision=prec, pin_sin1=pin_sin1,
                     stop_kind=kind, max_attempts=attempts, stop_rate=accept_rate, max_n=max_n,
-                    euclid=euclid) as p:
+                    euclid=euclid, sparse=sparse) as p:
         st = p.run()
         if subpaths:
             st = p.run_sub(max_attempts=sub_attempts or attempts, accept_rate=sub_accept_rate,
@@ -61,7 +61,7 @@ def pack(dim: int, T: int, *, terms: int = 2, seed: int = 12345, attempts: float
 
 def command(dim: int, T: int, *, attempts: float, seed: int, accept_rate: float,
             terms: int = 2, subpaths: bool = False, sub_attempts: float = 0.0,
-            euclid: bool = False, curve_path: str = "curve.csv",
+            euclid: bool = False, sparse: bool = False, curve_path: str = "curve.csv",
             binary: str | None = None) -> list[str]:
     """The argv braidlab.engine.build_command would produce, aimed at this package's binary."""
     exe = binary or os.path.join(BIN_DIR, "braid_ug" if dim == 2 else "braid_ug3d")
@@ -69,6 +69,8 @@ def command(dim: int, T: int, *, attempts: float, seed: int, accept_rate: float,
            "--seed", str(seed), "--until-accept-rate", repr(accept_rate)]
     if euclid:
         cmd += ["--euclid-collision"]
+    if sparse:
+        cmd += ["--sparse"]
     if terms != 2:
         cmd += ["--terms", str(terms)]
     if subpaths:
