@@ -36,10 +36,38 @@ def test_abi_version_and_error_string():
     assert isinstance(L.ug_last_error(), bytes)
 
 
-def test_config_struct_layout_matches_header():
-    # ug_pack_config: 4 i32, u64, 2 i32, 2 f64, 3 i64, f64, i64, 2 i32
-    assert ctypes.sizeof(abi.PackConfig) == 4 * 4 + 8 + 2 * 4 + 2 * 8 + 3 * 8 + 8 + 8 + 2 * 4
-    assert ctypes.sizeof(abi.PackStatus) == 3 * 8 + 2 * 4 + 2 * 8 + 2 * 8 + 2 * 8
+STRUCTS = {  # C struct name -> ctypes mirror
+    "ug_pack_config": abi.PackConfig, "ug_pack_status": abi.PackStatus, "ug_sub_config": abi.SubConfig,
+    "ug_kernel_stats": abi.KernelStats, "ug_front": abi.Front, "ug_observer": abi.Observer,
+    "ug_frame_info": abi.FrameInfo, "ug_knot_config": abi.KnotConfig, "ug_knot_status": abi.KnotStatus,
+}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    # every field's offset and every struct's size, as the C compiler lays them out from
+    # include/ugpack.h, against the ctypes mirrors
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "ugpack.h"', "int main(void) {"]
+    for name, st in STRUCTS.items():
+        lines.append(f'printf("{name} size %zu\\n", sizeof({name}));')
+        for f, _ in st._fields_:
+            lines.append(f'printf("{name} {f} %zu\\n", offsetof({name}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(ln.split()[:2]): int(ln.split()[2]) for ln in out if ln}
+    for name, st in STRUCTS.items():
+        assert got[(name, "size")] == ctypes.sizeof(st), name
+        for f, _ in st._fields_:
+            assert got[(name, f)] == getattr(st, f).offset, (name, f)
 
 
 def test_profile_and_kernel_constants_match_header():

@@ -259,21 +259,36 @@ def test_wrap_reference_examples():
 
 
 def test_rsa_predicate_reference_cases():
-    # tests/test_group_partners.py:65-82: constant comoving positions a1 across the seam
-    # collide; paths parked 1.0 apart never do.  Build them as pinned-wiggle-free packings
-    # through the characterization check on a hand-made acceptance list is not possible,
-    # so check the predicate via two accepted candidates of a real packing instead.
-    m = po.Model(2, 16, seed=1)
-    idx, gid, att = m.pack(5000)
-    X = {i: m.positions(int(i)) for i in idx[:10]}
-    r = m.r
-    for i in idx[:10]:
-        for j in idx[:10]:
-            if i >= j:
-                continue
-            d = np.abs(X[i] - X[j])
-            d = np.minimum(d, 2.0 - d)
-            assert not np.any(np.all(d <= r, axis=1)), "two accepted paths collide"
+    # The reference's contact cases (tests/test_group_partners.py:65-82, T = 50, exclusion
+    # 2/T), restated on the oracle's predicate with constant comoving positions.
+    m = po.Model(2, 50)
+    T = 50
+    a = np.array([0.3, 0.3])
+    # identical strands touch (from the first timestep)
+    assert m.touches(a, a)
+    # parked 1.0 apart on both axes: no contact, also with a small wiggle on each
+    b = np.array([-0.7, -0.7])
+    assert not m.touches(a, b)
+    assert not m.touches(a + 0.01, b - 0.01) and not m.touches(a - 0.01, b + 0.01)
+    # X = -0.999 against +0.999: the unwrapped gap 1.998 exceeds the cell, the wrapped gap
+    # 0.002 does not -- contact, and the two cells are neighbours across the seam, so the
+    # 3x3 cell scan finds it
+    x, y = np.array([-0.999, 0.0]), np.array([0.999, 0.0])
+    assert m.touches(x, y) and m.touches(y, x)
+    assert (m.cell(-0.999), m.cell(0.999)) == (0, T - 1)
+    # the boundary of the inclusive cube (T = 16, r = 1/8, exact in binary): exactly r apart on
+    # both axes touches, a step beyond does not
+    m16 = po.Model(2, 16)
+    p0 = np.array([0.25, 0.5])
+    assert m16.touches(p0, np.array([0.375, 0.375]))
+    assert not m16.touches(p0, np.array([0.25, np.nextafter(0.625, 1.0)]))
+    # and on a real packing no two accepted paths touch at any timestep
+    mp = po.Model(2, 16, seed=1)
+    idx, _, _ = mp.pack(5000)
+    X = mp.positions_of(idx[:12])
+    for i in range(len(X)):
+        for j in range(i):
+            assert not any(mp.touches(X[i][t], X[j][t]) for t in range(16))
 
 
 # ------------------------------------------------------------------ serial RSA
