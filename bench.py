@@ -282,33 +282,47 @@ def run_knot(args, ugpack, torch) -> int:
     candidates of seed 1 on the host cores, and the GPU the same sample, lists compared."""
     w = WORKLOADS["knot"]
 
-    def one(seed, budget):
-        k = ugpack.Knot(w["dim"], w["T"], seed=seed, pin_sin1=True, max_attempts=budget)
+    def one(seed, budget, prec):
+        k = ugpack.Knot(w["dim"], w["T"], seed=seed, pin_sin1=True, max_attempts=budget, precision=prec)
         st = k.run()
         idx, sp = k.strands()
         k.close()
         return st, idx
 
-    for i in range(args.warmup):
-        one(1001 + i, w["max_attempts"])
-    per, strands = [], []
-    for k in range(args.steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        st, _ = one(k + 1, w["max_attempts"])
-        torch.cuda.synchronize()
-        per.append(time.perf_counter() - t0)
-        strands.append(st.n)
-    wall = sum(per)
+    forms = [ugpack.PREC_SPLIT_BF16, ugpack.PREC_FP64] if args.precision == "both" else \
+        [ugpack.PREC_SPLIT_BF16 if args.precision == "split" else ugpack.PREC_FP64]
+    res = {}
+    for prec in forms:
+        for i in range(args.warmup):
+            one(1001 + i, w["max_attempts"], prec)
+        per, lists = [], []
+        for k in range(args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st, idx = one(k + 1, w["max_attempts"], prec)
+            torch.cuda.synchronize()
+            per.append(time.perf_counter() - t0)
+            lists.append(idx.tolist())
+        res[prec] = (sum(per), lists)
+    main = forms[0]
+    wall, lists = res[main]
+    split = main == ugpack.PREC_SPLIT_BF16
     line = {
         "metric": "candidates tested/s, knot regime (braid_knot), 2+1 F=32 pinned full band",
         "value": round(args.steps * w["max_attempts"] / wall, 1), "unit": "candidates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall / args.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3+f32acc, f64 test" if split else "f64",
         "data": "synthetic (Philox candidate stream, seeds 1..steps)",
-        "config": {"workload": w["text"], "dim": w["dim"], "T": w["T"], "candidates_per_step": w["max_attempts"]},
-        "strands_per_step": strands,
+        "config": {"workload": w["text"], "dim": w["dim"], "T": w["T"], "candidates_per_step": w["max_attempts"],
+                   "precision": "split-bf16" if split else "fp64"},
+        "strands_per_step": [len(x) for x in lists],
     }
+    if len(forms) == 2:
+        fw, fl = res[ugpack.PREC_FP64]
+        line["fp64_form"] = {"value": round(args.steps * w["max_attempts"] / fw, 1),
+                             "ms_per_step": round(1e3 * fw / args.steps, 3), "same_strands_as_split": fl == lists}
+        line["speedup_split_vs_fp64"] = round(fw / wall, 3)
     if args.no_cpu:
         line["cpu_baseline"] = None
     else:
@@ -321,7 +335,7 @@ def run_knot(args, ugpack, torch) -> int:
         dt = time.perf_counter() - t0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st, idx = one(1, M)
+        st, idx = one(1, M, main)
         torch.cuda.synchronize()
         gdt = time.perf_counter() - t0
         line["cpu_baseline"] = {

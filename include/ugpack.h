@@ -253,7 +253,8 @@ typedef struct ug_knot_config {
     double nu_attempts;  /* --nu-attempts: nu-species candidates after them (0 = off) */
     double nu_div;       /* --nu-div k >= 1: the nu exclusion is (2/T)/k */
     int32_t max_strands; /* strand capacity, 0 = 16384 (kMaxStrands, :94-97) */
-    int32_t reserved;
+    int32_t precision;   /* UG_PREC_FP64, or UG_PREC_SPLIT_BF16: full-band models test the first
+                            32 ordered timesteps on the split-BF16 prefilter first; same strands */
 } ug_knot_config;
 
 typedef struct ug_knot_status {

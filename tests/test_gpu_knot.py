@@ -27,16 +27,18 @@ CASES = [  # d, T, terms, seed, pin, budget, nu, nu_div, max_strands
 ]
 
 
+@pytest.mark.parametrize("precision", ["fp64", "split"])
 @pytest.mark.parametrize("case", CASES)
-def test_knot_equals_oracle(gpu, case):
+def test_knot_equals_oracle(gpu, case, precision):
     import ugpack
 
     d, T, terms, seed, pin, budget, nu, div, cap = case
-    print(f"knot {case}", flush=True)
+    print(f"knot {case} {precision}", flush=True)
     want = po.Model(d, T, terms=terms or None, seed=seed, pin=pin).pack_knot(
         budget, nu_attempts=nu, nu_div=div, max_strands=cap or 16384)
+    prec = ugpack.PREC_SPLIT_BF16 if precision == "split" else ugpack.PREC_FP64
     with ugpack.Knot(d, T, terms=terms, seed=seed, pin_sin1=pin, max_attempts=budget, nu_attempts=nu,
-                     nu_div=div, max_strands=cap) as k:
+                     nu_div=div, max_strands=cap, precision=prec) as k:
         st = k.run()
         idx, sp = k.strands()
         rows = k.batches()
@@ -53,7 +55,7 @@ def test_knot_primary_equals_the_grid_packer(gpu):
     import ugpack
 
     for d, T, pin in ((2, 32, True), (3, 16, True), (2, 48, False)):
-        with ugpack.Knot(d, T, seed=7, pin_sin1=pin, max_attempts=2e6) as k:
+        with ugpack.Knot(d, T, seed=7, pin_sin1=pin, max_attempts=2e6, precision=ugpack.PREC_SPLIT_BF16) as k:
             k.run()
             kidx, _ = k.strands()
         with ugpack.Packer(d, T, seed=7, pin_sin1=pin, max_attempts=2e6, precision=ugpack.PREC_SPLIT_BF16) as p:

@@ -8,7 +8,9 @@
 // dump with one row per strand: species, attempt, then per axis a2 and the nw (b, a, f)
 // triples.  Candidates come from this packer's Philox stream (its canonical draw, so slot j
 // carries frequency j + 2 in the full band), evaluated and compared in FP64 through
-// libugpack.so (include/ugpack.h).
+// libugpack.so (include/ugpack.h).  One flag beyond the reference's: --precision fp64|split
+// (default split: full-band models go through the split-BF16 prefilter first; the same
+// strands either way).
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -34,6 +36,7 @@ int main(int argc, char** argv) {
     double budget = 1e10, nu_attempts = 0.0, nu_div = 1.0;
     const char* curve_path = nullptr;
     const char* dump_path = nullptr;
+    int precision = UG_PREC_SPLIT_BF16;
     for (int i = 1; i < argc; i++) {
         const char* f = argv[i];
         const bool has = i + 1 < argc;
@@ -57,6 +60,17 @@ int main(int argc, char** argv) {
             curve_path = argv[++i];
         else if (!strcmp(f, "--dump") && has)
             dump_path = argv[++i];
+        else if (!strcmp(f, "--precision") && has) {
+            const char* p = argv[++i];
+            if (!strcmp(p, "fp64"))
+                precision = UG_PREC_FP64;
+            else if (!strcmp(p, "split"))
+                precision = UG_PREC_SPLIT_BF16;
+            else {
+                fprintf(stderr, "error: --precision must be fp64|split\n");
+                return 1;
+            }
+        }
         else {
             fprintf(stderr, "unknown flag %s\n", f);
             return 1;
@@ -88,6 +102,7 @@ int main(int argc, char** argv) {
     cfg.max_attempts = budget;
     cfg.nu_attempts = nu_attempts;
     cfg.nu_div = nu_div;
+    cfg.precision = precision;
     ug_knot* h = nullptr;
     if (ug_knot_create(&cfg, nullptr, &h) != UG_OK) {
         fprintf(stderr, "%s\n", ug_last_error());

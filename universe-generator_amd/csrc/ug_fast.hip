@@ -127,6 +127,10 @@ struct FastArgs {
     const bf16x8* bfrag;    // [nb][ks][2][64] basis fragments, columns in endpoint-first order
     const float* invk;      // [T+2] 1/k as float
     const int* colt;        // [nb*32] timestep of each column (-1 = padding)
+    // knot regime: the resident strands' FP32 positions [T][ks] (x, y, w, unused), nk of them
+    const float4* kpos32;
+    int64_t kstride;
+    int nk;
 };
 
 // Positions of the wave's 32 candidates at the 32 columns of N block nb into
@@ -200,7 +204,9 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 // Occupancy: 2+1 runs 4 waves per SIMD (at most 128 VGPRs, no spills; 4 blocks of 35 KB LDS
 // fit a CU), 2-4% faster per launch than 3 on one MI355X; 3+1 stays at 3, which is also what
 // its 52 KB of LDS per block allows.
-template <int D, bool SUB, bool SPARSE>
+// KNOT: no grid -- a visit compares the candidate with every resident strand (the knot
+// regime's brute force) and rejects it on a certain contact.
+template <int D, bool SUB, bool SPARSE, bool KNOT = false>
 __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
                                                         const int64_t* list, const int* dcount,
                                                         int cb0, int cb1, int par, uint8_t* flags,
@@ -313,6 +319,15 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
                         cert = gmn != gmx;  // two groups: nothing more to learn here
                     }
                 };
+                if constexpr (KNOT) {
+                    // the strands of this timestep are contiguous: two loads in flight at a time
+                    for (int sidx = 0; sidx < f.nk && !cert; sidx += 2) {
+                        const float4 P0 = f.kpos32[(size_t)t * f.kstride + sidx];
+                        const float4 P1 = f.kpos32[(size_t)t * f.kstride + min(sidx + 1, f.nk - 1)];
+                        point(sidx, P0);
+                        point(sidx + 1, P1);
+                    }
+                } else {
                 // occupancy of all neighbour cells first (independent loads, issued together)
                 // as one mask; then up to 2 occupied cells at a time load their heads together
                 // (from the dense table, or by probing the sparse grid) and walk their chains
@@ -349,6 +364,7 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
                             node[g] = __float_as_int(P[g].w);
                         }
                     }
+                }
                 }
             }
             if (!SUB) {
@@ -550,12 +566,14 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
     if (!rc && fs->lds_prefilter > 160 * 1024)
         rc = fail(UG_EINVAL, "split-BF16 tile needs %zu bytes of LDS", fs->lds_prefilter);
     if (!rc) {
-        const void* kf[4] = {
+        const void* kf[5] = {
             tb.d == 2 ? (const void*)k_fast_prefilter<2, false, false> : (const void*)k_fast_prefilter<3, false, false>,
             tb.d == 2 ? (const void*)k_fast_prefilter<2, true, false> : (const void*)k_fast_prefilter<3, true, false>,
             tb.d == 2 ? (const void*)k_fast_prefilter<2, false, true> : (const void*)k_fast_prefilter<3, false, true>,
-            tb.d == 2 ? (const void*)k_fast_prefilter<2, true, true> : (const void*)k_fast_prefilter<3, true, true>};
-        for (int v = 0; v < 4 && !rc; v++)
+            tb.d == 2 ? (const void*)k_fast_prefilter<2, true, true> : (const void*)k_fast_prefilter<3, true, true>,
+            tb.d == 2 ? (const void*)k_fast_prefilter<2, false, false, true>
+                      : (const void*)k_fast_prefilter<3, false, false, true>};
+        for (int v = 0; v < 5 && !rc; v++)
             rc = hip_check(hipFuncSetAttribute(kf[v], hipFuncAttributeMaxDynamicSharedMemorySize, (int)fs->lds_prefilter),
                            "lds attribute");
     }
@@ -585,6 +603,9 @@ static FastArgs fast_args(Pack* h) {
     f.bfrag = fs->d_bfrag;
     f.invk = fs->d_invk;
     f.colt = fs->d_colt;
+    f.kpos32 = nullptr;
+    f.kstride = 0;
+    f.nk = 0;
     return f;
 }
 
@@ -630,6 +651,28 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
         UG_FAST(3, true);
 #undef UG_FAST
     return hip_check(hipGetLastError(), "k_fast_prefilter");
+}
+
+int32_t launch_fast_knot(Pack* h, uint64_t base, int64_t count, uint8_t* flags, const float4* kpos32, int64_t ks,
+                         int nk) {
+    if (count <= 0)
+        return UG_OK;
+    if (!h->full || !h->d_basis16)
+        return fail(UG_ESTATE, "split-BF16 prefilter not set up for this model");
+    FastState* fs = (FastState*)h->d_basis16;
+    const int64_t ntiles = (count + TILE - 1) / TILE;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((ntiles + FAST_WAVES - 1) / FAST_WAVES, 8192));
+    FastArgs f = fast_args(h);
+    f.kpos32 = kpos32;
+    f.kstride = ks;
+    f.nk = nk;
+    if (h->d == 2)
+        UG_LAUNCH(h, (k_fast_prefilter<2, false, false, true>), dim3(blocks), dim3(256), fs->lds_prefilter, f, base,
+                  count, nullptr, nullptr, 0, 1, 0, flags, nullptr, nullptr, 0);
+    else
+        UG_LAUNCH(h, (k_fast_prefilter<3, false, false, true>), dim3(blocks), dim3(256), fs->lds_prefilter, f, base,
+                  count, nullptr, nullptr, 0, 1, 0, flags, nullptr, nullptr, 0);
+    return hip_check(hipGetLastError(), "k_fast_prefilter (knot)");
 }
 
 int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, int64_t nlimit) {

@@ -344,6 +344,10 @@ int fast_blocks(const Pack* h);  // N blocks of the model's ordered timesteps (0
 // among the rows < nlimit.
 int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, int64_t nlimit);
 int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* out);
+// Knot regime: flags[i] = 0 when candidate base + i certainly touches one of the nk strands
+// (FP32 positions kpos32[t][ks]) in the first 32 ordered timesteps.
+int32_t launch_fast_knot(Pack* h, uint64_t base, int64_t count, uint8_t* flags, const float4* kpos32, int64_t ks,
+                         int nk);
 // Conflict bits among a resolve chunk's s survivors (trajectories in d_spos, packed cells in
 // d_scell, mask rows cleared); dS: count bounded on the device; expect: survivors expected.
 void launch_conflicts(Pack* h, int s, const int* dS = nullptr, int64_t expect = -1);
@@ -354,8 +358,8 @@ int32_t launch_greedy(Pack* h, int s);
 int32_t launch_knot_test(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
                          const int* dcount, const int* kcell, const double* kpos, int64_t ks, int nk);
 // Appends the chunk's accepted survivors (d_acc_list, at most limit) of list as strands n0, ...
-int32_t launch_knot_append(Pack* h, const int64_t* list, int s, int64_t n0, int* kcell, double* kpos, int64_t ks,
-                           int64_t* kidx, int limit);
+int32_t launch_knot_append(Pack* h, const int64_t* list, int s, int64_t n0, int* kcell, double* kpos, float4* kpos32,
+                           int64_t ks, int64_t* kidx, int limit);
 // Handle set-up shared by the packer and the knot engine: with_table = false builds no cell
 // table, occupancy bits or path records (the knot engine keeps its own strands).
 int32_t create_handle(const ug_pack_config* cfg, void* hip_stream, bool with_table, Pack** out);

@@ -4,7 +4,9 @@
 // Candidates are counter-indexed on the packer's Philox stream and admitted in index order,
 // so a run is serial RSA per species and does not depend on batch sizes.  Each batch is
 // tested against the strands resident at its start by k_knot_test (one wave per candidate,
-// canonical FP64 evaluation, every strand compared at every visited timestep); the survivors
+// canonical FP64 evaluation, every strand compared at every visited timestep), in the
+// split-BF16 form after the packer's prefilter in its knot mode has removed the candidates
+// with a certain contact in the first 32 ordered timesteps; the survivors
 // are resolved in index order by the packer's own resolve kernels (trajectories, pairwise
 // conflict bits, greedy) and appended to the timestep-major strand arrays.  Two phases, as
 // cuda/braid_knot.cu:387-399: the primary species at the exclusion 2/T over candidates
@@ -30,6 +32,8 @@ struct ug_knot {
     ug_knot_config cfg{};
     int* d_kcell = nullptr;   // [T][ks] packed cells of the strands
     double* d_kpos = nullptr; // [T][ks][D] canonical positions
+    float4* d_kpos32 = nullptr;  // [T][ks] the same in FP32 (split-BF16 prefilter)
+    bool split = false;
     int64_t* d_kidx = nullptr;
     int64_t ks = 0, nk = 0, n_primary = 0, primary_attempts = 0;
     int64_t attempts = 0, batch = 4096;
@@ -56,6 +60,7 @@ void knot_release(ug_knot* K) {
         (void)hipStreamSynchronize(K->p->stream);
     free_cached(K->d_kcell);
     free_cached(K->d_kpos);
+    free_cached(K->d_kpos32);
     free_cached(K->d_kidx);
     free_cached(K->d_blist);
     if (K->p)
@@ -90,7 +95,7 @@ int32_t knot_resolve(ug_knot* K, int64_t* list, int64_t S, int64_t* admitted) {
         rc = launch_greedy(h, s);
         const int limit = (int)std::min<int64_t>(K->ks - K->nk, INT32_MAX);
         if (!rc)
-            rc = launch_knot_append(h, list, s, K->nk, K->d_kcell, K->d_kpos, K->ks, K->d_kidx, limit);
+            rc = launch_knot_append(h, list, s, K->nk, K->d_kcell, K->d_kpos, K->d_kpos32, K->ks, K->d_kidx, limit);
         if (!rc)
             rc = hip_check(hipMemcpyAsync(pin, h->d_nacc, 4, hipMemcpyDeviceToHost, h->stream), "knot count");
         if (!rc)
@@ -153,6 +158,17 @@ int32_t knot_batch(ug_knot* K, uint64_t base, int64_t b, int64_t* S_out, int64_t
         int64_t Sp = piece;
         if (nk0 == 0) {
             rc = all_survive(h, base + (uint64_t)off, piece, K->d_blist + S);
+        } else if (K->split) {
+            // certain contacts in the first 32 ordered timesteps on the split-BF16 prefilter,
+            // then the exact FP64 test of the rest
+            int64_t S1 = 0;
+            rc = launch_fast_knot(h, base + (uint64_t)off, piece, h->d_flags, K->d_kpos32, K->ks, nk0);
+            if (!rc)
+                rc = compact(h, h->d_flags, piece, base + (uint64_t)off, nullptr, h->d_list3, &S1);
+            if (!rc)
+                rc = launch_knot_test(h, 0, S1, h->d_list3, h->d_flags, nullptr, K->d_kcell, K->d_kpos, K->ks, nk0);
+            if (!rc)
+                rc = compact(h, h->d_flags, S1, 0, h->d_list3, K->d_blist + S, &Sp);
         } else {
             rc = launch_knot_test(h, base + (uint64_t)off, piece, nullptr, h->d_flags, nullptr, K->d_kcell, K->d_kpos,
                                   K->ks, nk0);
@@ -202,7 +218,9 @@ extern "C" int32_t ug_knot_create(const ug_knot_config* cfg_in, void* hip_stream
     pc.dim = cfg.dim;
     pc.T = cfg.T;
     pc.terms = cfg.terms;
-    pc.precision = UG_PREC_FP64;
+    if (cfg.precision != UG_PREC_FP64 && cfg.precision != UG_PREC_SPLIT_BF16)
+        return fail(UG_EINVAL, "error: unknown precision %d", cfg.precision);
+    pc.precision = cfg.precision;
     pc.seed = cfg.seed;
     pc.pin_sin1 = cfg.pin_sin1;
     pc.stop_kind = UG_STOP_BUDGET;
@@ -217,8 +235,9 @@ extern "C" int32_t ug_knot_create(const ug_knot_config* cfg_in, void* hip_stream
     K->ks = cfg.max_strands;
     K->r0 = h->dm.r;
     const size_t cells = (size_t)h->T * (size_t)K->ks;
+    K->split = cfg.precision == UG_PREC_SPLIT_BF16 && h->full;
     if (!(K->d_kcell = (int*)cache_alloc(cells * 4)) || !(K->d_kpos = (double*)cache_alloc(cells * h->d * 8)) ||
-        !(K->d_kidx = (int64_t*)cache_alloc((size_t)K->ks * 8)))
+        !(K->d_kpos32 = (float4*)cache_alloc(cells * 16)) || !(K->d_kidx = (int64_t*)cache_alloc((size_t)K->ks * 8)))
         rc = fail(UG_ENOMEM, "ug_knot_create: strand storage for %lld strands", (long long)K->ks);
     if (rc) {
         knot_release(K);

@@ -795,7 +795,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_knot_append(DevModel m, const double* pos, const int* cells,
                                                      const int* acc_list, const int* nacc, int limit,
                                                      const int64_t* surv_idx, int64_t n0, int* kcell, double* kpos,
-                                                     int64_t ks, int64_t* kidx) {
+                                                     float4* kpos32, int64_t ks, int64_t* kidx) {
     const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int na = min(*nacc, limit);
     const int k = (int)(item / m.T), t = (int)(item % m.T);
@@ -803,9 +803,13 @@ __global__ __launch_bounds__(256) void k_knot_append(DevModel m, const double* p
         return;
     const int sidx = acc_list[k];
     const size_t src = (size_t)sidx * m.T + t, dst = (size_t)t * ks + (size_t)(n0 + k);
+    float f[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int a = 0; a < D; a++)
+    for (int a = 0; a < D; a++) {
         kpos[dst * D + a] = pos[src * D + a];
+        f[a] = (float)pos[src * D + a];
+    }
+    kpos32[dst] = make_float4(f[0], f[1], f[2], 0.f);
     kcell[dst] = cells[src];
     if (t == 0)
         kidx[n0 + k] = surv_idx[sidx];
@@ -2213,15 +2217,15 @@ int32_t launch_knot_test(Pack* h, uint64_t base, int64_t count, const int64_t* l
     return hip_check(hipGetLastError(), "k_knot_test");
 }
 
-int32_t launch_knot_append(Pack* h, const int64_t* list, int s, int64_t n0, int* kcell, double* kpos, int64_t ks,
-                           int64_t* kidx, int limit) {
+int32_t launch_knot_append(Pack* h, const int64_t* list, int s, int64_t n0, int* kcell, double* kpos, float4* kpos32,
+                           int64_t ks, int64_t* kidx, int limit) {
     const int ib = grid_for((int64_t)s * h->T, 256);
     if (h->d == 2)
         hipLaunchKernelGGL(k_knot_append<2>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell,
-                           h->d_acc_list, h->d_nacc, limit, list, n0, kcell, kpos, ks, kidx);
+                           h->d_acc_list, h->d_nacc, limit, list, n0, kcell, kpos, kpos32, ks, kidx);
     else
         hipLaunchKernelGGL(k_knot_append<3>, dim3(ib), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell,
-                           h->d_acc_list, h->d_nacc, limit, list, n0, kcell, kpos, ks, kidx);
+                           h->d_acc_list, h->d_nacc, limit, list, n0, kcell, kpos, kpos32, ks, kidx);
     return hip_check(hipGetLastError(), "k_knot_append");
 }
 

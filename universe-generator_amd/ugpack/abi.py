@@ -61,7 +61,7 @@ class KernelStats(C.Structure):
 class KnotConfig(C.Structure):
     _fields_ = [("dim", C.c_int32), ("T", C.c_int32), ("terms", C.c_int32), ("pin_sin1", C.c_int32),
                 ("seed", C.c_uint64), ("max_attempts", C.c_double), ("nu_attempts", C.c_double),
-                ("nu_div", C.c_double), ("max_strands", C.c_int32), ("reserved", C.c_int32)]
+                ("nu_div", C.c_double), ("max_strands", C.c_int32), ("precision", C.c_int32)]
 
 
 class KnotStatus(C.Structure):
@@ -348,10 +348,10 @@ class Knot:
 
     def __init__(self, dim: int, T: int, terms: int = 0, seed: int = 1, pin_sin1: bool = False,
                  max_attempts: float = 1e10, nu_attempts: float = 0.0, nu_div: float = 1.0,
-                 max_strands: int = 0, stream: int = 0):
+                 max_strands: int = 0, stream: int = 0, precision: int = PREC_FP64):
         self._h = None
         cfg = KnotConfig(dim, T, terms, int(pin_sin1), seed, float(max_attempts), float(nu_attempts),
-                         float(nu_div), max_strands, 0)
+                         float(nu_div), max_strands, precision)
         h = C.c_void_p()
         check(lib().ug_knot_create(C.byref(cfg), C.c_void_p(stream), C.byref(h)))
         self._h = h
