@@ -197,6 +197,11 @@ def bake_frames(ugpack, torch, packer, nframes: int, px: int) -> dict:
             "raster_mass_ok": abs(mass - nframes * paths.n) < 0.5}
 
 
+def progress(msg: str) -> None:
+    """A step-done line on stderr (the JSON result line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def ensemble_configs(ugpack, n: int, cands: float, precision: int):
     """The ensemble's universes: T and dim drawn from a fixed-seed generator."""
     import numpy as np
@@ -215,13 +220,17 @@ def run_ensemble(args, ugpack, torch, forms, prec_main) -> int:
     w = WORKLOADS["cfg4"]
     n, cands, threads = w["universes"], w["max_attempts"], args.threads
     sweep = None
+    plan = {"threads": threads, "sub_target": 1024, "sub_max": 1 << 20}
     if threads <= 0:
-        # the in-flight universe count is tuned on this GPU: a 128-universe sample of the same
-        # mix (seeds 1..128) at 2*10^6 candidates each, outside the timed region
-        threads, sweep = tune.ensemble_threads(ensemble_configs(ugpack, 128, 2e6, prec_main))
+        # the in-flight universe count and the sub-batch sizing are tuned on this GPU: a
+        # 128-universe sample of the same mix (seeds 1..128) at 2*10^6 candidates each,
+        # outside the timed region
+        plan, sweep = tune.ensemble_plan(ensemble_configs(ugpack, 128, 2e6, prec_main))
+        threads = plan["threads"]
+        progress(f"ensemble tuned: {plan}")
     res = {}
     for prec in forms:
-        cfgs = ensemble_configs(ugpack, n, cands, prec)
+        cfgs = tune.with_sizing(ensemble_configs(ugpack, n, cands, prec), plan["sub_target"], plan["sub_max"])
         ugpack.ensemble(ensemble_configs(ugpack, 32, 1e6, prec), threads=threads)  # warmup
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -231,6 +240,7 @@ def run_ensemble(args, ugpack, torch, forms, prec_main) -> int:
         wall = (time.perf_counter() - t0) / args.steps
         res[prec] = dict(wall=wall, value=n * cands / wall, n=[s.n for s in st],
                          attempts=sum(s.attempts for s in st))
+        progress(f"ensemble precision {prec}: {wall:.3f} s per step")
     # roofline of the dominant universe class (T=128, 3+1) from a profiled pack at 10^7
     p = ugpack.Packer(3, 128, seed=1, precision=prec_main, max_attempts=cands)
     p.profile(True)
@@ -248,8 +258,8 @@ def run_ensemble(args, ugpack, torch, forms, prec_main) -> int:
         "dtype": "bf16x3+f32acc, f64 recheck" if split else "f64",
         "data": "synthetic (Philox candidate streams, seeds 1..512)",
         "config": {"workload": w["text"], "universes": n, "candidates_per_universe": cands,
-                   "threads": threads,
-                   "threads_sweep_s": {str(k): round(v, 4) for k, v in sweep.items()} if sweep else None,
+                   "threads": threads, "sub_target": plan["sub_target"], "sub_max": plan["sub_max"],
+                   "tuning_sweep_s": {str(k): round(v, 4) for k, v in sweep.items()} if sweep else None,
                    "T_counts": {str(T): sum(1 for c in cfgs if c.T == T) for T in (16, 32, 64, 128)},
                    "dim3_count": sum(1 for c in cfgs if c.dim == 3),
                    "precision": "split-bf16" if split else "fp64"},
@@ -644,6 +654,7 @@ def main() -> int:
             p.close()
         results[prec] = dict(value=cands / wall, wall=wall, per_pack=per_pack, accepted=accepted,
                              parity=parity, stats=stats, rstats=rstats, cands=cands, digests=digests)
+        progress(f"{args.workload} precision {prec}: {args.steps} packs, {cands / wall:.4g} candidates/s")
 
     main_r = results[prec_main]
     split = prec_main == ugpack.PREC_SPLIT_BF16

@@ -76,6 +76,9 @@ typedef struct ug_pack_config {
     int32_t sparse;      /* --sparse: the occupied cells only (O(N T) memory, a hash of
                             (timestep, cell) buckets) instead of the dense T^(d+1) cell table
                             (cuda/braid_cuda3d.cu:82-91,1262-1353); same accepted lists */
+    int32_t sub_target;  /* sub-batch sizing (budget and fixed-window rules): survivors aimed
+                            at per sub-batch, 0 = 1024; speed only, never results */
+    int32_t sub_max;     /* largest sub-batch, a power of two <= 2^20 (0 = 2^20) */
 } ug_pack_config;
 
 typedef struct ug_pack_status {

@@ -204,6 +204,12 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
         cfg.stop_window = 1 << 22;
     if (cfg.stop_kind == UG_STOP_FIXED_WINDOW && (cfg.stop_batch & (cfg.stop_batch - 1)))
         return fail(UG_EINVAL, "error: stop_batch must be a power of two");
+    if (cfg.sub_target <= 0)
+        cfg.sub_target = 1024;
+    if (cfg.sub_max <= 0)
+        cfg.sub_max = UG_SUB_MAX;
+    if (cfg.sub_max < 4096 || cfg.sub_max > UG_SUB_MAX || (cfg.sub_max & (cfg.sub_max - 1)))
+        return fail(UG_EINVAL, "error: sub_max must be a power of two in [4096, %d]", UG_SUB_MAX);
     if (!(cfg.max_attempts > 0))
         cfg.max_attempts = 9.2e18;
 
@@ -552,7 +558,7 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         fill_status(h, st);
         return rc;
     }
-    const int64_t sub_cap = std::min<int64_t>(UG_SUB_MAX, cfg.stop_kind == UG_STOP_FIXED_WINDOW ? cfg.stop_batch : UG_SUB_MAX);
+    const int64_t sub_cap = std::min<int64_t>(cfg.sub_max, cfg.stop_kind == UG_STOP_FIXED_WINDOW ? cfg.stop_batch : UG_SUB_MAX);
     int32_t rc = UG_OK;
     // The size of the sub-batch starting at `from`: h->sub, cut at the budget, the call's end
     // and (fixed window) the next stop-rule boundary; 0 when none may start there.
@@ -630,14 +636,15 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         } else {
             h->attempts += sub;
         }
-        // Sub-batch size from the measured rates, aiming at ~1024 survivors (one in-batch
-        // resolve chunk) and at most ~16k candidates for the FP64 recheck per sub-batch (the
+        // Sub-batch size from the measured rates, aiming at ~sub_target survivors (1024 by
+        // default: one in-batch resolve chunk) and at most ~16k candidates for the FP64 recheck
+        // per sub-batch (the
         // reference adapts toward ~256 survivors, cuda/braid_cuda.cu:920-924).  Results do
         // not depend on it.
         {
             const double surv = (double)std::max<int64_t>(1, S);
             const double rech = (double)std::max<int64_t>(1, h->rechecked - rechecked_before);
-            double want = std::min(1024.0 * (double)sub / surv, 16384.0 * (double)sub / rech);
+            double want = std::min((double)cfg.sub_target * (double)sub / surv, 16384.0 * (double)sub / rech);
             want = std::min(want, 8.0 * (double)sub);
             int64_t next = 4096;
             while (next * 2 <= (int64_t)want && next * 2 <= sub_cap)

@@ -36,6 +36,7 @@ class PackConfig(C.Structure):
         ("max_attempts", C.c_double), ("stop_rate", C.c_double), ("stop_batch", C.c_int64),
         ("stop_window", C.c_int64), ("max_n", C.c_int64), ("margin", C.c_double),
         ("max_paths", C.c_int64), ("euclid", C.c_int32), ("sparse", C.c_int32),
+        ("sub_target", C.c_int32), ("sub_max", C.c_int32),
     ]
 
 
@@ -198,11 +199,11 @@ class Packer:
                  stop_kind: int = STOP_BUDGET, max_attempts: float = 0.0,
                  stop_rate: float = 0.0, stop_batch: int = 1 << 20, stop_window: int = 1 << 22,
                  max_n: int = 0, margin: float = 0.0, max_paths: int = 0, stream: int = 0,
-                 euclid: bool = False, sparse: bool = False):
+                 euclid: bool = False, sparse: bool = False, sub_target: int = 0, sub_max: int = 0):
         self._h = None
         cfg = PackConfig(dim, T, terms, precision, seed, int(pin_sin1), stop_kind,
                          float(max_attempts), float(stop_rate), stop_batch, stop_window, max_n,
-                         float(margin), max_paths, int(euclid), int(sparse))
+                         float(margin), max_paths, int(euclid), int(sparse), sub_target, sub_max)
         h = C.c_void_p()
         check(lib().ug_pack_create(C.byref(cfg), C.c_void_p(stream), C.byref(h)))
         self._h = h
@@ -410,10 +411,11 @@ def pack_config(dim: int, T: int, terms: int = 0, seed: int = 1, precision: int 
                 pin_sin1: bool = False, stop_kind: int = STOP_BUDGET, max_attempts: float = 0.0,
                 stop_rate: float = 0.0, stop_batch: int = 1 << 20, stop_window: int = 1 << 22,
                 max_n: int = 0, margin: float = 0.0, max_paths: int = 0,
-                euclid: bool = False, sparse: bool = False) -> PackConfig:
+                euclid: bool = False, sparse: bool = False, sub_target: int = 0,
+                sub_max: int = 0) -> PackConfig:
     return PackConfig(dim, T, terms, precision, seed, int(pin_sin1), stop_kind, float(max_attempts),
                       float(stop_rate), stop_batch, stop_window, max_n, float(margin), max_paths,
-                      int(euclid), int(sparse))
+                      int(euclid), int(sparse), sub_target, sub_max)
 
 
 def list_digest(idx) -> int:
