@@ -168,6 +168,17 @@ int32_t ug_pack_rounds(const ug_pack* h, int64_t* rows5, int64_t cap, int64_t* n
 int32_t ug_eval_candidates(const ug_pack* h, const int64_t* dev_idx, int64_t count,
                            int32_t precision, double* dev_out);
 
+/* Accuracy of the split-BF16 prefilter's FP32 generation on the current device, measured
+ * exhaustively against FP64: out2[0] = the largest absolute error of (cos, sin) of a phase
+ * over all 2^32 phase words; out2[1] = the largest |e - e_FP64| / (1 + e_FP64) of an
+ * exponential spacing e over all 2^31 Dirichlet words.  With spacing sums W >= nw/4 (smaller
+ * ones go to the FP64 test) a coefficient is then within (out2[0] + 5 out2[1]) of the unit
+ * slope budget, and the certainty margin allots 2^-20 to that.  Runs on the given stream and
+ * synchronizes it.
+ * No counterpart in the reference (its FP32 prefilter uses a fixed one-sided margin,
+ * cuda/braid_cuda3d.cu:938-948). */
+int32_t ug_split_generation_error(void* hip_stream, double* out2);
+
 /* Per-kernel measurement of the candidate-test kernels.  When enabled, every launch of the
  * selected kernels is bracketed by a pair of timing-only HIP events on the handle's stream
  * (read back by ug_pack_kernel_stats, so the pack waits on nothing extra; each event still

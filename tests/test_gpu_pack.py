@@ -238,6 +238,21 @@ def test_split_positions_within_margin(gpu, d, T):
     assert err < margin
 
 
+def test_split_generation_error_inside_margin_budget(gpu):
+    """The prefilter's FP32 generation (hardware sin/cos of the phases, v_log of the Dirichlet
+    words) measured exhaustively on the device: with W >= nw/4 (smaller sums go to the FP64
+    test) a coefficient's error is at most (e_phase + 5 e_expo) times the unit budget, and the
+    certainty margin allots 2^-20 to it."""
+    import ugpack
+
+    print("split generation error (all 2^32 phase words)", flush=True)
+    e_phase, e_expo = ugpack.abi.split_generation_error()
+    print(f"phase {e_phase:.3e}, expo {e_expo:.3e}: {(e_phase + 5 * e_expo) / 2**-20:.3f} of 2^-20")
+    assert 0 < e_phase < 2.5e-7
+    assert 0 < e_expo
+    assert e_phase + 5 * e_expo <= 2**-20
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3, 4])
 def test_split_config1_equals_golden(gpu, seed):
     import ugpack

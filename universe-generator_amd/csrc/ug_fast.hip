@@ -2,13 +2,14 @@
 //
 // Evaluation is the dense contraction X[c][t] = a1[c] + (1/W_c) sum_r A[c][r] B[r][t]:
 // A holds, per frequency pair (k even, k+1 odd), the unnormalized coefficients
-//   (a~_k cos f_k, a~_k sin f_k, a~_{k+1}, 0),   a~ = +-e/k  (e the exponential spacing),
-// and B the canonical basis rows (S_k, C_k, S_{k+1}, 0) at the timesteps.  The reference
+//   (e~_k cos f_k, e~_k sin f_k, e~_{k+1}, 0),   e~ = +-e  (e the exponential spacing),
+// and B the canonical basis rows divided by their frequency, (S_k/k, C_k/k, S_{k+1}/(k+1), 0),
+// at the timesteps.  The reference
 // evaluates the same sum term by term with FP64 sin() (cuda/braid_cuda.cu:418-441) and
 // its --sparse mode from FP32 tables (cuda/braid_cuda3d.cu:630-717).
 //
 // One wave owns a tile of 32 candidates: lanes generate their A fragments directly in
-// registers (Philox + FP32 log / sincospi), split each value into bf16 hi + lo, and run
+// registers (Philox, FP32 log, hardware sin/cos of the phases), split each value into bf16 hi + lo, and run
 // the 3-term product hi*hi + hi*lo + lo*hi on v_mfma_f32_32x32x16_bf16 with FP32
 // accumulation against the basis of the first 32 endpoint-first timesteps, staged in LDS.
 // The lookup then rejects a candidate only on a CERTAIN contact (every axis within
@@ -36,42 +37,22 @@ constexpr int TILE = 32;        // candidates per wave tile (MFMA M)
 constexpr int NCOL = 32;        // timesteps per N block (MFMA N)
 constexpr int FAST_WAVES = 4;   // waves per workgroup
 
-// FP32 sin(pi r), cos(pi r) for |r| <= 1/4 (Taylor in r^2).
-__device__ __forceinline__ void sincospi_f32_small(float r, float& s, float& c) {
-    const float x = r * r;
-    float ps = 0.0821458866f;  // pi^9/9!
-    ps = fmaf(ps, x, -0.599264529f);
-    ps = fmaf(ps, x, 2.55016404f);
-    ps = fmaf(ps, x, -5.16771278f);
-    ps = fmaf(ps, x, 3.14159265f);
-    s = r * ps;
-    float pc = 0.235330630f;   // pi^8/8!
-    pc = fmaf(pc, x, -1.33526277f);
-    pc = fmaf(pc, x, 4.05871213f);
-    pc = fmaf(pc, x, -4.93480220f);
-    pc = fmaf(pc, x, 1.0f);
-    c = pc;
-}
-
-// cos(pi u), sin(pi u) for u = w 2^-32 in [0, 1).
+// cos(pi u), sin(pi u) for u = w 2^-32 in [0, 1), on the transcendental unit: v_cos_f32 and
+// v_sin_f32 take revolutions, and pi u = 2 pi (w 2^-33).  Over all 2^32 phase words the
+// absolute error against FP64 is at most 1.9e-7 on gfx950 (measured exhaustively on one
+// MI355X; ug_split_generation_error repeats the measurement), inside the 2^-20 generation term
+// of the certainty margin.
 __device__ __forceinline__ void phase_cs(uint32_t w, float& cf, float& sf) {
-    // q = nearest integer to 2u (0, 1 or 2) and r = u - q/2, exactly in 32-bit integers:
-    // w - q 2^31 wraps modulo 2^32 onto the int32 value num, |num| <= 2^30
-    const uint32_t q = ((w >> 30) + 1u) >> 1;
-    const int32_t num = (int32_t)(w - (q << 31));
-    const float r = (float)num * 0x1p-32f;
-    float s, c;
-    sincospi_f32_small(r, s, c);
-    // q = 0: (c, s); q = 1: (-s, c); q = 2: (-c, -s) -- selects, no branches
-    const bool odd = q == 1u;
-    const float c0 = odd ? s : c, s0 = odd ? c : s;
-    cf = q != 0u ? -c0 : c0;
-    sf = q == 2u ? -s0 : s0;
+    const float x = (float)w * 0x1p-33f;
+    cf = __builtin_amdgcn_cosf(x);
+    sf = __builtin_amdgcn_sinf(x);
 }
 
-__device__ __forceinline__ float expo_f32(uint32_t w) {
+// An exponential spacing in units of ln 2: -log2(1 - u), u = (w >> 1) 2^-31.  All spacings of
+// an axis share the scale, which cancels in the amplitudes e_j / W.
+__device__ __forceinline__ float expo2_f32(uint32_t w) {
     const float v = (float)(0x80000000u - (w >> 1)) * 0x1p-31f;
-    return -__log2f(v) * 0.693147182f;
+    return -__log2f(v);
 }
 
 __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
@@ -79,32 +60,30 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
     lo = (__bf16)(x - (float)hi);
 }
 
-// A fragments of one K step (pairs p0 = 4q + 2h, p0 + 1) for axis a, candidate idx.
+// A fragments of one K step (pairs p0 = 4q + 2h, p0 + 1) for axis a, candidate idx.  The
+// 1/k factors live in the basis rows (B = S_k/k, C_k/k, S_{k+1}/(k+1)), so a slot's value is
+// its signed exponential spacing, times cos f / sin f for even frequencies.  Missing slots
+// (past the last frequency) and padding candidates contribute zeros, selected without
+// branches; their basis rows are zero as well.
 __device__ __forceinline__ void gen_kstep(const DevModel& m, uint64_t idx, bool valid, int a,
-                                          int q, int h, bf16x8& Ahi, bf16x8& Alo, float& Wpart,
-                                          const float* invk) {
+                                          int q, int h, bf16x8& Ahi, bf16x8& Alo, float& Wpart) {
     const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
 #pragma unroll
     for (int pp = 0; pp < 2; pp++) {
         const int p = 4 * q + 2 * h + pp;
-        float v0 = 0.f, v1 = 0.f, v2 = 0.f;
-        if (valid && p < m.npairs) {
-            const u32x4 r = philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, m.key0, m.key1);
-            const int j0 = 2 * p;
-            const float e0 = expo_f32(r.x);
-            // bit 0 of the Dirichlet word is the slot's sign
-            const float ae = __uint_as_float(__float_as_uint(e0 * invk[j0 + 2]) ^ (r.x << 31));
-            float cf, sf;
-            phase_cs(r.z, cf, sf);
-            v0 = ae * cf;
-            v1 = ae * sf;
-            Wpart += e0;
-            if (j0 + 1 < m.nw) {
-                const float e1 = expo_f32(r.y);
-                v2 = __uint_as_float(__float_as_uint(e1 * invk[j0 + 3]) ^ (r.y << 31));
-                Wpart += e1;
-            }
-        }
+        const u32x4 r = philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, m.key0, m.key1);
+        const int j0 = 2 * p;
+        // slot 2p exists iff p < npairs, slot 2p + 1 iff 2p + 1 < nw
+        const float e0 = valid && p < m.npairs ? expo2_f32(r.x) : 0.f;
+        const float e1 = valid && j0 + 1 < m.nw ? expo2_f32(r.y) : 0.f;
+        // bit 0 of the Dirichlet word is the slot's sign
+        const float ae = __uint_as_float(__float_as_uint(e0) ^ (r.x << 31));
+        float cf, sf;
+        phase_cs(r.z, cf, sf);
+        const float v0 = ae * cf, v1 = ae * sf;
+        const float v2 = __uint_as_float(__float_as_uint(e1) ^ (r.y << 31));
+        Wpart += e0;
+        Wpart += e1;
         __bf16 h0, l0, h1, l1, h2, l2;
         split_bf16(v0, h0, l0);
         split_bf16(v1, h1, l1);
@@ -125,7 +104,6 @@ struct FastArgs {
     int ks;                 // K steps (4 pairs each)
     int ncols;              // ordered timesteps with a certainty margin (T, in N blocks of 32)
     const bf16x8* bfrag;    // [nb][ks][2][64] basis fragments, columns in endpoint-first order
-    const float* invk;      // [T+2] 1/k as float
     const int* colt;        // [nb*32] timestep of each column (-1 = padding)
     // knot regime: the resident strands' FP32 positions [T][ks] (x, y, w, unused), nk of them
     const float4* kpos32;
@@ -161,7 +139,7 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 #pragma unroll
         for (int a = 0; a < D; a++) {
             bf16x8 Ahi, Alo;
-            gen_kstep(m, idx, valid, a, q, h, Ahi, Alo, Wpart[a], f.invk);
+            gen_kstep(m, idx, valid, a, q, h, Ahi, Alo, Wpart[a]);
             acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Bhi, acc[a], 0, 0, 0);
             acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Blo, acc[a], 0, 0, 0);
             acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo, Bhi, acc[a], 0, 0, 0);
@@ -170,7 +148,8 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 #pragma unroll
     for (int a = 0; a < D; a++) {
         const float W = Wpart[a] + __shfl_xor(Wpart[a], 32, 64);
-        wbad = wbad || (valid && W < 0.25f * (float)m.nw);
+        // W sums spacings in units of ln 2: W ln2 < nw/4
+        wbad = wbad || (valid && W * 0.693147182f < 0.25f * (float)m.nw);
         if (h == 0) {
             sa1[mrow] = m.pin ? 0.f : (float)a1_of_word(wa[a]);
             sinvw[mrow] = 1.0f / W;
@@ -475,11 +454,42 @@ float bf16_to_float(uint16_t b) {
 
 struct FastState {
     bf16x8* d_bfrag = nullptr;
-    float* d_invk = nullptr;
     int* d_colt = nullptr;
     int ks = 0, nb = 0, ncols = 0;
     size_t lds_prefilter = 0;
 };
+
+__global__ void k_fill_u32x2_zero(uint32_t* p) {
+    if (threadIdx.x < 2)
+        p[threadIdx.x] = 0u;
+}
+
+// Exhaustive accuracy of the FP32 generation (ug_split_generation_error): the phases over all
+// 2^32 words, the exponential spacings over all 2^31 distinct Dirichlet words (bit 0 is the
+// sign); err[0] = max |(cos, sin) - FP64|, err[1] = max |e - FP64| / (1 + FP64) with e in
+// units of ln 2 (the scale the prefilter uses), as float bits.
+__global__ __launch_bounds__(256) void k_generation_error(uint32_t* err) {
+    float e0 = 0.f, e1 = 0.f;
+    for (uint64_t w = (uint64_t)blockIdx.x * 256 + threadIdx.x; w < (1ull << 32); w += (uint64_t)gridDim.x * 256) {
+        const double ang = 3.14159265358979323846 * (double)w * 0x1p-32;
+        float c, s;
+        phase_cs((uint32_t)w, c, s);
+        e0 = fmaxf(e0, (float)fmax(fabs((double)c - cos(ang)), fabs((double)s - sin(ang))));
+        if ((w & 1u) == 0) {
+            // in units of ln 2, as the prefilter computes it
+            const double ex = -log2((double)(0x80000000u - ((uint32_t)w >> 1)) * 0x1p-31);
+            e1 = fmaxf(e1, (float)(fabs((double)expo2_f32((uint32_t)w) - ex) / (1.0 + ex)));
+        }
+    }
+    for (int sh = 32; sh >= 1; sh >>= 1) {
+        e0 = fmaxf(e0, __shfl_xor(e0, sh, 64));
+        e1 = fmaxf(e1, __shfl_xor(e1, sh, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(err, __float_as_uint(e0));  // non-negative floats order like their bits
+        atomicMax(err + 1, __float_as_uint(e1));
+    }
+}
 
 }  // namespace
 
@@ -497,7 +507,8 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
     std::vector<int> colt((size_t)fs->nb * NCOL, -1);
     for (int oi = 0; oi < T; oi++)
         colt[oi] = order[oi];
-    // basis fragments: row 16q + 8hh + i of K step q is basis row r = 4 p + s, p = r / 4
+    // basis fragments: row 16q + 8hh + i of K step q is basis row r = 4 p + s, p = r / 4,
+    // holding S_k/k, C_k/k, S_{k+1}/(k+1), 0 (the amplitudes' 1/k folded in)
     std::vector<uint16_t> frag((size_t)fs->nb * fs->ks * 2 * 64 * 8, 0);
     auto basis = [&](int r, int t) -> double {
         const int p = r / 4, s = r % 4;
@@ -505,11 +516,11 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
         if (t < 0 || p >= npairs)
             return 0.0;
         if (s == 0)
-            return S[(size_t)t * W + ke];
+            return S[(size_t)t * W + ke] / ke;
         if (s == 1)
-            return C[(size_t)t * W + ke];
+            return C[(size_t)t * W + ke] / ke;
         if (s == 2)
-            return ko <= T ? S[(size_t)t * W + ko] : 0.0;
+            return ko <= T ? S[(size_t)t * W + ko] / ko : 0.0;
         return 0.0;
     };
     for (int nb = 0; nb < fs->nb; nb++)
@@ -526,9 +537,6 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
                     frag[base] = hb;
                     frag[base + (size_t)64 * 8] = lb;
                 }
-    std::vector<float> invk(T + 2, 0.f);
-    for (int k = 1; k <= T + 1; k++)
-        invk[k] = (float)(1.0 / (double)k);
     // certainty margin per examined column: B_t bounds sum |coef * basis| for unit budget
     const double n_add = 48.0 * fs->ks;
     const double eps = 3.1 * 0x1p-16 + n_add * 0x1p-24 + 0x1p-20;
@@ -551,13 +559,10 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
     tb.margin = mmax;
     tb.fast = fs;  // owned by the tables from here on (kept for the process)
     if (hipMalloc((void**)&fs->d_bfrag, frag.size() * 2) != hipSuccess ||
-        hipMalloc((void**)&fs->d_invk, invk.size() * 4) != hipSuccess ||
         hipMalloc((void**)&fs->d_colt, colt.size() * 4) != hipSuccess ||
         hipMalloc((void**)&tb.d_marg, marg.size() * 4) != hipSuccess)
         return fail(UG_ENOMEM, "split-BF16 table allocation failed");
     int32_t rc = hip_check(hipMemcpy(fs->d_bfrag, frag.data(), frag.size() * 2, hipMemcpyHostToDevice), "frag upload");
-    if (!rc)
-        rc = hip_check(hipMemcpy(fs->d_invk, invk.data(), invk.size() * 4, hipMemcpyHostToDevice), "invk upload");
     if (!rc)
         rc = hip_check(hipMemcpy(fs->d_colt, colt.data(), colt.size() * 4, hipMemcpyHostToDevice), "colt upload");
     if (!rc)
@@ -601,7 +606,6 @@ static FastArgs fast_args(Pack* h) {
     f.ks = fs->ks;
     f.ncols = fs->ncols;
     f.bfrag = fs->d_bfrag;
-    f.invk = fs->d_invk;
     f.colt = fs->d_colt;
     f.kpos32 = nullptr;
     f.kstride = 0;
@@ -722,3 +726,27 @@ int32_t launch_fast_eval(Pack* h, const int64_t* list, int64_t count, double* ou
 }
 
 }  // namespace ug
+
+extern "C" int32_t ug_split_generation_error(void* hip_stream, double* out2) {
+    if (!out2)
+        return ug::fail(UG_EINVAL, "ug_split_generation_error: NULL output");
+    hipStream_t st = (hipStream_t)hip_stream;
+    uint32_t* d = nullptr;
+    int32_t rc = ug::hip_check(hipMalloc((void**)&d, 8), "generation error buffer");
+    if (rc)
+        return rc;
+    hipLaunchKernelGGL(k_fill_u32x2_zero, dim3(1), dim3(64), 0, st, d);
+    hipLaunchKernelGGL(k_generation_error, dim3(16384), dim3(256), 0, st, d);
+    uint32_t h[2] = {0, 0};
+    rc = ug::hip_check(hipGetLastError(), "k_generation_error");
+    if (!rc)
+        rc = ug::hip_check(hipMemcpyAsync(h, d, 8, hipMemcpyDeviceToHost, st), "generation error copy");
+    if (!rc)
+        rc = ug::hip_check(hipStreamSynchronize(st), "generation error sync");
+    (void)hipFree(d);
+    float f[2];
+    memcpy(f, h, 8);
+    out2[0] = f[0];
+    out2[1] = f[1];
+    return rc;
+}

@@ -137,6 +137,7 @@ def lib() -> C.CDLL:
         "ug_knot_batches": ([vp, vp, i64, C.POINTER(i64)], i32),
         "ug_knot_destroy": ([vp], None),
         "ug_frame_bake_slices_noalloc": ([vp, vp, i32, vp, vp, i32], i32),
+        "ug_split_generation_error": ([vp, vp], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -153,6 +154,17 @@ def cache_trim() -> None:
     check(lib().ug_cache_trim())
 
 
+def split_generation_error() -> tuple[float, float]:
+    """Exhaustive accuracy of the split-BF16 prefilter's FP32 generation on the current device:
+    (max |(cos, sin) of a phase - FP64| over all 2^32 phase words,
+     max |exponential spacing - FP64| / (1 + value) over all 2^31 Dirichlet words)."""
+    import torch
+
+    out = (C.c_double * 2)()
+    check(lib().ug_split_generation_error(C.c_void_p(torch.cuda.current_stream().cuda_stream), out))
+    return float(out[0]), float(out[1])
+
+
 def exported_symbols() -> list[str]:
     return [
         "ug_abi_version", "ug_last_error", "ug_pack_create", "ug_pack_run", "ug_pack_run_sub",
@@ -163,6 +175,7 @@ def exported_symbols() -> list[str]:
         "ug_frame_bake_lightcone",
         "ug_frame_bake_slices", "ug_frame_bake_slices_noalloc", "ug_knot_create", "ug_knot_run",
         "ug_knot_get_status", "ug_knot_strands", "ug_knot_params", "ug_knot_batches", "ug_knot_destroy",
+        "ug_split_generation_error",
     ]
 
 
