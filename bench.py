@@ -36,7 +36,7 @@ WORKLOADS = {
                  text="3+1 pack to jamming, F=T=128, batches of 2^20, stop when accepts in the "
                       "last 2^22 candidates < 1e-4*2^22"),
     # 512 independent universes: T uniform in {16, 32, 64, 128}, dim 2 or 3 (p = 1/2), seeds
-    # 1..512, each packed to 10^7 candidates, 16 in flight (one stream each)
+    # 1..512, each packed to 10^7 candidates (engine and universes in flight tuned on the GPU)
     "cfg4": dict(dim=0, T=0, max_attempts=1e7, universes=512,
                  text="ensemble of 512 universes, T uniform in {16,32,64,128}, dim 2|3 with "
                       "p=1/2, seeds 1..512, each packed to 10^7 candidates"),
@@ -220,22 +220,25 @@ def run_ensemble(args, ugpack, torch, forms, prec_main) -> int:
     w = WORKLOADS["cfg4"]
     n, cands, threads = w["universes"], w["max_attempts"], args.threads
     sweep = None
-    plan = {"threads": threads, "sub_target": 1024, "sub_max": 1 << 20}
-    if threads <= 0:
-        # the in-flight universe count and the sub-batch sizing are tuned on this GPU: a
-        # 128-universe sample of the same mix (seeds 1..128) at 2*10^6 candidates each,
+    plan = {"engine": args.engine, "threads": threads, "lanes": args.lanes, "window": args.window,
+            "sub_target": 1024, "sub_max": 1 << 20}
+    if args.engine == "tuned":
+        # the engine, its universes in flight and the sub-batch sizing are tuned on this GPU:
+        # a 128-universe sample of the same mix (seeds 1..128) at 2*10^6 candidates each,
         # outside the timed region
         plan, sweep = tune.ensemble_plan(ensemble_configs(ugpack, 128, 2e6, prec_main))
-        threads = plan["threads"]
         progress(f"ensemble tuned: {plan}")
+    threads = plan["threads"] if plan["threads"] > 0 else 4
+    batched = plan["engine"] == "batched"
+    run_kw = dict(threads=threads, batched=batched, window=plan["window"], lanes=max(plan["lanes"], 1))
     res = {}
     for prec in forms:
         cfgs = tune.with_sizing(ensemble_configs(ugpack, n, cands, prec), plan["sub_target"], plan["sub_max"])
-        ugpack.ensemble(ensemble_configs(ugpack, 32, 1e6, prec), threads=threads)  # warmup
+        ugpack.ensemble(ensemble_configs(ugpack, 32, 1e6, prec), **run_kw)  # warmup
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
-            st = ugpack.ensemble(cfgs, threads=threads)
+            st = ugpack.ensemble(cfgs, **run_kw)
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
         res[prec] = dict(wall=wall, value=n * cands / wall, n=[s.n for s in st],
@@ -258,7 +261,9 @@ def run_ensemble(args, ugpack, torch, forms, prec_main) -> int:
         "dtype": "bf16x3+f32acc, f64 recheck" if split else "f64",
         "data": "synthetic (Philox candidate streams, seeds 1..512)",
         "config": {"workload": w["text"], "universes": n, "candidates_per_universe": cands,
-                   "threads": threads, "sub_target": plan["sub_target"], "sub_max": plan["sub_max"],
+                   "engine": plan["engine"], "threads": threads if not batched else None,
+                   "lanes": plan["lanes"] if batched else None, "window": plan["window"] if batched else None,
+                   "sub_target": plan["sub_target"], "sub_max": plan["sub_max"],
                    "tuning_sweep_s": {str(k): round(v, 4) for k, v in sweep.items()} if sweep else None,
                    "T_counts": {str(T): sum(1 for c in cfgs if c.T == T) for T in (16, 32, 64, 128)},
                    "dim3_count": sum(1 for c in cfgs if c.dim == 3),
@@ -555,8 +560,13 @@ def main() -> int:
     ap.add_argument("--precision", default="both", choices=["split", "fp64", "both"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
-    ap.add_argument("--threads", type=int, default=0,
-                    help="ensemble workers (cfg4); 0 = tuned on this GPU")
+    ap.add_argument("--engine", default="tuned", choices=["tuned", "batched", "threads"],
+                    help="ensemble engine (cfg4): tuned on this GPU, the batched lock-step engine, or "
+                         "worker threads")
+    ap.add_argument("--threads", type=int, default=0, help="ensemble workers (cfg4, --engine threads)")
+    ap.add_argument("--window", type=int, default=0,
+                    help="universes in flight per lane (cfg4, --engine batched; 0 = 32)")
+    ap.add_argument("--lanes", type=int, default=1, help="lock-step lanes (cfg4, --engine batched)")
     ap.add_argument("--calib", action="store_true",
                     help="first copy 256 MiB device-to-device (counter calibration for rocprof --pmc)")
     ap.add_argument("--sparse", action="store_true",

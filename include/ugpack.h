@@ -249,6 +249,22 @@ int32_t ug_cache_trim(void);
 int32_t ug_ensemble_run(const ug_pack_config* cfgs, int32_t n, int32_t threads, ug_pack** keep,
                         ug_pack_status* status, uint64_t* digests);
 
+/* Packs n independent universes like ug_ensemble_run with the batched engine: `lanes` host
+ * threads (0 = 1), each with a HIP stream of its own, advance up to `window` universes each
+ * (0 = 32, at most 64) in lock-step; finished universes are replaced from a shared queue in
+ * index order.  A step moves every universe of a lane by one sub-batch (or one retest of survivors left
+ * from its last resolve chunk), each stage one launch for all of them -- the split-BF16
+ * prefilter and the FP64 tests over a flattened (universe, tile / candidate) work queue,
+ * compaction and the resolve chunk one workgroup row per universe -- with survivor counts
+ * kept on the device and one host synchronisation per step.  Each universe's accepted
+ * list, status and digest are those of its own ug_pack_run.  The batched kernels cover
+ * full-spectrum universes with the dense cell table, T <= 160 and the budget or fixed-window
+ * stop; other configs run one at a time on a lane's stream after its batched universes.
+ * keep: as ug_ensemble_run (each kept handle gets a stream of its own).  No counterpart in
+ * the reference (one engine process per universe, braidlab/orchestrator.py:70-82,106-107). */
+int32_t ug_ensemble_run_batched(const ug_pack_config* cfgs, int32_t n, int32_t window, int32_t lanes,
+                                ug_pack** keep, ug_pack_status* status, uint64_t* digests);
+
 /* ------------------------------------------------------------------ knot regime
  * Replaces the braid_knot executable (cuda/braid_knot.cu): few strands, very many
  * candidates, no spatial grid.  Candidates are the packer's counter-indexed Philox

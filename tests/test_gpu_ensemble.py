@@ -130,10 +130,16 @@ def test_tuned_plan_and_sizing_leave_the_lists_unchanged(gpu):
     from ugpack import tune
 
     cfgs = _configs(ugpack, ugpack.PREC_SPLIT_BF16)
-    plan, sweep = tune.ensemble_plan(cfgs, choices=(1, 2, 4), targets=(512, 2048), sub_max=(1 << 16, 1 << 20))
-    assert plan["threads"] in (1, 2, 4) and plan["sub_target"] in (512, 1024, 2048)
-    assert len(sweep) == 3 + 4 and all(t > 0 for t in sweep.values())
+    plan, sweep = tune.ensemble_plan(cfgs, choices=(1, 2, 4), targets=(512, 2048), sub_max=(1 << 16, 1 << 20),
+                                     batched=((1, 2), (2, 6)))
+    assert plan["engine"] in ("threads", "batched") and plan["sub_target"] in (512, 1024, 2048)
+    assert plan["threads"] in (1, 2, 4)
+    if plan["engine"] == "batched":
+        assert (plan["lanes"], plan["window"]) in ((1, 2), (2, 6))
+    assert len(sweep) == 3 + 2 + 4 and all(t > 0 for t in sweep.values())
     want = [ugpack.abi.list_digest(po.Model(d, T, seed=s).pack(m)[0]) for d, T, s, m in MIX]
     for tg, sm in ((512, 1 << 16), (2048, 1 << 20), (plan["sub_target"], plan["sub_max"])):
-        _, dg = ugpack.ensemble(tune.with_sizing(cfgs, tg, sm), threads=plan["threads"], digests=True)
-        assert dg == want, (tg, sm)
+        for batched in (False, True):
+            _, dg = ugpack.ensemble(tune.with_sizing(cfgs, tg, sm), threads=plan["threads"], digests=True,
+                                    batched=batched, window=plan["window"], lanes=max(plan["lanes"], 1))
+            assert dg == want, (tg, sm, batched)

@@ -384,6 +384,10 @@ static bool fixed_window_stop(const Pack* h) {
     return (double)cnt < c.stop_rate * (double)c.stop_window;
 }
 
+void ug::pack_fill_status(const Pack* h, ug_pack_status* st) { fill_status(h, st); }
+bool ug::pack_fixed_window_stop(const Pack* h) { return fixed_window_stop(h); }
+void ug::pack_record_curve(Pack* h) { (void)record_curve(h); }
+
 // The smallest accepted count of the next sub-batch (sub candidates) for which the run goes
 // on: the loop stops after it iff its uniques number fewer (0: it cannot stop there).  The
 // fixed window counts accepts with index >= end - window, all of the sub-batch's among them.

@@ -19,7 +19,7 @@
 #include "ug_internal.h"
 
 // FNV-1a (64-bit) over the little-endian bytes of an accepted index list.
-static uint64_t list_digest(const std::vector<int64_t>& idx) {
+uint64_t ug::list_digest(const std::vector<int64_t>& idx) {
     uint64_t h = 1469598103934665603ull;
     for (const int64_t v : idx)
         for (int b = 0; b < 8; b++) {
@@ -92,7 +92,7 @@ extern "C" int32_t ug_ensemble_run(const ug_pack_config* cfgs, int32_t n, int32_
                 break;
             }
             if (digests)
-                digests[i] = list_digest(((const ug::Pack*)h)->acc_host);
+                digests[i] = ug::list_digest(((const ug::Pack*)h)->acc_host);
             if (keep)
                 keep[i] = h;
             else

@@ -176,21 +176,13 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
     return wbad;
 }
 
-// Prefilter: flags[i] = 0 when candidate i certainly collides, 1 when the FP64 test must decide.
-// SUB (subpath phase): flags[i] = 0 only when candidate i certainly touches two different
-// groups among the paths stored before row nlimit (the round-start packing) -- such a
-// candidate fails both the round-start test and the adoption test.
-// Occupancy: 2+1 runs 4 waves per SIMD (at most 128 VGPRs, no spills; 4 blocks of 35 KB LDS
-// fit a CU), 2-4% faster per launch than 3 on one MI355X; 3+1 stays at 3, which is also what
-// its 52 KB of LDS per block allows.
-// KNOT: no grid -- a visit compares the candidate with every resident strand (the knot
-// regime's brute force) and rejects it on a certain contact.
-template <int D, bool SUB, bool SPARSE, bool KNOT = false>
-__global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
-                                                        const int64_t* list, const int* dcount,
-                                                        int cb0, int cb1, int par, uint8_t* flags,
-                                                        unsigned long long* ctr, const int* acc_gid,
-                                                        int64_t nlimit) {
+// One work item of the prefilter: the wave's tile `tile` of 32 candidates over the N blocks
+// [nbs, nbe) of 32 ordered timesteps (count: the launch's candidates, already bounded).
+template <int D, bool SUB, bool SPARSE, bool KNOT>
+__device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, int nbs, int nbe, int par,
+                                               uint64_t base, int64_t count, const int64_t* list,
+                                               uint8_t* flags, const int* acc_gid, int64_t nlimit,
+                                               unsigned long long& nvis, unsigned long long& npts) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevModel& m = f.m;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -203,18 +195,8 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
     int* smin = (int*)(sinvw + 2 * TILE);  // after ulist: per-candidate group range of a round
     int* smax = smin + TILE;
     const int mrow = lane & 31, h = lane >> 5;
-    if (dcount)
-        count = min(count, (int64_t)*dcount);
-    const int64_t ntiles = (count + TILE - 1) / TILE;
-    // par: one work item per (tile, N block), flags preset to 1 and cleared by whichever block
-    // rejects; otherwise one item per tile walking its N blocks in order
-    const int nbk = par ? cb1 - cb0 : 1;
-    unsigned long long nvis = 0, npts = 0;
     const float r = (float)m.r;
-    for (int64_t item = (int64_t)blockIdx.x * FAST_WAVES + wv; item < ntiles * nbk;
-         item += (int64_t)gridDim.x * FAST_WAVES) {
-        const int64_t tile = item / nbk;
-        const int nbs = par ? cb0 + (int)(item % nbk) : cb0, nbe = par ? nbs + 1 : cb1;
+    {
         const int64_t i = tile * TILE + mrow;
         const bool valid = i < count;
         const uint64_t idx = list ? (valid ? (uint64_t)list[i] : 0ull) : base + (uint64_t)(valid ? i : 0);
@@ -388,6 +370,38 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
                 flags[i] = 0;
         }
     }
+}
+
+// Prefilter: flags[i] = 0 when candidate i certainly collides, 1 when the FP64 test must decide.
+// SUB (subpath phase): flags[i] = 0 only when candidate i certainly touches two different
+// groups among the paths stored before row nlimit (the round-start packing) -- such a
+// candidate fails both the round-start test and the adoption test.
+// Occupancy: 2+1 runs 4 waves per SIMD (at most 128 VGPRs, no spills; 4 blocks of 35 KB LDS
+// fit a CU), 2-4% faster per launch than 3 on one MI355X; 3+1 stays at 3, which is also what
+// its 52 KB of LDS per block allows.
+// KNOT: no grid -- a visit compares the candidate with every resident strand (the knot
+// regime's brute force) and rejects it on a certain contact.
+template <int D, bool SUB, bool SPARSE, bool KNOT = false>
+__global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
+                                                        const int64_t* list, const int* dcount,
+                                                        int cb0, int cb1, int par, uint8_t* flags,
+                                                        unsigned long long* ctr, const int* acc_gid,
+                                                        int64_t nlimit) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (dcount)
+        count = min(count, (int64_t)*dcount);
+    const int64_t ntiles = (count + TILE - 1) / TILE;
+    // par: one work item per (tile, N block), flags preset to 1 and cleared by whichever block
+    // rejects; otherwise one item per tile walking its N blocks in order
+    const int nbk = par ? cb1 - cb0 : 1;
+    unsigned long long nvis = 0, npts = 0;
+    for (int64_t item = (int64_t)blockIdx.x * FAST_WAVES + wv; item < ntiles * nbk;
+         item += (int64_t)gridDim.x * FAST_WAVES) {
+        const int64_t tile = item / nbk;
+        const int nbs = par ? cb0 + (int)(item % nbk) : cb0, nbe = par ? nbs + 1 : cb1;
+        prefilter_item<D, SUB, SPARSE, KNOT>(f, tile, nbs, nbe, par, base, count, list, flags, acc_gid, nlimit,
+                                             nvis, npts);
+    }
     if (ctr) {
         unsigned long long a = nvis, b = npts;
         for (int s = 32; s >= 1; s >>= 1) {
@@ -408,6 +422,39 @@ __global__ void k_flags_one(uint8_t* flags, int64_t count, const int* dcount) {
         count = min(count, (int64_t)*dcount);
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256)
         flags[i] = 1;
+}
+
+// One universe's share of a batched stage-1 prefilter launch (the batched engine).
+struct MFast {
+    FastArgs f;
+    uint64_t base;
+    int64_t count;
+    const int64_t* list;
+    uint8_t* flags;
+};
+
+// Stage-1 prefilter over the candidates of many universes (at most 64; one dimension): the
+// tiles of all of them as one flattened work queue, one tile per wave item, the same work
+// item as the single-universe kernel.
+template <int D>
+__global__ __launch_bounds__(256, D == 2 ? 4 : 3) void k_fast_prefilter_multi(const MFast* __restrict__ A, int U) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    BatchItems q;
+    q.scan(lane < U ? (A[lane].count + TILE - 1) / TILE : 0, U);
+    unsigned long long nvis = 0, npts = 0;
+    for (int64_t item = (int64_t)blockIdx.x * FAST_WAVES + wv; item < q.total;
+         item += (int64_t)gridDim.x * FAST_WAVES) {
+        int64_t tile = 0;
+        const int u = q.owner(item, &tile);
+        // the universe's record in registers, read once per item, its pointers global
+        const MFast* r = A + u;
+        MFast a = *r;
+        a.f.m = load_model(&r->f.m);
+        a.f.bfrag = ug_gptr(&r->f.bfrag);
+        a.f.colt = ug_gptr(&r->f.colt);
+        prefilter_item<D, false, false, false>(a.f, tile, 0, 1, 0, a.base, a.count, ug_gptr(&r->list),
+                                               ug_gptr(&r->flags), nullptr, 0, nvis, npts);
+    }
 }
 
 // Split-BF16 positions of listed candidates at every timestep (error reporting, tests).
@@ -611,6 +658,37 @@ static FastArgs fast_args(Pack* h) {
     f.kstride = 0;
     f.nk = 0;
     return f;
+}
+
+size_t fast_multi_arg_bytes() { return sizeof(MFast); }
+
+int32_t fast_multi_arg(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags, void* out) {
+    if (!h->full || !h->d_basis16)
+        return fail(UG_ESTATE, "split-BF16 prefilter not set up for this model");
+    if (h->dm.sparse)
+        return fail(UG_EINVAL, "the batched prefilter covers the dense cell table only");
+    MFast a;
+    a.f = fast_args(h);
+    a.base = base;
+    a.count = count;
+    a.list = list;
+    a.flags = flags;
+    memcpy(out, &a, sizeof a);
+    return UG_OK;
+}
+
+int32_t launch_fast_multi(hipStream_t st, int d, const void* dA, int U, int64_t tiles) {
+    if (U <= 0 || tiles <= 0)
+        return UG_OK;
+    if (U > 64)
+        return fail(UG_EINVAL, "batched prefilter of %d universes (at most 64 a launch)", U);
+    const size_t lds = (size_t)FAST_WAVES * (d * TILE * NCOL + 5 * TILE) * 4;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + FAST_WAVES - 1) / FAST_WAVES, 16384));
+    if (d == 2)
+        hipLaunchKernelGGL(k_fast_prefilter_multi<2>, dim3(blocks), dim3(256), lds, st, (const MFast*)dA, U);
+    else
+        hipLaunchKernelGGL(k_fast_prefilter_multi<3>, dim3(blocks), dim3(256), lds, st, (const MFast*)dA, U);
+    return hip_check(hipGetLastError(), "k_fast_prefilter_multi");
 }
 
 int fast_blocks(const Pack* h) {

@@ -136,6 +136,115 @@ __device__ __forceinline__ void row_heads3(const DevModel& m, int64_t row, int c
         node[k] = ((bits >> k) & 1u) ? tr[k] : -1;
 }
 
+// Work items of a batched launch over at most 64 universes: universe u owns the flattened
+// items [excl(u), excl(u) + n(u)).  Each wave scans the counts once (lane u holds universe u)
+// and then finds the owner of an item with one ballot: the universes whose items all lie
+// before it are a prefix of the lanes, because the running sums never decrease.
+struct BatchItems {
+    int64_t incl, excl, total;
+    int U;
+    __device__ __forceinline__ void scan(int64_t n_lane, int nU) {
+        const int lane = threadIdx.x & 63;
+        U = nU;
+        const int64_t c = lane < nU ? max(n_lane, (int64_t)0) : 0;
+        incl = c;
+        for (int s = 1; s < 64; s <<= 1) {
+            const int64_t v = __shfl_up(incl, s, 64);
+            if (lane >= s)
+                incl += v;
+        }
+        excl = incl - c;
+        total = __shfl(incl, 63, 64);
+    }
+    // universe of item (uniform over the wave), and the item's index within it
+    __device__ __forceinline__ int owner(int64_t item, int64_t* i) const {
+        const int lane = threadIdx.x & 63;
+        const int u = __popcll(__ballot(lane < U && incl <= item));
+        *i = item - __shfl(excl, u, 64);
+        return u;
+    }
+};
+
+// Batched launches read each universe's record (pointers included) from device memory; a
+// pointer loaded as a plain pointer is generic to the compiler and every access through it
+// becomes a flat access.  ug_gptr loads a pointer field of a record in device memory as a
+// global-address-space pointer, so accesses through it are global loads and stores, as for
+// pointers passed as kernel arguments.  Every pointer in these records is device memory (or
+// pinned host memory, which lies in the same aperture).
+#define UG_GAS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T* ug_gptr(T* const* field) {
+    return (T*)*(T UG_GAS* const*)field;
+}
+// A DevModel from one in device memory, its pointers global.
+__device__ __forceinline__ DevModel load_model(const DevModel* src) {
+    DevModel m = *src;
+    m.S = ug_gptr(&src->S);
+    m.C = ug_gptr(&src->C);
+    m.ST = ug_gptr(&src->ST);
+    m.CT = ug_gptr(&src->CT);
+    m.order = ug_gptr(&src->order);
+    m.tab = ug_gptr(&src->tab);
+    m.occ = ug_gptr(&src->occ);
+    m.pos64 = ug_gptr(&src->pos64);
+    m.pos32 = ug_gptr(&src->pos32);
+    m.marg = ug_gptr(&src->marg);
+    m.hslots = ug_gptr(&src->hslots);
+    return m;
+}
+
+// ---- batched ensemble engine (ug_batch.hip): one universe's share of a launch that serves
+// many universes; the launches read these from a device array indexed by universe.
+// k_cand64 work: fresh candidates base + i or list[i], i < min(count, *dcount).
+struct MCand {
+    DevModel m;
+    uint64_t base;
+    int64_t count;
+    const int64_t* list;
+    const int* dcount;
+    uint8_t* flags;
+    double* out;     // trajectories (full)
+    int* cells;      // their packed cells
+    uint64_t* mask;  // conflict rows cleared with them
+    int head;
+};
+// Ordered compaction of flags[0 .. min(count, *dcount)) into out (count into *total); bcount
+// holds the per-block counts (ceil(count / 4096) of them).
+struct MCompact {
+    const uint8_t* flags;
+    int64_t count;
+    const int* dcount;
+    uint64_t base;
+    const int64_t* list;
+    int* bcount;
+    int64_t* out;
+    int* total;
+};
+// A resolve chunk of min(S, counts[1]) survivors (trajectories pos, packed cells, cleared
+// conflict rows; counts = the filter's device counts {rechecks, survivors}): conflict bits
+// (tiled kernel when tiled -- the bits are the same either way), index-order greedy,
+// insertion of at most limit accepted paths as rows n0, ..., and the report: rep[0] =
+// accepted, rep[1] = survivors, rep[2..3] = occupied buckets (64 bits), rep[4] = rechecks,
+// then the accepted candidate indices from rep + 6 (as int64).
+struct MResolve {
+    DevModel m;
+    const double* pos;
+    const int* cells;
+    uint64_t* mask;
+    const int* counts;
+    int S, tiled, limit;
+    int* acc_list;
+    int* nacc;
+    const int64_t* surv_idx;
+    int64_t n0;
+    int64_t* acc_idx;
+    int32_t* acc_gid;
+    double* pos64;
+    float4* pos32;
+    unsigned long long* filled;
+    int* rep;
+};
+
 // Read-only tables of one model, built once per (device, dim, T, terms, form, margin) and
 // shared by its handles (ug_cache.hip).
 struct ModelTables {
@@ -360,6 +469,25 @@ int32_t launch_knot_test(Pack* h, uint64_t base, int64_t count, const int64_t* l
 // Appends the chunk's accepted survivors (d_acc_list, at most limit) of list as strands n0, ...
 int32_t launch_knot_append(Pack* h, const int64_t* list, int s, int64_t n0, int* kcell, double* kpos, float4* kpos32,
                            int64_t ks, int64_t* kidx, int limit);
+// Batched engine (ug_batch.hip): launches that serve many universes at once (at most 64 for
+// the candidate-test kernels), per-universe arguments in device arrays (MCand / MCompact /
+// MResolve, and the prefilter's opaque records of fast_multi_arg_bytes() bytes written by
+// fast_multi_arg).  items / tiles: an upper bound on the launch's work (sizes its grid).
+size_t cand64_lds_bytes(const Pack* h);
+int32_t launch_cand64_multi(hipStream_t st, int d, int pl, bool full, bool sb, const MCand* dA, int U,
+                            int64_t items, size_t lds);
+int32_t launch_compact_multi(hipStream_t st, const MCompact* dA, int U, int64_t max_count);
+int32_t launch_resolve_multi(hipStream_t st, int d, const MResolve* dA, int U, int maxS, int maxT,
+                             int64_t max_pairs, int max_tiles, bool any_plain, bool any_tiled);
+size_t fast_multi_arg_bytes();
+int32_t fast_multi_arg(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags, void* out);
+int32_t launch_fast_multi(hipStream_t st, int d, const void* dA, int U, int64_t tiles);
+int slot_pairs_per_lane(int npairs);
+// Pieces of the single-universe loop (ug_api.hip) that the batched engine shares.
+void pack_fill_status(const Pack* h, ug_pack_status* st);
+bool pack_fixed_window_stop(const Pack* h);  // the fixed-window rule fires at h->attempts
+void pack_record_curve(Pack* h);             // log-spaced curve rows up to h->attempts
+uint64_t list_digest(const std::vector<int64_t>& idx);  // FNV-1a of an accepted list (ug_ensemble.hip)
 // Handle set-up shared by the packer and the knot engine: with_table = false builds no cell
 // table, occupancy bits or path records (the knot engine keeps its own strands).
 int32_t create_handle(const ug_pack_config* cfg, void* hip_stream, bool with_table, Pack** out);

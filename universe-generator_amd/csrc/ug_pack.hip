@@ -428,14 +428,12 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
 // SB: the lane-per-timestep evaluation shares each slot group's basis loads across axes
 // (full spectra, one slot pair per lane).  A separate instantiation, because the shared
 // evaluation's extra registers would lower the occupancy of launches that do not use it.
-template <int D, bool FULL, int PL, bool SB = false>
-__global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_t count,
-                                               const int64_t* list, const int* dcount,
-                                               uint8_t* flags, double* out, int head,
-                                               unsigned long long* ctr, int* cells,
-                                               uint64_t* mask) {
-    if (dcount)
-        count = min(count, (int64_t)*dcount);
+// One candidate (item i of the launch, candidate index idx) of k_cand64; the wave is the
+// workgroup.  Shared by the single-universe kernel and the batched engine's launches.
+template <int D, bool FULL, int PL, bool SB>
+__device__ __forceinline__ void cand64_item(const DevModel& m, int64_t i, uint64_t idx, uint8_t* flags,
+                                            double* out, int head, int* cells, uint64_t* mask,
+                                            unsigned long long& nvis, unsigned long long& npts) {
     extern __shared__ __align__(16) int dyn_lds[];
     const int lane = threadIdx.x;
     const int T = m.T, nw = m.nw;
@@ -448,11 +446,9 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
     __shared__ int tsl[64];
     constexpr int R = D == 2 ? 3 : 9;
     constexpr int PER_ROUND = 64 / R;
-    unsigned long long nvis = 0, npts = 0;
-    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+    {
         if (FULL && mask && lane < UG_MASK_WORDS)
             mask[(size_t)i * UG_MASK_WORDS + lane] = 0ull;  // the conflict row, cleared
-        const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
         WaveCoef<D, PL> w;
         gen_wave64<D, PL>(m, idx, lane, w, sb, used);
         bool hit = false;
@@ -537,18 +533,67 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
             flags[i] = hit ? 0 : 1;
         __syncthreads();
     }
+}
+
+// Measurement counters of a k_cand64 wave (one atomic per wave and counter).
+__device__ __forceinline__ void cand64_counters(unsigned long long* ctr, unsigned long long nvis,
+                                                unsigned long long npts) {
+    unsigned long long* c = ctr + (blockIdx.x % UG_CTR_SLOTS) * UG_CTR_STRIDE;
+    unsigned long long pw = npts;
+    for (int h = 32; h >= 1; h >>= 1)
+        pw += __shfl_xor(pw, h, 64);
+    if (threadIdx.x == 0 && nvis) {
+        atomicAdd(&c[0], nvis);
+        atomicAdd(&c[1], pw);
+    }
+}
+
+template <int D, bool FULL, int PL, bool SB = false>
+__global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_t count,
+                                               const int64_t* list, const int* dcount,
+                                               uint8_t* flags, double* out, int head,
+                                               unsigned long long* ctr, int* cells,
+                                               uint64_t* mask) {
+    if (dcount)
+        count = min(count, (int64_t)*dcount);
+    unsigned long long nvis = 0, npts = 0;
+    for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
+        const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
+        cand64_item<D, FULL, PL, SB>(m, i, idx, flags, out, head, cells, mask, nvis, npts);
+    }
     if (!FULL && ctr) {
-        // exact counts for the algorithmic-bytes figure (one atomic per wave and counter)
-        unsigned long long* c = ctr + (blockIdx.x % UG_CTR_SLOTS) * UG_CTR_STRIDE;
-        if (dcount && blockIdx.x == 0 && lane == 0)
-            atomicAdd(&c[2], (unsigned long long)count);
-        unsigned long long pw = npts;
-        for (int h = 32; h >= 1; h >>= 1)
-            pw += __shfl_xor(pw, h, 64);
-        if (lane == 0 && nvis) {
-            atomicAdd(&c[0], nvis);
-            atomicAdd(&c[1], pw);
-        }
+        // exact counts for the algorithmic-bytes figure
+        if (dcount && blockIdx.x == 0 && threadIdx.x == 0)
+            atomicAdd(&ctr[(blockIdx.x % UG_CTR_SLOTS) * UG_CTR_STRIDE + 2], (unsigned long long)count);
+        cand64_counters(ctr, nvis, npts);
+    }
+}
+
+// k_cand64 over the candidates of many universes (the batched engine, at most 64 a launch):
+// universe u's min(count, *dcount) items as one flattened work queue, one wave per item.
+template <int D, bool FULL, int PL, bool SB>
+__global__ __launch_bounds__(64) void k_cand64_multi(const MCand* __restrict__ A, int U) {
+    const int lane = threadIdx.x;
+    int64_t c = 0;
+    if (lane < U) {
+        c = A[lane].count;
+        const int* dc = ug_gptr(&A[lane].dcount);
+        if (dc)
+            c = min(c, (int64_t)*dc);
+    }
+    BatchItems q;
+    q.scan(c, U);
+    unsigned long long nvis = 0, npts = 0;
+    for (int64_t item = blockIdx.x; item < q.total; item += gridDim.x) {
+        int64_t i = 0;
+        const int u = q.owner(item, &i);
+        // the universe's record in registers, read once per item, its pointers global
+        const MCand* r = A + u;
+        const DevModel m = load_model(&r->m);
+        const int64_t* list = ug_gptr(&r->list);
+        const uint64_t idx = list ? (uint64_t)list[i] : r->base + (uint64_t)i;
+        cand64_item<D, FULL, PL, SB>(m, i, idx, ug_gptr(&r->flags), ug_gptr(&r->out), r->head, ug_gptr(&r->cells),
+                                     ug_gptr(&r->mask), nvis, npts);
     }
 }
 
@@ -1038,12 +1083,12 @@ __global__ __launch_bounds__(64) void k_greedy_sub(const uint64_t* mask, int S, 
 // ordered compaction: out[k] = candidate index of the k-th flagged item (stable)
 constexpr int CMP_ITEMS = 4096;  // items per block (256 threads x 16)
 
-__global__ __launch_bounds__(256) void k_compact_count(const uint8_t* flags, int64_t count,
-                                                       const int* dcount, int* bcount) {
+__device__ __forceinline__ void compact_count_block(const uint8_t* flags, int64_t count, const int* dcount,
+                                                    int* bcount, int bx) {
     __shared__ int red[256];
     if (dcount)
         count = min(count, (int64_t)*dcount);
-    const int64_t b0 = (int64_t)blockIdx.x * CMP_ITEMS;
+    const int64_t b0 = (int64_t)bx * CMP_ITEMS;
     int n = 0;
     for (int k = threadIdx.x; k < CMP_ITEMS; k += 256) {
         const int64_t i = b0 + k;
@@ -1058,21 +1103,25 @@ __global__ __launch_bounds__(256) void k_compact_count(const uint8_t* flags, int
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        bcount[blockIdx.x] = red[0];
+        bcount[bx] = red[0];
 }
 
-__global__ __launch_bounds__(256) void k_compact_scatter(const uint8_t* flags, int64_t count,
-                                                         const int* dcount, uint64_t base,
-                                                         const int64_t* list, const int* bcount,
-                                                         int nblocks, int64_t* out, int* total,
-                                                         const int* vin, int* vout) {
+__global__ __launch_bounds__(256) void k_compact_count(const uint8_t* flags, int64_t count,
+                                                       const int* dcount, int* bcount) {
+    compact_count_block(flags, count, dcount, bcount, blockIdx.x);
+}
+
+__device__ __forceinline__ void compact_scatter_block(const uint8_t* flags, int64_t count, const int* dcount,
+                                                      uint64_t base, const int64_t* list, const int* bcount,
+                                                      int nblocks, int64_t* out, int* total, const int* vin,
+                                                      int* vout, int bx) {
     __shared__ int red[256];
     if (dcount)
         count = min(count, (int64_t)*dcount);
     __shared__ int scan[256];
     // offset of this block = sum of earlier block counts
     int s0 = 0;
-    for (int b = threadIdx.x; b < (int)blockIdx.x; b += 256)
+    for (int b = threadIdx.x; b < bx; b += 256)
         s0 += bcount[b];
     red[threadIdx.x] = s0;
     __syncthreads();
@@ -1083,7 +1132,7 @@ __global__ __launch_bounds__(256) void k_compact_scatter(const uint8_t* flags, i
     }
     const int boff = red[0];
     // each thread owns 16 consecutive items
-    const int64_t i0 = (int64_t)blockIdx.x * CMP_ITEMS + (int64_t)threadIdx.x * 16;
+    const int64_t i0 = (int64_t)bx * CMP_ITEMS + (int64_t)threadIdx.x * 16;
     int n = 0;
     for (int k = 0; k < 16; k++)
         if (i0 + k < count)
@@ -1105,8 +1154,35 @@ __global__ __launch_bounds__(256) void k_compact_scatter(const uint8_t* flags, i
             out[o++] = list ? list[i] : (int64_t)(base + (uint64_t)i);
         }
     }
-    if (blockIdx.x == nblocks - 1 && threadIdx.x == 255)
+    if (bx == nblocks - 1 && threadIdx.x == 255)
         *total = boff + scan[255];
+}
+
+__global__ __launch_bounds__(256) void k_compact_scatter(const uint8_t* flags, int64_t count,
+                                                         const int* dcount, uint64_t base,
+                                                         const int64_t* list, const int* bcount,
+                                                         int nblocks, int64_t* out, int* total,
+                                                         const int* vin, int* vout) {
+    compact_scatter_block(flags, count, dcount, base, list, bcount, nblocks, out, total, vin, vout, blockIdx.x);
+}
+
+// Batched compaction: blockIdx.y = universe, blockIdx.x = its block (ceil(count / 4096) of
+// them, at least one, so a universe with nothing to compact still gets *total = 0).
+__global__ __launch_bounds__(256) void k_compact_count_multi(const MCompact* A) {
+    const MCompact* r = A + blockIdx.y;
+    const int64_t count = r->count;
+    const int nb = (int)max((int64_t)1, (count + CMP_ITEMS - 1) / CMP_ITEMS);
+    if ((int)blockIdx.x < nb)
+        compact_count_block(ug_gptr(&r->flags), count, ug_gptr(&r->dcount), ug_gptr(&r->bcount), blockIdx.x);
+}
+__global__ __launch_bounds__(256) void k_compact_scatter_multi(const MCompact* A) {
+    const MCompact* r = A + blockIdx.y;
+    const int64_t count = r->count;
+    const int nb = (int)max((int64_t)1, (count + CMP_ITEMS - 1) / CMP_ITEMS);
+    if ((int)blockIdx.x < nb)
+        compact_scatter_block(ug_gptr(&r->flags), count, ug_gptr(&r->dcount), r->base, ug_gptr(&r->list),
+                              ug_gptr(&r->bcount), nb, ug_gptr(&r->out), ug_gptr(&r->total), nullptr, nullptr,
+                              blockIdx.x);
 }
 
 // =====================================================================================
@@ -1146,14 +1222,11 @@ __device__ __forceinline__ bool cells_adjacent(int u, int v, int T) {
 constexpr int CONF_T = 16;
 
 template <int D>
-__global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos, const int* cells, int S,
-                                                   const int* dS, uint64_t* mask) {
-    if (dS)
-        S = min(S, *dS);
+__device__ __forceinline__ void conflicts_body(const DevModel& m, const double* pos, const int* cells, int S,
+                                               uint64_t* mask, int bx, int gx) {
     const int nblk = (m.T + CONF_T - 1) / CONF_T;
     const int64_t items = (int64_t)S * (S - 1) / 2 * nblk;
-    for (int64_t it = (int64_t)blockIdx.x * 256 + threadIdx.x; it < items;
-         it += (int64_t)gridDim.x * 256) {
+    for (int64_t it = (int64_t)bx * 256 + threadIdx.x; it < items; it += (int64_t)gx * 256) {
         const int64_t k = it / nblk;
         const int t0 = (int)(it % nblk) * CONF_T;
         int64_t i = (int64_t)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
@@ -1186,6 +1259,14 @@ __global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos
     }
 }
 
+template <int D>
+__global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos, const int* cells, int S,
+                                                   const int* dS, uint64_t* mask) {
+    if (dS)
+        S = min(S, *dS);
+    conflicts_body<D>(m, pos, cells, S, mask, blockIdx.x, gridDim.x);
+}
+
 // The same conflict bits for large chunks: one workgroup per (64x64 tile (bi, bj) of the
 // lower triangle, block of CONF_TB timesteps).  The tile's packed cells and the j side's
 // FP64 positions are staged in LDS; thread (r, q) tests pairs (64 bi + r, 64 bj + 16 q + jj)
@@ -1200,13 +1281,11 @@ __host__ __device__ constexpr int conf_tiles(int S) {
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_conflicts_tiled(DevModel m, const double* pos, const int* cells, int S,
-                                                         const int* dS, uint64_t* mask) {
-    if (dS)
-        S = min(S, *dS);
+__device__ __forceinline__ void conflicts_tiled_body(const DevModel& m, const double* pos, const int* cells, int S,
+                                                     uint64_t* mask, int bx) {
     const int T = m.T;
     const int ntb = (T + CONF_TB - 1) / CONF_TB;
-    const int b = blockIdx.x / ntb, tb = blockIdx.x % ntb;
+    const int b = bx / ntb, tb = bx % ntb;
     if (b >= conf_tiles(S))
         return;  // uniform over the workgroup
     int bi = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
@@ -1273,6 +1352,14 @@ __global__ __launch_bounds__(256) void k_conflicts_tiled(DevModel m, const doubl
         atomicOr((unsigned long long*)&mask[(size_t)(64 * bi + tid) * UG_MASK_WORDS + bj], word[tid]);
 }
 
+template <int D>
+__global__ __launch_bounds__(256) void k_conflicts_tiled(DevModel m, const double* pos, const int* cells, int S,
+                                                         const int* dS, uint64_t* mask) {
+    if (dS)
+        S = min(S, *dS);
+    conflicts_tiled_body<D>(m, pos, cells, S, mask, blockIdx.x);
+}
+
 // A resolve chunk's results, stored straight into the pinned staging block (one dispatch in
 // place of four copies): [8] accepted count, [16] occupied (t, cell) buckets, [32] the
 // filter's four counts when they were still on the device, [64 ..] the accepted candidate
@@ -1303,10 +1390,7 @@ __device__ __forceinline__ void report_block(const int* nacc, const unsigned lon
 // conflicts with no earlier accepted survivor -- serial RSA restricted to the chunk.  Lane
 // i of block blk needs words 0..blk of its mask row; they are loaded together, and the next
 // block's while this one is decided, so the pass pays about one load latency in all.
-__global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, const int* dS, int* acc_list,
-                                               int* nacc) {
-    if (dS)
-        S = min(S, *dS);
+__device__ __forceinline__ void greedy_body(const uint64_t* mask, int S, int* acc_list, int* nacc) {
     __shared__ uint64_t accw[UG_MASK_WORDS];
     const int lane = threadIdx.x;
     uint64_t cur[UG_MASK_WORDS], nxt[UG_MASK_WORDS];
@@ -1362,6 +1446,13 @@ __global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, cons
         *nacc = na;
 }
 
+__global__ __launch_bounds__(64) void k_greedy(const uint64_t* mask, int S, const int* dS, int* acc_list,
+                                               int* nacc) {
+    if (dS)
+        S = min(S, *dS);
+    greedy_body(mask, S, acc_list, nacc);
+}
+
 // Sparse grid: path n becomes the head of the cell `key` (the slot claimed by compare-and-swap
 // when the cell was empty, *fresh set then); returns the previous head, -1 if none.  Slots of an
 // empty table hold {0, 0xffffffff}; a table is never more than half full.
@@ -1412,12 +1503,11 @@ __global__ __launch_bounds__(256) void k_hash_rebuild(DevModel m, int64_t rows, 
 // receives at most one of them except at rounding ties, where the atomic exchange keeps the
 // chain intact.  Row-edge cells are mirrored into the halo cells afterwards (k_halo).
 template <int D>
-__global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, const int* acc_list,
-                                                const int* nacc, int limit, const int64_t* surv_idx,
-                                                const int* surv_gid, int64_t n0, int64_t* acc_idx,
-                                                int32_t* acc_gid, double* pos64, float4* pos32,
-                                                unsigned long long* filled) {
-    const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void insert_item(const DevModel& m, const double* pos, const int* acc_list,
+                                            const int* nacc, int limit, const int64_t* surv_idx,
+                                            const int* surv_gid, int64_t n0, int64_t* acc_idx,
+                                            int32_t* acc_gid, double* pos64, float4* pos32,
+                                            unsigned long long* filled, int64_t item) {
     const int na = min(*nacc, limit);
     const int k = (int)(item / m.T), t = (int)(item % m.T);
     bool fresh = false;
@@ -1464,6 +1554,16 @@ __global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, c
 }
 
 template <int D>
+__global__ __launch_bounds__(256) void k_insert(DevModel m, const double* pos, const int* acc_list,
+                                                const int* nacc, int limit, const int64_t* surv_idx,
+                                                const int* surv_gid, int64_t n0, int64_t* acc_idx,
+                                                int32_t* acc_gid, double* pos64, float4* pos32,
+                                                unsigned long long* filled) {
+    insert_item<D>(m, pos, acc_list, nacc, limit, surv_idx, surv_gid, n0, acc_idx, acc_gid, pos64, pos32, filled,
+                   (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+template <int D>
 __device__ __forceinline__ void halo_item(const DevModel& m, const double* pos, const int* acc_list,
                                           const int* nacc, int limit, int64_t item) {
     const int na = min(*nacc, limit);
@@ -1505,6 +1605,83 @@ __global__ __launch_bounds__(256) void k_halo_report(DevModel m, const double* p
         report_block(nacc, filled, acc_idx, s, counts, pin, spec_count, spec_n, stop_lim, n_room);
     else
         halo_item<D>(m, pos, acc_list, nacc, limit, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+
+// ------------------------------------------------------------------ batched resolve
+// A resolve chunk of every universe of a batched step (MResolve; blockIdx.y = universe for
+// the per-item kernels, blockIdx.x = universe for the per-universe ones).  The same bodies as
+// the single-universe kernels, so each universe gets the bits it would get alone.
+// the chunk's survivors: at most S, and the filter's count when it is on the device
+__device__ __forceinline__ int resolve_count(const MResolve& a) {
+    return a.counts ? min(a.S, a.counts[1]) : a.S;
+}
+// A universe's resolve record, its pointers global (see ug_gptr).
+__device__ __forceinline__ MResolve resolve_rec(const MResolve* A, int u) {
+    const MResolve* r = A + u;
+    MResolve a = *r;
+    a.m = load_model(&r->m);
+    a.pos = ug_gptr(&r->pos);
+    a.cells = ug_gptr(&r->cells);
+    a.mask = ug_gptr(&r->mask);
+    a.counts = ug_gptr(&r->counts);
+    a.acc_list = ug_gptr(&r->acc_list);
+    a.nacc = ug_gptr(&r->nacc);
+    a.surv_idx = ug_gptr(&r->surv_idx);
+    a.acc_idx = ug_gptr(&r->acc_idx);
+    a.acc_gid = ug_gptr(&r->acc_gid);
+    a.pos64 = ug_gptr(&r->pos64);
+    a.pos32 = ug_gptr(&r->pos32);
+    a.filled = ug_gptr(&r->filled);
+    a.rep = ug_gptr(&r->rep);
+    return a;
+}
+template <int D>
+__global__ __launch_bounds__(256) void k_conflicts_multi(const MResolve* A) {
+    const MResolve a = resolve_rec(A, blockIdx.y);
+    const int S = resolve_count(a);
+    if (!a.tiled && S > 1)
+        conflicts_body<D>(a.m, a.pos, a.cells, S, a.mask, blockIdx.x, gridDim.x);
+}
+template <int D>
+__global__ __launch_bounds__(256) void k_conflicts_tiled_multi(const MResolve* A) {
+    const MResolve a = resolve_rec(A, blockIdx.y);
+    if (a.tiled)
+        conflicts_tiled_body<D>(a.m, a.pos, a.cells, resolve_count(a), a.mask, blockIdx.x);
+}
+__global__ __launch_bounds__(64) void k_greedy_multi(const MResolve* A) {
+    const MResolve a = resolve_rec(A, blockIdx.x);
+    greedy_body(a.mask, resolve_count(a), a.acc_list, a.nacc);
+}
+template <int D>
+__global__ __launch_bounds__(256) void k_insert_multi(const MResolve* A) {
+    const MResolve a = resolve_rec(A, blockIdx.y);
+    if ((int64_t)blockIdx.x * 256 >= (int64_t)a.S * a.m.T)
+        return;  // the whole workgroup is past this universe's items
+    insert_item<D>(a.m, a.pos, a.acc_list, a.nacc, a.limit, a.surv_idx, nullptr, a.n0, a.acc_idx, a.acc_gid,
+                   a.pos64, a.pos32, a.filled, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+template <int D>
+__global__ __launch_bounds__(256) void k_halo_multi(const MResolve* A) {
+    const MResolve a = resolve_rec(A, blockIdx.y);
+    const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (item < (int64_t)a.S * a.m.T)
+        halo_item<D>(a.m, a.pos, a.acc_list, a.nacc, a.limit, item);
+}
+// rep[0] = accepted (at most limit), rep[1] = the filter's survivors, rep[2..3] = occupied
+// buckets, rep[4] = the filter's FP64 rechecks, rep + 6: the accepted candidate indices
+// (int64) -- written into pinned host memory, read after the stream is synchronised.
+__global__ __launch_bounds__(256) void k_report_multi(const MResolve* A) {
+    const MResolve a = resolve_rec(A, blockIdx.x);
+    const int na = min(*a.nacc, a.limit);
+    if (threadIdx.x == 0) {
+        a.rep[0] = na;
+        a.rep[1] = a.counts ? a.counts[1] : a.S;
+        *(unsigned long long*)(a.rep + 2) = *a.filled;
+        a.rep[4] = a.counts ? a.counts[0] : 0;
+    }
+    int64_t* out = (int64_t*)(a.rep + 6);
+    for (int k = threadIdx.x; k < na; k += 256)
+        out[k] = a.acc_idx[a.n0 + k];
 }
 
 __global__ void k_fill_i32(int* p, int64_t n, int v) {
@@ -1564,6 +1741,89 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
 #undef UG_CAND64_PL
 #undef UG_CAND64
     return hip_check(hipGetLastError(), "k_cand64");
+}
+
+// ------------------------------------------------------------------ batched launches
+int slot_pairs_per_lane(int npairs) { return slot_pl(npairs); }
+
+size_t cand64_lds_bytes(const Pack* h) {
+    return (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
+}
+
+int32_t launch_cand64_multi(hipStream_t st, int d, int pl, bool full, bool sb, const MCand* dA, int U,
+                            int64_t items, size_t lds) {
+    if (U <= 0 || items <= 0)
+        return UG_OK;
+    if (U > 64)
+        return fail(UG_EINVAL, "batched candidate test of %d universes (at most 64 a launch)", U);
+    if (pl != 1 && pl != 2)
+        return fail(UG_EINVAL, "batched candidate test: %d slot pairs per lane (T <= 256 only)", pl);
+    const int blocks = (int)std::min<int64_t>(items, 256 * 32);
+#define UG_CM(DD, FF, PP, SS) \
+    hipLaunchKernelGGL((k_cand64_multi<DD, FF, PP, SS>), dim3(blocks), dim3(64), lds, st, dA, U)
+#define UG_CM_PL(DD, FF, SS)          \
+    do {                              \
+        if (pl == 1)                  \
+            UG_CM(DD, FF, 1, SS);     \
+        else                          \
+            UG_CM(DD, FF, 2, SS);     \
+    } while (0)
+    if (d == 2) {
+        if (full)
+            UG_CM_PL(2, true, true);
+        else if (sb)
+            UG_CM_PL(2, false, true);
+        else
+            UG_CM_PL(2, false, false);
+    } else {
+        if (full)
+            UG_CM_PL(3, true, true);
+        else if (sb)
+            UG_CM_PL(3, false, true);
+        else
+            UG_CM_PL(3, false, false);
+    }
+#undef UG_CM_PL
+#undef UG_CM
+    return hip_check(hipGetLastError(), "k_cand64_multi");
+}
+
+int32_t launch_compact_multi(hipStream_t st, const MCompact* dA, int U, int64_t max_count) {
+    if (U <= 0)
+        return UG_OK;
+    const int nb = (int)std::max<int64_t>(1, (max_count + CMP_ITEMS - 1) / CMP_ITEMS);
+    hipLaunchKernelGGL(k_compact_count_multi, dim3(nb, U), dim3(256), 0, st, dA);
+    hipLaunchKernelGGL(k_compact_scatter_multi, dim3(nb, U), dim3(256), 0, st, dA);
+    return hip_check(hipGetLastError(), "batched compaction");
+}
+
+int32_t launch_resolve_multi(hipStream_t st, int d, const MResolve* dA, int U, int maxS, int maxT,
+                             int64_t max_pairs, int max_tiles, bool any_plain, bool any_tiled) {
+    if (U <= 0)
+        return UG_OK;
+    const int64_t items = max_pairs * ((maxT + CONF_T - 1) / CONF_T);
+    const int pb = std::min(grid_for(items, 256), 4096);
+    const int tb = max_tiles * ((maxT + CONF_TB - 1) / CONF_TB);
+    const int ib = grid_for((int64_t)maxS * maxT, 256);
+    if (d == 2) {
+        if (any_plain && items > 0)
+            hipLaunchKernelGGL(k_conflicts_multi<2>, dim3(pb, U), dim3(256), 0, st, dA);
+        if (any_tiled && tb > 0)
+            hipLaunchKernelGGL(k_conflicts_tiled_multi<2>, dim3(tb, U), dim3(256), 0, st, dA);
+        hipLaunchKernelGGL(k_greedy_multi, dim3(U), dim3(64), 0, st, dA);
+        hipLaunchKernelGGL(k_insert_multi<2>, dim3(ib, U), dim3(256), 0, st, dA);
+        hipLaunchKernelGGL(k_halo_multi<2>, dim3(ib, U), dim3(256), 0, st, dA);
+    } else {
+        if (any_plain && items > 0)
+            hipLaunchKernelGGL(k_conflicts_multi<3>, dim3(pb, U), dim3(256), 0, st, dA);
+        if (any_tiled && tb > 0)
+            hipLaunchKernelGGL(k_conflicts_tiled_multi<3>, dim3(tb, U), dim3(256), 0, st, dA);
+        hipLaunchKernelGGL(k_greedy_multi, dim3(U), dim3(64), 0, st, dA);
+        hipLaunchKernelGGL(k_insert_multi<3>, dim3(ib, U), dim3(256), 0, st, dA);
+        hipLaunchKernelGGL(k_halo_multi<3>, dim3(ib, U), dim3(256), 0, st, dA);
+    }
+    hipLaunchKernelGGL(k_report_multi, dim3(U), dim3(256), 0, st, dA);
+    return hip_check(hipGetLastError(), "batched resolve");
 }
 
 // Event pairs of profiled launches outlive the handles that used them: a process-wide free

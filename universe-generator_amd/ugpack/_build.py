@@ -10,7 +10,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(PKG_DIR), "csrc")
 LIB_PATH = os.path.join(PKG_DIR, "libugpack.so")
 SOURCES = ["ug_pack.hip", "ug_fast.hip", "ug_api.hip", "ug_cache.hip", "ug_frames.hip", "ug_ensemble.hip",
-           "ug_knot.hip"]
+           "ug_knot.hip", "ug_batch.hip"]
 
 # -ffp-contract=off: the canonical FP64 arithmetic uses explicit fma only, so host and
 # device produce identical bits.  Packed FP32 instructions are disabled for all HIP code.

@@ -138,6 +138,8 @@ def lib() -> C.CDLL:
         "ug_knot_destroy": ([vp], None),
         "ug_frame_bake_slices_noalloc": ([vp, vp, i32, vp, vp, i32], i32),
         "ug_split_generation_error": ([vp, vp], i32),
+        "ug_ensemble_run_batched": ([C.POINTER(PackConfig), i32, i32, i32, C.POINTER(vp), C.POINTER(PackStatus),
+                                     vp], i32),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -175,7 +177,7 @@ def exported_symbols() -> list[str]:
         "ug_frame_bake_lightcone",
         "ug_frame_bake_slices", "ug_frame_bake_slices_noalloc", "ug_knot_create", "ug_knot_run",
         "ug_knot_get_status", "ug_knot_strands", "ug_knot_params", "ug_knot_batches", "ug_knot_destroy",
-        "ug_split_generation_error",
+        "ug_split_generation_error", "ug_ensemble_run_batched",
     ]
 
 
@@ -440,16 +442,23 @@ def list_digest(idx) -> int:
     return h
 
 
-def ensemble(configs: list[PackConfig], threads: int = 16, keep: bool = False, digests: bool = False):
-    """Pack independent universes `threads` at a time (ug_ensemble_run).  Returns the list
-    of final Status; with keep=True also the list of Packer handles, with digests=True also
-    each universe's accepted-list digest (list_digest)."""
+def ensemble(configs: list[PackConfig], threads: int = 16, keep: bool = False, digests: bool = False,
+             batched: bool = False, window: int = 0, lanes: int = 1):
+    """Pack independent universes: `threads` at a time on worker threads (ug_ensemble_run),
+    or with batched=True on the batched engine, `lanes` lock-step lanes of at most `window`
+    universes each (0 = 32, at most 64; ug_ensemble_run_batched).  Returns the list of final Status; with
+    keep=True also the list of Packer handles, with digests=True also each universe's
+    accepted-list digest (list_digest)."""
     n = len(configs)
     arr = (PackConfig * n)(*configs)
     st = (PackStatus * n)()
     handles = (C.c_void_p * n)() if keep else None
     dig = np.zeros(n, np.uint64)
-    check(lib().ug_ensemble_run(arr, n, threads, handles, st, dig.ctypes.data if digests else None))
+    dptr = dig.ctypes.data if digests else None
+    if batched:
+        check(lib().ug_ensemble_run_batched(arr, n, window, lanes, handles, st, dptr))
+    else:
+        check(lib().ug_ensemble_run(arr, n, threads, handles, st, dptr))
     out = [[Status.of(st[i]) for i in range(n)]]
     if keep:
         out.append([Packer._adopt(C.c_void_p(handles[i]), c.dim, c.T, c.terms) for i, c in enumerate(configs)])
