@@ -240,11 +240,15 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
     int64_t tab_len = (int64_t)T * T * (T + 2);
     if (h->d == 3)
         tab_len *= T;
-    const int occw = (T + 2 + 31) / 32 + 1;  // one spare word: a 3-bit window may cross a word
+    const int occw = (T + 2 + 31) / 32 + 1;  // 2+1: one spare word, a 3-bit window may cross a word
+    // 3+1: word planes of 4 x 8 row tiles (occ_index)
+    const int occ_nwq = h->d == 3 ? (T + 2 + 31) / 32 : 0, occ_nxb = (T + 3) / 4, occ_nyb = (T + 7) / 8;
     const bool sparse = with_table && cfg.sparse;
     // occupancy bits with either grid (a 32nd of the dense table), the dense head table only
     // without --sparse
-    h->occ_len = with_table ? tab_len / (T + 2) * occw : 0;
+    h->occ_len = !with_table ? 0
+                 : h->d == 3 ? (int64_t)T * occ_nwq * occ_nxb * occ_nyb * 32
+                             : tab_len / (T + 2) * occw;
     if (!with_table || sparse)
         tab_len = 0;
     h->tab_len = tab_len;
@@ -311,6 +315,9 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
     m.tab = h->d_tab;
     m.occ = h->d_occ;
     m.occw = occw;
+    m.occ_nwq = occ_nwq;
+    m.occ_nxb = occ_nxb;
+    m.occ_nyb = occ_nyb;
     m.pos64 = nullptr;
     m.pos32 = nullptr;
     m.marg = nullptr;

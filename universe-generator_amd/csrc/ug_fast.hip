@@ -297,7 +297,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                 uint32_t cells = 0;
 #pragma unroll
                 for (int rr = 0; rr < ROWS; rr++)
-                    cells |= occ_bits3(m, row_of(rr), c[D - 1]) << (3 * rr);
+                    cells |= occ_bits3(m, t, gx_of(rr), gy_of(rr), c[D - 1]) << (3 * rr);
                 while (cells && !cert) {
                     constexpr int NG = 2;
                     int node[NG];

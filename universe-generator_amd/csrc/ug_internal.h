@@ -40,8 +40,9 @@ struct DevModel {
     const double* CT; // [T+1][T]  C transposed
     const int* order; // [T] endpoint-first scan order
     int* tab;         // cell table: head path of each (t, cell) chain, -1 = empty
-    uint32_t* occ;    // occupancy bits of the same padded rows (bit i <-> head i), occw words a row
-    int occw;
+    uint32_t* occ;    // occupancy bits of the same padded rows (bit i <-> head i), see occ_index
+    int occw;         // 2+1: words a row (one spare: a 3-bit window may cross a word)
+    int occ_nwq, occ_nxb, occ_nyb;  // 3+1 tiles: words along a padded row, x tiles of 4, y tiles of 8
     const double* pos64;  // [n][T][D] canonical positions of accepted paths
     const float4* pos32;  // [n][T] (x, y, w, next path in the same cell as int bits)
     // split-BF16 form
@@ -83,14 +84,30 @@ __host__ __device__ inline int wrap_cell(int c, int d, int T) {
     return v < 0 ? v + T : (v >= T ? v - T : v);
 }
 
-// Occupancy of the padded cells c..c+2 (cells c-1..c+1 of the unpadded row) of table row
-// `row` (row = t*T + x in 2+1, (t*T + x)*T + y in 3+1) as bits 0..2: one two-word window, no
-// branches.  The bit rows are a 32nd of the head table, small enough to stay in the
-// last-level cache at 3+1 T=128, so empty neighbour cells cost no head-table traffic.
-__device__ __forceinline__ uint32_t occ_bits3(const DevModel& m, int64_t row, int c) {
-    const uint32_t* w = m.occ + row * m.occw + (c >> 5);
-    const uint64_t v = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
-    return (uint32_t)(v >> (c & 31)) & 7u;
+// Word wq (padded bits 32 wq .. 32 wq + 31) of the occupancy row of cells (t, x[, y], *).
+// 2+1: rows [t][x] of occw words, each row contiguous.  3+1: per timestep and word plane wq,
+// tiles of 4 x 8 rows ((x mod 4, y mod 8), one word each: 128 bytes, one cache line), so the
+// 3 x 3 rows around a cell usually lie in one or two lines instead of three rows a whole
+// x-plane apart -- the lookup of the 17 GB table at 3+1 T=256 then fetches about 2 lines of
+// bits per visited timestep instead of about 5.  The bits are a 32nd of the head table (42 MB
+// at 3+1 T=128, inside the last-level cache), so empty neighbour cells cost no head traffic.
+__host__ __device__ __forceinline__ int64_t occ_index(const DevModel& m, int t, int x, int y, int wq) {
+    if (m.occ_nwq == 0)
+        return ((int64_t)t * m.T + x) * m.occw + wq;
+    return ((((int64_t)t * m.occ_nwq + wq) * m.occ_nxb + (x >> 2)) * m.occ_nyb + (y >> 3)) * 32 + (x & 3) * 8 +
+           (y & 7);
+}
+
+// Occupancy of the padded cells c..c+2 (cells c-1..c+1 of the unpadded row, wrapped by the
+// halo bits at the row ends) of row (t, x[, y]) as bits 0..2.  The next word is read only
+// where the window crosses into it (it exists then: the padded row has T + 2 bits).
+__device__ __forceinline__ uint32_t occ_bits3(const DevModel& m, int t, int x, int y, int c) {
+    const int wq = c >> 5, sh = c & 31;
+    const uint64_t lo = m.occ[occ_index(m, t, x, y, wq)];
+    uint64_t hi = 0;
+    if (sh > 29)
+        hi = m.occ[occ_index(m, t, x, y, wq + 1)];
+    return (uint32_t)(((hi << 32) | lo) >> sh) & 7u;
 }
 
 // Sparse grid: the key of cell c (0 <= c < T) of table row `row`, and the slot probe sequence
@@ -119,11 +136,11 @@ __device__ __forceinline__ int hash_head(const DevModel& m, uint64_t key) {
     }
 }
 
-// Heads of the cells c-1, c, c+1 (wrapped) of table row `row` (t*T + x in 2+1, (t*T + x)*T + y
-// in 3+1), -1 for empty cells: the occupancy bits first, then the occupied cells' heads from
-// the dense table's padded row or from the sparse grid.
-__device__ __forceinline__ void row_heads3(const DevModel& m, int64_t row, int c, int node[3]) {
-    const uint32_t bits = occ_bits3(m, row, c);
+// Heads of the cells c-1, c, c+1 (wrapped) of table row (t, x[, y]) (row = t*T + x in 2+1,
+// (t*T + x)*T + y in 3+1), -1 for empty cells: the occupancy bits first, then the occupied
+// cells' heads from the dense table's padded row or from the sparse grid.
+__device__ __forceinline__ void row_heads3(const DevModel& m, int t, int x, int y, int64_t row, int c, int node[3]) {
+    const uint32_t bits = occ_bits3(m, t, x, y, c);
     if (m.sparse) {
 #pragma unroll
         for (int k = 0; k < 3; k++)
