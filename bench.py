@@ -652,11 +652,18 @@ def main() -> int:
         wall = sum(per_pack)
         stats, rstats = sum_stats(pstats), sum_stats(rstats_l)
         if prec == ugpack.PREC_SPLIT_BF16:  # the FP64 recheck timed in one untimed pack (seed 1)
-            p, _ = pack_once(ugpack, 1, prec, profile=ugpack.abi.PROFILE_ALL, workload=args.workload)
+            p, _ = pack_once(ugpack, 1, prec, profile=ugpack.abi.PROFILE_ALL | ugpack.abi.PROFILE_REASONS,
+                             workload=args.workload)
             one = p.kernel_stats(ugpack.abi.KERNEL_RECHECK)
+            pre1 = p.kernel_stats(ugpack.abi.KERNEL_PREFILTER)
             p.close()
             rstats = {"items": rstats["items"], "visits": rstats["visits"], "points": rstats["points"],
-                      "timed_pack_seed1": one}
+                      "per_candidate_tested": round(rstats["items"] / max(stats["items"], 1), 6),
+                      # why stage 1 left candidates to the FP64 test, in that pack
+                      "left_by_reason_seed1": {"spacing_sum": pre1["left_wsum"],
+                                               "possible_contact": pre1["left_possible"],
+                                               "no_possible_contact": pre1["left_none"]},
+                      "timed_pack_seed1": {k: one[k] for k in ("launches", "items", "ms", "visits", "points")}}
         wl = WORKLOADS[args.workload]
         if wl.get("frames") and prec == prec_main:
             p, _ = pack_once(ugpack, 1, prec, workload=args.workload)

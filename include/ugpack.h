@@ -192,6 +192,11 @@ typedef struct ug_kernel_stats {
     double ms;        /* summed event time */
     int64_t visits;   /* (candidate, timestep) pairs visited */
     int64_t points;   /* grid points compared */
+    /* split-BF16 prefilter, stage 1 (counted in every launch with UG_PROFILE_REASONS): the
+     * candidates it leaves to the FP64 test, by reason -- spacing sum below the error bound's
+     * premise; a possible contact (within r + margin) but no certain one among the examined
+     * timesteps; no possible contact there at all.  Zero for the other kernels. */
+    int64_t left_wsum, left_possible, left_none;
 } ug_kernel_stats;
 
 enum {
@@ -200,14 +205,16 @@ enum {
 };
 
 /* ug_pack_profile's `kernels`: which launches are timed (bit UG_KERNEL_* set); any nonzero
- * value turns the counters on, 0 turns measurement off.  Values above UG_PROFILE_ALL are
- * rejected.  In ug_kernel_stats, launches, ms and the prefilter's items cover the timed
+ * value turns the counters on, 0 turns measurement off.  UG_PROFILE_REASONS also counts why
+ * the split-BF16 prefilter leaves candidates to the FP64 test (ug_kernel_stats left_*; it
+ * slows the prefilter, so timed packs leave it off).  Other bits are rejected.  In ug_kernel_stats, launches, ms and the prefilter's items cover the timed
  * launches only; visits, points and the recheck's items are counted in every launch while
  * measurement is on. */
 enum {
     UG_PROFILE_PREFILTER = 1 << UG_KERNEL_PREFILTER,
     UG_PROFILE_RECHECK = 1 << UG_KERNEL_RECHECK,
-    UG_PROFILE_ALL = UG_PROFILE_PREFILTER | UG_PROFILE_RECHECK
+    UG_PROFILE_ALL = UG_PROFILE_PREFILTER | UG_PROFILE_RECHECK,
+    UG_PROFILE_REASONS = 1 << 2
 };
 
 int32_t ug_pack_profile(ug_pack* h, int32_t kernels);

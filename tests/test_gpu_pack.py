@@ -201,9 +201,32 @@ def test_profiling_selects_the_timed_kernels(gpu):
     counts = {k: (v[0]["visits"], v[0]["points"], v[1]["items"], v[1]["visits"]) for k, v in seen.items()}
     assert len(set(counts.values())) == 1
     with ugpack.Packer(2, 16, seed=1, max_attempts=100) as p:
-        for bad in (-1, abi.PROFILE_ALL + 1):
+        for bad in (-1, (abi.PROFILE_ALL | abi.PROFILE_REASONS) + 1):
             with pytest.raises(ugpack.UgError, match="UG_PROFILE"):
                 abi.check(abi.lib().ug_pack_profile(p._h, bad))
+
+
+@pytest.mark.parametrize("d,T,cands", [(2, 64, 3 << 20), (3, 32, 1 << 20)])
+def test_prefilter_reasons_account_for_every_recheck(gpu, d, T, cands):
+    """With UG_PROFILE_REASONS the prefilter sorts every candidate it leaves to the FP64 test
+    by reason; with one prefilter stage (T <= 160) they are exactly the recheck's items, and
+    counting them changes neither the lists nor the other counters."""
+    import ugpack
+    from ugpack import abi
+
+    print(f"reasons d={d} T={T}", flush=True)
+    out = {}
+    for kernels in (abi.PROFILE_ALL, abi.PROFILE_ALL | abi.PROFILE_REASONS):
+        with ugpack.Packer(d, T, seed=3, max_attempts=cands, precision=ugpack.PREC_SPLIT_BF16) as p:
+            p.profile(True, kernels=kernels)
+            p.run()
+            pre, re = p.kernel_stats(abi.KERNEL_PREFILTER), p.kernel_stats(abi.KERNEL_RECHECK)
+            out[kernels] = (list(p.accepted()[0]), pre, re)
+    (la, pa, ra), (lb, pb, rb) = out[abi.PROFILE_ALL], out[abi.PROFILE_ALL | abi.PROFILE_REASONS]
+    assert la == lb
+    assert (pa["visits"], pa["points"], ra["items"]) == (pb["visits"], pb["points"], rb["items"])
+    assert pa["left_wsum"] == pa["left_possible"] == pa["left_none"] == 0
+    assert pb["left_wsum"] + pb["left_possible"] + pb["left_none"] == rb["items"] > 0
 
 
 def test_invalid_config_is_an_error_not_an_exit(gpu):

@@ -807,7 +807,7 @@ extern "C" int32_t ug_pack_check(ug_pack* hp, uint64_t first, int64_t count, int
 extern "C" int32_t ug_pack_profile(ug_pack* hp, int32_t kernels) {
     if (!hp)
         return fail(UG_EINVAL, "ug_pack_profile: NULL handle");
-    if (kernels < 0 || kernels > UG_PROFILE_ALL)
+    if (kernels < 0 || (kernels & ~(UG_PROFILE_ALL | UG_PROFILE_REASONS)))
         return fail(UG_EINVAL, "ug_pack_profile: kernels=%d is not a set of UG_PROFILE_* bits", kernels);
     Pack* h = (Pack*)hp;
     if (kernels && !h->d_ctr) {

@@ -178,11 +178,15 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
 
 // One work item of the prefilter: the wave's tile `tile` of 32 candidates over the N blocks
 // [nbs, nbe) of 32 ordered timesteps (count: the launch's candidates, already bounded).
-template <int D, bool SUB, bool SPARSE, bool KNOT>
+// RSN (measurement): lane 0's cls[0..2] count the candidates left to the FP64 test by reason:
+// spacing sum below the bound's premise, a possible contact (within r + m_t) but no certain one
+// among the examined timesteps, no possible contact there at all.
+template <int D, bool SUB, bool SPARSE, bool KNOT, bool RSN = false>
 __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, int nbs, int nbe, int par,
                                                uint64_t base, int64_t count, const int64_t* list,
                                                uint8_t* flags, const int* acc_gid, int64_t nlimit,
-                                               unsigned long long& nvis, unsigned long long& npts) {
+                                               unsigned long long& nvis, unsigned long long& npts,
+                                               unsigned long long* cls) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevModel& m = f.m;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -200,7 +204,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
         const int64_t i = tile * TILE + mrow;
         const bool valid = i < count;
         const uint64_t idx = list ? (valid ? (uint64_t)list[i] : 0ull) : base + (uint64_t)(valid ? i : 0);
-        uint32_t rejm = 0;
+        uint32_t rejm = 0, graym = 0, wbadm = 0;
         int g1 = -1;  // SUB, lanes < 32: the one group candidate `lane` certainly touches so far
         // N blocks of 32 ordered timesteps each; the coefficients are generated again per
         // block (a later block only sees the candidates no earlier one rejected)
@@ -212,6 +216,8 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
         // spacings sum below nw/4 (wbad) is outside the error bound's premise and goes
         // straight to the FP64 test.
         uint32_t unres = (uint32_t)__ballot(lane < 32 && valid && !wbad && !((rejm >> lane) & 1u));
+        if (RSN)
+            wbadm |= (uint32_t)__ballot(lane < 32 && valid && wbad);
         if (!unres)
             break;
         const int col0 = NCOL * nb, bcols = min(NCOL, f.ncols - col0);  // this block's columns
@@ -232,7 +238,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
             const int rk = lane / per, off = lane % per;
             const int cm = rk < U ? ulist[rk] : 0;
             const int col = __shfl(ncol, cm, 64) + off;
-            bool cert = false;
+            bool cert = false, maybe = false;  // maybe: a possible contact (measurement only)
             int gmn = INT32_MAX, gmx = -1;  // SUB: groups of this lane's certain contacts
             if (rk < U && col < bcols) {
                 const int t = f.colt[col0 + col];
@@ -270,6 +276,8 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                         q = fmaf(d, d, q);
                     }
                     npts++;
+                    if (RSN)
+                        maybe = maybe || dmax <= r + mt;
                     const bool sure = m.euclid ? q <= rb2 : dmax <= r - mt;
                     if (!SUB) {
                         cert = cert || sure;
@@ -330,11 +338,14 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
             }
             if (!SUB) {
                 const uint64_t hits = __ballot(cert);
+                const uint64_t maybes = RSN ? __ballot(maybe) : 0ull;
                 if (lane < 32 && ((unres >> lane) & 1u)) {
                     const int myrk = __popc(unres & ((1u << lane) - 1u));
                     const uint64_t segmask = per >= 64 ? ~0ull : ((1ull << per) - 1ull);
                     if ((hits >> (myrk * per)) & segmask)
                         rejm |= 1u << lane;
+                    if ((maybes >> (myrk * per)) & segmask)
+                        graym |= 1u << lane;
                     ncol += per;
                 }
             } else {
@@ -369,6 +380,14 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
             else if ((rejm >> lane) & 1u)
                 flags[i] = 0;
         }
+        if (RSN && !par && !SUB) {
+            const uint32_t left = (uint32_t)__ballot(valid && h == 0 && !((rejm >> lane) & 1u));
+            if (lane == 0) {
+                cls[0] += __popc(left & wbadm);
+                cls[1] += __popc(left & ~wbadm & graym);
+                cls[2] += __popc(left & ~wbadm & ~graym);
+            }
+        }
     }
 }
 
@@ -381,7 +400,8 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
 // its 52 KB of LDS per block allows.
 // KNOT: no grid -- a visit compares the candidate with every resident strand (the knot
 // regime's brute force) and rejects it on a certain contact.
-template <int D, bool SUB, bool SPARSE, bool KNOT = false>
+// RSN: also count why candidates are left to the FP64 test (measurement; ctr must be set).
+template <int D, bool SUB, bool SPARSE, bool KNOT = false, bool RSN = false>
 __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilter(FastArgs f, uint64_t base, int64_t count,
                                                         const int64_t* list, const int* dcount,
                                                         int cb0, int cb1, int par, uint8_t* flags,
@@ -395,12 +415,13 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
     // rejects; otherwise one item per tile walking its N blocks in order
     const int nbk = par ? cb1 - cb0 : 1;
     unsigned long long nvis = 0, npts = 0;
+    unsigned long long cls[3] = {0, 0, 0};  // lane 0: the wave's FP64-test candidates by reason
     for (int64_t item = (int64_t)blockIdx.x * FAST_WAVES + wv; item < ntiles * nbk;
          item += (int64_t)gridDim.x * FAST_WAVES) {
         const int64_t tile = item / nbk;
         const int nbs = par ? cb0 + (int)(item % nbk) : cb0, nbe = par ? nbs + 1 : cb1;
-        prefilter_item<D, SUB, SPARSE, KNOT>(f, tile, nbs, nbe, par, base, count, list, flags, acc_gid, nlimit,
-                                             nvis, npts);
+        prefilter_item<D, SUB, SPARSE, KNOT, RSN>(f, tile, nbs, nbe, par, base, count, list, flags, acc_gid,
+                                                  nlimit, nvis, npts, cls);
     }
     if (ctr) {
         unsigned long long a = nvis, b = npts;
@@ -413,6 +434,10 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
             atomicAdd(&c[0], a);
             atomicAdd(&c[1], b);
         }
+        if (RSN && lane == 0)
+            for (int k = 0; k < 3; k++)
+                if (cls[k])
+                    atomicAdd(&c[3 + k], cls[k]);
     }
 }
 
@@ -453,7 +478,7 @@ __global__ __launch_bounds__(256, D == 2 ? 4 : 3) void k_fast_prefilter_multi(co
         a.f.bfrag = ug_gptr(&r->f.bfrag);
         a.f.colt = ug_gptr(&r->f.colt);
         prefilter_item<D, false, false, false>(a.f, tile, 0, 1, 0, a.base, a.count, ug_gptr(&r->list),
-                                               ug_gptr(&r->flags), nullptr, 0, nvis, npts);
+                                               ug_gptr(&r->flags), nullptr, 0, nvis, npts, nullptr);
     }
 }
 
@@ -618,14 +643,16 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
     if (!rc && fs->lds_prefilter > 160 * 1024)
         rc = fail(UG_EINVAL, "split-BF16 tile needs %zu bytes of LDS", fs->lds_prefilter);
     if (!rc) {
-        const void* kf[5] = {
+        const void* kf[6] = {
             tb.d == 2 ? (const void*)k_fast_prefilter<2, false, false> : (const void*)k_fast_prefilter<3, false, false>,
             tb.d == 2 ? (const void*)k_fast_prefilter<2, true, false> : (const void*)k_fast_prefilter<3, true, false>,
             tb.d == 2 ? (const void*)k_fast_prefilter<2, false, true> : (const void*)k_fast_prefilter<3, false, true>,
             tb.d == 2 ? (const void*)k_fast_prefilter<2, true, true> : (const void*)k_fast_prefilter<3, true, true>,
             tb.d == 2 ? (const void*)k_fast_prefilter<2, false, false, true>
-                      : (const void*)k_fast_prefilter<3, false, false, true>};
-        for (int v = 0; v < 5 && !rc; v++)
+                      : (const void*)k_fast_prefilter<3, false, false, true>,
+            tb.d == 2 ? (const void*)k_fast_prefilter<2, false, false, false, true>
+                      : (const void*)k_fast_prefilter<3, false, false, false, true>};
+        for (int v = 0; v < 6 && !rc; v++)
             rc = hip_check(hipFuncSetAttribute(kf[v], hipFuncAttributeMaxDynamicSharedMemorySize, (int)fs->lds_prefilter),
                            "lds attribute");
     }
@@ -723,7 +750,16 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
 #define UG_FAST(DD, SP)                                                                                     \
     UG_LAUNCH(h, (k_fast_prefilter<DD, false, SP>), dim3(nblk), dim3(256), fs->lds_prefilter, f, base, count, \
               list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0)
-    if (h->d == 2 && !f.m.sparse)
+    // the reason counts (UG_PROFILE_REASONS) in their own instantiation, off the timed path
+    const bool rsn = ctr && ((h->profile_mask >> 2) & 1) && !par && !f.m.sparse;
+#define UG_FASTR(DD)                                                                                      \
+    UG_LAUNCH(h, (k_fast_prefilter<DD, false, false, false, true>), dim3(nblk), dim3(256), fs->lds_prefilter, \
+              f, base, count, list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0)
+    if (rsn && h->d == 2)
+        UG_FASTR(2);
+    else if (rsn)
+        UG_FASTR(3);
+    else if (h->d == 2 && !f.m.sparse)
         UG_FAST(2, false);
     else if (h->d == 2)
         UG_FAST(2, true);
@@ -731,6 +767,7 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
         UG_FAST(3, false);
     else
         UG_FAST(3, true);
+#undef UG_FASTR
 #undef UG_FAST
     return hip_check(hipGetLastError(), "k_fast_prefilter");
 }

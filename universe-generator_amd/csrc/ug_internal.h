@@ -99,13 +99,16 @@ __host__ __device__ __forceinline__ int64_t occ_index(const DevModel& m, int t, 
 }
 
 // Occupancy of the padded cells c..c+2 (cells c-1..c+1 of the unpadded row, wrapped by the
-// halo bits at the row ends) of row (t, x[, y]) as bits 0..2.  The next word is read only
-// where the window crosses into it (it exists then: the padded row has T + 2 bits).
+// halo bits at the row ends) of row (t, x[, y]) as bits 0..2.  3+1 reads the next word plane
+// only where the window crosses into it (it exists then: the padded row has T + 2 bits); a
+// 2+1 row's next word (or its spare word) is always read, it shares the row's line.
 __device__ __forceinline__ uint32_t occ_bits3(const DevModel& m, int t, int x, int y, int c) {
     const int wq = c >> 5, sh = c & 31;
     const uint64_t lo = m.occ[occ_index(m, t, x, y, wq)];
     uint64_t hi = 0;
-    if (sh > 29)
+    if (m.occ_nwq == 0)
+        hi = m.occ[occ_index(m, t, x, y, wq + 1)];  // 2+1: the row's next word, in the same line
+    else if (sh > 29)
         hi = m.occ[occ_index(m, t, x, y, wq + 1)];
     return (uint32_t)(((hi << 32) | lo) >> sh) & 7u;
 }
@@ -191,7 +194,7 @@ struct BatchItems {
 #define UG_GAS __attribute__((address_space(1)))
 template <class T>
 __device__ __forceinline__ T* ug_gptr(T* const* field) {
-    return (T*)*(T UG_GAS* const*)field;
+    return (T*)*(T UG_GAS* const*)(const void*)field;
 }
 // A DevModel from one in device memory, its pointers global.
 __device__ __forceinline__ DevModel load_model(const DevModel* src) {

@@ -1921,14 +1921,17 @@ int32_t timed_collect(Pack* h) {
         if (rc)
             return rc;
         for (int w = 0; w < 2; w++) {
-            int64_t sum[3] = {0, 0, 0};
+            int64_t sum[6] = {0, 0, 0, 0, 0, 0};
             for (int k = 0; k < UG_CTR_SLOTS; k++)
-                for (int j = 0; j < 3; j++)
+                for (int j = 0; j < 6; j++)
                     sum[j] += (int64_t)c[(size_t)(w * UG_CTR_SLOTS + k) * UG_CTR_STRIDE + j];
             h->kstats[w].visits = sum[0];
             h->kstats[w].points = sum[1];
             if (w == UG_KERNEL_RECHECK)
                 h->kstats[w].items = sum[2];
+            h->kstats[w].left_wsum = sum[3];
+            h->kstats[w].left_possible = sum[4];
+            h->kstats[w].left_none = sum[5];
         }
     }
     return UG_OK;

@@ -56,7 +56,8 @@ class SubConfig(C.Structure):
 
 class KernelStats(C.Structure):
     _fields_ = [("launches", C.c_int64), ("items", C.c_int64), ("ms", C.c_double),
-                ("visits", C.c_int64), ("points", C.c_int64)]
+                ("visits", C.c_int64), ("points", C.c_int64), ("left_wsum", C.c_int64),
+                ("left_possible", C.c_int64), ("left_none", C.c_int64)]
 
 
 class KnotConfig(C.Structure):
@@ -74,6 +75,7 @@ class KnotStatus(C.Structure):
 KERNEL_PREFILTER, KERNEL_RECHECK = 0, 1
 PROFILE_PREFILTER, PROFILE_RECHECK = 1 << KERNEL_PREFILTER, 1 << KERNEL_RECHECK
 PROFILE_ALL = PROFILE_PREFILTER | PROFILE_RECHECK
+PROFILE_REASONS = 1 << 2  # also count why the prefilter leaves candidates to the FP64 test
 SUB_BUDGET, SUB_ACCEPT_RATE, SUB_FILL_RATE = 1, 2, 3
 
 
@@ -316,7 +318,8 @@ class Packer:
         ks = KernelStats()
         check(lib().ug_pack_kernel_stats(self._h, which, C.byref(ks)))
         return {"launches": ks.launches, "items": ks.items, "ms": ks.ms, "visits": ks.visits,
-                "points": ks.points}
+                "points": ks.points, "left_wsum": ks.left_wsum, "left_possible": ks.left_possible,
+                "left_none": ks.left_none}
 
     def curve(self) -> np.ndarray:
         n = C.c_int64()
