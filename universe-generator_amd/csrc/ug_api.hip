@@ -123,6 +123,9 @@ void release(Pack* h) {
     free_dev(h->d_list3);
     free_dev(h->d_list4);
     free_dev(h->d_listb);
+    free_dev(h->d_skip);
+    free_dev(h->d_skip3);
+    free_dev(h->d_skip4);
     free_dev(h->d_spec_count);
     if (h->ev_report)
         (void)hipEventDestroy(h->ev_report);
@@ -270,6 +273,9 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
     UG_ALLOC(h->d_list3, (size_t)UG_SUB_MAX * 8);
     UG_ALLOC(h->d_list4, (size_t)UG_SUB_MAX * 8);
     UG_ALLOC(h->d_listb, (size_t)UG_SUB_MAX * 8);
+    UG_ALLOC(h->d_skip, (size_t)UG_SUB_MAX * 4);
+    UG_ALLOC(h->d_skip3, (size_t)UG_SUB_MAX * 4);
+    UG_ALLOC(h->d_skip4, (size_t)UG_SUB_MAX * 4);
     UG_ALLOC(h->d_spec_count, 4);
     h->cmp_blocks_cap = UG_SUB_MAX / 4096 + 1;
     UG_ALLOC(h->d_bcount, (size_t)h->cmp_blocks_cap * 4);

@@ -214,7 +214,7 @@ int32_t Engine::stage_filter(std::vector<Uni*>& act) {
                 continue;
             args.resize(args.size() + fast_multi_arg_bytes());
             rc = fast_multi_arg(h, (uint64_t)h->attempts, cnt_of(u), u->fresh ? nullptr : u->pend, h->d_flags,
-                                args.data() + args.size() - fast_multi_arg_bytes());
+                                h->d_skip, args.data() + args.size() - fast_multi_arg_bytes());
             if (rc)
                 return rc;
             tiles += (cnt_of(u) + 31) / 32;
@@ -232,7 +232,8 @@ int32_t Engine::stage_filter(std::vector<Uni*>& act) {
         for (Uni* u : split) {
             Pack* h = u->h;
             cs.push_back(MCompact{h->d_flags, cnt_of(u), nullptr, (uint64_t)h->attempts, u->fresh ? nullptr : u->pend,
-                                  h->d_bcount, h->d_list3, h->d_count + 0});
+                                  h->d_bcount, h->d_list3, h->d_count + 0, (const int*)h->d_skip,
+                                  (int*)h->d_skip3});
             mx = std::max(mx, cnt_of(u));
         }
         UG_STAGE(dA, cs);
@@ -266,6 +267,7 @@ int32_t Engine::stage_filter(std::vector<Uni*>& act) {
         a.list = h->d_list3;
         a.dcount = h->d_count + 0;
         a.flags = h->d_flags;
+        a.skip = h->d_skip3;
         a.head = h->d == 2 ? 14 : 13;
         add(h->d, slot_pairs_per_lane(h->npairs), true, a, cand64_lds_bytes(h));
     }
@@ -294,13 +296,13 @@ int32_t Engine::stage_filter(std::vector<Uni*>& act) {
         for (Uni* u : split) {
             Pack* h = u->h;
             cs.push_back(MCompact{h->d_flags, cnt_of(u), h->d_count + 0, 0, h->d_list3, h->d_bcount, u->list,
-                                  h->d_count + 1});
+                                  h->d_count + 1, nullptr, nullptr});
             mx = std::max(mx, cnt_of(u));
         }
         for (Uni* u : fp64) {
             Pack* h = u->h;
             cs.push_back(MCompact{h->d_flags, cnt_of(u), nullptr, (uint64_t)h->attempts, u->fresh ? nullptr : u->pend,
-                                  h->d_bcount, u->list, h->d_count + 1});
+                                  h->d_bcount, u->list, h->d_count + 1, nullptr, nullptr});
             mx = std::max(mx, cnt_of(u));
         }
         UG_STAGE(dA, cs);

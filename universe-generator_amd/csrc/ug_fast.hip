@@ -186,7 +186,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                                                uint64_t base, int64_t count, const int64_t* list,
                                                uint8_t* flags, const int* acc_gid, int64_t nlimit,
                                                unsigned long long& nvis, unsigned long long& npts,
-                                               unsigned long long* cls) {
+                                               unsigned long long* cls, uint32_t* skip) {
     extern __shared__ __align__(16) unsigned char lds[];
     const DevModel& m = f.m;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -205,6 +205,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
         const bool valid = i < count;
         const uint64_t idx = list ? (valid ? (uint64_t)list[i] : 0ull) : base + (uint64_t)(valid ? i : 0);
         uint32_t rejm = 0, graym = 0, wbadm = 0;
+        uint32_t known = 0;  // lanes < 32: block-0 columns proven free of any FP64 contact
         int g1 = -1;  // SUB, lanes < 32: the one group candidate `lane` certainly touches so far
         // N blocks of 32 ordered timesteps each; the coefficients are generated again per
         // block (a later block only sees the candidates no earlier one rejected)
@@ -238,7 +239,9 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
             const int rk = lane / per, off = lane % per;
             const int cm = rk < U ? ulist[rk] : 0;
             const int col = __shfl(ncol, cm, 64) + off;
-            bool cert = false, maybe = false;  // maybe: a possible contact (measurement only)
+            // cert: a certain contact; maybe: a possible one (within r + m_t); clr: this
+            // column provably has no FP64 contact (see the interior test below)
+            bool cert = false, maybe = false, clr = false;
             int gmn = INT32_MAX, gmx = -1;  // SUB: groups of this lane's certain contacts
             if (rk < U && col < bcols) {
                 const int t = f.colt[col0 + col];
@@ -247,6 +250,8 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                 // the FP32 sum of squares is within (D+1)*2^-24 of its exact value
                 const float rb = r - sqrtf((float)D) * mt;
                 const float rb2 = rb > 0.f ? rb * rb * (1.0f - 0x1p-19f) : -1.0f;
+                const float rp = r + sqrtf((float)D) * mt;  // a possible ball contact
+                const float rp2 = rp * rp * (1.0f + 0x1p-19f);
                 float X[D];
                 int c[D];
 #pragma unroll
@@ -276,8 +281,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                         q = fmaf(d, d, q);
                     }
                     npts++;
-                    if (RSN)
-                        maybe = maybe || dmax <= r + mt;
+                    maybe = maybe || (m.euclid ? q <= rp2 : dmax <= r + mt);
                     const bool sure = m.euclid ? q <= rb2 : dmax <= r - mt;
                     if (!SUB) {
                         cert = cert || sure;
@@ -334,11 +338,27 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                         }
                     }
                 }
+                // the column provably holds no FP64 contact: no point of the scanned cells
+                // within r + m_t, and the FP32 position more than m_t (plus its rounding) inside
+                // its cell on every axis -- the FP64 position then lies in the same cell, the
+                // FP64 test scans the same neighbour cells, and every point there is farther
+                // than r in FP64
+                if (!cert && !maybe) {
+                    bool inner = true;
+#pragma unroll
+                    for (int a = 0; a < D; a++) {
+                        const float u = (X[a] + 1.0f) * (float)m.halfT;
+                        const float fr = u - floorf(u);
+                        inner = inner && fminf(fr, 1.0f - fr) * r > mt + 0x1p-20f;
+                    }
+                    clr = inner;
+                }
                 }
             }
             if (!SUB) {
                 const uint64_t hits = __ballot(cert);
                 const uint64_t maybes = RSN ? __ballot(maybe) : 0ull;
+                const uint64_t clrs = KNOT ? 0ull : __ballot(clr);
                 if (lane < 32 && ((unres >> lane) & 1u)) {
                     const int myrk = __popc(unres & ((1u << lane) - 1u));
                     const uint64_t segmask = per >= 64 ? ~0ull : ((1ull << per) - 1ull);
@@ -346,6 +366,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                         rejm |= 1u << lane;
                     if ((maybes >> (myrk * per)) & segmask)
                         graym |= 1u << lane;
+                    known |= (uint32_t)((((clrs >> (myrk * per)) & segmask) << ncol) & 0xffffffffull);
                     ncol += per;
                 }
             } else {
@@ -379,6 +400,8 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                 flags[i] = ((rejm >> lane) & 1u) ? 0 : 1;
             else if ((rejm >> lane) & 1u)
                 flags[i] = 0;
+            if (skip && !par && nbs == 0 && !((rejm >> lane) & 1u))
+                skip[i] = known;
         }
         if (RSN && !par && !SUB) {
             const uint32_t left = (uint32_t)__ballot(valid && h == 0 && !((rejm >> lane) & 1u));
@@ -406,7 +429,7 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
                                                         const int64_t* list, const int* dcount,
                                                         int cb0, int cb1, int par, uint8_t* flags,
                                                         unsigned long long* ctr, const int* acc_gid,
-                                                        int64_t nlimit) {
+                                                        int64_t nlimit, uint32_t* skip) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (dcount)
         count = min(count, (int64_t)*dcount);
@@ -421,7 +444,7 @@ __global__ __launch_bounds__(256, (D == 2 && !SUB) ? 4 : 3) void k_fast_prefilte
         const int64_t tile = item / nbk;
         const int nbs = par ? cb0 + (int)(item % nbk) : cb0, nbe = par ? nbs + 1 : cb1;
         prefilter_item<D, SUB, SPARSE, KNOT, RSN>(f, tile, nbs, nbe, par, base, count, list, flags, acc_gid,
-                                                  nlimit, nvis, npts, cls);
+                                                  nlimit, nvis, npts, cls, skip);
     }
     if (ctr) {
         unsigned long long a = nvis, b = npts;
@@ -456,6 +479,7 @@ struct MFast {
     int64_t count;
     const int64_t* list;
     uint8_t* flags;
+    uint32_t* skip;  // per flagged candidate: its block-0 columns proven clear
 };
 
 // Stage-1 prefilter over the candidates of many universes (at most 64; one dimension): the
@@ -478,7 +502,8 @@ __global__ __launch_bounds__(256, D == 2 ? 4 : 3) void k_fast_prefilter_multi(co
         a.f.bfrag = ug_gptr(&r->f.bfrag);
         a.f.colt = ug_gptr(&r->f.colt);
         prefilter_item<D, false, false, false>(a.f, tile, 0, 1, 0, a.base, a.count, ug_gptr(&r->list),
-                                               ug_gptr(&r->flags), nullptr, 0, nvis, npts, nullptr);
+                                               ug_gptr(&r->flags), nullptr, 0, nvis, npts, nullptr,
+                                               ug_gptr(&r->skip));
     }
 }
 
@@ -689,7 +714,8 @@ static FastArgs fast_args(Pack* h) {
 
 size_t fast_multi_arg_bytes() { return sizeof(MFast); }
 
-int32_t fast_multi_arg(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags, void* out) {
+int32_t fast_multi_arg(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags, uint32_t* skip,
+                       void* out) {
     if (!h->full || !h->d_basis16)
         return fail(UG_ESTATE, "split-BF16 prefilter not set up for this model");
     if (h->dm.sparse)
@@ -700,6 +726,7 @@ int32_t fast_multi_arg(Pack* h, uint64_t base, int64_t count, const int64_t* lis
     a.count = count;
     a.list = list;
     a.flags = flags;
+    a.skip = skip;
     memcpy(out, &a, sizeof a);
     return UG_OK;
 }
@@ -723,7 +750,7 @@ int fast_blocks(const Pack* h) {
 }
 
 int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
-                    const int* dcount, int cb0, int cb1) {
+                    const int* dcount, int cb0, int cb1, uint32_t* skip) {
     if (count <= 0)
         return UG_OK;
     if (!h->full || !h->d_basis16)
@@ -749,12 +776,12 @@ int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, 
         hipLaunchKernelGGL(k_flags_one, dim3(grid_for(count, 256 * 16)), dim3(256), 0, h->stream, flags, count, dcount);
 #define UG_FAST(DD, SP)                                                                                     \
     UG_LAUNCH(h, (k_fast_prefilter<DD, false, SP>), dim3(nblk), dim3(256), fs->lds_prefilter, f, base, count, \
-              list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0)
+              list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0, skip)
     // the reason counts (UG_PROFILE_REASONS) in their own instantiation, off the timed path
     const bool rsn = ctr && ((h->profile_mask >> 2) & 1) && !par && !f.m.sparse;
 #define UG_FASTR(DD)                                                                                      \
     UG_LAUNCH(h, (k_fast_prefilter<DD, false, false, false, true>), dim3(nblk), dim3(256), fs->lds_prefilter, \
-              f, base, count, list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0)
+              f, base, count, list, dcount, cb0, cb1, par, flags, ctr, nullptr, 0, skip)
     if (rsn && h->d == 2)
         UG_FASTR(2);
     else if (rsn)
@@ -787,10 +814,10 @@ int32_t launch_fast_knot(Pack* h, uint64_t base, int64_t count, uint8_t* flags, 
     f.nk = nk;
     if (h->d == 2)
         UG_LAUNCH(h, (k_fast_prefilter<2, false, false, true>), dim3(blocks), dim3(256), fs->lds_prefilter, f, base,
-                  count, nullptr, nullptr, 0, 1, 0, flags, nullptr, nullptr, 0);
+                  count, nullptr, nullptr, 0, 1, 0, flags, nullptr, nullptr, 0, nullptr);
     else
         UG_LAUNCH(h, (k_fast_prefilter<3, false, false, true>), dim3(blocks), dim3(256), fs->lds_prefilter, f, base,
-                  count, nullptr, nullptr, 0, 1, 0, flags, nullptr, nullptr, 0);
+                  count, nullptr, nullptr, 0, 1, 0, flags, nullptr, nullptr, 0, nullptr);
     return hip_check(hipGetLastError(), "k_fast_prefilter (knot)");
 }
 
@@ -805,7 +832,7 @@ int32_t launch_fast_sub(Pack* h, uint64_t base, int64_t count, uint8_t* flags, i
     const FastArgs f = fast_args(h);
 #define UG_FASTSUB(DD, SP)                                                                                         \
     hipLaunchKernelGGL((k_fast_prefilter<DD, true, SP>), dim3(blocks), dim3(256), fs->lds_prefilter, h->stream, f, \
-                       base, count, nullptr, nullptr, 0, 1, 0, flags, nullptr, h->d_acc_gid, nlimit)
+                       base, count, nullptr, nullptr, 0, 1, 0, flags, nullptr, h->d_acc_gid, nlimit, nullptr)
     if (h->d == 2 && !f.m.sparse)
         UG_FASTSUB(2, false);
     else if (h->d == 2)

@@ -226,6 +226,7 @@ struct MCand {
     double* out;     // trajectories (full)
     int* cells;      // their packed cells
     uint64_t* mask;  // conflict rows cleared with them
+    const uint32_t* skip;  // per listed candidate: scan positions < 32 proven clear (or null)
     int head;
 };
 // Ordered compaction of flags[0 .. min(count, *dcount)) into out (count into *total); bcount
@@ -239,6 +240,8 @@ struct MCompact {
     int* bcount;
     int64_t* out;
     int* total;
+    const int* vin;  // with vout: a value per item, compacted alongside (or null)
+    int* vout;
 };
 // A resolve chunk of min(S, counts[1]) survivors (trajectories pos, packed cells, cleared
 // conflict rows; counts = the filter's device counts {rechecks, survivors}): conflict bits
@@ -311,6 +314,9 @@ struct Pack {
     int64_t* d_list3 = nullptr;
     int64_t* d_list4 = nullptr;
     int64_t* d_listb = nullptr;     // survivors of a filter run ahead (alternates with d_list)
+    uint32_t* d_skip = nullptr;     // per candidate of a prefilter launch: block-0 columns proven clear
+    uint32_t* d_skip3 = nullptr;    // the same for the stage-1 survivors (d_list3)
+    uint32_t* d_skip4 = nullptr;    // and for the stage-2 survivors (d_list4)
     int* d_spec_count = nullptr;    // that filter's candidate count, set on the device
     hipEvent_t ev_report = nullptr;  // after a resolve chunk's report (timing disabled)
     int recheck_slot = 0;  // d_count slot holding the last filter's FP64 recheck count
@@ -398,7 +404,7 @@ int grid_for(int64_t items, int per_block);
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
                       double* out, bool full, unsigned long long* ctr = nullptr,
                       const int* dcount = nullptr, int head = -1, int* cells = nullptr,
-                      uint64_t* mask = nullptr);
+                      uint64_t* mask = nullptr, const uint32_t* skip = nullptr);
 // Profiled timing of one launch: timed_begin arms the next UG_LAUNCH on h's stream, which
 // then records an event pair directly before and after the dispatch (timing-only events
 // without the system-scope fence, never waited on inside a pack); timed_end books the pair
@@ -465,9 +471,11 @@ int32_t fast_build(ModelTables& t, const std::vector<double>& S, const std::vect
 int32_t fast_setup(Pack* h);
 // Split-BF16 certain-contact test of fresh candidates base + i (list == nullptr) or of
 // list[i], i < count (bounded by *dcount when given), over the N blocks [cb0, cb1) of 32
-// ordered timesteps: flags[i] = 0 when candidate i certainly collides.
+// ordered timesteps: flags[i] = 0 when candidate i certainly collides.  With skip (first
+// stage, cb0 = 0, sequential blocks), skip[i] of each candidate left to the FP64 test = the
+// block-0 columns (bit c: scan position c) proven to hold no FP64 contact.
 int32_t launch_fast(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
-                    const int* dcount = nullptr, int cb0 = 0, int cb1 = 1);
+                    const int* dcount = nullptr, int cb0 = 0, int cb1 = 1, uint32_t* skip = nullptr);
 int fast_blocks(const Pack* h);  // N blocks of the model's ordered timesteps (0: no split form)
 // Subpath-phase prefilter: flags[i] = 0 when candidate base + i certainly touches two groups
 // among the rows < nlimit.
@@ -500,7 +508,8 @@ int32_t launch_compact_multi(hipStream_t st, const MCompact* dA, int U, int64_t 
 int32_t launch_resolve_multi(hipStream_t st, int d, const MResolve* dA, int U, int maxS, int maxT,
                              int64_t max_pairs, int max_tiles, bool any_plain, bool any_tiled);
 size_t fast_multi_arg_bytes();
-int32_t fast_multi_arg(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags, void* out);
+int32_t fast_multi_arg(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags, uint32_t* skip,
+                       void* out);
 int32_t launch_fast_multi(hipStream_t st, int d, const void* dA, int U, int64_t tiles);
 int slot_pairs_per_lane(int npairs);
 // Pieces of the single-universe loop (ug_api.hip) that the batched engine shares.

@@ -430,10 +430,14 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
 // evaluation's extra registers would lower the occupancy of launches that do not use it.
 // One candidate (item i of the launch, candidate index idx) of k_cand64; the wave is the
 // workgroup.  Shared by the single-universe kernel and the batched engine's launches.
+// known >= 0: the candidate's scan positions < min(T, 32) were examined by the split-BF16
+// prefilter and bit c of known marks position c proven to hold no contact; only the others
+// (usually a handful) are evaluated across the wave before the bulk scan starts at 32.
 template <int D, bool FULL, int PL, bool SB>
 __device__ __forceinline__ void cand64_item(const DevModel& m, int64_t i, uint64_t idx, uint8_t* flags,
                                             double* out, int head, int* cells, uint64_t* mask,
-                                            unsigned long long& nvis, unsigned long long& npts) {
+                                            unsigned long long& nvis, unsigned long long& npts,
+                                            int64_t known = -1) {
     extern __shared__ __align__(16) int dyn_lds[];
     const int lane = threadIdx.x;
     const int T = m.T, nw = m.nw;
@@ -453,7 +457,34 @@ __device__ __forceinline__ void cand64_item(const DevModel& m, int64_t i, uint64
         gen_wave64<D, PL>(m, idx, lane, w, sb, used);
         bool hit = false;
         int o0 = 0;
-        if (!FULL) {
+        const int kn = min(T, 32);
+        uint32_t rest = known >= 0 ? ~(uint32_t)known & (kn >= 32 ? ~0u : ((1u << kn) - 1u)) : 0u;
+        if (!FULL && known >= 0 && __popc(rest) <= head) {
+            // positions the prefilter could not clear, PER_ROUND per lookup round, then the
+            // bulk scan from position kn
+            while (!hit && rest) {
+                int G = 0;
+                while (rest && G < PER_ROUND) {
+                    const int pidx = __builtin_ctz(rest);
+                    rest &= rest - 1u;
+                    double X[D];
+                    const int t = m.order[pidx];
+                    eval_wave64<D, PL>(m, w, t, X);
+                    if (lane == G) {
+                        tsl[G] = t;
+#pragma unroll
+                        for (int a = 0; a < D; a++)
+                            pos[G * D + a] = X[a];
+                    }
+                    G++;
+                }
+                __syncthreads();
+                hit = lookup_round<D>(m, pos, tsl, G, lane, npts);
+                nvis += (unsigned long long)G;
+                __syncthreads();
+            }
+            o0 = kn;
+        } else if (!FULL) {
             // head: chunks of 2, 4, 8 timesteps evaluated across the wave, one lookup round
             // each (the chunk stops growing at PER_ROUND: a head of T = 256 in 3+1 takes 37
             // rounds, and an ever-doubling int would overflow)
@@ -553,13 +584,14 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
                                                const int64_t* list, const int* dcount,
                                                uint8_t* flags, double* out, int head,
                                                unsigned long long* ctr, int* cells,
-                                               uint64_t* mask) {
+                                               uint64_t* mask, const uint32_t* skip) {
     if (dcount)
         count = min(count, (int64_t)*dcount);
     unsigned long long nvis = 0, npts = 0;
     for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
         const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
-        cand64_item<D, FULL, PL, SB>(m, i, idx, flags, out, head, cells, mask, nvis, npts);
+        cand64_item<D, FULL, PL, SB>(m, i, idx, flags, out, head, cells, mask, nvis, npts,
+                                     skip ? (int64_t)skip[i] : -1);
     }
     if (!FULL && ctr) {
         // exact counts for the algorithmic-bytes figure
@@ -592,8 +624,9 @@ __global__ __launch_bounds__(64) void k_cand64_multi(const MCand* __restrict__ A
         const DevModel m = load_model(&r->m);
         const int64_t* list = ug_gptr(&r->list);
         const uint64_t idx = list ? (uint64_t)list[i] : r->base + (uint64_t)i;
+        const uint32_t* skip = ug_gptr(&r->skip);
         cand64_item<D, FULL, PL, SB>(m, i, idx, ug_gptr(&r->flags), ug_gptr(&r->out), r->head, ug_gptr(&r->cells),
-                                     ug_gptr(&r->mask), nvis, npts);
+                                     ug_gptr(&r->mask), nvis, npts, skip ? (int64_t)skip[i] : -1);
     }
 }
 
@@ -1181,8 +1214,8 @@ __global__ __launch_bounds__(256) void k_compact_scatter_multi(const MCompact* A
     const int nb = (int)max((int64_t)1, (count + CMP_ITEMS - 1) / CMP_ITEMS);
     if ((int)blockIdx.x < nb)
         compact_scatter_block(ug_gptr(&r->flags), count, ug_gptr(&r->dcount), r->base, ug_gptr(&r->list),
-                              ug_gptr(&r->bcount), nb, ug_gptr(&r->out), ug_gptr(&r->total), nullptr, nullptr,
-                              blockIdx.x);
+                              ug_gptr(&r->bcount), nb, ug_gptr(&r->out), ug_gptr(&r->total), ug_gptr(&r->vin),
+                              ug_gptr(&r->vout), blockIdx.x);
 }
 
 // =====================================================================================
@@ -1699,7 +1732,7 @@ int grid_for(int64_t items, int per_block) {
 
 int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags,
                       double* out, bool full, unsigned long long* ctr, const int* dcount, int head,
-                      int* cells, uint64_t* mask) {
+                      int* cells, uint64_t* mask, const uint32_t* skip) {
     if (count <= 0)
         return UG_OK;
     if (head < 0)
@@ -1710,7 +1743,7 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
     const int pl = slot_pl(h->npairs);
 #define UG_CAND64(DD, FF, PP, SS)                                                                         \
     UG_LAUNCH(h, (k_cand64<DD, FF, PP, SS>), dim3(blocks), dim3(64), lds, h->dm, base, count, list, dcount, \
-              flags, out, head, ctr, cells, mask)
+              flags, out, head, ctr, cells, mask, skip)
     // shared basis loads where the lane-per-timestep evaluation does the work: rechecks and
     // retests of a list, and trajectories (a fresh first pass scans its head across the wave)
     const bool sb = pl <= 2 && h->full && (full || list);
@@ -2018,12 +2051,14 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
     // (count in d_count[0])
     rc = timed_begin(h, UG_KERNEL_PREFILTER);
     if (!rc)
-        rc = launch_fast(h, base, count, list, h->d_flags, dfresh);
+        rc = launch_fast(h, base, count, list, h->d_flags, dfresh, 0, 1, h->d_skip);
     if (!rc)
         rc = timed_end(h, UG_KERNEL_PREFILTER, count);
     if (!rc)
-        rc = compact(h, h->d_flags, count, base, list, h->d_list3, nullptr, dfresh, 0, false);
+        rc = compact(h, h->d_flags, count, base, list, h->d_list3, nullptr, dfresh, 0, false, (const int*)h->d_skip,
+                     (int*)h->d_skip3);
     const int64_t* rlist = h->d_list3;
+    const uint32_t* rskip = h->d_skip3;
     int rslot = 0;
     if (!rc && fast_blocks(h) >= 6) {
         // stage 2 (T > 160): the remaining ordered timesteps, split-BF16 as well, for the few
@@ -2037,15 +2072,18 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
         if (!rc)
             rc = timed_end(h, UG_KERNEL_PREFILTER, 0);  // items: stage-1 survivors, on the device
         if (!rc)
-            rc = compact(h, h->d_flags, count, 0, h->d_list3, h->d_list4, nullptr, h->d_count, 2, false);
+            rc = compact(h, h->d_flags, count, 0, h->d_list3, h->d_list4, nullptr, h->d_count, 2, false,
+                         (const int*)h->d_skip3, (int*)h->d_skip4);
         rlist = h->d_list4;
+        rskip = h->d_skip4;
         rslot = 2;
     }
     h->recheck_slot = rslot;
     if (!rc)
         rc = timed_begin(h, UG_KERNEL_RECHECK);
     if (!rc)
-        rc = launch_cand64(h, 0, count, rlist, h->d_flags, nullptr, false, rctr, h->d_count + rslot, -1);
+        rc = launch_cand64(h, 0, count, rlist, h->d_flags, nullptr, false, rctr, h->d_count + rslot, -1, nullptr,
+                           nullptr, rskip);
     if (!rc)
         rc = timed_end(h, UG_KERNEL_RECHECK, 0);  // items: counted on the device
     if (!rc)
