@@ -107,6 +107,24 @@ def test_sparse_config3_to_jam_at_full_size(gpu):
     assert np.array_equal(lists[True][0], lists[False][0]) and lists[True][1] == lists[False][1]
 
 
+def test_sparse_config5_to_jam_equals_dense(gpu):
+    # 3+1 T=256 to jamming: the sparse grid's table is rebuilt as it grows past 65,536 paths x
+    # 256 timesteps (16.7M entries, more than one entry per thread of a capped grid); the list
+    # must stay the dense table's (whose whole range test_gpu_pack checks)
+    import ugpack
+
+    out = {}
+    for sparse in (True, False):
+        print(f"cfg5 jam sparse={sparse}", flush=True)
+        with ugpack.Packer(3, 256, seed=1, precision=ugpack.PREC_SPLIT_BF16, sparse=sparse,
+                           stop_kind=ugpack.STOP_FIXED_WINDOW, stop_rate=1e-4) as p:
+            st = p.run()
+            out[sparse] = (st.n, st.attempts, st.filled, p.accepted()[0])
+    assert out[True][0] > 65536
+    assert out[True][:3] == out[False][:3]
+    assert np.array_equal(out[True][3], out[False][3])
+
+
 def test_braid_ug3d_sparse_flag(gpu, tmp_path):
     import ugpack
 

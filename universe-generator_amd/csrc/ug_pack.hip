@@ -1509,25 +1509,27 @@ __global__ void k_hash_clear(ulonglong2* slots, uint64_t n) {
 // Rebuild of the sparse grid into a larger (cleared) table: every stored (path, timestep)
 // is inserted again from its canonical position and its chain link rewritten.
 template <int D>
+// Grid-stride: a table of 3+1 T=256 outgrows one item per thread of a capped grid (65,535
+// blocks of 256 cover 16.7M entries, about 65k paths).
 __global__ __launch_bounds__(256) void k_hash_rebuild(DevModel m, int64_t rows, float4* pos32) {
-    const int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (item >= rows * m.T)
-        return;
-    const int64_t n = item / m.T;
-    const int t = (int)(item % m.T);
-    const size_t q = (size_t)n * m.T + t;
-    int c[D];
+    for (int64_t item = (int64_t)blockIdx.x * 256 + threadIdx.x; item < rows * m.T;
+         item += (int64_t)gridDim.x * 256) {
+        const int64_t n = item / m.T;
+        const int t = (int)(item % m.T);
+        const size_t q = (size_t)n * m.T + t;
+        int c[D];
 #pragma unroll
-    for (int a = 0; a < D; a++)
-        c[a] = cell_of(m.pos64[q * D + a], m.halfT, m.T);
-    int64_t rid = (int64_t)t * m.T + c[0];
-    if (D == 3)
-        rid = rid * m.T + c[1];
-    bool fresh = false;
-    const int prev = hash_insert(m, cell_hkey(rid, c[D - 1], m.T), (int)n, &fresh);
-    float4 r = pos32[q];
-    r.w = __int_as_float(prev);
-    pos32[q] = r;
+        for (int a = 0; a < D; a++)
+            c[a] = cell_of(m.pos64[q * D + a], m.halfT, m.T);
+        int64_t rid = (int64_t)t * m.T + c[0];
+        if (D == 3)
+            rid = rid * m.T + c[1];
+        bool fresh = false;
+        const int prev = hash_insert(m, cell_hkey(rid, c[D - 1], m.T), (int)n, &fresh);
+        float4 r = pos32[q];
+        r.w = __int_as_float(prev);
+        pos32[q] = r;
+    }
 }
 
 // Insert the chunk's accepted survivors into the cell table, one thread per (path, timestep):
