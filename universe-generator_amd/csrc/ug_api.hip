@@ -401,6 +401,64 @@ void ug::pack_fill_status(const Pack* h, ug_pack_status* st) { fill_status(h, st
 bool ug::pack_fixed_window_stop(const Pack* h) { return fixed_window_stop(h); }
 void ug::pack_record_curve(Pack* h) { (void)record_curve(h); }
 
+static int64_t sub_cap_of(const ug_pack_config& cfg) {
+    return std::min<int64_t>(cfg.sub_max, cfg.stop_kind == UG_STOP_FIXED_WINDOW ? cfg.stop_batch : UG_SUB_MAX);
+}
+
+// h->sub, cut at the budget, the call's end and (fixed window) the next stop-rule boundary;
+// when the rest up to that boundary is within the size the last sizing aimed at, the rest
+// whole (one sub-batch instead of a power of two and its remainder).  0 when none may start.
+int64_t ug::pack_size_at(const Pack* h, int64_t from, int64_t call_end) {
+    const ug_pack_config& cfg = h->cfg;
+    const int64_t budget = (int64_t)std::min(cfg.max_attempts, 9.2e18);
+    if (from >= budget || from >= call_end)
+        return 0;
+    const int64_t cap = sub_cap_of(cfg);
+    int64_t sub = std::min(h->sub, cap);
+    if (cfg.stop_kind == UG_STOP_FIXED_WINDOW) {
+        const int64_t rest = (from / cfg.stop_batch + 1) * cfg.stop_batch - from;
+        sub = rest <= std::min<double>(h->sub_want, (double)cap) ? rest : std::min(sub, rest);
+    }
+    sub = std::min(sub, budget - from);
+    return std::min(sub, call_end - from);
+}
+
+void ug::pack_note_rate(Pack* h, int64_t at, int64_t count, int64_t S) {
+    if (at > 0 && count > 0) {
+        h->rate_at = at;
+        h->rate_p = (double)S / (double)count;
+    }
+}
+
+// Aims at ~sub_target survivors (1024 by default: one in-batch resolve chunk) and at most
+// ~16k candidates for the FP64 recheck per sub-batch (the reference adapts toward ~256
+// survivors, cuda/braid_cuda.cu:920-924).  The survival rate of the next sub-batch is the last
+// observed one scaled by (observed at) / (next start): while the table fills, the rate of a
+// random sequential packing falls about as the inverse of the candidates drawn (2+1 T=64
+// from 1024 to 2^20 candidates: the rate times the candidate index stays within 16..53, while
+// the rate falls 1700-fold), so sizing by the last rate alone lags by the growth in between.
+// Every survivor is rechecked in FP64 (split form), so the recheck rate is the survival rate
+// plus that of the candidates the prefilter could not decide but which collide; only the
+// survival part is scaled.
+void ug::pack_size_next(Pack* h, int64_t sub, int64_t S, int64_t rech) {
+    const ug_pack_config& cfg = h->cfg;
+    const int64_t cap = sub_cap_of(cfg);
+    const double L = (double)std::max<int64_t>(1, sub);
+    double p = (double)std::max<int64_t>(1, S) / L;
+    if (h->rate_at > 0 && h->attempts > 0)
+        p = std::max(h->rate_p * (double)h->rate_at / (double)std::max(h->rate_at, h->attempts), 1e-12);
+    const double gray = rech > 0 ? (double)std::max<int64_t>(0, rech - S) / L : 0.0;
+    double want = (double)cfg.sub_target / p;
+    if (rech > 0)
+        want = std::min(want, 16384.0 / std::max(p + gray, 1.0 / L));
+    want = std::min(want, 64.0 * L);
+    int64_t next = 4096;
+    while (next * 2 <= (int64_t)want && next * 2 <= cap)
+        next *= 2;
+    h->sub = next;
+    h->sub_want = want;
+}
+
 // The smallest accepted count of the next sub-batch (sub candidates) for which the run goes
 // on: the loop stops after it iff its uniques number fewer (0: it cannot stop there).  The
 // fixed window counts accepts with index >= end - window, all of the sub-batch's among them.
@@ -575,22 +633,9 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         fill_status(h, st);
         return rc;
     }
-    const int64_t sub_cap = std::min<int64_t>(cfg.sub_max, cfg.stop_kind == UG_STOP_FIXED_WINDOW ? cfg.stop_batch : UG_SUB_MAX);
+    const int64_t sub_cap = sub_cap_of(cfg);
     int32_t rc = UG_OK;
-    // The size of the sub-batch starting at `from`: h->sub, cut at the budget, the call's end
-    // and (fixed window) the next stop-rule boundary; 0 when none may start there.
-    auto size_at = [&](int64_t from) -> int64_t {
-        if (from >= budget || from >= call_end)
-            return 0;
-        int64_t sub = std::min(h->sub, sub_cap);
-        sub = std::min(sub, budget - from);
-        sub = std::min(sub, call_end - from);
-        if (cfg.stop_kind == UG_STOP_FIXED_WINDOW) {
-            const int64_t nb = (from / cfg.stop_batch + 1) * cfg.stop_batch;
-            sub = std::min(sub, nb - from);
-        }
-        return sub;
-    };
+    auto size_at = [&](int64_t from) -> int64_t { return pack_size_at(h, from, call_end); };
     // Sub-batches are pipelined one deep: while the host reads sub-batch k's results, the
     // device already filters sub-batch k+1 against the table as it stands after k's first
     // resolve chunk (all of k's inserts when that chunk was the only one; otherwise the
@@ -620,6 +665,8 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
             sub = size_at(h->attempts);
             cur_list = h->d_list;
             int64_t S0 = 0;
+            h->sub_base = h->attempts;
+            h->sub_from_empty = h->n + h->nsub == 0;
             // Survivors stay on the device: the resolve reads their count there, and it comes
             // back with the resolve's results (one host round trip per sub-batch).  An empty
             // table rejects nothing, so the first sub-batch goes straight to the resolve.
@@ -633,7 +680,12 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         // Only in the steady state (full-size sub-batches, few survivors): earlier, sizes still
         // grow from the measured rates and a sub-batch often needs several resolve chunks,
         // which would void the run ahead.
-        const bool steady = h->sub >= sub_cap && h->last_S <= UG_S_MAX / 4;
+        // (survivors of this sub-batch expected from the last survival-rate observation, else
+        // the last sub-batch's count)
+        double expect_S = (double)h->last_S;
+        if (h->rate_at > 0)
+            expect_S = h->rate_p * (double)h->rate_at / (double)std::max<int64_t>(h->rate_at, (int64_t)base) * (double)sub;
+        const bool steady = h->sub >= sub_cap && expect_S <= UG_S_MAX / 4;
         ahead.count = steady ? size_at(h->attempts + sub) : 0;
         ahead.out = cur_list == h->d_list ? h->d_listb : h->d_list;
         ahead.dcount = h->d_spec_count;
@@ -653,21 +705,12 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         } else {
             h->attempts += sub;
         }
-        // Sub-batch size from the measured rates, aiming at ~sub_target survivors (1024 by
-        // default: one in-batch resolve chunk) and at most ~16k candidates for the FP64 recheck
-        // per sub-batch (the
-        // reference adapts toward ~256 survivors, cuda/braid_cuda.cu:920-924).  Results do
-        // not depend on it.
-        {
-            const double surv = (double)std::max<int64_t>(1, S);
-            const double rech = (double)std::max<int64_t>(1, h->rechecked - rechecked_before);
-            double want = std::min((double)cfg.sub_target * (double)sub / surv, 16384.0 * (double)sub / rech);
-            want = std::min(want, 8.0 * (double)sub);
-            int64_t next = 4096;
-            while (next * 2 <= (int64_t)want && next * 2 <= sub_cap)
-                next *= 2;
-            h->sub = next;
-        }
+        // next sub-batch size (results do not depend on it); a sub-batch filtered against an
+        // empty table observes nothing (its retest after the first chunk does)
+        if (!h->sub_from_empty)
+            pack_note_rate(h, (int64_t)base, sub, S);
+        h->sub_from_empty = false;
+        pack_size_next(h, sub, S, h->rechecked - rechecked_before);
         rc = record_curve(h);
     }
     if (rc == UG_OK)

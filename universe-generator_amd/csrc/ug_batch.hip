@@ -105,43 +105,14 @@ struct Uni {
 
 int64_t budget_of(const Pack* h) { return (int64_t)std::min(h->cfg.max_attempts, 9.2e18); }
 
-// Size of the sub-batch that starts at h->attempts (ug_pack_run's rule): h->sub, cut at the
-// budget and (fixed window) at the next stop-rule boundary.
-int64_t size_at(const Pack* h) {
-    const ug_pack_config& cfg = h->cfg;
-    const int64_t sub_cap =
-        std::min<int64_t>(cfg.sub_max, cfg.stop_kind == UG_STOP_FIXED_WINDOW ? cfg.stop_batch : UG_SUB_MAX);
-    const int64_t from = h->attempts;
-    int64_t sub = std::min(h->sub, sub_cap);
-    sub = std::min(sub, budget_of(h) - from);
-    if (cfg.stop_kind == UG_STOP_FIXED_WINDOW) {
-        const int64_t nb = (from / cfg.stop_batch + 1) * cfg.stop_batch;
-        sub = std::min(sub, nb - from);
-    }
-    return sub;
-}
+// Size of the sub-batch that starts at h->attempts (ug_pack_run's rule).
+int64_t size_at(const Pack* h) { return pack_size_at(h, h->attempts, INT64_MAX); }
 
 // The run has ended before the next sub-batch (stop rule, budget or max_n).
 bool finished(Pack* h) {
     if (h->cfg.stop_kind == UG_STOP_FIXED_WINDOW && pack_fixed_window_stop(h))
         return true;
     return h->attempts >= budget_of(h) || (h->max_n > 0 && h->n >= h->max_n);
-}
-
-// Next sub-batch size from the finished one's rates (ug_pack_run's rule; results do not
-// depend on it).
-void size_next(Pack* h, const Uni& u) {
-    const ug_pack_config& cfg = h->cfg;
-    const int64_t sub_cap =
-        std::min<int64_t>(cfg.sub_max, cfg.stop_kind == UG_STOP_FIXED_WINDOW ? cfg.stop_batch : UG_SUB_MAX);
-    const double surv = (double)std::max<int64_t>(1, u.S_first);
-    const double rech = (double)std::max<int64_t>(1, u.rech_sub);
-    double want = std::min((double)cfg.sub_target * (double)u.sub / surv, 16384.0 * (double)u.sub / rech);
-    want = std::min(want, 8.0 * (double)u.sub);
-    int64_t next = 4096;
-    while (next * 2 <= (int64_t)want && next * 2 <= sub_cap)
-        next *= 2;
-    h->sub = next;
 }
 
 // A config the batched kernels cover: dense cell table, full spectrum, T <= 160 (one
@@ -447,6 +418,11 @@ int32_t Engine::step(std::vector<Uni*>& act) {
         if (u->fresh) {
             u->S_first = u->S;
             h->survivors += u->S;
+        } else if (h->sub_from_empty) {
+            // the sub-batch met an empty table, so its list was the whole range: this retest
+            // of its last npend candidates is the table's first survival-rate observation
+            pack_note_rate(h, h->attempts + u->sub - u->npend, u->npend, u->S);
+            h->sub_from_empty = false;
         }
         h->filled = *(const unsigned long long*)(rep + 2);
         const int64_t* ai = (const int64_t*)(rep + 6);
@@ -513,8 +489,11 @@ bool end_sub(Uni* u) {
         h->attempts = h->acc_host[(size_t)h->max_n - 1] + 1;
         return true;
     }
+    if (!h->sub_from_empty)
+        pack_note_rate(h, h->attempts, u->sub, u->S_first);
+    h->sub_from_empty = false;
     h->attempts += u->sub;
-    size_next(h, *u);
+    pack_size_next(h, u->sub, u->S_first, u->rech_sub);
     pack_record_curve(h);
     return false;
 }
@@ -555,6 +534,7 @@ int32_t run_lane(Shared& S, Lane& L) {
                 u->sub = size_at(u->h);
                 u->S_first = 0;
                 u->rech_sub = 0;
+                u->h->sub_from_empty = u->h->n == 0;
             }
             live.push_back(u);
         }

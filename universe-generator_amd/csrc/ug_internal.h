@@ -350,6 +350,14 @@ struct Pack {
     std::vector<std::array<int64_t, 4>> curve;
     int64_t next_ms = 1000;
     int64_t attempts = 0, max_n = 0, sub = 4096;
+    // sub-batch sizing: the last survival-rate observation (rate_p survivors per candidate
+    // filtered against the table as it stood at candidate rate_at; rate_at 0 = none), the
+    // size the last sizing aimed at, and the current sub-batch's start and whether the table
+    // was empty there
+    int64_t rate_at = 0;
+    double rate_p = 0.0, sub_want = 0.0;
+    int64_t sub_base = 0;
+    bool sub_from_empty = false;
     bool done = false;
     int64_t survivors = 0, rechecked = 0;
     double seconds = 0.0;
@@ -516,6 +524,13 @@ int slot_pairs_per_lane(int npairs);
 void pack_fill_status(const Pack* h, ug_pack_status* st);
 bool pack_fixed_window_stop(const Pack* h);  // the fixed-window rule fires at h->attempts
 void pack_record_curve(Pack* h);             // log-spaced curve rows up to h->attempts
+// Sub-batch sizing shared by ug_pack_run and the batched engine (results do not depend on it):
+// the size of the sub-batch starting at `from` (call_end: the call's last candidate), a
+// survival-rate observation, and the next size after a sub-batch of `sub` candidates with S
+// survivors of its fresh filter and `rech` FP64 rechecks.
+int64_t pack_size_at(const Pack* h, int64_t from, int64_t call_end);
+void pack_note_rate(Pack* h, int64_t at, int64_t count, int64_t S);
+void pack_size_next(Pack* h, int64_t sub, int64_t S, int64_t rech);
 uint64_t list_digest(const std::vector<int64_t>& idx);  // FNV-1a of an accepted list (ug_ensemble.hip)
 // Handle set-up shared by the packer and the knot engine: with_table = false builds no cell
 // table, occupancy bits or path records (the knot engine keeps its own strands).

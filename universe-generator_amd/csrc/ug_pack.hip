@@ -2293,6 +2293,12 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
             rc = filter_candidates(h, 0, S - s, list + s, h->d_list2, &S2);
             if (rc)
                 return rc;
+            if (h->sub_from_empty) {
+                // the sub-batch met an empty table, so its list is the whole range: the retest
+                // of candidates sub_base + s .. is the table's first survival-rate observation
+                pack_note_rate(h, h->sub_base + s, S - s, S2);
+                h->sub_from_empty = false;
+            }
             rc = hip_check(hipMemcpyAsync(list, h->d_list2, (size_t)S2 * 8, hipMemcpyDeviceToDevice, h->stream),
                            "list copy");
             if (rc)
