@@ -55,9 +55,23 @@ __device__ __forceinline__ float expo2_f32(uint32_t w) {
     return -__log2f(v);
 }
 
-__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
-    hi = (__bf16)x;
-    lo = (__bf16)(x - (float)hi);
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32v4;
+
+// Two values split into bf16 hi (round to nearest even) and the bf16 of the exact FP32
+// remainder, packed as (hi of x, hi of y) and (lo of x, lo of y): one packed conversion per
+// word, the hi halves widened back by shifts.
+__device__ __forceinline__ void split_bf16x2(float x, float y, uint32_t& hi, uint32_t& lo) {
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){x, y}, bf16x2));
+    const float xl = x - __uint_as_float(hi << 16), yl = y - __uint_as_float(hi & 0xffff0000u);
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){xl, yl}, bf16x2));
+}
+// The same for (x, 0).
+__device__ __forceinline__ void split_bf16x1(float x, uint32_t& hi, uint32_t& lo) {
+    hi = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){x, 0.f}, bf16x2));
+    const float xl = x - __uint_as_float(hi << 16);
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){xl, 0.f}, bf16x2));
 }
 
 // A fragments of one K step (pairs p0 = 4q + 2h, p0 + 1) for axis a, candidate idx.  The
@@ -68,6 +82,7 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
 __device__ __forceinline__ void gen_kstep(const DevModel& m, uint64_t idx, bool valid, int a,
                                           int q, int h, bf16x8& Ahi, bf16x8& Alo, float& Wpart) {
     const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
+    uint32_t hw[4], lw[4];
 #pragma unroll
     for (int pp = 0; pp < 2; pp++) {
         const int p = 4 * q + 2 * h + pp;
@@ -76,27 +91,20 @@ __device__ __forceinline__ void gen_kstep(const DevModel& m, uint64_t idx, bool 
         // slot 2p exists iff p < npairs, slot 2p + 1 iff 2p + 1 < nw
         const float e0 = valid && p < m.npairs ? expo2_f32(r.x) : 0.f;
         const float e1 = valid && j0 + 1 < m.nw ? expo2_f32(r.y) : 0.f;
-        // bit 0 of the Dirichlet word is the slot's sign
-        const float ae = __uint_as_float(__float_as_uint(e0) ^ (r.x << 31));
+        // bit 0 of the Dirichlet word is the slot's sign (e0, e1 >= 0: OR sets it)
+        const float ae = __uint_as_float(__float_as_uint(e0) | (r.x << 31));
         float cf, sf;
         phase_cs(r.z, cf, sf);
         const float v0 = ae * cf, v1 = ae * sf;
-        const float v2 = __uint_as_float(__float_as_uint(e1) ^ (r.y << 31));
+        const float v2 = __uint_as_float(__float_as_uint(e1) | (r.y << 31));
         Wpart += e0;
         Wpart += e1;
-        __bf16 h0, l0, h1, l1, h2, l2;
-        split_bf16(v0, h0, l0);
-        split_bf16(v1, h1, l1);
-        split_bf16(v2, h2, l2);
-        Ahi[4 * pp + 0] = h0;
-        Ahi[4 * pp + 1] = h1;
-        Ahi[4 * pp + 2] = h2;
-        Ahi[4 * pp + 3] = (__bf16)0.f;
-        Alo[4 * pp + 0] = l0;
-        Alo[4 * pp + 1] = l1;
-        Alo[4 * pp + 2] = l2;
-        Alo[4 * pp + 3] = (__bf16)0.f;
+        // slots (v0, v1) and (v2, 0) of the fragment, hi and lo words
+        split_bf16x2(v0, v1, hw[2 * pp], lw[2 * pp]);
+        split_bf16x1(v2, hw[2 * pp + 1], lw[2 * pp + 1]);
     }
+    Ahi = __builtin_bit_cast(bf16x8, (u32v4){hw[0], hw[1], hw[2], hw[3]});
+    Alo = __builtin_bit_cast(bf16x8, (u32v4){lw[0], lw[1], lw[2], lw[3]});
 }
 
 struct FastArgs {
