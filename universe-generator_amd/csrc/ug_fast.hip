@@ -167,14 +167,25 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // D layout of 32x32 MFMA: lane holds column n = lane & 31, rows 8(v/4) + 4h + (v%4)
         // per-lane offsets kept opaque to the compiler here, so that the 16 stores use one
-        // base register with immediate offsets instead of 16 hoisted addresses
+        // base register with immediate offsets instead of 16 hoisted addresses.  The rows'
+        // 1/W and a1 are read first, four consecutive rows per LDS read.
         int poff = (a * TILE + 4 * h) * NCOL + mrow, roff = 4 * h;
         asm volatile("" : "+v"(poff), "+v"(roff));
+        float4 rw[4], r1[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            rw[j] = *(const float4*)&sinvw[roff + 8 * j];
+            r1[j] = *(const float4*)&sa1[roff + 8 * j];
+        }
 #pragma unroll
         for (int v = 0; v < 16; v++) {
             const int ro = 8 * (v >> 2) + (v & 3);
-            float X = fmaf(acc[a][v], sinvw[roff + ro], sa1[roff + ro]);
-            X = X - 2.0f * floorf((X + 1.0f) * 0.5f);
+            const float w = v % 4 == 0 ? rw[v >> 2].x : v % 4 == 1 ? rw[v >> 2].y : v % 4 == 2 ? rw[v >> 2].z : rw[v >> 2].w;
+            const float c = v % 4 == 0 ? r1[v >> 2].x : v % 4 == 1 ? r1[v >> 2].y : v % 4 == 2 ? r1[v >> 2].z : r1[v >> 2].w;
+            float X = fmaf(acc[a][v], w, c);
+            // X - 2 floor((X + 1) / 2), each fma rounding exactly as the separate operations
+            // (halving and doubling are exact)
+            X = fmaf(-2.0f, floorf(fmaf(X, 0.5f, 0.5f)), X);
             pos[poff + ro * NCOL] = X;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
