@@ -246,7 +246,10 @@ void or_candidate(const or_model* m, uint64_t idx, double* a1, int* b, double* c
                 ba[j] = bb;
             }
         }
-        /* Per slot pair p: Dirichlet words (bit 0 = sign), phase words. */
+        /* Per slot pair p: Dirichlet words (bit 0 = sign), phase words.  Partial spectra: block
+         * p = {Dirichlet(2p), Dirichlet(2p+1), phase(2p), phase(2p+1)}.  Full pool (only even
+         * slots have a phase): the axis' blocks are one word stream (word i = component i % 4
+         * of block i / 4) and pair p takes words 3p, 3p+1, 3p+2. */
         double e[4096];
         uint32_t neg[4096], ph[4096];
         double Q[64];
@@ -254,7 +257,19 @@ void or_candidate(const or_model* m, uint64_t idx, double* a1, int* b, double* c
             Q[i] = 0.0;
         for (int p = 0; p < npairs; p++) {
             uint32_t w[4];
-            philox_at(m, idx, (uint32_t)p, (1u << 8) | (uint32_t)a, w);
+            if (nw == m->modmax) {
+                uint32_t b0[4], b1[4];
+                const uint32_t i0 = 3u * (uint32_t)p;
+                philox_at(m, idx, i0 >> 2, (1u << 8) | (uint32_t)a, b0);
+                philox_at(m, idx, (i0 >> 2) + 1u, (1u << 8) | (uint32_t)a, b1);
+                for (int k = 0; k < 3; k++) {
+                    const uint32_t wi = (i0 & 3u) + (uint32_t)k;
+                    w[k] = wi < 4 ? b0[wi] : b1[wi - 4];
+                }
+                w[3] = 0u;
+            } else {
+                philox_at(m, idx, (uint32_t)p, (1u << 8) | (uint32_t)a, w);
+            }
             for (int q = 0; q < 2; q++) {
                 int j = 2 * p + q;
                 if (j >= nw)

@@ -219,6 +219,49 @@ def test_candidate_invariants_partial_and_pin():
     assert np.all(pinned.candidate(3).a1 == 0.0)
 
 
+def _words(seed, idx, a, n):
+    """The first n stage-1 words of axis a: components of consecutive Philox blocks."""
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    out = []
+    for k in range((n + 3) // 4):
+        out.extend(int(x) for x in po.philox([idx & 0xFFFFFFFF, idx >> 32, k, (1 << 8) | a], key))
+    return out[:n]
+
+
+def test_slot_word_layout():
+    # full pool: pair p reads words 3p, 3p+1 (Dirichlet words of slots 2p, 2p+1) and 3p+2 (the
+    # even slot's phase) of the axis' stream; partial spectra: block p whole, phases in z, w.
+    # A layout slip moves the spacing shares and phases, checked here against the words.
+    def shares(words):
+        e = np.array([-np.log((0x80000000 - (w >> 1)) * 2.0**-31) for w in words])
+        return e / e.sum(), np.array([w & 1 for w in words])
+    for d, T, seed, idx in ((2, 16, 5, 777), (3, 12, 2, 31), (2, 64, 1, 123456)):
+        m = po.Model(d, T, seed=seed)
+        cd = m.candidate(idx)
+        nw = m.nw
+        assert nw == T - 1
+        npairs = (nw + 1) // 2
+        for a in range(d):
+            w = _words(seed, idx, a, 3 * npairs)
+            dw = [w[3 * (j // 2) + (j % 2)] for j in range(nw)]
+            sh, neg = shares(dw)
+            assert np.allclose(np.abs(cd.amp[a]) * cd.b[a], sh, rtol=1e-12, atol=0)
+            assert np.array_equal(cd.amp[a] < 0, neg == 1)
+            for j in range(0, nw, 2):
+                assert cd.phase[a][j] == w[3 * (j // 2) + 2] * 2.0**-32 * np.pi
+    m = po.Model(2, 24, terms=5, seed=3)
+    cd = m.candidate(99)
+    nw = m.nw
+    for a in range(2):
+        w = _words(3, 99, a, 4 * ((nw + 1) // 2))
+        dw = [w[4 * (j // 2) + (j % 2)] for j in range(nw)]
+        sh, neg = shares(dw)
+        assert np.allclose(np.abs(cd.amp[a]) * cd.b[a], sh, rtol=1e-12, atol=0)
+        for j in range(nw):
+            if cd.b[a][j] % 2 == 0:
+                assert cd.phase[a][j] == w[4 * (j // 2) + 2 + (j % 2)] * 2.0**-32 * np.pi
+
+
 def test_candidate_distributions():
     d, T, n = 2, 32, 3000
     m = po.Model(d, T, seed=21)

@@ -66,7 +66,8 @@ void ug::gen_params_host(const Pack* h, uint64_t idx, double* out_row) {
         for (int i = 0; i < 64; i++)
             Q[i] = 0.0;
         for (int p = 0; p < npairs; p++) {
-            const u32x4 r = philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, k0, k1);
+            const u32x4 r = h->full ? pair_words_full(lo, hi, p, a, k0, k1)
+                                    : philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, k0, k1);
             const uint32_t dw[2] = {r.x, r.y}, pw[2] = {r.z, r.w};
             for (int k = 0; k < 2; k++) {
                 const int j = 2 * p + k;

@@ -23,9 +23,13 @@ namespace ug {
 
 // ----------------------------------------------------------------------------------
 // Philox4x32-10.  Counter words: (idx lo, idx hi, pair or draw block, stage<<8 | axis).
-// Stage 0: sin1 amplitudes (word a = axis a).  Stage 1: per slot pair p, words
-// {Dirichlet(2p), Dirichlet(2p+1), phase(2p), phase(2p+1)}; bit 0 of a Dirichlet word
-// is that slot's sign.  Stage 2: frequency draws for partial spectra, 4 per block.
+// Stage 0: sin1 amplitudes (word a = axis a).  Stage 1: slot words; bit 0 of a Dirichlet word
+// is that slot's sign.  Partial spectra: block p holds slot pair p's words {Dirichlet(2p),
+// Dirichlet(2p+1), phase(2p), phase(2p+1)}.  Full-pool spectra (slot j has frequency j + 2, so
+// only even slots have a phase): the axis' blocks form one word stream, word i = component
+// i % 4 of block i / 4, and pair p takes words {3p, 3p+1, 3p+2} = {Dirichlet(2p),
+// Dirichlet(2p+1), phase(2p)} (pair_words_full).  Stage 2: frequency draws for partial
+// spectra, 4 per block.
 struct u32x4 {
     uint32_t x, y, z, w;
 };
@@ -56,6 +60,22 @@ UG_HD u32x4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t 
         k1 += 0xBB67AE85u;
     }
     return u32x4{c0, c1, c2, c3};
+}
+
+// Stage-1 words {Dirichlet(2p), Dirichlet(2p+1), phase(2p), 0} of slot pair p of axis a in a
+// full-pool spectrum: words 3p .. 3p+2 of the axis' stream (one block, or the end of one and
+// the start of the next).
+UG_HD u32x4 pair_words_full(uint32_t lo, uint32_t hi, int p, int a, uint32_t k0, uint32_t k1) {
+    const uint32_t i = 3u * (uint32_t)p, b = i >> 2, o = i & 3u, c3 = (1u << 8) | (uint32_t)a;
+    const u32x4 r = philox(lo, hi, b, c3, k0, k1);
+    if (o == 0)
+        return u32x4{r.x, r.y, r.z, 0u};
+    if (o == 1)
+        return u32x4{r.y, r.z, r.w, 0u};
+    const u32x4 n = philox(lo, hi, b + 1, c3, k0, k1);
+    if (o == 2)
+        return u32x4{r.z, r.w, n.x, 0u};
+    return u32x4{r.w, n.x, n.y, 0u};
 }
 
 // ----------------------------------------------------------------------------------

@@ -74,19 +74,19 @@ __device__ __forceinline__ void split_bf16x1(float x, uint32_t& hi, uint32_t& lo
     lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){xl, 0.f}, bf16x2));
 }
 
-// A fragments of one K step (pairs p0 = 4q + 2h, p0 + 1) for axis a, candidate idx.  The
-// 1/k factors live in the basis rows (B = S_k/k, C_k/k, S_{k+1}/(k+1)), so a slot's value is
-// its signed exponential spacing, times cos f / sin f for even frequencies.  Missing slots
-// (past the last frequency) and padding candidates contribute zeros, selected without
-// branches; their basis rows are zero as well.
-__device__ __forceinline__ void gen_kstep(const DevModel& m, uint64_t idx, bool valid, int a,
-                                          int q, int h, bf16x8& Ahi, bf16x8& Alo, float& Wpart) {
-    const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
+// A fragments of one K step from the words of its two slot pairs p0, p0 + 1 (u0, u1:
+// {Dirichlet(2p), Dirichlet(2p+1), phase(2p)} of a full-pool spectrum).  The 1/k factors live in
+// the basis rows (B = S_k/k, C_k/k, S_{k+1}/(k+1)), so a slot's value is its signed exponential
+// spacing, times cos f / sin f for even frequencies.  Missing slots (past the last frequency)
+// and padding candidates contribute zeros, selected without branches; their basis rows are
+// zero as well.
+__device__ __forceinline__ void gen_kstep(const DevModel& m, bool valid, int p0, const u32x4& u0, const u32x4& u1,
+                                          bf16x8& Ahi, bf16x8& Alo, float& Wpart) {
     uint32_t hw[4], lw[4];
 #pragma unroll
     for (int pp = 0; pp < 2; pp++) {
-        const int p = 4 * q + 2 * h + pp;
-        const u32x4 r = philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, m.key0, m.key1);
+        const int p = p0 + pp;
+        const u32x4 r = pp ? u1 : u0;
         const int j0 = 2 * p;
         // slot 2p exists iff p < npairs, slot 2p + 1 iff 2p + 1 < nw
         const float e0 = valid && p < m.npairs ? expo2_f32(r.x) : 0.f;
@@ -141,16 +141,45 @@ __device__ __forceinline__ bool tile_positions(const FastArgs& f, uint64_t idx, 
         for (int v = 0; v < 16; v++)
             acc[a][v] = 0.f;
     }
-    for (int q = 0; q < f.ks; q++) {
-        const bf16x8 Bhi = bsrc[((size_t)(nb * f.ks + q) * 2 + 0) * 64 + lane];
-        const bf16x8 Blo = bsrc[((size_t)(nb * f.ks + q) * 2 + 1) * 64 + lane];
+    // K steps Q, Q + 1 (Q even) of lane half h take the four slot pairs 4g .. 4g + 3, g = Q + h,
+    // whose words are blocks 3g .. 3g + 2 of the axis' stream: two blocks at step Q (its two
+    // pairs and the first two words of the next pair, kept), one at step Q + 1
+    const uint32_t ilo = (uint32_t)idx, ihi = (uint32_t)(idx >> 32);
+    for (int Q = 0; Q < f.ks; Q += 2) {
+        const int g = Q + h;
+        uint32_t keep[D][2];
+        {
+            const bf16x8 Bhi = bsrc[((size_t)(nb * f.ks + Q) * 2 + 0) * 64 + lane];
+            const bf16x8 Blo = bsrc[((size_t)(nb * f.ks + Q) * 2 + 1) * 64 + lane];
 #pragma unroll
-        for (int a = 0; a < D; a++) {
-            bf16x8 Ahi, Alo;
-            gen_kstep(m, idx, valid, a, q, h, Ahi, Alo, Wpart[a]);
-            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Bhi, acc[a], 0, 0, 0);
-            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Blo, acc[a], 0, 0, 0);
-            acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo, Bhi, acc[a], 0, 0, 0);
+            for (int a = 0; a < D; a++) {
+                const uint32_t c3 = (1u << 8) | (uint32_t)a;
+                const u32x4 b0 = philox(ilo, ihi, 3u * g, c3, m.key0, m.key1);
+                const u32x4 b1 = philox(ilo, ihi, 3u * g + 1u, c3, m.key0, m.key1);
+                keep[a][0] = b1.z;
+                keep[a][1] = b1.w;
+                bf16x8 Ahi, Alo;
+                gen_kstep(m, valid, 4 * g, u32x4{b0.x, b0.y, b0.z, 0u}, u32x4{b0.w, b1.x, b1.y, 0u}, Ahi, Alo,
+                          Wpart[a]);
+                acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Bhi, acc[a], 0, 0, 0);
+                acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Blo, acc[a], 0, 0, 0);
+                acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo, Bhi, acc[a], 0, 0, 0);
+            }
+        }
+        {
+            const bf16x8 Bhi = bsrc[((size_t)(nb * f.ks + Q + 1) * 2 + 0) * 64 + lane];
+            const bf16x8 Blo = bsrc[((size_t)(nb * f.ks + Q + 1) * 2 + 1) * 64 + lane];
+#pragma unroll
+            for (int a = 0; a < D; a++) {
+                const uint32_t c3 = (1u << 8) | (uint32_t)a;
+                const u32x4 b2 = philox(ilo, ihi, 3u * g + 2u, c3, m.key0, m.key1);
+                bf16x8 Ahi, Alo;
+                gen_kstep(m, valid, 4 * g + 2, u32x4{keep[a][0], keep[a][1], b2.x, 0u}, u32x4{b2.y, b2.z, b2.w, 0u},
+                          Ahi, Alo, Wpart[a]);
+                acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Bhi, acc[a], 0, 0, 0);
+                acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Ahi, Blo, acc[a], 0, 0, 0);
+                acc[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Alo, Bhi, acc[a], 0, 0, 0);
+            }
         }
     }
 #pragma unroll
@@ -617,14 +646,15 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
                    const std::vector<int>& order, double margin_req) {
     const int T = tb.T, W = T + 1, npairs = (tb.nw + 1) / 2;
     FastState* fs = new FastState();
-    fs->ks = (npairs + 3) / 4;
+    fs->ks = 2 * ((npairs + 7) / 8);  // even: a lane's four pairs span two K steps
     fs->nb = (T + NCOL - 1) / NCOL;
     fs->ncols = T;
     std::vector<int> colt((size_t)fs->nb * NCOL, -1);
     for (int oi = 0; oi < T; oi++)
         colt[oi] = order[oi];
-    // basis fragments: row 16q + 8hh + i of K step q is basis row r = 4 p + s, p = r / 4,
-    // holding S_k/k, C_k/k, S_{k+1}/(k+1), 0 (the amplitudes' 1/k folded in)
+    // basis fragments: row 8hh + i of K step q holds slot s = i % 4 of pair
+    // p = 8 (q / 2) + 4 hh + 2 (q % 2) + i / 4 (the generation's pair order), i.e. basis row
+    // r = 4 p + s: S_k/k, C_k/k, S_{k+1}/(k+1), 0 (the amplitudes' 1/k folded in)
     std::vector<uint16_t> frag((size_t)fs->nb * fs->ks * 2 * 64 * 8, 0);
     auto basis = [&](int r, int t) -> double {
         const int p = r / 4, s = r % 4;
@@ -643,7 +673,7 @@ int32_t fast_build(ModelTables& tb, const std::vector<double>& S, const std::vec
         for (int q = 0; q < fs->ks; q++)
             for (int lane = 0; lane < 64; lane++)
                 for (int i = 0; i < 8; i++) {
-                    const int r = 16 * q + 8 * (lane >> 5) + i;
+                    const int r = 4 * (8 * (q >> 1) + 4 * (lane >> 5) + 2 * (q & 1) + (i >> 2)) + (i & 3);
                     const int t = colt[(size_t)nb * NCOL + (lane & 31)];
                     const double b = basis(r, t);
                     const uint16_t hb = bf16_bits_rne((float)b);

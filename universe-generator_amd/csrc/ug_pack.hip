@@ -136,7 +136,8 @@ __device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D
                 w.b[a][i][k] = 0;
             }
             if (p < m.npairs) {
-                const u32x4 r = philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, m.key0, m.key1);
+                const u32x4 r = m.full ? pair_words_full(lo, hi, p, a, m.key0, m.key1)
+                                       : philox(lo, hi, (uint32_t)p, (1u << 8) | (uint32_t)a, m.key0, m.key1);
                 const uint32_t dw[2] = {r.x, r.y}, pw[2] = {r.z, r.w};
 #pragma unroll
                 for (int k = 0; k < 2; k++) {
