@@ -355,9 +355,23 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                 // together, so a visit costs about three dependent memory steps (bits, heads,
                 // points)
                 uint32_t cells = 0;
+                if constexpr (D == 2) {
+                    // the three rows' word pairs loaded before any is used
+                    const int wq = c[1] >> 5, sh = c[1] & 31;
+                    uint64_t raw[3];
 #pragma unroll
-                for (int rr = 0; rr < ROWS; rr++)
-                    cells |= occ_bits3(m, t, gx_of(rr), gy_of(rr), c[D - 1]) << (3 * rr);
+                    for (int rr = 0; rr < 3; rr++) {
+                        const uint32_t* wp = m.occ + occ_index<2>(m, t, gx_of(rr), 0, wq);
+                        raw[rr] = (uint64_t)wp[0] | ((uint64_t)wp[1] << 32);
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < 3; rr++)
+                        cells |= ((uint32_t)(raw[rr] >> sh) & 7u) << (3 * rr);
+                } else {
+#pragma unroll
+                    for (int rr = 0; rr < ROWS; rr++)
+                        cells |= occ_bits3<D>(m, t, gx_of(rr), gy_of(rr), c[D - 1]) << (3 * rr);
+                }
                 while (cells && !cert) {
                     constexpr int NG = 2;
                     int node[NG];

@@ -91,8 +91,10 @@ __host__ __device__ inline int wrap_cell(int c, int d, int T) {
 // x-plane apart -- the lookup of the 17 GB table at 3+1 T=256 then fetches about 2 lines of
 // bits per visited timestep instead of about 5.  The bits are a 32nd of the head table (42 MB
 // at 3+1 T=128, inside the last-level cache), so empty neighbour cells cost no head traffic.
+// (The layout follows the dimension: D = 2 rows, D = 3 tiles.)
+template <int D>
 __host__ __device__ __forceinline__ int64_t occ_index(const DevModel& m, int t, int x, int y, int wq) {
-    if (m.occ_nwq == 0)
+    if constexpr (D == 2)
         return ((int64_t)t * m.T + x) * m.occw + wq;
     return ((((int64_t)t * m.occ_nwq + wq) * m.occ_nxb + (x >> 2)) * m.occ_nyb + (y >> 3)) * 32 + (x & 3) * 8 +
            (y & 7);
@@ -102,14 +104,15 @@ __host__ __device__ __forceinline__ int64_t occ_index(const DevModel& m, int t, 
 // halo bits at the row ends) of row (t, x[, y]) as bits 0..2.  3+1 reads the next word plane
 // only where the window crosses into it (it exists then: the padded row has T + 2 bits); a
 // 2+1 row's next word (or its spare word) is always read, it shares the row's line.
+template <int D>
 __device__ __forceinline__ uint32_t occ_bits3(const DevModel& m, int t, int x, int y, int c) {
     const int wq = c >> 5, sh = c & 31;
-    const uint64_t lo = m.occ[occ_index(m, t, x, y, wq)];
+    const uint64_t lo = m.occ[occ_index<D>(m, t, x, y, wq)];
     uint64_t hi = 0;
-    if (m.occ_nwq == 0)
-        hi = m.occ[occ_index(m, t, x, y, wq + 1)];  // 2+1: the row's next word, in the same line
+    if constexpr (D == 2)
+        hi = m.occ[occ_index<D>(m, t, x, y, wq + 1)];  // the row's next word, in the same line
     else if (sh > 29)
-        hi = m.occ[occ_index(m, t, x, y, wq + 1)];
+        hi = m.occ[occ_index<D>(m, t, x, y, wq + 1)];
     return (uint32_t)(((hi << 32) | lo) >> sh) & 7u;
 }
 
@@ -142,8 +145,9 @@ __device__ __forceinline__ int hash_head(const DevModel& m, uint64_t key) {
 // Heads of the cells c-1, c, c+1 (wrapped) of table row (t, x[, y]) (row = t*T + x in 2+1,
 // (t*T + x)*T + y in 3+1), -1 for empty cells: the occupancy bits first, then the occupied
 // cells' heads from the dense table's padded row or from the sparse grid.
+template <int D>
 __device__ __forceinline__ void row_heads3(const DevModel& m, int t, int x, int y, int64_t row, int c, int node[3]) {
-    const uint32_t bits = occ_bits3(m, t, x, y, c);
+    const uint32_t bits = occ_bits3<D>(m, t, x, y, c);
     if (m.sparse) {
 #pragma unroll
         for (int k = 0; k < 3; k++)

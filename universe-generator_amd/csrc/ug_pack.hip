@@ -390,7 +390,7 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
         const int64_t rid = D == 2 ? (int64_t)t * T + gx : ((int64_t)t * T + gx) * T + gy;
         // the occupied cells' heads load together and their chains are walked together
         int node[3];
-        row_heads3(m, t, gx, gy, rid, c[D - 1], node);
+        row_heads3<D>(m, t, gx, gy, rid, c[D - 1], node);
         while (!h && (node[0] >= 0 || node[1] >= 0 || node[2] >= 0)) {
             double P[3][D];
             int nxt[3];
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t fir
                     const int gy = D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, T);
                     const int64_t rid = D == 2 ? (int64_t)t * T + gx : ((int64_t)t * T + gx) * T + gy;
                     int tr[3];
-                    row_heads3(m, t, gx, gy, rid, c[D - 1], tr);
+                    row_heads3<D>(m, t, gx, gy, rid, c[D - 1], tr);
                     for (int k = 0; k < 3 && !h; k++)
                         for (int nn = tr[k]; nn >= 0 && !h;) {
                             const size_t q = (size_t)nn * T + t;
@@ -1013,7 +1013,7 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
                     const int gy = D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, T);
                     const int64_t rid = D == 2 ? (int64_t)t * T + gx : ((int64_t)t * T + gx) * T + gy;
                     int tr[3];
-                    row_heads3(m, t, gx, gy, rid, c[D - 1], tr);
+                    row_heads3<D>(m, t, gx, gy, rid, c[D - 1], tr);
                     for (int k = 0; k < 3; k++)
                         for (int n = tr[k]; n >= 0;) {
                             const size_t q = (size_t)n * T + t;
@@ -1573,11 +1573,11 @@ __device__ __forceinline__ void insert_item(const DevModel& m, const double* pos
         }
         // occupancy bit of the padded cell, and of its halo copy at the row ends
         const int oy = D == 3 ? c[1] : 0;
-        atomicOr(&m.occ[occ_index(m, t, c[0], oy, (cl + 1) >> 5)], 1u << ((cl + 1) & 31));
+        atomicOr(&m.occ[occ_index<D>(m, t, c[0], oy, (cl + 1) >> 5)], 1u << ((cl + 1) & 31));
         if (cl == 0)
-            atomicOr(&m.occ[occ_index(m, t, c[0], oy, (m.T + 1) >> 5)], 1u << ((m.T + 1) & 31));
+            atomicOr(&m.occ[occ_index<D>(m, t, c[0], oy, (m.T + 1) >> 5)], 1u << ((m.T + 1) & 31));
         if (cl == m.T - 1)
-            atomicOr(&m.occ[occ_index(m, t, c[0], oy, 0)], 1u);
+            atomicOr(&m.occ[occ_index<D>(m, t, c[0], oy, 0)], 1u);
         pos32[o] = make_float4(f[0], f[1], f[2], __int_as_float(prev));
         if (t == 0) {
             acc_idx[n] = surv_idx[s];
