@@ -34,6 +34,10 @@ typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 namespace {
 
 constexpr int TILE = 32;        // candidates per wave tile (MFMA M)
+// A lookup round's split of the wave among U unresolved candidates: per = 64 / U lanes each,
+// and M = ceil(2^16 / per), with lane / per = (lane * M) >> 16 exactly for every lane < 64;
+// entry U = per | M << 8.
+__constant__ uint32_t k_round_split[33] = {0, 262208, 524320, 798997, 1048592, 1398284, 1677834, 1864201, 2097160, 2396935, 2796294, 3355653, 3355653, 4194308, 4194308, 4194308, 4194308, 5592579, 5592579, 5592579, 5592579, 5592579, 8388610, 8388610, 8388610, 8388610, 8388610, 8388610, 8388610, 8388610, 8388610, 8388610, 8388610};
 constexpr int NCOL = 32;        // timesteps per N block (MFMA N)
 constexpr int FAST_WAVES = 4;   // waves per workgroup
 
@@ -274,7 +278,8 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
         int* ulist = (int*)(sinvw + TILE);
         while (unres) {
             const int U = __popc(unres);
-            const int per = 64 / U;
+            const uint32_t split = k_round_split[U];
+            const int per = (int)(split & 255u);
             if (lane < 32 && ((unres >> lane) & 1u))
                 ulist[__popc(unres & ((1u << lane) - 1u))] = lane;
             if (SUB && lane < 32) {
@@ -284,7 +289,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const int rk = lane / per, off = lane % per;
+            const int rk = (int)(((uint32_t)lane * (split >> 8)) >> 16), off = lane - rk * per;
             const int cm = rk < U ? ulist[rk] : 0;
             const int col = __shfl(ncol, cm, 64) + off;
             // cert: a certain contact; maybe: a possible one (within r + m_t); clr: this
@@ -316,6 +321,14 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                 auto gy_of = [&](int rr) { return D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, m.T); };
                 auto row_of = [&](int rr) -> int64_t {
                     return D == 2 ? (int64_t)t * m.T + gx_of(rr) : ((int64_t)t * m.T + gx_of(rr)) * m.T + gy_of(rr);
+                };
+                // head-table slot of neighbour bit b (row b / 3, padded cell c + b % 3): below
+                // 2^31 in 2+1 (T <= 512 rows of T + 2 slots per timestep), 32-bit math there
+                auto tab_at = [&](int b) -> int64_t {
+                    if constexpr (D == 2)
+                        return (int64_t)(((uint32_t)t * (uint32_t)m.T + (uint32_t)gx_of(b / 3)) * (uint32_t)m.rowlen +
+                                         (uint32_t)(c[1] + b % 3));
+                    return row_of(b / 3) * m.rowlen + c[D - 1] + b % 3;
                 };
                 // one grid point of this lane's visit: certain contact (or, SUB, its group)
                 auto point = [&](int node, const float4& P) {
@@ -382,7 +395,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                             const int b = __builtin_ctz(cells);
                             cells &= cells - 1u;
                             node[g] = SPARSE ? hash_head(m, cell_hkey(row_of(b / 3), wrap_cell(c[D - 1], b % 3 - 1, m.T), m.T))
-                                             : m.tab[row_of(b / 3) * m.rowlen + c[D - 1] + b % 3];
+                                             : m.tab[tab_at(b)];
                         }
                     }
                     while (!cert && (node[0] >= 0 || node[1] >= 0)) {
@@ -390,7 +403,7 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
 #pragma unroll
                         for (int g = 0; g < NG; g++)
                             if (node[g] >= 0)
-                                P[g] = m.pos32[(size_t)node[g] * m.T + t];
+                                P[g] = m.pos32[(uint64_t)(uint32_t)node[g] * (uint32_t)m.T + (uint32_t)t];
 #pragma unroll
                         for (int g = 0; g < NG; g++) {
                             if (node[g] < 0)

@@ -94,8 +94,8 @@ __host__ __device__ inline int wrap_cell(int c, int d, int T) {
 // (The layout follows the dimension: D = 2 rows, D = 3 tiles.)
 template <int D>
 __host__ __device__ __forceinline__ int64_t occ_index(const DevModel& m, int t, int x, int y, int wq) {
-    if constexpr (D == 2)
-        return ((int64_t)t * m.T + x) * m.occw + wq;
+    if constexpr (D == 2)  // below 2^32 (T <= 512): 32-bit math
+        return (int64_t)(((uint32_t)t * (uint32_t)m.T + (uint32_t)x) * (uint32_t)m.occw + (uint32_t)wq);
     return ((((int64_t)t * m.occ_nwq + wq) * m.occ_nxb + (x >> 2)) * m.occ_nyb + (y >> 3)) * 32 + (x & 3) * 8 +
            (y & 7);
 }
