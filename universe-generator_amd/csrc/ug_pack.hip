@@ -12,7 +12,7 @@
 //       classification, FP64 recheck of everything uncertain       (ug_fast.hip)]
 //   2. ordered compaction of survivors by candidate index          (k_compact_*)
 //   3. full FP64 trajectories of the survivors                     (k_cand64<FULL>)
-//   4. pairwise conflicts among survivors + greedy in index order  (k_conflicts, k_greedy)
+//   4. pairwise conflicts among survivors + greedy in index order  (k_conflicts_cells, k_greedy)
 //   5. insertion into the per-timestep cell table                  (k_insert)
 // so the accepted list equals serial RSA in candidate-index order (cuda/braid_knot.cu:433-462).
 #include <stdarg.h>
@@ -1293,13 +1293,6 @@ __device__ __forceinline__ void conflicts_body(const DevModel& m, const double* 
     }
 }
 
-template <int D>
-__global__ __launch_bounds__(256) void k_conflicts(DevModel m, const double* pos, const int* cells, int S,
-                                                   const int* dS, uint64_t* mask) {
-    if (dS)
-        S = min(S, *dS);
-    conflicts_body<D>(m, pos, cells, S, mask, blockIdx.x, gridDim.x);
-}
 
 // The same conflict bits for large chunks: one workgroup per (64x64 tile (bi, bj) of the
 // lower triangle, block of CONF_TB timesteps).  The tile's packed cells and the j side's
@@ -1386,12 +1379,83 @@ __device__ __forceinline__ void conflicts_tiled_body(const DevModel& m, const do
         atomicOr((unsigned long long*)&mask[(size_t)(64 * bi + tid) * UG_MASK_WORDS + bj], word[tid]);
 }
 
+
+// The same conflict bits by cells: one workgroup per timestep t hashes the chunk's survivors by
+// their packed cell at t in LDS (open addressing, 2048 slots for at most 1024 survivors, a
+// chain per cell), then each survivor probes the 3^D neighbouring cells (wrapped) and runs the
+// FP64 contact test on the earlier survivors found there -- the pairs whose cells neighbour at
+// t, exactly those the pair kernels test, in O(S 3^D) per timestep instead of O(S^2).
+constexpr int CONF_HSLOTS = 2048;
+
+__device__ __forceinline__ int conf_hslot(int key) {
+    return (int)(((uint32_t)key * 0x9E3779B1u) >> (32 - 11));
+}
+
 template <int D>
-__global__ __launch_bounds__(256) void k_conflicts_tiled(DevModel m, const double* pos, const int* cells, int S,
+__global__ __launch_bounds__(256) void k_conflicts_cells(DevModel m, const double* pos, const int* cells, int S,
                                                          const int* dS, uint64_t* mask) {
+    __shared__ int hkey[CONF_HSLOTS];
+    __shared__ int hhead[CONF_HSLOTS];
+    __shared__ short nxt[UG_S_MAX];
+    __shared__ double sp[UG_S_MAX * D];
     if (dS)
         S = min(S, *dS);
-    conflicts_tiled_body<D>(m, pos, cells, S, mask, blockIdx.x);
+    // workgroup (t, g): the whole chunk hashed at timestep t, probes for survivors 256 g ..
+    const int T = m.T, t = blockIdx.x, s0 = (int)blockIdx.y * 256;
+    if (s0 >= S)
+        return;  // uniform over the workgroup
+    for (int k = threadIdx.x; k < CONF_HSLOTS; k += 256) {
+        hkey[k] = -1;
+        hhead[k] = -1;
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < S; s += 256) {
+        const int key = cells[(size_t)s * T + t];
+#pragma unroll
+        for (int a = 0; a < D; a++)
+            sp[s * D + a] = pos[((size_t)s * T + t) * D + a];
+        int h = conf_hslot(key);
+        for (;;) {
+            const int prev = atomicCAS(&hkey[h], -1, key);
+            if (prev == -1 || prev == key)
+                break;
+            h = (h + 1) & (CONF_HSLOTS - 1);
+        }
+        nxt[s] = (short)atomicExch(&hhead[h], s);
+    }
+    __syncthreads();
+    const int s = s0 + (int)threadIdx.x;
+    if (s >= S)
+        return;
+    const int key = cells[(size_t)s * T + t];
+    int c[D];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        c[a] = (key >> (10 * a)) & 1023;
+    double x[D];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        x[a] = sp[s * D + a];
+    constexpr int NB = D == 2 ? 9 : 27;
+    for (int nb = 0; nb < NB; nb++) {
+        int nk = 0, q = nb;
+#pragma unroll
+        for (int a = 0; a < D; a++) {
+            int v = c[a] + q % 3 - 1;
+            q /= 3;
+            v = v < 0 ? v + T : (v >= T ? v - T : v);
+            nk |= v << (10 * a);
+        }
+        int h = conf_hslot(nk);
+        int kk;
+        while ((kk = hkey[h]) != -1 && kk != nk)
+            h = (h + 1) & (CONF_HSLOTS - 1);
+        if (kk != nk)
+            continue;
+        for (int j = hhead[h]; j >= 0; j = nxt[j])
+            if (j < s && contact<D>(m, x, &sp[j * D]))
+                atomicOr((unsigned long long*)&mask[(size_t)s * UG_MASK_WORDS + (j >> 6)], 1ull << (j & 63));
+    }
 }
 
 // A resolve chunk's results, stored straight into the pinned staging block (one dispatch in
@@ -2173,31 +2237,19 @@ int32_t ensure_hash(Pack* h, int64_t points) {
 // The chunk's packed cells (d_scell) and cleared mask rows come from its trajectory launch.
 // expect: the survivors the host expects (s itself when known); large chunks take the
 // tiled kernel, small ones the pair-per-thread kernel -- the bits are the same either way.
+// Conflict bits of a chunk's first s survivors (at most *dS): the cell-hash kernel, one
+// workgroup per timestep.  (The pair kernels above serve the batched engine.)
 void launch_conflicts(Pack* h, int s, const int* dS, int64_t expect) {
-    const int64_t npairs = (int64_t)s * (s - 1) / 2;
-    if (npairs <= 0)
+    (void)expect;
+    if ((int64_t)s * (s - 1) / 2 <= 0)
         return;
-    if (expect < 0)
-        expect = s;
-    const bool tiled = expect >= 512;
-    const int64_t items = npairs * ((h->T + CONF_T - 1) / CONF_T);
-    const int kb = tiled ? conf_tiles(s) * ((h->T + CONF_TB - 1) / CONF_TB)
-                         : (dS ? std::min(grid_for(items, 256), 2048) : grid_for(items, 256));
-    if (h->d == 2) {
-        if (tiled)
-            hipLaunchKernelGGL(k_conflicts_tiled<2>, dim3(kb), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell,
-                               s, dS, h->d_mask);
-        else
-            hipLaunchKernelGGL(k_conflicts<2>, dim3(kb), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell, s,
-                               dS, h->d_mask);
-    } else {
-        if (tiled)
-            hipLaunchKernelGGL(k_conflicts_tiled<3>, dim3(kb), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell,
-                               s, dS, h->d_mask);
-        else
-            hipLaunchKernelGGL(k_conflicts<3>, dim3(kb), dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell, s,
-                               dS, h->d_mask);
-    }
+    const dim3 grid(h->T, (s + 255) / 256);
+    if (h->d == 2)
+        hipLaunchKernelGGL(k_conflicts_cells<2>, grid, dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell, s, dS,
+                           h->d_mask);
+    else
+        hipLaunchKernelGGL(k_conflicts_cells<3>, grid, dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell, s, dS,
+                           h->d_mask);
 }
 
 // Resolve the survivors list[0..S) (already clear of the cell table) in index order and
