@@ -96,7 +96,6 @@ struct Uni {
     int64_t rech_sub = 0;       // FP64 rechecks during the sub-batch (sizing)
     int chunk = 0;              // this step's resolve chunk
     int64_t bound = 0;          // upper bound on this step's chunk (sizes the resolve grids)
-    int64_t expect = 0;         // survivors expected (picks the conflict kernel's form)
     int slot = 0;               // position among the step's universes (report block)
     bool in_b = false;          // list lives in the handle's d_listb (else d_list)
     bool all = false;           // filter against an empty table
@@ -164,8 +163,6 @@ int32_t Engine::stage_filter(std::vector<Uni*>& act) {
         u->all = u->fresh && h->n == 0;
         const int64_t cnt = u->fresh ? u->sub : u->npend;
         u->bound = std::min<int64_t>(cnt, UG_S_MAX);
-        // a fresh sub-batch was sized for sub_target survivors at the last one's rate
-        u->expect = u->all ? cnt : (u->fresh ? std::min<int64_t>(cnt, h->cfg.sub_target) : u->bound);
         if (u->all)
             all.push_back(u);
         else if (h->cfg.precision == UG_PREC_SPLIT_BF16)
@@ -333,9 +330,7 @@ int32_t Engine::stage_resolve(std::vector<Uni*>& act) {
     // grids cover each universe's bound on the chunk, the kernels stop at the device count
     for (int d = 2; d <= 3; d++) {
         std::vector<MResolve> a;
-        int maxS = 0, maxT = 0, max_tiles = 0;
-        int64_t max_pairs = 0;
-        bool any_plain = false, any_tiled = false;
+        int maxS = 0, maxT = 0;
         for (Uni* u : act) {
             Pack* h = u->h;
             if (h->d != d)
@@ -348,8 +343,6 @@ int32_t Engine::stage_resolve(std::vector<Uni*>& act) {
             x.mask = h->d_mask;
             x.counts = h->d_count;
             x.S = (int)u->bound;
-            // the tiled kernel pays off from several hundred survivors (same bits either way)
-            x.tiled = std::min<int64_t>(u->expect, u->bound) >= 512 ? 1 : 0;
             x.limit = h->max_n > 0 ? (int)std::min<int64_t>(h->max_n - h->n, INT32_MAX) : INT32_MAX;
             x.acc_list = h->d_acc_list;
             x.nacc = h->d_nacc;
@@ -364,20 +357,13 @@ int32_t Engine::stage_resolve(std::vector<Uni*>& act) {
             a.push_back(x);
             maxS = std::max(maxS, x.S);
             maxT = std::max(maxT, h->T);
-            if (x.tiled) {
-                any_tiled = true;
-                max_tiles = std::max(max_tiles, ((x.S + 63) / 64) * ((x.S + 63) / 64 + 1) / 2);
-            } else {
-                any_plain = true;
-                max_pairs = std::max(max_pairs, (int64_t)x.S * (x.S - 1) / 2);
-            }
         }
         if (a.empty())
             continue;
         UG_STAGE(dA, a);
         const int n = (int)a.size();
         launches.push_back([=]() {
-            return launch_resolve_multi(st, d, dA, n, maxS, maxT, max_pairs, max_tiles, any_plain, any_tiled);
+            return launch_resolve_multi(st, d, dA, n, maxS, maxT);
         });
     }
     return UG_OK;

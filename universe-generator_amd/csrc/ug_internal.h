@@ -248,8 +248,8 @@ struct MCompact {
     int* vout;
 };
 // A resolve chunk of min(S, counts[1]) survivors (trajectories pos, packed cells, cleared
-// conflict rows; counts = the filter's device counts {rechecks, survivors}): conflict bits
-// (tiled kernel when tiled -- the bits are the same either way), index-order greedy,
+// conflict rows; counts = the filter's device counts {rechecks, survivors}): conflict bits,
+// index-order greedy,
 // insertion of at most limit accepted paths as rows n0, ..., and the report: rep[0] =
 // accepted, rep[1] = survivors, rep[2..3] = occupied buckets (64 bits), rep[4] = rechecks,
 // then the accepted candidate indices from rep + 6 (as int64).
@@ -259,7 +259,7 @@ struct MResolve {
     const int* cells;
     uint64_t* mask;
     const int* counts;
-    int S, tiled, limit;
+    int S, limit;
     int* acc_list;
     int* nacc;
     const int64_t* surv_idx;
@@ -499,7 +499,7 @@ int32_t launch_fast_knot(Pack* h, uint64_t base, int64_t count, uint8_t* flags, 
                          int nk);
 // Conflict bits among a resolve chunk's s survivors (trajectories in d_spos, packed cells in
 // d_scell, mask rows cleared); dS: count bounded on the device; expect: survivors expected.
-void launch_conflicts(Pack* h, int s, const int* dS = nullptr, int64_t expect = -1);
+void launch_conflicts(Pack* h, int s, const int* dS = nullptr);
 // Index-order greedy over the chunk's conflict bits -> d_acc_list, d_nacc.
 int32_t launch_greedy(Pack* h, int s);
 // Knot regime: brute-force test of fresh candidates base + i (list == nullptr) or list[i]
@@ -517,8 +517,7 @@ size_t cand64_lds_bytes(const Pack* h);
 int32_t launch_cand64_multi(hipStream_t st, int d, int pl, bool full, bool sb, const MCand* dA, int U,
                             int64_t items, size_t lds);
 int32_t launch_compact_multi(hipStream_t st, const MCompact* dA, int U, int64_t max_count);
-int32_t launch_resolve_multi(hipStream_t st, int d, const MResolve* dA, int U, int maxS, int maxT,
-                             int64_t max_pairs, int max_tiles, bool any_plain, bool any_tiled);
+int32_t launch_resolve_multi(hipStream_t st, int d, const MResolve* dA, int U, int maxS, int maxT);
 size_t fast_multi_arg_bytes();
 int32_t fast_multi_arg(Pack* h, uint64_t base, int64_t count, const int64_t* list, uint8_t* flags, uint32_t* skip,
                        void* out);

@@ -91,7 +91,7 @@ int32_t knot_resolve(ug_knot* K, int64_t* list, int64_t S, int64_t* admitted) {
         rc = launch_cand64(h, 0, s, list, nullptr, h->d_spos, true, nullptr, nullptr, 0, h->d_scell, h->d_mask);
         if (rc)
             break;
-        launch_conflicts(h, s, nullptr, s);
+        launch_conflicts(h, s, nullptr);
         rc = launch_greedy(h, s);
         const int limit = (int)std::min<int64_t>(K->ks - K->nk, INT32_MAX);
         if (!rc)

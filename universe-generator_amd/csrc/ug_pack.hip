@@ -1221,19 +1221,6 @@ __global__ __launch_bounds__(256) void k_compact_scatter_multi(const MCompact* A
 
 // =====================================================================================
 // in-batch accept
-// Canonical pair predicate at one timestep: neighbouring cells on every axis and the
-// contact predicate -- exactly what a cell-table lookup would find.
-template <int D>
-__device__ __forceinline__ bool pair_hit(const DevModel& m, const double* x, const double* y) {
-    bool h = true;
-#pragma unroll
-    for (int a = 0; a < D; a++) {
-        int dc = cell_of(x[a], m.halfT, m.T) - cell_of(y[a], m.halfT, m.T);
-        dc = dc < 0 ? -dc : dc;
-        h = h && (dc <= 1 || dc == m.T - 1);
-    }
-    return h && contact<D>(m, x, y);
-}
 
 // Neighbouring cells on every axis (torus), from packed cells (10 bits per axis, written
 // with a resolve chunk's trajectories).
@@ -1249,142 +1236,14 @@ __device__ __forceinline__ bool cells_adjacent(int u, int v, int T) {
     return adj;
 }
 
-// Conflict bits among the first S survivors: mask[i][j/64] bit j%64 set iff i > j collide.
-// One thread per (pair, block of CONF_T timesteps): packed cells first (two ints per
-// timestep), the FP64 contact test only where the cells neighbour -- the canonical pair
-// predicate.
-constexpr int CONF_T = 16;
-
-template <int D>
-__device__ __forceinline__ void conflicts_body(const DevModel& m, const double* pos, const int* cells, int S,
-                                               uint64_t* mask, int bx, int gx) {
-    const int nblk = (m.T + CONF_T - 1) / CONF_T;
-    const int64_t items = (int64_t)S * (S - 1) / 2 * nblk;
-    for (int64_t it = (int64_t)bx * 256 + threadIdx.x; it < items; it += (int64_t)gx * 256) {
-        const int64_t k = it / nblk;
-        const int t0 = (int)(it % nblk) * CONF_T;
-        int64_t i = (int64_t)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
-        while (i * (i - 1) / 2 > k)
-            i--;
-        while ((i + 1) * i / 2 <= k)
-            i++;
-        const int64_t j = k - i * (i - 1) / 2;
-        const int* ci = cells + (size_t)i * m.T;
-        const int* cj = cells + (size_t)j * m.T;
-        const int t1 = min(t0 + CONF_T, m.T);
-        bool hit = false;
-        for (int t = t0; t < t1 && !hit; t += 4) {
-            int a[4], b[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int tt = min(t + u, t1 - 1);
-                a[u] = ci[tt];
-                b[u] = cj[tt];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int tt = min(t + u, t1 - 1);
-                if (!hit && cells_adjacent<D>(a[u], b[u], m.T))
-                    hit = contact<D>(m, pos + ((size_t)i * m.T + tt) * D, pos + ((size_t)j * m.T + tt) * D);
-            }
-        }
-        if (hit)
-            atomicOr((unsigned long long*)&mask[(size_t)i * UG_MASK_WORDS + (j >> 6)], 1ull << (j & 63));
-    }
-}
-
-
-// The same conflict bits for large chunks: one workgroup per (64x64 tile (bi, bj) of the
-// lower triangle, block of CONF_TB timesteps).  The tile's packed cells and the j side's
-// FP64 positions are staged in LDS; thread (r, q) tests pairs (64 bi + r, 64 bj + 16 q + jj)
-// at all CONF_TB timesteps at once (independent adjacency tests, then FP64 contacts only
-// where cells neighbour) and the tile ORs one 64-bit word per row into the cleared mask.
-// Faster than the pair-per-thread kernel once the chunk has several hundred survivors
-// (1024 survivors at 2+1 T=64: 40 us against 90 us), slower for a handful.
-constexpr int CONF_TB = 16;
-
-__host__ __device__ constexpr int conf_tiles(int S) {
-    return ((S + 63) / 64) * ((S + 63) / 64 + 1) / 2;
-}
-
-template <int D>
-__device__ __forceinline__ void conflicts_tiled_body(const DevModel& m, const double* pos, const int* cells, int S,
-                                                     uint64_t* mask, int bx) {
-    const int T = m.T;
-    const int ntb = (T + CONF_TB - 1) / CONF_TB;
-    const int b = bx / ntb, tb = bx % ntb;
-    if (b >= conf_tiles(S))
-        return;  // uniform over the workgroup
-    int bi = (int)((sqrtf(8.0f * (float)b + 1.0f) - 1.0f) * 0.5f);
-    while (bi * (bi + 1) / 2 > b)
-        bi--;
-    while ((bi + 1) * (bi + 2) / 2 <= b)
-        bi++;
-    const int bj = b - bi * (bi + 1) / 2;
-    const int t0 = tb * CONF_TB, nt = min(CONF_TB, T - t0);
-    __shared__ int sci[CONF_TB][64];
-    __shared__ int scj[CONF_TB][64];
-    __shared__ double spj[CONF_TB][64][D];
-    __shared__ unsigned long long word[64];
-    const int tid = threadIdx.x, r = tid & 63, q = tid >> 6;
-    if (tid < 64)
-        word[tid] = 0ull;
-    for (int e = tid; e < 64 * CONF_TB; e += 256) {
-        const int row = e / CONF_TB, tt = e % CONF_TB;
-        const int ri = 64 * bi + row, rj = 64 * bj + row;
-        const bool vi = ri < S && tt < nt, vj = rj < S && tt < nt;
-        sci[tt][row] = vi ? cells[(size_t)ri * T + t0 + tt] : -1;
-        scj[tt][row] = vj ? cells[(size_t)rj * T + t0 + tt] : -1;
-#pragma unroll
-        for (int a = 0; a < D; a++)
-            spj[tt][row][a] = vj ? pos[((size_t)rj * T + t0 + tt) * D + a] : 0.0;
-    }
-    __syncthreads();
-    const int i = 64 * bi + r;
-    uint32_t hits = 0;
-    if (i < S) {
-        int ci[CONF_TB];
-#pragma unroll
-        for (int tt = 0; tt < CONF_TB; tt++)
-            ci[tt] = sci[tt][r];
-        const uint32_t tmask = nt >= 32 ? ~0u : ((1u << nt) - 1u);
-        for (int jj = 0; jj < 16; jj++) {
-            const int c = 16 * q + jj, jx = 64 * bj + c;
-            if (jx >= i || jx >= S)
-                continue;
-            uint32_t adj = 0;
-#pragma unroll
-            for (int tt = 0; tt < CONF_TB; tt++)
-                adj |= cells_adjacent<D>(ci[tt], scj[tt][c], T) ? (1u << tt) : 0u;
-            adj &= tmask;
-            while (adj) {
-                const int tt = __ffs(adj) - 1;
-                adj &= adj - 1;
-                double x[D];
-#pragma unroll
-                for (int a = 0; a < D; a++)
-                    x[a] = pos[((size_t)i * T + t0 + tt) * D + a];
-                if (contact<D>(m, x, &spj[tt][c][0])) {
-                    hits |= 1u << jj;
-                    break;
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if (hits)
-        atomicOr(&word[r], (unsigned long long)hits << (16 * q));
-    __syncthreads();
-    if (tid < 64 && word[tid] && 64 * bi + tid < S)
-        atomicOr((unsigned long long*)&mask[(size_t)(64 * bi + tid) * UG_MASK_WORDS + bj], word[tid]);
-}
-
-
-// The same conflict bits by cells: one workgroup per timestep t hashes the chunk's survivors by
-// their packed cell at t in LDS (open addressing, 2048 slots for at most 1024 survivors, a
-// chain per cell), then each survivor probes the 3^D neighbouring cells (wrapped) and runs the
-// FP64 contact test on the earlier survivors found there -- the pairs whose cells neighbour at
-// t, exactly those the pair kernels test, in O(S 3^D) per timestep instead of O(S^2).
+// Conflict bits among a resolve chunk's first S survivors: mask[i][j/64] bit j%64 set iff
+// i > j collide at some timestep -- their cells neighbour on every axis (torus) and the
+// canonical FP64 contact predicate holds, what a cell-table lookup would find.  Workgroup
+// (t, g) hashes the whole chunk by packed cell at timestep t in LDS (open addressing, 2048
+// slots for at most 1024 survivors, a chain per cell, positions beside), then survivor
+// 256 g + lane probes its 3^D neighbouring cells (wrapped) and tests the earlier survivors
+// found there: O(S 3^D) per timestep instead of O(S^2) pair tests (a pair-per-thread kernel
+// and a 64x64-tile kernel over all pairs took 9-45 us where this takes a few).
 constexpr int CONF_HSLOTS = 2048;
 
 __device__ __forceinline__ int conf_hslot(int key) {
@@ -1392,17 +1251,15 @@ __device__ __forceinline__ int conf_hslot(int key) {
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_conflicts_cells(DevModel m, const double* pos, const int* cells, int S,
-                                                         const int* dS, uint64_t* mask) {
+__device__ __forceinline__ void conflicts_cells_body(const DevModel& m, const double* pos, const int* cells, int S,
+                                                     uint64_t* mask, int t, int g) {
     __shared__ int hkey[CONF_HSLOTS];
     __shared__ int hhead[CONF_HSLOTS];
     __shared__ short nxt[UG_S_MAX];
     __shared__ double sp[UG_S_MAX * D];
-    if (dS)
-        S = min(S, *dS);
     // workgroup (t, g): the whole chunk hashed at timestep t, probes for survivors 256 g ..
-    const int T = m.T, t = blockIdx.x, s0 = (int)blockIdx.y * 256;
-    if (s0 >= S)
+    const int T = m.T, s0 = g * 256;
+    if (t >= T || s0 >= S)
         return;  // uniform over the workgroup
     for (int k = threadIdx.x; k < CONF_HSLOTS; k += 256) {
         hkey[k] = -1;
@@ -1456,6 +1313,14 @@ __global__ __launch_bounds__(256) void k_conflicts_cells(DevModel m, const doubl
             if (j < s && contact<D>(m, x, &sp[j * D]))
                 atomicOr((unsigned long long*)&mask[(size_t)s * UG_MASK_WORDS + (j >> 6)], 1ull << (j & 63));
     }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_conflicts_cells(DevModel m, const double* pos, const int* cells, int S,
+                                                         const int* dS, uint64_t* mask) {
+    if (dS)
+        S = min(S, *dS);
+    conflicts_cells_body<D>(m, pos, cells, S, mask, blockIdx.x, blockIdx.y);
 }
 
 // A resolve chunk's results, stored straight into the pinned staging block (one dispatch in
@@ -1737,16 +1602,10 @@ __device__ __forceinline__ MResolve resolve_rec(const MResolve* A, int u) {
 }
 template <int D>
 __global__ __launch_bounds__(256) void k_conflicts_multi(const MResolve* A) {
-    const MResolve a = resolve_rec(A, blockIdx.y);
+    const MResolve a = resolve_rec(A, blockIdx.z);
     const int S = resolve_count(a);
-    if (!a.tiled && S > 1)
-        conflicts_body<D>(a.m, a.pos, a.cells, S, a.mask, blockIdx.x, gridDim.x);
-}
-template <int D>
-__global__ __launch_bounds__(256) void k_conflicts_tiled_multi(const MResolve* A) {
-    const MResolve a = resolve_rec(A, blockIdx.y);
-    if (a.tiled)
-        conflicts_tiled_body<D>(a.m, a.pos, a.cells, resolve_count(a), a.mask, blockIdx.x);
+    if (S > 1)
+        conflicts_cells_body<D>(a.m, a.pos, a.cells, S, a.mask, blockIdx.x, blockIdx.y);
 }
 __global__ __launch_bounds__(64) void k_greedy_multi(const MResolve* A) {
     const MResolve a = resolve_rec(A, blockIdx.x);
@@ -1897,27 +1756,20 @@ int32_t launch_compact_multi(hipStream_t st, const MCompact* dA, int U, int64_t 
     return hip_check(hipGetLastError(), "batched compaction");
 }
 
-int32_t launch_resolve_multi(hipStream_t st, int d, const MResolve* dA, int U, int maxS, int maxT,
-                             int64_t max_pairs, int max_tiles, bool any_plain, bool any_tiled) {
+int32_t launch_resolve_multi(hipStream_t st, int d, const MResolve* dA, int U, int maxS, int maxT) {
     if (U <= 0)
         return UG_OK;
-    const int64_t items = max_pairs * ((maxT + CONF_T - 1) / CONF_T);
-    const int pb = std::min(grid_for(items, 256), 4096);
-    const int tb = max_tiles * ((maxT + CONF_TB - 1) / CONF_TB);
+    const dim3 cg(maxT, (maxS + 255) / 256, U);
     const int ib = grid_for((int64_t)maxS * maxT, 256);
     if (d == 2) {
-        if (any_plain && items > 0)
-            hipLaunchKernelGGL(k_conflicts_multi<2>, dim3(pb, U), dim3(256), 0, st, dA);
-        if (any_tiled && tb > 0)
-            hipLaunchKernelGGL(k_conflicts_tiled_multi<2>, dim3(tb, U), dim3(256), 0, st, dA);
+        if (maxS > 1)
+            hipLaunchKernelGGL(k_conflicts_multi<2>, cg, dim3(256), 0, st, dA);
         hipLaunchKernelGGL(k_greedy_multi, dim3(U), dim3(64), 0, st, dA);
         hipLaunchKernelGGL(k_insert_multi<2>, dim3(ib, U), dim3(256), 0, st, dA);
         hipLaunchKernelGGL(k_halo_multi<2>, dim3(ib, U), dim3(256), 0, st, dA);
     } else {
-        if (any_plain && items > 0)
-            hipLaunchKernelGGL(k_conflicts_multi<3>, dim3(pb, U), dim3(256), 0, st, dA);
-        if (any_tiled && tb > 0)
-            hipLaunchKernelGGL(k_conflicts_tiled_multi<3>, dim3(tb, U), dim3(256), 0, st, dA);
+        if (maxS > 1)
+            hipLaunchKernelGGL(k_conflicts_multi<3>, cg, dim3(256), 0, st, dA);
         hipLaunchKernelGGL(k_greedy_multi, dim3(U), dim3(64), 0, st, dA);
         hipLaunchKernelGGL(k_insert_multi<3>, dim3(ib, U), dim3(256), 0, st, dA);
         hipLaunchKernelGGL(k_halo_multi<3>, dim3(ib, U), dim3(256), 0, st, dA);
@@ -2239,8 +2091,7 @@ int32_t ensure_hash(Pack* h, int64_t points) {
 // tiled kernel, small ones the pair-per-thread kernel -- the bits are the same either way.
 // Conflict bits of a chunk's first s survivors (at most *dS): the cell-hash kernel, one
 // workgroup per timestep.  (The pair kernels above serve the batched engine.)
-void launch_conflicts(Pack* h, int s, const int* dS, int64_t expect) {
-    (void)expect;
+void launch_conflicts(Pack* h, int s, const int* dS) {
     if ((int64_t)s * (s - 1) / 2 <= 0)
         return;
     const dim3 grid(h->T, (s + 255) / 256);
@@ -2275,7 +2126,7 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         if (rc)
             return rc;
         // with the count still on the device, the last filter's survivors are the estimate
-        launch_conflicts(h, s, cs, dev ? h->last_S : s);
+        launch_conflicts(h, s, cs);
         hipLaunchKernelGGL(k_greedy, dim3(1), dim3(64), 0, h->stream, h->d_mask, s, cs, h->d_acc_list,
                            h->d_nacc);
         const int limit = h->max_n > 0 ? (int)std::min<int64_t>(h->max_n - h->n, INT32_MAX) : INT32_MAX;
