@@ -1241,10 +1241,11 @@ __device__ __forceinline__ bool cells_adjacent(int u, int v, int T) {
 // canonical FP64 contact predicate holds, what a cell-table lookup would find.  Workgroup
 // (t, g) hashes the whole chunk by packed cell at timestep t in LDS (open addressing, 2048
 // slots for at most 1024 survivors, a chain per cell, positions beside), then survivor
-// 256 g + lane probes its 3^D neighbouring cells (wrapped) and tests the earlier survivors
+// CONF_WG g + thread probes its 3^D neighbouring cells (wrapped) and tests the earlier survivors
 // found there: O(S 3^D) per timestep instead of O(S^2) pair tests (a pair-per-thread kernel
 // and a 64x64-tile kernel over all pairs took 9-45 us where this takes a few).
 constexpr int CONF_HSLOTS = 2048;
+constexpr int CONF_WG = 1024;  // threads per workgroup: one survivor each, one group per timestep
 
 __device__ __forceinline__ int conf_hslot(int key) {
     return (int)(((uint32_t)key * 0x9E3779B1u) >> (32 - 11));
@@ -1257,16 +1258,16 @@ __device__ __forceinline__ void conflicts_cells_body(const DevModel& m, const do
     __shared__ int hhead[CONF_HSLOTS];
     __shared__ short nxt[UG_S_MAX];
     __shared__ double sp[UG_S_MAX * D];
-    // workgroup (t, g): the whole chunk hashed at timestep t, probes for survivors 256 g ..
-    const int T = m.T, s0 = g * 256;
+    // workgroup (t, g): the whole chunk hashed at timestep t, probes for survivors CONF_WG g ..
+    const int T = m.T, s0 = g * CONF_WG;
     if (t >= T || s0 >= S)
         return;  // uniform over the workgroup
-    for (int k = threadIdx.x; k < CONF_HSLOTS; k += 256) {
+    for (int k = threadIdx.x; k < CONF_HSLOTS; k += CONF_WG) {
         hkey[k] = -1;
         hhead[k] = -1;
     }
     __syncthreads();
-    for (int s = threadIdx.x; s < S; s += 256) {
+    for (int s = threadIdx.x; s < S; s += CONF_WG) {
         const int key = cells[(size_t)s * T + t];
 #pragma unroll
         for (int a = 0; a < D; a++)
@@ -1316,7 +1317,7 @@ __device__ __forceinline__ void conflicts_cells_body(const DevModel& m, const do
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_conflicts_cells(DevModel m, const double* pos, const int* cells, int S,
+__global__ __launch_bounds__(CONF_WG) void k_conflicts_cells(DevModel m, const double* pos, const int* cells, int S,
                                                          const int* dS, uint64_t* mask) {
     if (dS)
         S = min(S, *dS);
@@ -1601,7 +1602,7 @@ __device__ __forceinline__ MResolve resolve_rec(const MResolve* A, int u) {
     return a;
 }
 template <int D>
-__global__ __launch_bounds__(256) void k_conflicts_multi(const MResolve* A) {
+__global__ __launch_bounds__(CONF_WG) void k_conflicts_multi(const MResolve* A) {
     const MResolve a = resolve_rec(A, blockIdx.z);
     const int S = resolve_count(a);
     if (S > 1)
@@ -1759,17 +1760,17 @@ int32_t launch_compact_multi(hipStream_t st, const MCompact* dA, int U, int64_t 
 int32_t launch_resolve_multi(hipStream_t st, int d, const MResolve* dA, int U, int maxS, int maxT) {
     if (U <= 0)
         return UG_OK;
-    const dim3 cg(maxT, (maxS + 255) / 256, U);
+    const dim3 cg(maxT, (maxS + CONF_WG - 1) / CONF_WG, U);
     const int ib = grid_for((int64_t)maxS * maxT, 256);
     if (d == 2) {
         if (maxS > 1)
-            hipLaunchKernelGGL(k_conflicts_multi<2>, cg, dim3(256), 0, st, dA);
+            hipLaunchKernelGGL(k_conflicts_multi<2>, cg, dim3(CONF_WG), 0, st, dA);
         hipLaunchKernelGGL(k_greedy_multi, dim3(U), dim3(64), 0, st, dA);
         hipLaunchKernelGGL(k_insert_multi<2>, dim3(ib, U), dim3(256), 0, st, dA);
         hipLaunchKernelGGL(k_halo_multi<2>, dim3(ib, U), dim3(256), 0, st, dA);
     } else {
         if (maxS > 1)
-            hipLaunchKernelGGL(k_conflicts_multi<3>, cg, dim3(256), 0, st, dA);
+            hipLaunchKernelGGL(k_conflicts_multi<3>, cg, dim3(CONF_WG), 0, st, dA);
         hipLaunchKernelGGL(k_greedy_multi, dim3(U), dim3(64), 0, st, dA);
         hipLaunchKernelGGL(k_insert_multi<3>, dim3(ib, U), dim3(256), 0, st, dA);
         hipLaunchKernelGGL(k_halo_multi<3>, dim3(ib, U), dim3(256), 0, st, dA);
@@ -2094,13 +2095,13 @@ int32_t ensure_hash(Pack* h, int64_t points) {
 void launch_conflicts(Pack* h, int s, const int* dS) {
     if ((int64_t)s * (s - 1) / 2 <= 0)
         return;
-    const dim3 grid(h->T, (s + 255) / 256);
+    const dim3 grid(h->T, (s + CONF_WG - 1) / CONF_WG);
     if (h->d == 2)
-        hipLaunchKernelGGL(k_conflicts_cells<2>, grid, dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell, s, dS,
-                           h->d_mask);
+        hipLaunchKernelGGL(k_conflicts_cells<2>, grid, dim3(CONF_WG), 0, h->stream, h->dm, h->d_spos, h->d_scell, s,
+                           dS, h->d_mask);
     else
-        hipLaunchKernelGGL(k_conflicts_cells<3>, grid, dim3(256), 0, h->stream, h->dm, h->d_spos, h->d_scell, s, dS,
-                           h->d_mask);
+        hipLaunchKernelGGL(k_conflicts_cells<3>, grid, dim3(CONF_WG), 0, h->stream, h->dm, h->d_spos, h->d_scell, s,
+                           dS, h->d_mask);
 }
 
 // Resolve the survivors list[0..S) (already clear of the cell table) in index order and
