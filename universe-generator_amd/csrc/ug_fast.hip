@@ -381,9 +381,26 @@ __device__ __forceinline__ void prefilter_item(const FastArgs& f, int64_t tile, 
                     for (int rr = 0; rr < 3; rr++)
                         cells |= ((uint32_t)(raw[rr] >> sh) & 7u) << (3 * rr);
                 } else {
+                    // the nine rows' words loaded before any is used; the next word plane only
+                    // where the window crosses into it (the same for all nine rows)
+                    const int wq = c[2] >> 5, sh = c[2] & 31;
+                    uint32_t lo[ROWS];
 #pragma unroll
                     for (int rr = 0; rr < ROWS; rr++)
-                        cells |= occ_bits3<D>(m, t, gx_of(rr), gy_of(rr), c[D - 1]) << (3 * rr);
+                        lo[rr] = m.occ[occ_index<3>(m, t, gx_of(rr), gy_of(rr), wq)];
+                    if (sh > 29) {
+                        uint32_t hi[ROWS];
+#pragma unroll
+                        for (int rr = 0; rr < ROWS; rr++)
+                            hi[rr] = m.occ[occ_index<3>(m, t, gx_of(rr), gy_of(rr), wq + 1)];
+#pragma unroll
+                        for (int rr = 0; rr < ROWS; rr++)
+                            cells |= ((uint32_t)((((uint64_t)hi[rr] << 32) | lo[rr]) >> sh) & 7u) << (3 * rr);
+                    } else {
+#pragma unroll
+                        for (int rr = 0; rr < ROWS; rr++)
+                            cells |= ((lo[rr] >> sh) & 7u) << (3 * rr);
+                    }
                 }
                 while (cells && !cert) {
                     constexpr int NG = 2;

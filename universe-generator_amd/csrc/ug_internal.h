@@ -94,10 +94,14 @@ __host__ __device__ inline int wrap_cell(int c, int d, int T) {
 // (The layout follows the dimension: D = 2 rows, D = 3 tiles.)
 template <int D>
 __host__ __device__ __forceinline__ int64_t occ_index(const DevModel& m, int t, int x, int y, int wq) {
-    if constexpr (D == 2)  // below 2^32 (T <= 512): 32-bit math
+    // below 2^32 words for T <= 512 (3+1 T=512: 2.3G), so 32-bit unsigned math
+    if constexpr (D == 2)
         return (int64_t)(((uint32_t)t * (uint32_t)m.T + (uint32_t)x) * (uint32_t)m.occw + (uint32_t)wq);
-    return ((((int64_t)t * m.occ_nwq + wq) * m.occ_nxb + (x >> 2)) * m.occ_nyb + (y >> 3)) * 32 + (x & 3) * 8 +
-           (y & 7);
+    return (int64_t)((((((uint32_t)t * (uint32_t)m.occ_nwq + (uint32_t)wq) * (uint32_t)m.occ_nxb + (uint32_t)(x >> 2)) *
+                          (uint32_t)m.occ_nyb +
+                      (uint32_t)(y >> 3)) *
+                         32u +
+                     (uint32_t)((x & 3) * 8 + (y & 7))));
 }
 
 // Occupancy of the padded cells c..c+2 (cells c-1..c+1 of the unpadded row, wrapped by the
