@@ -417,6 +417,74 @@ __device__ __forceinline__ bool lookup_round(const DevModel& m, const double* po
     return __ballot(h) != 0;
 }
 
+// One position's whole neighbourhood by one lane (the bulk phase, one lane per timestep): the
+// occupancy words of all its rows first, then the heads of the occupied cells NG at a time,
+// their chains walked together -- about three dependent memory steps for 64 timesteps at
+// once, where lookup_round spreads one timestep's rows over lanes and needs a round per
+// 64 / 3^(D-1) timesteps.  The same cells as row_heads3; true on a contact.
+template <int D>
+__device__ __forceinline__ bool lookup_lane(const DevModel& m, const double* x, int t, unsigned long long& npts) {
+    constexpr int R = D == 2 ? 3 : 9;
+    const int T = m.T;
+    int c[D];
+#pragma unroll
+    for (int a = 0; a < D; a++)
+        c[a] = cell_of(x[a], m.halfT, T);
+    auto gx_of = [&](int rr) { return wrap_cell(c[0], (D == 2 ? rr : rr / 3) - 1, T); };
+    auto gy_of = [&](int rr) { return D == 2 ? 0 : wrap_cell(c[1], rr % 3 - 1, T); };
+    uint32_t cells = 0;
+#pragma unroll
+    for (int rr = 0; rr < R; rr++)
+        cells |= occ_bits3<D>(m, t, gx_of(rr), gy_of(rr), c[D - 1]) << (3 * rr);
+    bool h = false;
+    constexpr int NG = 4;
+    while (cells && !h) {
+        int node[NG];
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            node[g] = -1;
+            if (cells) {
+                const int b = __builtin_ctz(cells);
+                cells &= cells - 1u;
+                const int rr = b / 3, k = b % 3;
+                const int64_t rid =
+                    D == 2 ? (int64_t)t * T + gx_of(rr) : ((int64_t)t * T + gx_of(rr)) * T + gy_of(rr);
+                node[g] = m.sparse ? hash_head(m, cell_hkey(rid, wrap_cell(c[D - 1], k - 1, T), T))
+                                   : m.tab[rid * m.rowlen + c[D - 1] + k];
+            }
+        }
+        bool any = false;
+#pragma unroll
+        for (int g = 0; g < NG; g++)
+            any = any || node[g] >= 0;
+        while (!h && any) {
+            double P[NG][D];
+            int nx[NG];
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                nx[g] = -1;
+                if (node[g] >= 0) {
+                    const size_t q = (size_t)node[g] * T + t;
+#pragma unroll
+                    for (int a = 0; a < D; a++)
+                        P[g][a] = m.pos64[q * D + a];
+                    nx[g] = __float_as_int(m.pos32[q].w);
+                }
+            }
+            any = false;
+#pragma unroll
+            for (int g = 0; g < NG; g++)
+                if (node[g] >= 0) {
+                    h = h || contact<D>(m, x, P[g]);
+                    npts++;
+                    node[g] = nx[g];
+                    any = any || node[g] >= 0;
+                }
+        }
+    }
+    return h;
+}
+
 // One wave per candidate.  Lanes generate the coefficients over slot pairs.  The first
 // `head` endpoint-first timesteps (where most candidates meet a contact) are evaluated across
 // the wave (canonical lane tree) in chunks of 2, 4, 8, ... (at most 64 / 3^(D-1)), each
@@ -525,6 +593,7 @@ __device__ __forceinline__ void cand64_item(const DevModel& m, int64_t i, uint64
             for (int ps = o0; ps < T && !hit; ps += 64) {
                 const int oi = ps + lane;
                 const int pend = min(64, T - ps);
+                bool hl = false;
                 if (oi < T) {
                     const int t = FULL ? oi : m.order[oi];
                     double X[D];
@@ -544,21 +613,13 @@ __device__ __forceinline__ void cand64_item(const DevModel& m, int64_t i, uint64
                         if (cells)
                             cells[(size_t)i * T + t] = v;
                     } else {
-                        tsl[lane] = t;
-#pragma unroll
-                        for (int a = 0; a < D; a++)
-                            pos[lane * D + a] = X[a];
+                        hl = lookup_lane<D>(m, X, t, npts);
                     }
                 }
                 if (FULL)
                     continue;
-                __syncthreads();
-                for (int r0 = 0; r0 < pend && !hit; r0 += PER_ROUND) {
-                    const int G = min(PER_ROUND, pend - r0);
-                    hit = lookup_round<D>(m, pos + r0 * D, tsl + r0, G, lane, npts);
-                    nvis += (unsigned long long)G;
-                }
-                __syncthreads();
+                hit = __ballot(hl) != 0;
+                nvis += (unsigned long long)pend;
             }
         }
         if (!FULL && lane == 0)
