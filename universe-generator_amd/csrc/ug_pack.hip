@@ -30,6 +30,11 @@
 #include "ug_arith.h"
 #include "ug_internal.h"
 
+// Second split-BF16 stage from this many N blocks of 32 ordered timesteps (T > 160).
+#ifndef UG_STAGE2_MIN_BLOCKS
+#define UG_STAGE2_MIN_BLOCKS 6
+#endif
+
 using namespace ug;
 
 // =====================================================================================
@@ -2041,7 +2046,7 @@ int32_t filter_candidates(Pack* h, uint64_t base, int64_t count, const int64_t* 
     const int64_t* rlist = h->d_list3;
     const uint32_t* rskip = h->d_skip3;
     int rslot = 0;
-    if (!rc && fast_blocks(h) >= 6) {
+    if (!rc && fast_blocks(h) >= UG_STAGE2_MIN_BLOCKS) {
         // stage 2 (T > 160): the remaining ordered timesteps, split-BF16 as well, for the few
         // that stage 1 could not reject -> d_list4 (count in d_count[2]); the FP64 recheck
         // then sees only candidates no timestep certainly rejects.  Measured on one MI355X:
