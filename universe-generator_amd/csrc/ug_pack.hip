@@ -494,8 +494,9 @@ __device__ __forceinline__ bool lookup_lane(const DevModel& m, const double* x, 
 // `head` endpoint-first timesteps (where most candidates meet a contact) are evaluated across
 // the wave (canonical lane tree) in chunks of 2, 4, 8, ... (at most 64 / 3^(D-1)), each
 // chunk looked up in one round; a candidate still clear after them has every remaining
-// timestep evaluated in parallel, one lane per timestep, and looked up 64 / 3^(D-1)
-// timesteps per round (cuda/braid_cuda.cu:418-462).  Bulk tests of fresh candidates use
+// timestep evaluated in parallel, one lane per timestep, each lane looking up its own
+// timestep's neighbourhood (lookup_lane; cuda/braid_cuda.cu:418-462).  Bulk tests of fresh
+// candidates use
 // head = T; lists of likely survivors use a short head.
 // FULL = false: flags[i] = 1 iff candidate i survives the cell table.
 // FULL = true: out[i][T][D] = canonical trajectory (no lookup).
@@ -2151,13 +2152,9 @@ int32_t ensure_hash(Pack* h, int64_t points) {
     return rc;
 }
 
-// Conflict bits among the chunk's s survivors (trajectories in d_spos, mask cleared); with
-// dS the count is min(s, *dS), known only on the device (grid-stride launches).
-// The chunk's packed cells (d_scell) and cleared mask rows come from its trajectory launch.
-// expect: the survivors the host expects (s itself when known); large chunks take the
-// tiled kernel, small ones the pair-per-thread kernel -- the bits are the same either way.
-// Conflict bits of a chunk's first s survivors (at most *dS): the cell-hash kernel, one
-// workgroup per timestep.  (The pair kernels above serve the batched engine.)
+// Conflict bits among the chunk's s survivors (trajectories in d_spos, packed cells in
+// d_scell, mask rows cleared: all from its trajectory launch); with dS the count is
+// min(s, *dS), known only on the device.  The cell-hash kernel, one workgroup per timestep.
 void launch_conflicts(Pack* h, int s, const int* dS) {
     if ((int64_t)s * (s - 1) / 2 <= 0)
         return;
