@@ -221,7 +221,7 @@ def run_ensemble(args, ugpack, torch, forms, prec_main) -> int:
     n, cands, threads = w["universes"], w["max_attempts"], args.threads
     sweep = None
     plan = {"engine": args.engine, "threads": threads, "lanes": args.lanes, "window": args.window,
-            "sub_target": 1024, "sub_max": 1 << 20}
+            "sub_target": tune.DEFAULT_SUB_TARGET, "sub_max": 1 << 20}
     if args.engine == "tuned":
         # the engine, its universes in flight and the sub-batch sizing are tuned on this GPU:
         # a 128-universe sample of the same mix (seeds 1..128) at 2*10^6 candidates each,

@@ -77,7 +77,8 @@ typedef struct ug_pack_config {
                             (timestep, cell) buckets) instead of the dense T^(d+1) cell table
                             (cuda/braid_cuda3d.cu:82-91,1262-1353); same accepted lists */
     int32_t sub_target;  /* sub-batch sizing (budget and fixed-window rules): survivors aimed
-                            at per sub-batch, 0 = 1024; speed only, never results */
+                            at per sub-batch, 0 = 2048 in 3+1 (one resolve chunk), 1024 in
+                            2+1; speed only, never results */
     int32_t sub_max;     /* largest sub-batch, a power of two <= 2^20 (0 = 2^20) */
 } ug_pack_config;
 

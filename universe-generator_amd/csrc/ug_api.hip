@@ -208,8 +208,11 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
         cfg.stop_window = 1 << 22;
     if (cfg.stop_kind == UG_STOP_FIXED_WINDOW && (cfg.stop_batch & (cfg.stop_batch - 1)))
         return fail(UG_EINVAL, "error: stop_batch must be a power of two");
+    // survivors aimed at per sub-batch: one full resolve chunk in 3+1 (fewer chunks of fixed
+    // launch costs: 3+1 T=128 packs in 57.3 ms against 60.2 ms at half a chunk), half a chunk
+    // in 2+1 (its early sub-batches then stay small: 2+1 T=64 1.715 against 1.743 ms)
     if (cfg.sub_target <= 0)
-        cfg.sub_target = 1024;
+        cfg.sub_target = cfg.dim == 3 ? UG_S_MAX : UG_S_MAX / 2;
     if (cfg.sub_max <= 0)
         cfg.sub_max = UG_SUB_MAX;
     if (cfg.sub_max < 4096 || cfg.sub_max > UG_SUB_MAX || (cfg.sub_max & (cfg.sub_max - 1)))
@@ -431,7 +434,7 @@ void ug::pack_note_rate(Pack* h, int64_t at, int64_t count, int64_t S) {
     }
 }
 
-// Aims at ~sub_target survivors (1024 by default: one in-batch resolve chunk) and at most
+// Aims at ~sub_target survivors (UG_S_MAX by default: one in-batch resolve chunk) and at most
 // ~16k candidates for the FP64 recheck per sub-batch (the reference adapts toward ~256
 // survivors, cuda/braid_cuda.cu:920-924).  The survival rate of the next sub-batch is the last
 // observed one scaled by (observed at) / (next start): while the table fills, the rate of a

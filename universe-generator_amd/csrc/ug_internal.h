@@ -14,7 +14,7 @@
 
 #define UG_MAX_T 512      // basis pairs per lane per axis <= 4 (T <= 512)
 #define MAXPL 4
-#define UG_S_MAX 1024     // survivors resolved per in-batch chunk
+#define UG_S_MAX 2048     // survivors resolved per in-batch chunk
 #define UG_MASK_WORDS (UG_S_MAX / 64)
 #define UG_SUB_MAX (1 << 20)  // largest candidate sub-batch
 // Measurement counters: a wave adds into slot blockIdx % UG_CTR_SLOTS (UG_CTR_STRIDE words

@@ -1311,20 +1311,26 @@ __device__ __forceinline__ bool cells_adjacent(int u, int v, int T) {
 // CONF_WG g + thread probes its 3^D neighbouring cells (wrapped) and tests the earlier survivors
 // found there: O(S 3^D) per timestep instead of O(S^2) pair tests (a pair-per-thread kernel
 // and a 64x64-tile kernel over all pairs took 9-45 us where this takes a few).
-constexpr int CONF_HSLOTS = 2048;
+constexpr int CONF_HBITS = 12;
+constexpr int CONF_HSLOTS = 1 << CONF_HBITS;  // at least twice the survivors of a chunk
+static_assert(CONF_HSLOTS >= 2 * UG_S_MAX, "conflict hash load above one half");
+constexpr size_t conf_lds_bytes(int D) { return (size_t)CONF_HSLOTS * 8 + (size_t)UG_S_MAX * (2 + 8 * D); }
+static_assert((CONF_HSLOTS * 8 + UG_S_MAX * 2) % 8 == 0, "positions after the chains stay aligned");
 constexpr int CONF_WG = 1024;  // threads per workgroup: one survivor each, one group per timestep
 
 __device__ __forceinline__ int conf_hslot(int key) {
-    return (int)(((uint32_t)key * 0x9E3779B1u) >> (32 - 11));
+    return (int)(((uint32_t)key * 0x9E3779B1u) >> (32 - CONF_HBITS));
 }
 
 template <int D>
 __device__ __forceinline__ void conflicts_cells_body(const DevModel& m, const double* pos, const int* cells, int S,
                                                      uint64_t* mask, int t, int g) {
-    __shared__ int hkey[CONF_HSLOTS];
-    __shared__ int hhead[CONF_HSLOTS];
-    __shared__ short nxt[UG_S_MAX];
-    __shared__ double sp[UG_S_MAX * D];
+    // (dynamic LDS, conf_lds_bytes: the hash, the chains, the positions)
+    extern __shared__ __align__(16) int dyn_lds[];
+    int* hkey = dyn_lds;
+    int* hhead = hkey + CONF_HSLOTS;
+    short* nxt = (short*)(hhead + CONF_HSLOTS);
+    double* sp = (double*)(nxt + UG_S_MAX);
     // workgroup (t, g): the whole chunk hashed at timestep t, probes for survivors CONF_WG g ..
     const int T = m.T, s0 = g * CONF_WG;
     if (t >= T || s0 >= S)
@@ -1824,20 +1830,36 @@ int32_t launch_compact_multi(hipStream_t st, const MCompact* dA, int U, int64_t 
     return hip_check(hipGetLastError(), "batched compaction");
 }
 
+// The conflict kernels' LDS (above the 64 KB default for D = 3), allowed once per process.
+void conf_lds_attributes() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute((const void*)k_conflicts_cells<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)conf_lds_bytes(2));
+        (void)hipFuncSetAttribute((const void*)k_conflicts_cells<3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)conf_lds_bytes(3));
+        (void)hipFuncSetAttribute((const void*)k_conflicts_multi<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)conf_lds_bytes(2));
+        (void)hipFuncSetAttribute((const void*)k_conflicts_multi<3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)conf_lds_bytes(3));
+    });
+}
+
 int32_t launch_resolve_multi(hipStream_t st, int d, const MResolve* dA, int U, int maxS, int maxT) {
     if (U <= 0)
         return UG_OK;
+    conf_lds_attributes();
     const dim3 cg(maxT, (maxS + CONF_WG - 1) / CONF_WG, U);
     const int ib = grid_for((int64_t)maxS * maxT, 256);
     if (d == 2) {
         if (maxS > 1)
-            hipLaunchKernelGGL(k_conflicts_multi<2>, cg, dim3(CONF_WG), 0, st, dA);
+            hipLaunchKernelGGL(k_conflicts_multi<2>, cg, dim3(CONF_WG), conf_lds_bytes(2), st, dA);
         hipLaunchKernelGGL(k_greedy_multi, dim3(U), dim3(64), 0, st, dA);
         hipLaunchKernelGGL(k_insert_multi<2>, dim3(ib, U), dim3(256), 0, st, dA);
         hipLaunchKernelGGL(k_halo_multi<2>, dim3(ib, U), dim3(256), 0, st, dA);
     } else {
         if (maxS > 1)
-            hipLaunchKernelGGL(k_conflicts_multi<3>, cg, dim3(CONF_WG), 0, st, dA);
+            hipLaunchKernelGGL(k_conflicts_multi<3>, cg, dim3(CONF_WG), conf_lds_bytes(3), st, dA);
         hipLaunchKernelGGL(k_greedy_multi, dim3(U), dim3(64), 0, st, dA);
         hipLaunchKernelGGL(k_insert_multi<3>, dim3(ib, U), dim3(256), 0, st, dA);
         hipLaunchKernelGGL(k_halo_multi<3>, dim3(ib, U), dim3(256), 0, st, dA);
@@ -2158,13 +2180,14 @@ int32_t ensure_hash(Pack* h, int64_t points) {
 void launch_conflicts(Pack* h, int s, const int* dS) {
     if ((int64_t)s * (s - 1) / 2 <= 0)
         return;
+    conf_lds_attributes();
     const dim3 grid(h->T, (s + CONF_WG - 1) / CONF_WG);
     if (h->d == 2)
-        hipLaunchKernelGGL(k_conflicts_cells<2>, grid, dim3(CONF_WG), 0, h->stream, h->dm, h->d_spos, h->d_scell, s,
-                           dS, h->d_mask);
+        hipLaunchKernelGGL(k_conflicts_cells<2>, grid, dim3(CONF_WG), conf_lds_bytes(2), h->stream, h->dm, h->d_spos,
+                           h->d_scell, s, dS, h->d_mask);
     else
-        hipLaunchKernelGGL(k_conflicts_cells<3>, grid, dim3(CONF_WG), 0, h->stream, h->dm, h->d_spos, h->d_scell, s,
-                           dS, h->d_mask);
+        hipLaunchKernelGGL(k_conflicts_cells<3>, grid, dim3(CONF_WG), conf_lds_bytes(3), h->stream, h->dm, h->d_spos,
+                           h->d_scell, s, dS, h->d_mask);
 }
 
 // Resolve the survivors list[0..S) (already clear of the cell table) in index order and
