@@ -208,11 +208,12 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
         cfg.stop_window = 1 << 22;
     if (cfg.stop_kind == UG_STOP_FIXED_WINDOW && (cfg.stop_batch & (cfg.stop_batch - 1)))
         return fail(UG_EINVAL, "error: stop_batch must be a power of two");
-    // survivors aimed at per sub-batch: one full resolve chunk in 3+1 (fewer chunks of fixed
-    // launch costs: 3+1 T=128 packs in 57.3 ms against 60.2 ms at half a chunk), half a chunk
-    // in 2+1 (its early sub-batches then stay small: 2+1 T=64 1.715 against 1.743 ms)
+    // survivors aimed at per sub-batch: one resolve chunk -- 2048 in 3+1 (fewer chunks of fixed
+    // launch costs: 3+1 T=128 packs in 57.3 ms against 60.2 ms with 1024), 1024 in 2+1 (its
+    // early sub-batches then stay small: 2+1 T=64 1.715 against 1.743 ms with 2048)
+    const int chunk_max = cfg.dim == 3 ? UG_S_MAX : UG_S_MAX / 2;
     if (cfg.sub_target <= 0)
-        cfg.sub_target = cfg.dim == 3 ? UG_S_MAX : UG_S_MAX / 2;
+        cfg.sub_target = chunk_max;
     if (cfg.sub_max <= 0)
         cfg.sub_max = UG_SUB_MAX;
     if (cfg.sub_max < 4096 || cfg.sub_max > UG_SUB_MAX || (cfg.sub_max & (cfg.sub_max - 1)))
@@ -229,6 +230,7 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
     h->nw = cfg.terms - 1;
     h->npairs = (h->nw + 1) / 2;
     h->modmax = modmax;
+    h->chunk_max = chunk_max;
     h->full = h->nw == modmax;
     h->max_n = cfg.max_n;
     const int T = h->T;
@@ -689,7 +691,7 @@ extern "C" int32_t ug_pack_run(ug_pack* hp, int64_t max_candidates, ug_pack_stat
         double expect_S = (double)h->last_S;
         if (h->rate_at > 0)
             expect_S = h->rate_p * (double)h->rate_at / (double)std::max<int64_t>(h->rate_at, (int64_t)base) * (double)sub;
-        const bool steady = h->sub >= sub_cap && expect_S <= UG_S_MAX / 4;
+        const bool steady = h->sub >= sub_cap && expect_S <= h->chunk_max / 4;
         ahead.count = steady ? size_at(h->attempts + sub) : 0;
         ahead.out = cur_list == h->d_list ? h->d_listb : h->d_list;
         ahead.dcount = h->d_spec_count;

@@ -162,7 +162,7 @@ int32_t Engine::stage_filter(std::vector<Uni*>& act) {
         u->slot = i;
         u->all = u->fresh && h->n == 0;
         const int64_t cnt = u->fresh ? u->sub : u->npend;
-        u->bound = std::min<int64_t>(cnt, UG_S_MAX);
+        u->bound = std::min<int64_t>(cnt, h->chunk_max);
         if (u->all)
             all.push_back(u);
         else if (h->cfg.precision == UG_PREC_SPLIT_BF16)
@@ -375,7 +375,7 @@ int32_t Engine::step(std::vector<Uni*>& act) {
     int32_t rc = UG_OK;
     for (Uni* u : act) {
         const int64_t cnt = u->fresh ? u->sub : u->npend;
-        rc = ensure_paths(u->h, u->h->n + std::min<int64_t>(cnt, UG_S_MAX));
+        rc = ensure_paths(u->h, u->h->n + std::min<int64_t>(cnt, u->h->chunk_max));
         if (rc)
             return rc;
     }
@@ -397,7 +397,7 @@ int32_t Engine::step(std::vector<Uni*>& act) {
         const int* rep = (const int*)(h_rep + REP_BYTES * (size_t)u->slot);
         const int na = rep[0];
         u->S = rep[1];
-        u->chunk = (int)std::min<int64_t>(u->S, UG_S_MAX);
+        u->chunk = (int)std::min<int64_t>(u->S, h->chunk_max);
         const int64_t rech = h->cfg.precision == UG_PREC_SPLIT_BF16 && !u->all ? rep[4] : 0;
         h->rechecked += rech;
         u->rech_sub += rech;

@@ -344,6 +344,9 @@ struct Pack {
     int* d_rscount = nullptr;   // survivors of a round against its round-start packing
     int sub_done = 0;  // UG_SUB_* once the subpath phase stopped
     int64_t last_S = UG_S_MAX;  // survivors of the last filter whose count stayed on the device
+    // survivors per resolve chunk: UG_S_MAX in 3+1, half of it in 2+1 (whose early chunks of
+    // candidates from an empty table are cheaper small)
+    int chunk_max = UG_S_MAX;
                                 // (a fresh table lets a whole first sub-batch survive)
     // sub_batch: the round size, adapted as cuda/braid_cuda.cu:952,1000-1003;
     // sub_nlimit: rows stored before the current round started

@@ -2198,10 +2198,10 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
     int32_t rc;
     *acc_total = 0;
     // dS: the survivor count is still on the device (filter without a host round trip); the
-    // first chunk runs on min(*dS, UG_S_MAX) and the count comes back with its results
+    // first chunk runs on min(*dS, chunk_max) and the count comes back with its results
     bool dev = dS != nullptr;
     while (dev || S > 0) {
-        const int s = dev ? UG_S_MAX : (int)std::min<int64_t>(S, UG_S_MAX);
+        const int s = dev ? h->chunk_max : (int)std::min<int64_t>(S, h->chunk_max);
         const int* cs = dev ? dS : nullptr;
         rc = ensure_paths(h, h->n + s);
         if (!rc)
@@ -2275,7 +2275,7 @@ int32_t resolve_and_insert(Pack* h, int64_t* list, int64_t S, int64_t* acc_total
         *acc_total += na;
         if (h->max_n > 0 && h->n >= h->max_n)
             return UG_OK;
-        if (S > UG_S_MAX) {
+        if (S > h->chunk_max) {
             // retest the rest against the updated table (a filter run ahead saw the table
             // before these inserts: it is dropped)
             if (spec)
