@@ -1426,11 +1426,11 @@ __device__ __forceinline__ void report_block(const int* nacc, const unsigned lon
 // Greedy in index order over the conflict bits (one wave): survivor i is accepted iff it
 // conflicts with no earlier accepted survivor -- serial RSA restricted to the chunk.  Lane
 // i of block blk needs words 0..blk of its mask row; they are loaded together, and the next
-// block's while this one is decided, so the pass pays about one load latency in all.
+// two blocks' while this one is decided, so the pass pays about one load latency in all.
 __device__ __forceinline__ void greedy_body(const uint64_t* mask, int S, int* acc_list, int* nacc) {
     __shared__ uint64_t accw[UG_MASK_WORDS];
     const int lane = threadIdx.x;
-    uint64_t cur[UG_MASK_WORDS], nxt[UG_MASK_WORDS];
+    uint64_t cur[UG_MASK_WORDS], nxt[UG_MASK_WORDS], nx2[UG_MASK_WORDS];
     auto load = [&](uint64_t* w, int blk) {
         const int i = blk * 64 + lane;
 #pragma unroll
@@ -1438,10 +1438,12 @@ __device__ __forceinline__ void greedy_body(const uint64_t* mask, int S, int* ac
             w[wd] = (i < S && wd <= blk) ? mask[(size_t)i * UG_MASK_WORDS + wd] : 0ull;
     };
     load(cur, 0);
+    if (64 < S)
+        load(nxt, 1);
     int na = 0;
     for (int blk = 0; blk * 64 < S; blk++) {
-        if ((blk + 1) * 64 < S)
-            load(nxt, blk + 1);
+        if ((blk + 2) * 64 < S)
+            load(nx2, blk + 2);
         const int i = blk * 64 + lane;
         const bool valid = i < S;
         uint64_t killed = 0ull, inrow = 0ull;
@@ -1476,8 +1478,10 @@ __device__ __forceinline__ void greedy_body(const uint64_t* mask, int S, int* ac
         }
         na += __popcll(accb);
 #pragma unroll
-        for (int wd = 0; wd < UG_MASK_WORDS; wd++)
+        for (int wd = 0; wd < UG_MASK_WORDS; wd++) {
             cur[wd] = nxt[wd];
+            nxt[wd] = nx2[wd];
+        }
     }
     if (lane == 0)
         *nacc = na;
