@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define UG_ABI_VERSION 4
+#define UG_ABI_VERSION 5
 
 enum {
     UG_OK = 0,
@@ -41,6 +41,15 @@ enum {
 enum {
     UG_PREC_FP64 = 0,      /* plain all-FP64 GPU form */
     UG_PREC_SPLIT_BF16 = 1 /* split-BF16 MFMA evaluation + FP64 recheck near the radius */
+};
+
+/* Candidate stream (ug_pack_config.rng_kind, ug_knot_config.rng_kind). */
+enum {
+    UG_RNG_PHILOX = 0,      /* Philox4x32-10 counters per (candidate, stage, axis, slot pair) */
+    UG_RNG_XOSHIRO_REF = 1  /* cross-check mode: candidate idx draws the reference's own serial
+                               xoshiro256** stream, seeded as cuda/braid_knot.cu:207, in the
+                               draw order of its proposal (cuda/braid_knot.cu:112-166); FP64
+                               precision only (UG_EINVAL with UG_PREC_SPLIT_BF16) */
 };
 
 /* Stop rules. */
@@ -80,6 +89,7 @@ typedef struct ug_pack_config {
                             at per sub-batch, 0 = 2048 in 3+1 (one resolve chunk), 1024 in
                             2+1; speed only, never results */
     int32_t sub_max;     /* largest sub-batch, a power of two <= 2^20 (0 = 2^20) */
+    int32_t rng_kind;    /* UG_RNG_* (0 = Philox) */
 } ug_pack_config;
 
 typedef struct ug_pack_status {
@@ -293,6 +303,8 @@ typedef struct ug_knot_config {
     int32_t max_strands; /* strand capacity, 0 = 16384 (kMaxStrands, :94-97) */
     int32_t precision;   /* UG_PREC_FP64, or UG_PREC_SPLIT_BF16: full-band models test the first
                             32 ordered timesteps on the split-BF16 prefilter first; same strands */
+    int32_t rng_kind;    /* UG_RNG_* (0 = Philox; UG_RNG_XOSHIRO_REF draws the reference knot
+                            tool's own candidates, cuda/braid_knot.cu:151-166,207) */
 } ug_knot_config;
 
 typedef struct ug_knot_status {

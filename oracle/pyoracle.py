@@ -41,6 +41,12 @@ def lib() -> C.CDLL:
         L.or_model_init.restype = C.c_int
         L.or_model_set_euclid.argtypes = [C.c_void_p, C.c_int]
         L.or_model_set_euclid.restype = None
+        L.or_model_set_rng.argtypes = [C.c_void_p, C.c_int]
+        L.or_model_set_rng.restype = None
+        L.or_xoshiro_seed.argtypes = [C.c_uint64, C.c_void_p]
+        L.or_xoshiro_seed.restype = None
+        L.or_xoshiro_next.argtypes = [C.c_void_p]
+        L.or_xoshiro_next.restype = C.c_uint64
         L.or_touches.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_touches.restype = C.c_int
         L.or_model_free.argtypes = [C.c_void_p]
@@ -89,6 +95,7 @@ class _Model(C.Structure):
         ("seed", C.c_uint64), ("r", C.c_double), ("halfT", C.c_double),
         ("S", C.POINTER(C.c_double)), ("C", C.POINTER(C.c_double)),
         ("sz", C.POINTER(C.c_double)), ("order", C.POINTER(C.c_int)), ("euclid", C.c_int),
+        ("rng", C.c_int),
     ]
 
 
@@ -109,6 +116,19 @@ def log(v: float) -> float:
     return lib().or_log(v)
 
 
+def xoshiro_seed(seed: int) -> np.ndarray:
+    """The xoshiro256** state seeded from `seed` (four splitmix64 outputs)."""
+    st = np.zeros(4, np.uint64)
+    lib().or_xoshiro_seed(seed, st.ctypes.data)
+    return st
+
+
+def xoshiro_next(st: np.ndarray) -> int:
+    """One xoshiro256** output; advances the state array in place."""
+    assert st.dtype == np.uint64 and st.flags.c_contiguous and st.size == 4
+    return int(lib().or_xoshiro_next(st.ctypes.data))
+
+
 def _p(a: np.ndarray) -> int:
     return a.ctypes.data
 
@@ -127,7 +147,7 @@ class Model:
     """Oracle model for (d, T, terms, seed, pin)."""
 
     def __init__(self, d: int, T: int, terms: int | None = None, seed: int = 1, pin: bool = False,
-                 euclid: bool = False):
+                 euclid: bool = False, rng: int = 0):
         self.d, self.T = d, T
         self.terms = T if terms is None else terms
         self._m = _Model()
@@ -136,7 +156,10 @@ class Model:
             raise ValueError(f"or_model_init failed ({rc}) for d={d} T={T} terms={self.terms}")
         if euclid:
             lib().or_model_set_euclid(C.byref(self._m), 1)
+        if rng:
+            lib().or_model_set_rng(C.byref(self._m), int(rng))
         self.euclid = bool(euclid)
+        self.rng = int(rng)
         self.nw = self._m.nw
         self.r = self._m.r
 

@@ -186,6 +186,107 @@ static int touches(const or_model* m, const double* x, const double* y);
 
 void or_model_set_euclid(or_model* m, int euclid) { m->euclid = euclid ? 1 : 0; }
 
+void or_model_set_rng(or_model* m, int rng) { m->rng = rng ? 1 : 0; }
+
+/* splitmix64 (Steele, Lea and Flood): the state advances by the golden-ratio increment and
+ * each output is the state passed through two xor-shift-multiply rounds and a final
+ * xor-shift; the four outputs from `seed` are xoshiro's state (cuda/braid_knot.cu:62-70). */
+void or_xoshiro_seed(uint64_t seed, uint64_t s[4]) {
+    uint64_t x = seed;
+    for (int i = 0; i < 4; i++) {
+        x += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        s[i] = z ^ (z >> 31);
+    }
+}
+
+static uint64_t rol64(uint64_t v, unsigned k) { return (v << k) | (v >> (64u - k)); }
+
+/* xoshiro256** (Blackman and Vigna): output = rotl(s1 * 5, 7) * 9, then the linear state
+ * update (cuda/braid_knot.cu:71-82). */
+uint64_t or_xoshiro_next(uint64_t s[4]) {
+    const uint64_t result = rol64(s[1] * 5u, 7) * 9u;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rol64(s[3], 45);
+    return result;
+}
+
+/* 53-bit uniform k of one draw: u = k 2^-53 in [0, 1) (cuda/braid_knot.cu:84-86). */
+static uint64_t xo_u53(uint64_t s[4]) { return or_xoshiro_next(s) >> 11; }
+
+/* Candidate idx on the reference's stream: per axis, in stream order (cuda/braid_knot.cu:
+ * 151-166), the frequencies (draw_freqs :124-149: Fisher-Yates over 2..modmax+1 for the full
+ * pool, else up to 64 rejection draws per slot, the last kept even when a duplicate), the nw
+ * exponential spacings -log(1 - u) summed serially and divided by their sum (split_unit
+ * :112-120), per slot the amplitude w_j / b_j, its sign draw and, for an even frequency, its
+ * phase draw f = pi u, then the sin1 amplitude 2u - 1 unless pinned.  The phase's sine and
+ * cosine are taken of the exact pi k 2^-53 (sincospi of the rational k / 2^53). */
+static void candidate_xref(const or_model* m, uint64_t idx, double* a1, int* b, double* c, double* s,
+                           double* amp, double* phase) {
+    const int d = m->d, nw = m->nw;
+    uint64_t st[4];
+    or_xoshiro_seed(m->seed ^ (idx * 0x9E3779B97F4A7C15ULL + 0x1234567ULL), st);
+    double* e = (double*)malloc((size_t)nw * sizeof(double));
+    for (int a = 0; a < d; a++) {
+        int* ba = b + (size_t)a * nw;
+        if (nw == m->modmax) {
+            for (int j = 0; j < nw; j++)
+                ba[j] = j + 2;
+            for (int j = nw - 1; j >= 1; j--) {
+                const int k = (int)(or_xoshiro_next(st) % (uint64_t)(j + 1));
+                const int tmp = ba[k];
+                ba[k] = ba[j];
+                ba[j] = tmp;
+            }
+        } else {
+            for (int j = 0; j < nw; j++) {
+                int cand = 0, clash = 1;
+                for (int tries = 0; clash && tries < 64; tries++) {
+                    cand = 2 + (int)(or_xoshiro_next(st) % (uint64_t)m->modmax);
+                    clash = 0;
+                    for (int k = 0; k < j && !clash; k++)
+                        clash = ba[k] == cand;
+                }
+                ba[j] = cand;
+            }
+        }
+        double sum = 0.0;
+        for (int j = 0; j < nw; j++) {
+            const uint64_t k = xo_u53(st);
+            e[j] = -or_log(1.0 - (double)k * 0x1p-53);
+            sum += e[j];
+        }
+        for (int j = 0; j < nw; j++) {
+            double aj = (e[j] / sum) / (double)ba[j];
+            if (or_xoshiro_next(st) & 1u)
+                aj = -aj;
+            double cj = aj, sj = 0.0, fj = 0.0;
+            if (ba[j] % 2 == 0) {
+                const uint64_t k = xo_u53(st);
+                double sf, cf;
+                or_sincospi_rat(k, 1ULL << 53, &sf, &cf);
+                cj = aj * cf;
+                sj = aj * sf;
+                fj = (double)k * 0x1p-53 * 3.141592653589793;
+            }
+            c[(size_t)a * nw + j] = cj;
+            s[(size_t)a * nw + j] = sj;
+            if (amp)
+                amp[(size_t)a * nw + j] = aj;
+            if (phase)
+                phase[(size_t)a * nw + j] = fj;
+        }
+        a1[a] = m->pin ? 0.0 : (double)xo_u53(st) * 0x1p-53 * 2.0 - 1.0;
+    }
+    free(e);
+}
+
 int or_touches(const or_model* m, const double* x, const double* y) { return touches(m, x, y); }
 
 void or_model_free(or_model* m) {
@@ -204,6 +305,10 @@ static void philox_at(const or_model* m, uint64_t idx, uint32_t c2, uint32_t c3,
 
 void or_candidate(const or_model* m, uint64_t idx, double* a1, int* b, double* c, double* s,
                   double* amp, double* phase) {
+    if (m->rng) {
+        candidate_xref(m, idx, a1, b, c, s, amp, phase);
+        return;
+    }
     const int d = m->d, nw = m->nw;
     const int npairs = (nw + 1) / 2;
     uint32_t w0[4];

@@ -43,6 +43,7 @@ typedef struct or_model {
     double* sz;     /* [T]: sin z_t */
     int* order;     /* [T]: endpoint-first scan order */
     int euclid;     /* contact: 0 = Chebyshev cube, 1 = L2 ball (--euclid-collision) */
+    int rng;        /* candidate stream: 0 = Philox, 1 = the reference's xoshiro256** stream */
 } or_model;
 
 int or_model_init(or_model* m, int d, int T, int terms, uint64_t seed, int pin);
@@ -51,6 +52,16 @@ void or_model_free(or_model* m);
 /* --euclid-collision (cuda/braid_cuda3d.cu:605-608): contact iff
  * dx*dx + dy*dy [+ dw*dw] <= r*r, summed in that order with fma. */
 void or_model_set_euclid(or_model* m, int euclid);
+
+/* Candidate stream (rng_kind 1 = UG_RNG_XOSHIRO_REF): candidate idx draws xoshiro256**
+ * (cuda/braid_knot.cu:56-90) seeded with splitmix64 from seed ^ (idx*0x9E3779B97F4A7C15 +
+ * 0x1234567) (:207), in the draw order of the knot tool's proposal (:112-166). */
+void or_model_set_rng(or_model* m, int rng);
+
+/* The four splitmix64 outputs that seed xoshiro256** from `seed` (cuda/braid_knot.cu:62-70),
+ * and one xoshiro256** step on that state (:71-82). */
+void or_xoshiro_seed(uint64_t seed, uint64_t s[4]);
+uint64_t or_xoshiro_next(uint64_t s[4]);
 
 /* The contact predicate of the model at one timestep (points x[d], y[d]). */
 int or_touches(const or_model* m, const double* x, const double* y);

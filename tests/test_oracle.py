@@ -289,6 +289,113 @@ def test_candidates_are_counter_addressed():
     assert np.array_equal(a.c, b.c) and not np.array_equal(a.c, c.c)
 
 
+# ------------------------------------------------------------------ the reference's stream
+def test_xoshiro_published_vectors():
+    # splitmix64 from 0 (its published first outputs) seeds the state; xoshiro256** from the
+    # state {1, 2, 3, 4} (the generator's published first ten outputs)
+    assert [int(x) for x in po.xoshiro_seed(0)[:3]] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4,
+                                                       0x06C45D188009454F]
+    st = np.array([1, 2, 3, 4], np.uint64)
+    want = [11520, 0, 1509978240, 1215971899390074240, 1216172134540287360, 607988272756665600,
+            16172922978634559625, 8476171486693032832, 10595114339597558777, 2904607092377533576]
+    assert [po.xoshiro_next(st) for _ in range(10)] == want
+
+
+def _xref_draws(seed, idx, d, nw, modmax, pin):
+    """The knot tool's proposal (cuda/braid_knot.cu:124-166) as raw draws, in stream order,
+    from the generator pinned above: per axis the frequencies, the spacing uniforms, per slot
+    a sign and (even frequency) a phase uniform, then the sin1 uniform."""
+    M64 = (1 << 64) - 1
+    st = po.xoshiro_seed(seed ^ ((idx * 0x9E3779B97F4A7C15 + 0x1234567) & M64))
+    nxt = lambda: po.xoshiro_next(st)
+    axes = []
+    for _ in range(d):
+        if nw == modmax:
+            b = list(range(2, nw + 2))
+            for j in range(nw - 1, 0, -1):
+                k = nxt() % (j + 1)
+                b[j], b[k] = b[k], b[j]
+        else:
+            b = []
+            for j in range(nw):
+                for _g in range(64):
+                    c = nxt() % modmax + 2
+                    if c not in b:
+                        break
+                b.append(c)
+        u = [(nxt() >> 11) * 2.0**-53 for _ in range(nw)]
+        sign, ph = [], []
+        for j in range(nw):
+            sign.append(nxt() & 1)
+            ph.append((nxt() >> 11) * 2.0**-53 if b[j] % 2 == 0 else 0.0)
+        a1 = 0.0 if pin else (nxt() >> 11) * 2.0**-53 * 2 - 1
+        axes.append((np.array(b), np.array(u), np.array(sign), np.array(ph), a1))
+    return axes
+
+
+@pytest.mark.parametrize("d,T,terms,pin,idx", [(2, 16, None, False, 5), (3, 12, None, True, 77),
+                                               (2, 24, 5, False, 123456), (3, 64, None, False, 9)])
+def test_xref_candidate_follows_the_stream_order(d, T, terms, pin, idx):
+    # which draw feeds which quantity: a slip in the order (signs before spacings, the sin1
+    # draw first as in the engines' proposal, phases for odd frequencies) moves these
+    m = po.Model(d, T, terms=terms, seed=11, pin=pin, rng=1)
+    cd = m.candidate(idx)
+    for a, (b, u, sign, ph, a1) in enumerate(_xref_draws(11, idx, d, m.nw, max(T - 1, 2), pin)):
+        assert np.array_equal(cd.b[a], b)
+        e = -np.log1p(-u)
+        assert np.allclose(np.abs(cd.amp[a]) * b, e / e.sum(), rtol=1e-13, atol=0)
+        assert np.array_equal(cd.amp[a] < 0, sign == 1)
+        assert np.allclose(cd.phase[a], ph * np.pi, rtol=1e-15, atol=0)
+        assert cd.a1[a] == a1
+
+
+def test_xref_candidate_invariants_and_distributions():
+    for d, T in ((2, 16), (3, 32)):
+        m = po.Model(d, T, seed=5, rng=1)
+        for idx in range(0, 200, 9):
+            cd = m.candidate(idx)
+            assert np.all((cd.a1 >= -1.0) & (cd.a1 < 1.0))
+            for a in range(d):
+                assert sorted(cd.b[a]) == list(range(2, T + 1))
+                assert abs(np.sum(np.abs(cd.amp[a]) * cd.b[a]) - 1.0) < 1e-13
+                odd = cd.b[a] % 2 == 1
+                assert np.all(cd.phase[a][odd] == 0.0) and np.all(cd.s[a][odd] == 0.0)
+                assert np.allclose(cd.c[a], cd.amp[a] * np.cos(cd.phase[a]), rtol=0, atol=1e-16)
+                assert np.allclose(cd.s[a], cd.amp[a] * np.sin(cd.phase[a]), rtol=0, atol=1e-16)
+    assert np.all(po.Model(2, 16, seed=4, pin=True, rng=1).candidate(3).a1 == 0.0)
+    # the measure: sin1 uniform on [-1, 1), Dirichlet shares, uniform phases, fair signs, and
+    # the Fisher-Yates slot of a frequency uniform over the slots
+    T, n = 32, 3000
+    m = po.Model(2, T, seed=21, rng=1)
+    a1, share, ph, neg, where2 = [], [], [], [], []
+    for idx in range(n):
+        cd = m.candidate(idx)
+        a1.extend(cd.a1)
+        share.append((np.abs(cd.amp[0]) * cd.b[0])[5])
+        even = cd.b[0] % 2 == 0
+        ph.extend(cd.phase[0][even][:3])
+        neg.extend(cd.amp[0] < 0)
+        where2.append(int(np.flatnonzero(cd.b[1] == 2)[0]))
+    assert stats.kstest(a1, "uniform", args=(-1, 2)).pvalue > 1e-3
+    assert stats.kstest(share, "beta", args=(1, T - 2)).pvalue > 1e-3
+    assert stats.kstest(np.array(ph) / np.pi, "uniform").pvalue > 1e-3
+    assert abs(np.mean(neg) - 0.5) < 0.01
+    assert stats.chisquare(np.bincount(where2, minlength=T - 1)).pvalue > 1e-3
+
+
+def test_xref_pack_characterization():
+    # serial RSA on the reference's stream: no two accepted collide, every rejected candidate
+    # collides with an earlier accepted one, and the list differs from the Philox stream's
+    for d, T, terms, cands in ((2, 16, None, 6000), (3, 12, None, 5000), (2, 24, 5, 6000)):
+        m = po.Model(d, T, terms=terms, seed=3, rng=1)
+        idx, _, att = m.pack(cands)
+        assert att == cands and len(idx) > 20
+        assert m.check_rsa(cands, idx) == 0
+        assert m.check_rsa(cands, idx[:-1]) != 0
+        philox, _, _ = po.Model(d, T, terms=terms, seed=3).pack(cands)
+        assert not np.array_equal(idx, philox)
+
+
 # ------------------------------------------------------------------ wrap and predicate
 def test_wrap_reference_examples():
     m = po.Model(2, 16)

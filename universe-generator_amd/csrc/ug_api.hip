@@ -26,6 +26,32 @@ void ug::gen_params_host(const Pack* h, uint64_t idx, double* out_row) {
     const int D = h->d, nw = h->nw, npairs = h->npairs;
     const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
     const uint32_t k0 = h->dm.key0, k1 = h->dm.key1;
+    if (h->dm.xref) {
+        // the reference's serial stream: every axis' draws, then the arithmetic of gen_wave64
+        xo256 r = xo_candidate((uint64_t)k0 | (uint64_t)k1 << 32, idx);
+        std::vector<int> b(nw);
+        std::vector<uint64_t> ek(nw), pk(nw);
+        for (int a = 0; a < D; a++) {
+            double* row = out_row + (size_t)a * (1 + 3 * nw);
+            const uint64_t ua = xo_axis(r, h->modmax, nw, h->dm.pin, b.data(), ek.data(), pk.data(), 1);
+            row[0] = h->dm.pin ? 0.0 : xo_a1(ua);
+            std::vector<double> e(nw);
+            double sum = 0.0;
+            for (int j = 0; j < nw; j++) {
+                e[j] = xo_spacing(ek[j]);
+                sum = sum + e[j];
+            }
+            for (int j = 0; j < nw; j++) {
+                double aj = (e[j] / sum) / (double)b[j];
+                if (pk[j] >> 63)
+                    aj = -aj;
+                row[1 + 3 * j] = aj;
+                row[2 + 3 * j] = (double)b[j];
+                row[3 + 3 * j] = (b[j] & 1) ? 0.0 : (double)(pk[j] & ((1ull << 53) - 1)) * 0x1p-53 * 3.141592653589793;
+            }
+        }
+        return;
+    }
     const u32x4 w0 = philox(lo, hi, 0u, 0u, k0, k1);
     const uint32_t wa[3] = {w0.x, w0.y, w0.z};
     std::vector<int> b(nw);
@@ -198,6 +224,11 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
         return fail(UG_EINVAL, "error: unknown precision %d", cfg.precision);
     if (cfg.stop_kind < UG_STOP_BUDGET || cfg.stop_kind > UG_STOP_REF_WINDOW)
         return fail(UG_EINVAL, "error: unknown stop rule %d", cfg.stop_kind);
+    if (cfg.rng_kind != UG_RNG_PHILOX && cfg.rng_kind != UG_RNG_XOSHIRO_REF)
+        return fail(UG_EINVAL, "error: unknown rng_kind %d", cfg.rng_kind);
+    if (cfg.rng_kind == UG_RNG_XOSHIRO_REF && cfg.precision != UG_PREC_FP64)
+        return fail(UG_EINVAL, "error: rng_kind UG_RNG_XOSHIRO_REF runs in the FP64 form only (the split-BF16 "
+                               "prefilter draws Philox words inside its MFMA loop)");
     if (cfg.stop_kind == UG_STOP_FIXED_WINDOW && !(cfg.stop_rate > 0))
         return fail(UG_EINVAL, "error: the stop rule needs stop_rate > 0");
     if (cfg.stop_kind == UG_STOP_REF_WINDOW && cfg.stop_rate < 0)
@@ -313,6 +344,7 @@ int32_t ug::create_handle(const ug_pack_config* cfg_in, void* hip_stream, bool w
     m.full = h->full ? 1 : 0;
     m.key0 = (uint32_t)cfg.seed;
     m.key1 = (uint32_t)(cfg.seed >> 32);
+    m.xref = cfg.rng_kind == UG_RNG_XOSHIRO_REF ? 1 : 0;
     m.r = 2.0 / (double)T;
     m.euclid = cfg.euclid ? 1 : 0;
     m.r2 = m.r * m.r;
@@ -821,6 +853,9 @@ extern "C" int32_t ug_eval_candidates(const ug_pack* hp, const int64_t* dev_idx,
     int32_t rc;
     if (precision == UG_PREC_FP64)
         rc = launch_cand64(h, 0, count, dev_idx, nullptr, dev_out, true);
+    else if (precision == UG_PREC_SPLIT_BF16 && h->dm.xref)
+        return fail(UG_EINVAL, "ug_eval_candidates: the split-BF16 evaluation draws the Philox stream; this "
+                               "handle draws the reference's (rng_kind UG_RNG_XOSHIRO_REF, FP64 form only)");
     else if (precision == UG_PREC_SPLIT_BF16)
         rc = launch_fast_eval(h, dev_idx, count, dev_out);
     else

@@ -11,8 +11,10 @@
 //   Dirichlet(1) by exponential spacings cuda/braid_cuda.cu:250-264
 //   a_j = w_j / b_j, random signs        cuda/braid_cuda.cu:348-359
 //   phases on even frequencies only      cuda/braid_cuda.cu:366-371
-// The RNG is new: Philox4x32-10 keyed by the seed, one counter per (candidate,
-// stage, axis, slot pair), so every draw is addressable without a serial stream.
+// The default RNG is new: Philox4x32-10 keyed by the seed, one counter per (candidate,
+// stage, axis, slot pair), so every draw is addressable without a serial stream.  The
+// reference's own serial xoshiro256** stream and draw order (its knot tool's proposal) are
+// available as a cross-check mode in the FP64 forms (xo_* below).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -79,7 +81,7 @@ UG_HD u32x4 pair_words_full(uint32_t lo, uint32_t hi, int p, int a, uint32_t k0,
 }
 
 // ----------------------------------------------------------------------------------
-// sin(pi n/m), cos(pi n/m) for integers n >= 0, 0 < m <= 2^32: exact integer reduction
+// sin(pi n/m), cos(pi n/m) for integers n >= 0, 0 < m <= 2^53: exact integer reduction
 // to pi (r + q/2), |r| <= 1/4, then Taylor polynomials of sin(pi r) / cos(pi r) in r^2.
 struct sc2 {
     double s, c;
@@ -169,6 +171,98 @@ UG_HD double expo_of_word(uint32_t w) {  // -log(1 - u), u = (w >> 1) 2^-31
     const double v = (double)(0x80000000u - (w >> 1)) * 0x1p-31;
     return -log_pos(v);
 }
+
+// ----------------------------------------------------------------------------------
+// The reference's candidate-indexed proposal stream (rng_kind UG_RNG_XOSHIRO_REF), the one its
+// knot tool draws (cuda/braid_knot.cu:56-90 Rng, :207 seed): xoshiro256** (Blackman and
+// Vigna) whose four state words are successive splitmix64 outputs from the candidate's seed
+// seed ^ (idx * 0x9E3779B97F4A7C15 + 0x1234567).
+struct xo256 {
+    uint64_t s0, s1, s2, s3;
+};
+
+UG_HD uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+UG_HD uint64_t splitmix64_next(uint64_t& x) {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+UG_HD xo256 xo_seeded(uint64_t seed) {
+    xo256 r;
+    r.s0 = splitmix64_next(seed);
+    r.s1 = splitmix64_next(seed);
+    r.s2 = splitmix64_next(seed);
+    r.s3 = splitmix64_next(seed);
+    return r;
+}
+
+UG_HD uint64_t xo_next(xo256& r) {
+    const uint64_t out = rotl64(r.s1 * 5u, 7) * 9u;
+    const uint64_t sh = r.s1 << 17;
+    r.s2 ^= r.s0;
+    r.s3 ^= r.s1;
+    r.s1 ^= r.s2;
+    r.s0 ^= r.s3;
+    r.s2 ^= sh;
+    r.s3 = rotl64(r.s3, 45);
+    return out;
+}
+
+UG_HD xo256 xo_candidate(uint64_t seed, uint64_t idx) {
+    return xo_seeded(seed ^ (idx * 0x9E3779B97F4A7C15ull + 0x1234567ull));
+}
+
+// One axis of the knot tool's proposal (cuda/braid_knot.cu:124-166) as raw draws, in stream
+// order: the nw frequencies (Fisher-Yates over 2..modmax+1 for the full pool, else up to 64
+// rejection draws each, a duplicate kept after the 64th), then nw 53-bit uniforms of the
+// exponential spacings (ek[j * st]), then per slot a sign draw and, for an even frequency, a
+// 53-bit phase uniform (pk[j * st] = sign << 63 | phase uniform, the uniform 0 for odd
+// frequencies), then the sin1 uniform unless pinned (returned; 0 when pinned).
+UG_HD uint64_t xo_axis(xo256& r, int modmax, int nw, int pin, int* b, uint64_t* ek, uint64_t* pk, int st) {
+    if (nw == modmax) {
+        for (int j = 0; j < nw; j++)
+            b[j] = j + 2;
+        for (int j = nw - 1; j > 0; j--) {
+            const int k = (int)(xo_next(r) % (uint64_t)(j + 1));
+            const int t = b[j];
+            b[j] = b[k];
+            b[k] = t;
+        }
+    } else {
+        for (int j = 0; j < nw; j++) {
+            int bn = 0;
+            bool dup = true;
+            for (int guard = 0; dup && guard < 64; guard++) {
+                bn = (int)(xo_next(r) % (uint64_t)modmax) + 2;
+                dup = false;
+                for (int k = 0; k < j; k++)
+                    dup = dup || b[k] == bn;
+            }
+            b[j] = bn;
+        }
+    }
+    for (int j = 0; j < nw; j++)
+        ek[j * st] = xo_next(r) >> 11;
+    for (int j = 0; j < nw; j++) {
+        uint64_t v = (xo_next(r) & 1u) << 63;
+        if ((b[j] & 1) == 0)
+            v |= xo_next(r) >> 11;
+        pk[j * st] = v;
+    }
+    return pin ? 0u : xo_next(r) >> 11;
+}
+
+// The arithmetic on those draws (cuda/braid_knot.cu:112-120,155-164), canonical here as for the
+// Philox stream: spacing e = -log(1 - u), u = k 2^-53 (1 - u exact); the weights e_j / sum with
+// the sum taken serially in slot order; amplitude (e_j / sum) / b_j, both divisions IEEE;
+// phase angle pi k 2^-53 through sincospi_rat (the reference rounds u pi first); sin1
+// amplitude 2u - 1 (exact).
+UG_HD double xo_spacing(uint64_t k) { return -log_pos(1.0 - (double)k * 0x1p-53); }
+UG_HD double xo_a1(uint64_t k) { return (double)k * 0x1p-52 - 1.0; }
 
 // torus_wrap / torus_delta (cuda/braid_cuda.cu:158-168).
 UG_HD double torus_wrap(double x) { return x - 2.0 * floor((x + 1.0) * 0.5); }

@@ -115,11 +115,11 @@ bool finished(Pack* h) {
 }
 
 // A config the batched kernels cover: dense cell table, full spectrum, T <= 160 (one
-// prefilter stage), budget or fixed-window stop, unique mode.
+// prefilter stage), budget or fixed-window stop, unique mode, the Philox stream.
 bool batchable(const ug_pack_config& c) {
     const int terms = c.terms == 0 ? c.T : c.terms;
     return (c.dim == 2 || c.dim == 3) && c.T >= 4 && c.T <= 160 && terms == c.T && !c.sparse &&
-           (c.stop_kind == UG_STOP_BUDGET || c.stop_kind == UG_STOP_FIXED_WINDOW);
+           (c.stop_kind == UG_STOP_BUDGET || c.stop_kind == UG_STOP_FIXED_WINDOW) && c.rng_kind == UG_RNG_PHILOX;
 }
 
 struct Engine {

@@ -53,6 +53,9 @@ struct DevModel {
     int sparse;
     ulonglong2* hslots;
     uint64_t hmask;       // slots - 1 (a power of two)
+    // candidate stream: 0 = Philox (key0, key1), 1 = the reference's xoshiro256** stream
+    // seeded from key0 | key1 << 32 (FP64 forms only)
+    int xref;
 };
 
 // Canonical FP64 contact predicate between two points at one timestep (torus min-image

@@ -223,6 +223,7 @@ extern "C" int32_t ug_knot_create(const ug_knot_config* cfg_in, void* hip_stream
     pc.precision = cfg.precision;
     pc.seed = cfg.seed;
     pc.pin_sin1 = cfg.pin_sin1;
+    pc.rng_kind = cfg.rng_kind;
     pc.stop_kind = UG_STOP_BUDGET;
     pc.max_attempts = cfg.max_attempts;
     Pack* h = nullptr;

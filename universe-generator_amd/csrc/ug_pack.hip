@@ -110,9 +110,100 @@ __device__ void draw_freqs_partial(const DevModel& m, uint64_t idx, int* sb, uin
     }
 }
 
+// The slot arrays of the lane-per-timestep evaluation use a padded stride of 128 PL slots per
+// axis (PL = slot pairs per lane: 1 up to 64 pairs, 2 up to 128, else 4).
+__host__ __device__ inline int slot_pl(int npairs) { return npairs <= 64 ? 1 : (npairs <= 128 ? 2 : 4); }
+
+template <int D>
+__host__ __device__ inline int cand64_lds_head(const DevModel& m, int pl) {
+    const int draws = D * m.nw + (m.modmax + 2 + 31) / 32;
+    const int slots = D * 128 * pl;
+    return ((draws > slots ? draws : slots) + 3) / 4 * 4;
+}
+
+template <int D>
+__host__ __device__ inline int cand64_lds_words(const DevModel& m) {
+    const int pl = slot_pl(m.npairs);
+    return cand64_lds_head<D>(m, pl) + D * 128 * pl * 4 + 64 * D * 2;
+}
+
+// The reference's serial stream (rng_kind UG_RNG_XOSHIRO_REF; the XR instantiations of the
+// kernels, so the Philox ones keep their register allocation): lane 0 replays the
+// candidate's draws (xo_axis, every axis in stream order) into LDS -- the frequencies at
+// sb[a nw + j], the spacing and sign/phase uniforms of slot j of axis a at
+// xr[2 (a NWP + j)] and xr[2 (a NWP + j) + 1], past the frequency area -- then the lanes turn
+// their own slots into coefficients.  The spacings are summed serially in slot order (as
+// cuda/braid_knot.cu:114-117), every lane the same sum.  The callers' dynamic LDS holds
+// cand64_lds_words words.
 template <int D, int PL>
+__device__ __forceinline__ void gen_wave64_xref(const DevModel& m, uint64_t idx, int lane, WaveCoef<D, PL>& w,
+                                                int* sb) {
+    constexpr int NWP = 128 * PL;
+    const int nw = m.nw;
+    uint64_t* xr = (uint64_t*)(sb + cand64_lds_head<D>(m, PL));
+    __shared__ uint64_t xa[3];
+    if (lane == 0) {
+        xo256 r = xo_candidate((uint64_t)m.key0 | (uint64_t)m.key1 << 32, idx);
+        for (int a = 0; a < D; a++)
+            xa[a] = xo_axis(r, m.modmax, nw, m.pin, sb + a * nw, xr + 2 * a * NWP, xr + 2 * a * NWP + 1, 2);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < D; a++) {
+        w.a1[a] = m.pin ? 0.0 : xo_a1(xa[a]);
+        double e[PL][2];
+#pragma unroll
+        for (int i = 0; i < PL; i++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int j = 2 * (lane + 64 * i) + k;
+                e[i][k] = 0.0;
+                if (j < nw) {
+                    // each lane rewrites only its own slots' entries
+                    e[i][k] = xo_spacing(xr[2 * (a * NWP + j)]);
+                    xr[2 * (a * NWP + j)] = (uint64_t)__double_as_longlong(e[i][k]);
+                }
+            }
+        __syncthreads();
+        double sum = 0.0;
+        for (int j = 0; j < nw; j++)
+            sum = sum + __longlong_as_double((long long)xr[2 * (a * NWP + j)]);
+#pragma unroll
+        for (int i = 0; i < PL; i++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int j = 2 * (lane + 64 * i) + k;
+                double cj = 0.0, sj = 0.0;
+                int bj = 2;  // unused slots: zero coefficients on a valid basis row
+                if (j < nw) {
+                    bj = sb[a * nw + j];
+                    const uint64_t pk = xr[2 * (a * NWP + j) + 1];
+                    double aj = (e[i][k] / sum) / (double)bj;
+                    if (pk >> 63)
+                        aj = -aj;
+                    if ((bj & 1) == 0) {
+                        const sc2 f = sincospi_rat(pk & ((1ull << 53) - 1ull), 1ull << 53);
+                        cj = aj * f.c;
+                        sj = aj * f.s;
+                    } else {
+                        cj = aj;
+                    }
+                }
+                w.c[a][i][k] = cj;
+                w.s[a][i][k] = sj;
+                w.b[a][i][k] = bj;
+            }
+    }
+    __syncthreads();  // the caller reuses both areas
+}
+
+template <int D, int PL, bool XR = false>
 __device__ void gen_wave64(const DevModel& m, uint64_t idx, int lane, WaveCoef<D, PL>& w, int* sb,
                            uint32_t* used) {
+    if constexpr (XR) {
+        gen_wave64_xref<D, PL>(m, idx, lane, w, sb);
+        return;
+    }
     const uint32_t lo = (uint32_t)idx, hi = (uint32_t)(idx >> 32);
     const u32x4 w0 = philox(lo, hi, 0u, 0u, m.key0, m.key1);
     const uint32_t wa[3] = {w0.x, w0.y, w0.z};
@@ -355,23 +446,6 @@ __device__ __forceinline__ void eval_lane_full(const DevModel& m, const double2*
         X[a] = torus_wrap(a1[a] + total[a]);
 }
 
-// The slot arrays of the lane-per-timestep evaluation use a padded stride of 128 PL slots per
-// axis (PL = slot pairs per lane: 1 up to 64 pairs, 2 up to 128, else 4).
-__host__ __device__ inline int slot_pl(int npairs) { return npairs <= 64 ? 1 : (npairs <= 128 ? 2 : 4); }
-
-template <int D>
-__host__ __device__ inline int cand64_lds_head(const DevModel& m, int pl) {
-    const int draws = D * m.nw + (m.modmax + 2 + 31) / 32;
-    const int slots = D * 128 * pl;
-    return ((draws > slots ? draws : slots) + 3) / 4 * 4;
-}
-
-template <int D>
-__host__ __device__ inline int cand64_lds_words(const DevModel& m) {
-    const int pl = slot_pl(m.npairs);
-    return cand64_lds_head<D>(m, pl) + D * 128 * pl * 4 + 64 * D * 2;
-}
-
 // Lookup round: lanes take (timestep slot sl, 3-cell row rr) items of the timesteps whose
 // positions are in pos[]; returns true when any lane finds a contact (canonical predicate).
 template <int D>
@@ -508,7 +582,7 @@ __device__ __forceinline__ bool lookup_lane(const DevModel& m, const double* x, 
 // known >= 0: the candidate's scan positions < min(T, 32) were examined by the split-BF16
 // prefilter and bit c of known marks position c proven to hold no contact; only the others
 // (usually a handful) are evaluated across the wave before the bulk scan starts at 32.
-template <int D, bool FULL, int PL, bool SB>
+template <int D, bool FULL, int PL, bool SB, bool XR = false>
 __device__ __forceinline__ void cand64_item(const DevModel& m, int64_t i, uint64_t idx, uint8_t* flags,
                                             double* out, int head, int* cells, uint64_t* mask,
                                             unsigned long long& nvis, unsigned long long& npts,
@@ -529,7 +603,7 @@ __device__ __forceinline__ void cand64_item(const DevModel& m, int64_t i, uint64
         if (FULL && mask && lane < UG_MASK_WORDS)
             mask[(size_t)i * UG_MASK_WORDS + lane] = 0ull;  // the conflict row, cleared
         WaveCoef<D, PL> w;
-        gen_wave64<D, PL>(m, idx, lane, w, sb, used);
+        gen_wave64<D, PL, XR>(m, idx, lane, w, sb, used);
         bool hit = false;
         int o0 = 0;
         const int kn = min(T, 32);
@@ -647,7 +721,7 @@ __device__ __forceinline__ void cand64_counters(unsigned long long* ctr, unsigne
     }
 }
 
-template <int D, bool FULL, int PL, bool SB = false>
+template <int D, bool FULL, int PL, bool SB = false, bool XR = false>
 __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_t count,
                                                const int64_t* list, const int* dcount,
                                                uint8_t* flags, double* out, int head,
@@ -658,8 +732,8 @@ __global__ __launch_bounds__(64) void k_cand64(DevModel m, uint64_t base, int64_
     unsigned long long nvis = 0, npts = 0;
     for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
         const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
-        cand64_item<D, FULL, PL, SB>(m, i, idx, flags, out, head, cells, mask, nvis, npts,
-                                     skip ? (int64_t)skip[i] : -1);
+        cand64_item<D, FULL, PL, SB, XR>(m, i, idx, flags, out, head, cells, mask, nvis, npts,
+                                         skip ? (int64_t)skip[i] : -1);
     }
     if (!FULL && ctr) {
         // exact counts for the algorithmic-bytes figure
@@ -704,7 +778,7 @@ __global__ __launch_bounds__(64) void k_cand64_multi(const MCand* __restrict__ A
 // paths collide).  mode 1, candidate first + i: skipped when accepted; otherwise it must touch
 // a unique accepted before it (rows < #accepted with smaller index) -- no contact is a
 // violation.  out[0..1] = violations of modes 0/1, out[2..3] = items checked.
-template <int D, int PL>
+template <int D, int PL, bool XR = false>
 __global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t first, int64_t count,
                                               const int64_t* acc_idx, int64_t n,
                                               unsigned long long* out) {
@@ -742,7 +816,7 @@ __global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t fir
             limit = lo;
         }
         WaveCoef<D, PL> w;
-        gen_wave64<D, PL>(m, idx, lane, w, sb, used);
+        gen_wave64<D, PL, XR>(m, idx, lane, w, sb, used);
 #pragma unroll
         for (int a = 0; a < D; a++)
 #pragma unroll
@@ -761,7 +835,9 @@ __global__ __launch_bounds__(64) void k_check(DevModel m, int mode, uint64_t fir
             if (oi < T) {
                 const int t = m.order[oi];
                 double X[D];
-                if (PL == 1 && m.full)  // (more slot pairs per lane would spill registers)
+                // (more slot pairs per lane would spill registers; the reference's stream
+                // shuffles the full pool over the slots)
+                if (!XR && PL == 1 && m.full)
                     eval_lane_full<D, PL>(m, cs, sb, w.a1, t, X);
                 else
                     eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
@@ -856,7 +932,7 @@ __device__ __forceinline__ bool knot_round(const DevModel& m, const double* pos,
 // One wave per candidate, as k_cand64's test (head chunks of 2, 4, ... 64 timesteps across
 // the wave, then one lane per timestep) with the brute-force lookup: flags[i] = 1 iff
 // candidate i touches none of the nk strands.
-template <int D, int PL, bool SB>
+template <int D, int PL, bool SB, bool XR = false>
 __global__ __launch_bounds__(64) void k_knot_test(DevModel m, uint64_t base, int64_t count, const int64_t* list,
                                                   const int* dcount, uint8_t* flags, int head, const int* kcell,
                                                   const double* kpos, int64_t ks, int nk) {
@@ -875,7 +951,7 @@ __global__ __launch_bounds__(64) void k_knot_test(DevModel m, uint64_t base, int
     for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
         const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
         WaveCoef<D, PL> w;
-        gen_wave64<D, PL>(m, idx, lane, w, sb, used);
+        gen_wave64<D, PL, XR>(m, idx, lane, w, sb, used);
         bool hit = false;
         int o0 = 0;
         for (int chunk = 2; !hit && o0 < min(head, T); chunk = min(2 * chunk, 64)) {
@@ -963,7 +1039,7 @@ __global__ __launch_bounds__(256) void k_knot_append(DevModel m, const double* p
 
 // Baker coefficients of stored path p (one wave per path): the canonical per-slot
 // (a cos f, a sin f) of its candidate, scattered by frequency into dense rows.
-template <int D, int PL>
+template <int D, int PL, bool XR = false>
 __global__ __launch_bounds__(64) void k_paths_from_pack(DevModel m, const int64_t* acc_idx, int64_t n,
                                                         double* a2, double* A, double* B, int kmax,
                                                         int* bmax) {
@@ -975,7 +1051,7 @@ __global__ __launch_bounds__(64) void k_paths_from_pack(DevModel m, const int64_
     int bm = 0;
     for (int64_t p = blockIdx.x; p < n; p += gridDim.x) {
         WaveCoef<D, PL> w;
-        gen_wave64<D, PL>(m, (uint64_t)acc_idx[p], lane, w, sb, used);
+        gen_wave64<D, PL, XR>(m, (uint64_t)acc_idx[p], lane, w, sb, used);
 #pragma unroll
         for (int a = 0; a < D; a++) {
             if (lane == 0)
@@ -1010,7 +1086,7 @@ __global__ __launch_bounds__(64) void k_paths_from_pack(DevModel m, const int64_
 // touch nothing there, :463-464); gid_out[i] = the touched group.  When rs_count is given,
 // it counts the candidates the round-start grid alone lets through (the reference's
 // survivor count, which drives its batch size).
-template <int D, int PL>
+template <int D, int PL, bool XR = false>
 __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int64_t count,
                                                  const int64_t* list, const int* dcount,
                                                  const int32_t* acc_gid, int64_t nlimit, int* rs_count,
@@ -1032,7 +1108,7 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
     for (int64_t i = blockIdx.x; i < count; i += gridDim.x) {
         const uint64_t idx = list ? (uint64_t)list[i] : base + (uint64_t)i;
         WaveCoef<D, PL> w;
-        gen_wave64<D, PL>(m, idx, lane, w, sb, used);
+        gen_wave64<D, PL, XR>(m, idx, lane, w, sb, used);
 #pragma unroll
         for (int a = 0; a < D; a++)
 #pragma unroll
@@ -1053,7 +1129,9 @@ __global__ __launch_bounds__(64) void k_sub_test(DevModel m, uint64_t base, int6
             if (oi < T) {
                 const int t = m.order[oi];
                 double X[D];
-                if (PL == 1 && m.full)  // (more slot pairs per lane would spill registers)
+                // (more slot pairs per lane would spill registers; the reference's stream
+                // shuffles the full pool over the slots)
+                if (!XR && PL == 1 && m.full)
                     eval_lane_full<D, PL>(m, cs, sb, w.a1, t, X);
                 else
                     eval_lane<D, PL>(m, cs, sb, w.a1, t, X);
@@ -1745,9 +1823,17 @@ int32_t launch_cand64(Pack* h, uint64_t base, int64_t count, const int64_t* list
     const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
     // slot pairs per lane: 1 up to 64 pairs (T <= 129), 2 up to 128, else 4
     const int pl = slot_pl(h->npairs);
-#define UG_CAND64(DD, FF, PP, SS)                                                                         \
-    UG_LAUNCH(h, (k_cand64<DD, FF, PP, SS>), dim3(blocks), dim3(64), lds, h->dm, base, count, list, dcount, \
-              flags, out, head, ctr, cells, mask, skip)
+    // the reference's stream (rng_kind UG_RNG_XOSHIRO_REF): its own instantiations, one lane
+    // per timestep without the shared basis loads
+#define UG_CAND64(DD, FF, PP, SS)                                                                              \
+    do {                                                                                                       \
+        if (h->dm.xref)                                                                                        \
+            UG_LAUNCH(h, (k_cand64<DD, FF, PP, false, true>), dim3(blocks), dim3(64), lds, h->dm, base, count, \
+                      list, dcount, flags, out, head, ctr, cells, mask, skip);                                 \
+        else                                                                                                   \
+            UG_LAUNCH(h, (k_cand64<DD, FF, PP, SS>), dim3(blocks), dim3(64), lds, h->dm, base, count, list,    \
+                      dcount, flags, out, head, ctr, cells, mask, skip);                                       \
+    } while (0)
     // shared basis loads where the lane-per-timestep evaluation does the work: rechecks and
     // retests of a list, and trajectories (a fresh first pass scans its head across the wave)
     const bool sb = pl <= 2 && h->full && (full || list);
@@ -2329,9 +2415,16 @@ int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, i
     const int blocks = (int)std::min<int64_t>(count, 256 * 32);
     const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
     const int pl = slot_pl(h->npairs);
-#define UG_SUBT(DD, PP)                                                                                      \
-    hipLaunchKernelGGL((k_sub_test<DD, PP>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, list, \
+#define UG_SUBT_X(DD, PP, XX)                                                                                    \
+    hipLaunchKernelGGL((k_sub_test<DD, PP, XX>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, list, \
                        dcount, h->d_acc_gid, h->sub_nlimit, rs_count, h->d_flags, h->d_gflag)
+#define UG_SUBT(DD, PP)               \
+    do {                              \
+        if (h->dm.xref)               \
+            UG_SUBT_X(DD, PP, true);  \
+        else                          \
+            UG_SUBT_X(DD, PP, false); \
+    } while (0)
     if (h->d == 2) {
         if (pl == 1)
             UG_SUBT(2, 1);
@@ -2348,6 +2441,7 @@ int32_t sub_filter(Pack* h, uint64_t base, int64_t count, const int64_t* list, i
             UG_SUBT(3, 4);
     }
 #undef UG_SUBT
+#undef UG_SUBT_X
     rc = hip_check(hipGetLastError(), "k_sub_test");
     if (!rc)
         rc = compact(h, h->d_flags, count, base, list, out, S, dcount, 1, true, h->d_gflag, gout);
@@ -2435,11 +2529,19 @@ int32_t launch_paths_from_pack(Pack* h, hipStream_t stream, double* a2, double* 
     if (rc)
         return rc;
     const int blocks = (int)std::min<int64_t>(n, 256 * 32);
-    const size_t lds = (size_t)4 * (h->d * h->nw + (h->modmax + 2 + 31) / 32);
+    // the frequency draws; the whole candidate area with the reference's stream (gen_wave64_xref)
+    const size_t lds = h->dm.xref ? cand64_lds_bytes(h) : (size_t)4 * (h->d * h->nw + (h->modmax + 2 + 31) / 32);
     const int pl = slot_pl(h->npairs);
-#define UG_PFP(DD, PP)                                                                                   \
-    hipLaunchKernelGGL((k_paths_from_pack<DD, PP>), dim3(blocks), dim3(64), lds, stream, h->dm, h->d_acc_idx, n, \
-                       a2, A, B, kmax, d_bmax)
+#define UG_PFP_X(DD, PP, XX)                                                                                 \
+    hipLaunchKernelGGL((k_paths_from_pack<DD, PP, XX>), dim3(blocks), dim3(64), lds, stream, h->dm, h->d_acc_idx, \
+                       n, a2, A, B, kmax, d_bmax)
+#define UG_PFP(DD, PP)               \
+    do {                             \
+        if (h->dm.xref)              \
+            UG_PFP_X(DD, PP, true);  \
+        else                         \
+            UG_PFP_X(DD, PP, false); \
+    } while (0)
     if (h->d == 2) {
         if (pl == 1)
             UG_PFP(2, 1);
@@ -2456,6 +2558,7 @@ int32_t launch_paths_from_pack(Pack* h, hipStream_t stream, double* a2, double* 
             UG_PFP(3, 4);
     }
 #undef UG_PFP
+#undef UG_PFP_X
     return hip_check(hipGetLastError(), "k_paths_from_pack");
 }
 
@@ -2465,9 +2568,16 @@ int32_t launch_check(Pack* h, int mode, uint64_t first, int64_t count, unsigned 
     const int blocks = (int)std::min<int64_t>(count, 256 * 32);
     const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
     const int pl = slot_pl(h->npairs);
-#define UG_CHK(DD, PP)                                                                                       \
-    hipLaunchKernelGGL((k_check<DD, PP>), dim3(blocks), dim3(64), lds, h->stream, h->dm, mode, first, count, \
+#define UG_CHK_X(DD, PP, XX)                                                                                     \
+    hipLaunchKernelGGL((k_check<DD, PP, XX>), dim3(blocks), dim3(64), lds, h->stream, h->dm, mode, first, count, \
                        h->d_acc_idx, h->n, d_out)
+#define UG_CHK(DD, PP)               \
+    do {                             \
+        if (h->dm.xref)              \
+            UG_CHK_X(DD, PP, true);  \
+        else                         \
+            UG_CHK_X(DD, PP, false); \
+    } while (0)
     if (h->d == 2) {
         if (pl == 1)
             UG_CHK(2, 1);
@@ -2484,6 +2594,7 @@ int32_t launch_check(Pack* h, int mode, uint64_t first, int64_t count, unsigned 
             UG_CHK(3, 4);
     }
 #undef UG_CHK
+#undef UG_CHK_X
     return hip_check(hipGetLastError(), "k_check");
 }
 
@@ -2496,9 +2607,15 @@ int32_t launch_knot_test(Pack* h, uint64_t base, int64_t count, const int64_t* l
     const size_t lds = (size_t)4 * (h->d == 2 ? cand64_lds_words<2>(h->dm) : cand64_lds_words<3>(h->dm));
     const int pl = slot_pl(h->npairs);
     const bool sb = pl <= 2 && h->full && list;
-#define UG_KNOT(DD, PP, SS)                                                                                \
-    hipLaunchKernelGGL((k_knot_test<DD, PP, SS>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base, count, \
-                       list, dcount, flags, head, kcell, kpos, ks, nk)
+#define UG_KNOT(DD, PP, SS)                                                                                      \
+    do {                                                                                                         \
+        if (h->dm.xref)                                                                                          \
+            hipLaunchKernelGGL((k_knot_test<DD, PP, false, true>), dim3(blocks), dim3(64), lds, h->stream, h->dm, \
+                               base, count, list, dcount, flags, head, kcell, kpos, ks, nk);                     \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_knot_test<DD, PP, SS>), dim3(blocks), dim3(64), lds, h->stream, h->dm, base,    \
+                               count, list, dcount, flags, head, kcell, kpos, ks, nk);                           \
+    } while (0)
 #define UG_KNOT_PL(DD)                \
     do {                              \
         if (pl == 1 && sb)            \

@@ -7,7 +7,7 @@ braidlab/engine.py call surface.
 
 from ._build import build, LIB_PATH  # noqa: F401
 from .abi import (  # noqa: F401
-    PREC_FP64, PREC_SPLIT_BF16, STOP_BUDGET, STOP_FIXED_WINDOW, STOP_REF_WINDOW,
+    PREC_FP64, PREC_SPLIT_BF16, RNG_PHILOX, RNG_XOSHIRO_REF, STOP_BUDGET, STOP_FIXED_WINDOW, STOP_REF_WINDOW,
     SUB_ACCEPT_RATE, SUB_BUDGET, SUB_FILL_RATE, Knot, Packer, Status, UgError, ensemble, lib,
     pack_config,
 )

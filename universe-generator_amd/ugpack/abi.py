@@ -18,7 +18,8 @@ from ._build import LIB_PATH
 UG_OK, UG_EINVAL, UG_ENOMEM, UG_EHIP, UG_EOVERFLOW, UG_ESTATE = 0, -1, -2, -3, -4, -5
 PREC_FP64, PREC_SPLIT_BF16 = 0, 1
 STOP_BUDGET, STOP_FIXED_WINDOW, STOP_REF_WINDOW = 0, 1, 2
-ABI_VERSION = 4
+RNG_PHILOX, RNG_XOSHIRO_REF = 0, 1
+ABI_VERSION = 5
 
 
 class UgError(RuntimeError):
@@ -36,7 +37,7 @@ class PackConfig(C.Structure):
         ("max_attempts", C.c_double), ("stop_rate", C.c_double), ("stop_batch", C.c_int64),
         ("stop_window", C.c_int64), ("max_n", C.c_int64), ("margin", C.c_double),
         ("max_paths", C.c_int64), ("euclid", C.c_int32), ("sparse", C.c_int32),
-        ("sub_target", C.c_int32), ("sub_max", C.c_int32),
+        ("sub_target", C.c_int32), ("sub_max", C.c_int32), ("rng_kind", C.c_int32),
     ]
 
 
@@ -63,7 +64,8 @@ class KernelStats(C.Structure):
 class KnotConfig(C.Structure):
     _fields_ = [("dim", C.c_int32), ("T", C.c_int32), ("terms", C.c_int32), ("pin_sin1", C.c_int32),
                 ("seed", C.c_uint64), ("max_attempts", C.c_double), ("nu_attempts", C.c_double),
-                ("nu_div", C.c_double), ("max_strands", C.c_int32), ("precision", C.c_int32)]
+                ("nu_div", C.c_double), ("max_strands", C.c_int32), ("precision", C.c_int32),
+                ("rng_kind", C.c_int32)]
 
 
 class KnotStatus(C.Structure):
@@ -216,11 +218,13 @@ class Packer:
                  stop_kind: int = STOP_BUDGET, max_attempts: float = 0.0,
                  stop_rate: float = 0.0, stop_batch: int = 1 << 20, stop_window: int = 1 << 22,
                  max_n: int = 0, margin: float = 0.0, max_paths: int = 0, stream: int = 0,
-                 euclid: bool = False, sparse: bool = False, sub_target: int = 0, sub_max: int = 0):
+                 euclid: bool = False, sparse: bool = False, sub_target: int = 0, sub_max: int = 0,
+                 rng_kind: int = RNG_PHILOX):
         self._h = None
         cfg = PackConfig(dim, T, terms, precision, seed, int(pin_sin1), stop_kind,
                          float(max_attempts), float(stop_rate), stop_batch, stop_window, max_n,
-                         float(margin), max_paths, int(euclid), int(sparse), sub_target, sub_max)
+                         float(margin), max_paths, int(euclid), int(sparse), sub_target, sub_max,
+                         rng_kind)
         h = C.c_void_p()
         check(lib().ug_pack_create(C.byref(cfg), C.c_void_p(stream), C.byref(h)))
         self._h = h
@@ -367,10 +371,11 @@ class Knot:
 
     def __init__(self, dim: int, T: int, terms: int = 0, seed: int = 1, pin_sin1: bool = False,
                  max_attempts: float = 1e10, nu_attempts: float = 0.0, nu_div: float = 1.0,
-                 max_strands: int = 0, stream: int = 0, precision: int = PREC_FP64):
+                 max_strands: int = 0, stream: int = 0, precision: int = PREC_FP64,
+                 rng_kind: int = RNG_PHILOX):
         self._h = None
         cfg = KnotConfig(dim, T, terms, int(pin_sin1), seed, float(max_attempts), float(nu_attempts),
-                         float(nu_div), max_strands, precision)
+                         float(nu_div), max_strands, precision, rng_kind)
         h = C.c_void_p()
         check(lib().ug_knot_create(C.byref(cfg), C.c_void_p(stream), C.byref(h)))
         self._h = h
@@ -430,10 +435,10 @@ def pack_config(dim: int, T: int, terms: int = 0, seed: int = 1, precision: int 
                 stop_rate: float = 0.0, stop_batch: int = 1 << 20, stop_window: int = 1 << 22,
                 max_n: int = 0, margin: float = 0.0, max_paths: int = 0,
                 euclid: bool = False, sparse: bool = False, sub_target: int = 0,
-                sub_max: int = 0) -> PackConfig:
+                sub_max: int = 0, rng_kind: int = RNG_PHILOX) -> PackConfig:
     return PackConfig(dim, T, terms, precision, seed, int(pin_sin1), stop_kind, float(max_attempts),
                       float(stop_rate), stop_batch, stop_window, max_n, float(margin), max_paths,
-                      int(euclid), int(sparse), sub_target, sub_max)
+                      int(euclid), int(sparse), sub_target, sub_max, rng_kind)
 
 
 def list_digest(idx) -> int:
