@@ -137,3 +137,20 @@ def test_xref_needs_the_fp64_form(gpu):
     with ugpack.Packer(2, 16, rng_kind=ugpack.RNG_XOSHIRO_REF) as p:
         with pytest.raises(ugpack.UgError, match="FP64 form only"):
             p.eval_candidates(torch.arange(4, dtype=torch.int64), precision=ugpack.PREC_SPLIT_BF16)
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_xref_universes_in_an_ensemble(gpu, batched):
+    # the batched engine hands reference-stream universes to its single-universe loop; either
+    # runner gives each universe the oracle's list on its own stream
+    import ugpack
+
+    mix = [(2, 16, 1, 20000, 1), (3, 12, 2, 15000, 0), (2, 32, 3, 20000, 1), (3, 16, 4, 10000, 1),
+           (2, 16, 5, 20000, 0)]
+    print(f"xref ensemble batched={batched}", flush=True)
+    cfgs = [ugpack.pack_config(d, T, seed=s, max_attempts=m, rng_kind=x) for d, T, s, m, x in mix]
+    status, digests = ugpack.ensemble(cfgs, threads=3, digests=True, batched=batched, window=4)
+    for (d, T, s, m, x), st, dg in zip(mix, status, digests):
+        ref, _, _ = po.Model(d, T, seed=s, rng=x).pack(m)
+        assert st.attempts == m and st.done and st.n == len(ref)
+        assert dg == ugpack.abi.list_digest(ref), (d, T, s, x)
