@@ -317,7 +317,7 @@ def test_split_config2_pack_to_jam_equals_golden(gpu, seed):
 # ------------------------------------------------------------------ 3+1 at config-3 scale
 @pytest.mark.parametrize("precision", ["fp64", "split"])
 def test_config3_prefix_equals_golden(gpu, precision):
-    """3+1, F = T = 128: the first 2^20 candidates of seed 1 (N = 39,818 accepts)."""
+    """3+1, F = T = 128: the first 2^20 candidates of seed 1 (N = 39,810 accepts)."""
     import ugpack
 
     print(f"cfg3 prefix {precision}", flush=True)

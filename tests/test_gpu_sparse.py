@@ -44,7 +44,7 @@ def test_sparse_equals_golden(gpu, name, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "split"])
 def test_sparse_config3_prefix_through_table_growth(gpu, precision):
-    # 39,818 paths x 128 timesteps from a table sized for 1024 paths: five rebuilds
+    # 39,810 paths x 128 timesteps from a table sized for 1024 paths: five rebuilds
     import ugpack
 
     g = _gold("cfg3_d3_T128_s1_prefix1M")
