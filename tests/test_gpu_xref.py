@@ -154,3 +154,33 @@ def test_xref_universes_in_an_ensemble(gpu, batched):
         ref, _, _ = po.Model(d, T, seed=s, rng=x).pack(m)
         assert st.attempts == m and st.done and st.n == len(ref)
         assert dg == ugpack.abi.list_digest(ref), (d, T, s, x)
+
+
+def test_braid_knot_ug_draws_the_reference_stream(gpu):
+    # --rng xoshiro-ref: the knot front end's phase lines and curve follow the oracle's packing
+    # of the reference's own candidates (the FP64 form chosen by the flag)
+    import os
+    import re
+    import subprocess
+
+    import ugpack
+
+    d, T, seed, budget, nu, div = 2, 16, 3, 1e6, 3e5, 2.0
+    want = po.Model(d, T, seed=seed, pin=True, rng=1).pack_knot(budget, nu_attempts=nu, nu_div=div)
+    philox = po.Model(d, T, seed=seed, pin=True).pack_knot(budget, nu_attempts=nu, nu_div=div)
+    assert not np.array_equal(want["idx"], philox["idx"])
+    cmd = [os.path.join(ugpack.BIN_DIR, "braid_knot_ug"), "-t", str(T), "--dim", str(d), "--pin-sin1",
+           "--seed", str(seed), "--attempts", repr(budget), "--nu-attempts", repr(nu), "--nu-div", repr(div),
+           "--rng", "xoshiro-ref", "--curve"]
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as tmp:
+        curve = os.path.join(tmp, "k.csv")
+        r = subprocess.run(cmd + [curve], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        rows = np.loadtxt(curve, delimiter=",", skiprows=1, dtype=np.int64, ndmin=2)
+    n, npri = len(want["idx"]), want["n_primary"]
+    assert re.search(rf"done: N={n} \(primary {npri}, nu {n - npri}\) in {want['attempts']} attempts", r.stderr)
+    assert np.array_equal(rows[:, 0], want["idx"] + 1) and np.array_equal(rows[:, 2], want["species"])
+    bad = subprocess.run(cmd[:-1] + ["--rng", "mt19937"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode != 0 and "--rng must be" in bad.stderr

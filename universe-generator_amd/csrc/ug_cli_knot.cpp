@@ -8,9 +8,12 @@
 // dump with one row per strand: species, attempt, then per axis a2 and the nw (b, a, f)
 // triples.  Candidates come from this packer's Philox stream (its canonical draw, so slot j
 // carries frequency j + 2 in the full band), evaluated and compared in FP64 through
-// libugpack.so (include/ugpack.h).  One flag beyond the reference's: --precision fp64|split
+// libugpack.so (include/ugpack.h).  Two flags beyond the reference's: --precision fp64|split
 // (default split: full-band models go through the split-BF16 prefilter first; the same
-// strands either way).
+// strands either way), and --rng philox|xoshiro-ref: xoshiro-ref draws the reference tool's
+// own candidates (its xoshiro256** stream and draw order, cuda/braid_knot.cu:151-166,207),
+// so the strands can be compared with a reference run of the same seed; it runs in the FP64
+// form, which it selects unless --precision says otherwise.
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -36,7 +39,8 @@ int main(int argc, char** argv) {
     double budget = 1e10, nu_attempts = 0.0, nu_div = 1.0;
     const char* curve_path = nullptr;
     const char* dump_path = nullptr;
-    int precision = UG_PREC_SPLIT_BF16;
+    int precision = UG_PREC_SPLIT_BF16, rng = UG_RNG_PHILOX;
+    bool precision_set = false;
     for (int i = 1; i < argc; i++) {
         const char* f = argv[i];
         const bool has = i + 1 < argc;
@@ -70,12 +74,26 @@ int main(int argc, char** argv) {
                 fprintf(stderr, "error: --precision must be fp64|split\n");
                 return 1;
             }
+            precision_set = true;
+        }
+        else if (!strcmp(f, "--rng") && has) {
+            const char* r = argv[++i];
+            if (!strcmp(r, "philox"))
+                rng = UG_RNG_PHILOX;
+            else if (!strcmp(r, "xoshiro-ref"))
+                rng = UG_RNG_XOSHIRO_REF;
+            else {
+                fprintf(stderr, "error: --rng must be philox|xoshiro-ref\n");
+                return 1;
+            }
         }
         else {
             fprintf(stderr, "unknown flag %s\n", f);
             return 1;
         }
     }
+    if (rng == UG_RNG_XOSHIRO_REF && !precision_set)
+        precision = UG_PREC_FP64;
     if (nu_div < 1.0) {
         fprintf(stderr, "error: --nu-div must be >= 1\n");
         return 1;
@@ -103,6 +121,7 @@ int main(int argc, char** argv) {
     cfg.nu_attempts = nu_attempts;
     cfg.nu_div = nu_div;
     cfg.precision = precision;
+    cfg.rng_kind = rng;
     ug_knot* h = nullptr;
     if (ug_knot_create(&cfg, nullptr, &h) != UG_OK) {
         fprintf(stderr, "%s\n", ug_last_error());
