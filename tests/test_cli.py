@@ -85,3 +85,5 @@ def test_knot_argv_errors():
     assert rc == 1 and "--dim must be 2 or 3" in err
     rc, err = run(exe, "-t", "16", "--terms", "40")
     assert rc == 1 and "terms 40" in err
+    rc, err = run(exe, "--rng", "mt19937")  # the one flag of its own besides --precision
+    assert rc == 1 and "--rng must be philox|xoshiro-ref" in err
